@@ -1,0 +1,169 @@
+/*
+ * inspire_mi355x.h -- C ABI of the MI355X (gfx950) InsPIRe server respond path.
+ *
+ * This is the drop-in boundary for the respond path of igor53627/inspire-rs
+ * (Rust crate `inspire` 0.1.2).  Each entry point replaces one public Rust
+ * function; the replaced function is cited as file:line in the reference.  A
+ * Rust caller binds these with `extern "C"`.
+ *
+ * Conventions
+ *  - Polynomials cross the boundary as the reference's `Poly.coeffs`
+ *    (src/math/poly.rs:63-77): uint64_t[K][d], residue-major, canonical
+ *    residues in [0, q_k).  An RLWE ciphertext is (a, b) = uint64_t[2][K][d];
+ *    an RGSW ciphertext is its 2*ell rows, uint64_t[2*ell][2][K][d]
+ *    (src/rgsw/types.rs:104-110).
+ *  - NTT-domain values crossing the boundary are in the reference's
+ *    representation (Montgomery form with R = 2^64, src/math/ntt.rs:223-246).
+ *  - Every call returns INSPIRE_OK (0) or an error code; the message text of
+ *    the last error on the calling thread is inspire_last_error().  Error
+ *    codes that mirror the reference's PirError strings (src/pir/error.rs)
+ *    carry the same text.
+ *  - All inputs are borrowed and read-only; outputs are caller-owned buffers.
+ *    A context may be shared by several threads: calls are serialised inside.
+ *  - The context keeps the preprocessed databases, CRS-derived packing tables
+ *    and registered client keys resident in device memory (HBM).
+ */
+#ifndef INSPIRE_MI355X_H
+#define INSPIRE_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define INSPIRE_OK 0
+#define INSPIRE_ERR_INVALID_ARGUMENT 1
+#define INSPIRE_ERR_INVALID_PARAMS 2       /* InspireParams::validate (src/params.rs:348-387) */
+#define INSPIRE_ERR_UNSUPPORTED 3          /* valid for the reference, not on this path (message says why) */
+#define INSPIRE_ERR_SHARD_NOT_FOUND 4      /* "Shard {id} not found" (src/pir/respond.rs:82) */
+#define INSPIRE_ERR_PACKING_KEYS_MISSING 5 /* "InspiRING client packing keys missing from query" (respond.rs:299) */
+#define INSPIRE_ERR_PACKING_KEYS_EMPTY 6   /* "InspiRING packing keys invalid: y_all and y_body are both empty" (respond.rs:367-369) */
+#define INSPIRE_ERR_PACKING_KEYS_MISSING_TREE 7 /* "InspiRING packing keys missing (set packing_mode=tree to use tree packing)" (respond.rs:141-143) */
+#define INSPIRE_ERR_UNKNOWN_DATABASE 8
+#define INSPIRE_ERR_OUT_OF_MEMORY 9
+#define INSPIRE_ERR_DEVICE 10
+#define INSPIRE_ERR_CAPACITY 11            /* an output buffer is too small */
+
+/* InspireVariant (src/params.rs:78-103) */
+#define INSPIRE_VARIANT_NO_PACKING 0
+#define INSPIRE_VARIANT_ONE_PACKING 1
+#define INSPIRE_VARIANT_TWO_PACKING 2
+
+/* PackingMode (src/pir/query.rs:27-40) */
+#define INSPIRE_PACKING_INSPIRING 0
+#define INSPIRE_PACKING_TREE 1
+
+/* inspire_params.flags */
+/* Apply NTT-domain automorphisms to every residue (a corrected mode).  The
+ * default reproduces the reference bit for bit, including its rebuild of
+ * residue 1 from residue 0 in apply_automorphism_ntt
+ * (src/inspiring/inspiring2.rs:279-296). */
+#define INSPIRE_FLAG_RESIDUE_WISE_AUTOMORPHISM 0x1u
+
+typedef struct inspire_ctx inspire_ctx;
+
+/* InspireParams (src/params.rs:134-185): the fields the respond path reads.
+ * sigma and security_level only matter to setup and clients. */
+typedef struct {
+    uint32_t ring_dim;       /* d: 256, 512, 1024, 2048 or 4096 on this path */
+    uint32_t crt_count;      /* 1 or 2; each modulus below 2^30 on this path */
+    uint64_t crt_moduli[2];
+    uint64_t q;              /* product of crt_moduli */
+    uint64_t p;              /* plaintext modulus */
+    uint64_t gadget_base;    /* power of two */
+    uint32_t gadget_len;     /* ell */
+    uint32_t flags;          /* INSPIRE_FLAG_* */
+} inspire_params;
+
+/* Query: ClientQuery / SeededClientQuery (src/pir/query.rs:119-150). */
+typedef struct {
+    uint32_t db_id;          /* which resident EncodedDatabase */
+    uint32_t shard_id;       /* ClientQuery.shard_id */
+    uint32_t seeded;         /* 0: ClientQuery, 1: SeededClientQuery */
+    uint32_t packing_mode;   /* INSPIRE_PACKING_* */
+    /* seeded == 0: rgsw = RgswCiphertext rows, uint64_t[2*ell][2][K][d].
+     * seeded == 1: rgsw = the b polys of the SeededRlweCiphertext rows,
+     *              uint64_t[2*ell][K][d], and rgsw_seeds = uint8_t[2*ell][32]. */
+    const uint64_t *rgsw;
+    const uint8_t *rgsw_seeds;
+    /* inspiring_packing_keys: Option<ClientPackingKeys> (inspiring2.rs:603-628) */
+    uint32_t has_packing_keys;
+    uint32_t y_all_len;      /* number of y_all rotations (0 = empty) */
+    const uint64_t *y_body;  /* uint64_t[ell][K][d], or NULL when empty */
+    const uint64_t *y_all;   /* uint64_t[y_all_len][ell][K][d], or NULL */
+} inspire_query;
+
+/* ServerResponse (src/pir/respond.rs:30-37). */
+typedef struct {
+    uint64_t *ciphertext;          /* out: uint64_t[2][K][d] */
+    uint64_t *column_ciphertexts;  /* out: uint64_t[column_capacity][2][K][d], may be NULL */
+    uint32_t column_capacity;
+    uint32_t num_column_ciphertexts; /* out: the reference's column_ciphertexts.len() */
+} inspire_response;
+
+/* ---- context ---- */
+
+/* Validates the parameters (params.rs:348-387), builds the NTT tables for the
+ * reference's roots (ntt.rs:116-164) and the InspiRING tables derived from the
+ * CRS seed w_seed (ServerCrs.inspiring_w_seed, src/pir/setup.rs:197-201). */
+int inspire_ctx_create(const inspire_params *params, const uint8_t w_seed[32], inspire_ctx **out);
+void inspire_ctx_destroy(inspire_ctx *ctx);
+const char *inspire_last_error(void);
+/* Device memory held by the context, in bytes. */
+uint64_t inspire_ctx_device_bytes(const inspire_ctx *ctx);
+
+/* ---- resident databases: EncodedDatabase / ShardData (src/pir/setup.rs:105-120) ---- */
+
+/* Loads shards given as encoded polynomials: polys = uint64_t[num_shards]
+ * [polys_per_shard][K][d] (ShardData.polynomials), shard_ids = ShardData.id.
+ * polys_per_shard == 0 loads empty shards.  Replaces a database with the same
+ * db_id.  The preprocessing (Delta scaling, gadget digits, forward NTT) runs on
+ * the device; the digits stay resident. */
+int inspire_db_load(inspire_ctx *ctx, uint32_t db_id, uint32_t num_shards, const uint32_t *shard_ids,
+                    uint32_t polys_per_shard, const uint64_t *polys);
+/* encode_database (src/pir/encode_db.rs:106-164) of raw entry bytes followed
+ * by inspire_db_load; shards hold d entries (setup.rs:149-153). */
+int inspire_db_load_entries(inspire_ctx *ctx, uint32_t db_id, const uint8_t *data, uint64_t len,
+                            uint32_t entry_size);
+int inspire_db_unload(inspire_ctx *ctx, uint32_t db_id);
+
+/* ---- respond (src/pir/respond.rs) ---- */
+
+/* respond (respond.rs:71-115): InsPIRe^0, full query. */
+int inspire_respond(inspire_ctx *ctx, const inspire_query *q, inspire_response *r);
+/* respond_seeded (respond.rs:408-415). */
+int inspire_respond_seeded(inspire_ctx *ctx, const inspire_query *q, inspire_response *r);
+/* respond_inspiring (respond.rs:284-386): InspiRING packed response. */
+int inspire_respond_inspiring(inspire_ctx *ctx, const inspire_query *q, inspire_response *r);
+/* respond_seeded_inspiring (respond.rs:391-398). */
+int inspire_respond_seeded_inspiring(inspire_ctx *ctx, const inspire_query *q, inspire_response *r);
+/* respond_with_variant (respond.rs:129-151) and respond_seeded_with_variant
+ * (respond.rs:161-183); q->seeded selects between them. */
+int inspire_respond_with_variant(inspire_ctx *ctx, const inspire_query *q, uint32_t variant,
+                                 inspire_response *r);
+/* Many queries (any variants, databases and shards) in one batched launch
+ * sequence.  status[i] receives each query's code; the return value is
+ * INSPIRE_OK when the batch ran (individual queries may still have failed). */
+int inspire_respond_batch(inspire_ctx *ctx, uint32_t n, const inspire_query *qs, const uint32_t *variants,
+                          inspire_response *rs, int32_t *status);
+
+/* ---- math entry points (NttContext, external_product) ---- */
+
+/* NttContext::forward / inverse (src/math/ntt.rs:223-246, 308-325) over
+ * `count` polynomials uint64_t[count][K][d], in place, reference NTT-domain
+ * representation. */
+int inspire_ntt_forward(inspire_ctx *ctx, uint64_t *polys, uint64_t count);
+int inspire_ntt_inverse(inspire_ctx *ctx, uint64_t *polys, uint64_t count);
+/* external_product (src/rgsw/external_product.rs:88-150) for `count`
+ * independent pairs: rlwe uint64_t[count][2][K][d], rgsw
+ * uint64_t[count][2*ell][2][K][d] -> out uint64_t[count][2][K][d]. */
+int inspire_external_product(inspire_ctx *ctx, uint64_t count, const uint64_t *rlwe, const uint64_t *rgsw,
+                             uint64_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

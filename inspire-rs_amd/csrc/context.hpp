@@ -1,0 +1,93 @@
+// Host-side context of the respond path: parameters, device tables, resident
+// databases, packing tables and workspaces.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/inspire_mi355x.h"
+#include "kernels.hpp"
+
+namespace inspire {
+
+// Error carrying an inspire status code and the reference's message text.
+struct Error {
+    int code;
+    std::string msg;
+};
+
+void set_last_error(const std::string& msg);
+
+// Owning device allocation.
+class DevBuf {
+   public:
+    DevBuf() = default;
+    explicit DevBuf(size_t bytes) { alloc(bytes); }
+    ~DevBuf() { release(); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) { release(); p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0; }
+        return *this;
+    }
+    void alloc(size_t bytes);
+    // Grows to at least `bytes` (contents are not preserved).
+    void reserve(size_t bytes) { if (bytes > n_) { release(); alloc(bytes); } }
+    void release();
+    template <typename T = void>
+    T* get() const { return reinterpret_cast<T*>(p_); }
+    size_t bytes() const { return n_; }
+
+   private:
+    void* p_ = nullptr;
+    size_t n_ = 0;
+};
+
+// One resident EncodedDatabase.
+struct Database {
+    uint32_t num_shards = 0;
+    uint32_t cols = 0;                                // polys per shard (gamma)
+    std::unordered_map<uint32_t, uint32_t> slot_of;   // ShardData.id -> slot (first match)
+    DevBuf digits;                                    // [shard][col][ell][K][N] u32, NTT domain
+};
+
+// InspiRING tables for one number of packed columns gamma (PackParams +
+// OfflinePackingKeys, inspiring2.rs:107-182, 448-506), resident.
+struct PackTables {
+    uint32_t gamma = 0;
+    DevBuf step1;      // [gamma][gamma][K][N] u32: gamma^-1 * NTT(X^e_ij), Montgomery form
+    DevBuf perm;       // [gamma][N] u32: automorphism table for g^i
+    DevBuf w_all;      // [(gamma-1)][ell][K][N] u32 NTT domain, Montgomery form
+    std::vector<uint32_t> gen_pows;  // g^i mod 2n, i < n
+};
+
+struct HostModulus {
+    uint64_t q = 0, psi = 0;
+    std::vector<uint32_t> tw, twp, itw, itwp;
+};
+
+}  // namespace inspire
+
+struct inspire_ctx {
+    inspire_params params{};
+    uint32_t N = 0, K = 0, logn = 0, ell = 0;
+    uint64_t q = 0, delta = 0;
+    uint8_t w_seed[32]{};
+    inspire::HostModulus hm[inspire::kMaxK];
+    inspire::DevBuf tables;   // twiddles of every modulus
+    inspire::CtxDev dev{};
+    hipStream_t stream = nullptr;
+    std::recursive_mutex mu;
+    std::map<uint32_t, inspire::Database> dbs;
+    std::map<uint32_t, std::unique_ptr<inspire::PackTables>> packs;
+    // scratch reused across calls
+    inspire::DevBuf ws0, ws1, ws2, ws3, ws4;
+    std::vector<uint32_t> hbuf;
+};

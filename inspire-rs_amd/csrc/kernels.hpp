@@ -1,0 +1,49 @@
+// Host-visible kernel interface: constant bundles passed by value and launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "ntt.hpp"
+
+namespace inspire {
+
+constexpr int kMaxK = 2;      // CRT moduli
+constexpr int kMaxEll = 4;    // gadget digits
+
+// Everything a kernel needs about the parameter set.
+struct CtxDev {
+    ModDev m[kMaxK];
+    uint32_t K, logn, n, ell;
+    uint32_t gadget_log;          // log2(gadget_base), at most 32
+    uint32_t gadget_mask;         // gadget_base - 1
+    uint32_t q0_inv_q1, q0_inv_q1p;  // CRT: q0^-1 mod q1 (Shoup pair, mod q1)
+    uint32_t one_q1p;                // Shoup companion of 1 mod q1 (x mod q1 for x < 2^32)
+    uint64_t q;                   // composite modulus
+    uint32_t delta[kMaxK], deltap[kMaxK];  // Delta mod q_k (Shoup pairs)
+    uint32_t mont[kMaxK], montp[kMaxK];    // 2^32 mod q_k (Montgomery-form conversion)
+};
+
+// Per-residue output scaling.  Forward: x -> c x when enable.  Inverse: the
+// last stage computes c (X + Y) and cw (X - Y).
+struct ScaleDev {
+    uint32_t enable;
+    uint32_t c[kMaxK], cp[kMaxK], cw[kMaxK], cwp[kMaxK];
+};
+
+hipError_t launch_ntt(uint32_t* data, size_t count, const CtxDev& C, const ScaleDev& sc, bool inverse,
+                      hipStream_t st);
+hipError_t launch_db_digits(const uint32_t* in, uint32_t* out, size_t P, const CtxDev& C, hipStream_t st);
+hipError_t launch_rlwe_digits(const uint32_t* in, uint32_t* out, size_t P, const CtxDev& C, hipStream_t st);
+hipError_t launch_first_dim(const uint32_t* digits, const uint32_t* rows, const int32_t* slot, uint32_t* out,
+                            uint32_t Q, uint32_t cols, uint32_t out_stride, uint32_t out_off, const CtxDev& C,
+                            hipStream_t st);
+hipError_t launch_ext_mac(const uint32_t* dig, const uint32_t* rgsw, uint32_t* out, size_t P, const CtxDev& C,
+                          hipStream_t st);
+// seeds [count][8] u32 key words; seed i expands into out + i*out_stride ([K][N]).
+hipError_t launch_expand_seeds(const uint32_t* seeds, uint32_t* out, size_t count, size_t out_stride, uint32_t* err,
+                               const CtxDev& C, hipStream_t st);
+hipError_t launch_sum_cols(uint32_t* resp, uint32_t Q, uint32_t cols, const CtxDev& C, hipStream_t st);
+
+}  // namespace inspire

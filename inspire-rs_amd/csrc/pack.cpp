@@ -1,0 +1,339 @@
+// InspiRING tables and the packed-respond driver (respond_inspiring,
+// src/pir/respond.rs:284-386).
+//
+// Everything query-independent that the reference rebuilds per query is built
+// once per (context, gamma) and kept resident:
+//   PackParams::new             (inspiring2.rs:107-182): generator, g^i, the
+//                               NTT-domain automorphism tables, gamma^-1 and
+//                               the NTT-form monomials, folded here into the
+//                               step-1 table gamma^-1 * NTT(X^(j g^(n-i)));
+//   OfflinePackingKeys::generate (inspiring2.rs:448-506): w_mask from w_seed
+//                               (ChaCha20, inspiring2.rs:816-835) and its
+//                               rotations w_all_ntt.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "context.hpp"
+#include "kernels.hpp"
+#include "modarith.hpp"
+#include "pack.hpp"
+#include "respond.hpp"
+
+namespace inspire {
+
+namespace {
+
+uint32_t bitrev(uint32_t x, uint32_t bits) {
+    uint32_t r = 0;
+    for (uint32_t i = 0; i < bits; i++) { r = (r << 1) | (x & 1); x >>= 1; }
+    return r;
+}
+
+// ChaCha20 block, 64-bit counter in words 12-13, stream 0 (rand_chacha 0.3).
+void chacha_block(const uint32_t key[8], uint64_t ctr, uint32_t out[16]) {
+    uint32_t x[16] = {0x61707865u, 0x3320646eu, 0x79622d32u, 0x6b206574u, key[0], key[1], key[2], key[3],
+                      key[4], key[5], key[6], key[7], (uint32_t)ctr, (uint32_t)(ctr >> 32), 0u, 0u};
+    uint32_t s[16];
+    std::memcpy(s, x, sizeof(s));
+    auto rotl = [](uint32_t v, int n) { return (v << n) | (v >> (32 - n)); };
+    auto qr = [&](int a, int b, int c, int d) {
+        x[a] += x[b]; x[d] = rotl(x[d] ^ x[a], 16);
+        x[c] += x[d]; x[b] = rotl(x[b] ^ x[c], 12);
+        x[a] += x[b]; x[d] = rotl(x[d] ^ x[a], 8);
+        x[c] += x[d]; x[b] = rotl(x[b] ^ x[c], 7);
+    };
+    for (int r = 0; r < 10; r++) {
+        qr(0, 4, 8, 12); qr(1, 5, 9, 13); qr(2, 6, 10, 14); qr(3, 7, 11, 15);
+        qr(0, 5, 10, 15); qr(1, 6, 11, 12); qr(2, 7, 8, 13); qr(3, 4, 9, 14);
+    }
+    for (int i = 0; i < 16; i++) out[i] = x[i] + s[i];
+}
+
+// ChaCha20Rng::next_u64 stream: u64 t = word[2t] | word[2t+1] << 32.
+struct ChaChaStream {
+    uint32_t key[8];
+    uint64_t block = 0;
+    uint32_t buf[16];
+    int idx = 16;
+    explicit ChaChaStream(const uint8_t seed[32]) {
+        for (int i = 0; i < 8; i++)
+            key[i] = (uint32_t)seed[4 * i] | ((uint32_t)seed[4 * i + 1] << 8) | ((uint32_t)seed[4 * i + 2] << 16) |
+                     ((uint32_t)seed[4 * i + 3] << 24);
+    }
+    uint64_t next_u64() {
+        if (idx >= 16) { chacha_block(key, block++, buf); idx = 0; }
+        uint64_t v = (uint64_t)buf[idx] | ((uint64_t)buf[idx + 1] << 32);
+        idx += 2;
+        return v;
+    }
+};
+
+// apply_automorphism (galois.rs:22-55) on composite coefficients, exactly as
+// written: also covers even exponents, where the map is not a permutation.
+std::vector<uint64_t> coeff_automorphism(const inspire_ctx* ctx, const uint64_t* poly, uint64_t g) {
+    const uint32_t d = ctx->N;
+    const uint64_t q = ctx->q;
+    std::vector<uint64_t> res(d, 0), outp((size_t)ctx->K * d);
+    const uint64_t q0 = ctx->hm[0].q;
+    for (uint32_t i = 0; i < d; i++) {
+        uint64_t c = poly[i];
+        if (ctx->K == 2) {  // Poly::coeff (poly.rs:359-372)
+            const uint64_t q1 = ctx->hm[1].q;
+            const uint64_t a1 = poly[d + i], a0m = c % q1;
+            const uint64_t diff = a1 >= a0m ? a1 - a0m : a1 + q1 - a0m;
+            c = c + q0 * mulmod64(diff, powmod64(q0 % q1, q1 - 2, q1), q1);
+            c %= q;
+        }
+        if (c == 0) continue;
+        const uint64_t ni = (g * i) % (2ull * d);
+        if (ni < d) res[ni] = (uint64_t)(((unsigned __int128)res[ni] + c) % q);
+        else {
+            const uint64_t ai = ni - d;
+            res[ai] = res[ai] >= c ? res[ai] - c : q - c + res[ai];
+        }
+    }
+    for (uint32_t k = 0; k < ctx->K; k++)
+        for (uint32_t i = 0; i < d; i++) outp[(size_t)k * d + i] = res[i] % ctx->hm[k].q;
+    return outp;
+}
+
+uint64_t mod_inverse_u64(uint64_t a, uint64_t m) {  // inspiring2.rs:1633-1640
+    __int128 r0 = a, r1 = m, s0 = 1, s1 = 0;
+    while (r1 != 0) {
+        __int128 qq = r0 / r1, t;
+        t = r1; r1 = r0 - qq * r1; r0 = t;
+        t = s1; s1 = s0 - qq * s1; s0 = t;
+    }
+    if (r0 != 1) throw Error{INSPIRE_ERR_UNSUPPORTED, "num_to_pack must be invertible mod q"};
+    __int128 x = s0 % (__int128)m;
+    if (x < 0) x += m;
+    return (uint64_t)x;
+}
+
+struct GenInfo {
+    uint32_t generator;
+    std::vector<uint32_t> gen_pows;  // [n]
+};
+
+GenInfo gen_info(uint32_t n, uint32_t gamma) {  // inspiring2.rs:114-128
+    GenInfo g;
+    g.generator = gamma < n ? (2 * n / gamma) + 1 : 5;
+    g.gen_pows.resize(n);
+    uint64_t v = 1;
+    for (uint32_t i = 0; i < n; i++) { g.gen_pows[i] = (uint32_t)v; v = (v * g.generator) % (2ull * n); }
+    return g;
+}
+
+}  // namespace
+
+PackTables& pack_tables(inspire_ctx* ctx, uint32_t gamma) {
+    auto it = ctx->packs.find(gamma);
+    if (it != ctx->packs.end()) return *it->second;
+    const uint32_t n = ctx->N, K = ctx->K, ell = ctx->ell, logn = ctx->logn;
+    const size_t KN = (size_t)K * n;
+    std::unique_ptr<PackTables> T(new PackTables());
+    T->gamma = gamma;
+    const GenInfo G = gen_info(n, gamma);
+    // automorphism tables, get_automorph_table(t) = tables[(t-1)/2] (inspiring2.rs:186-198)
+    std::vector<uint32_t> perm((size_t)gamma * n);
+    for (uint32_t i = 0; i < gamma; i++) {
+        const uint64_t t = 2ull * ((G.gen_pows[i] - 1) / 2) + 1;
+        for (uint32_t s = 0; s < n; s++) {
+            const uint64_t e = (t * (2ull * bitrev(s, logn) + 1)) % (2ull * n);
+            perm[(size_t)i * n + s] = bitrev((uint32_t)((e - 1) / 2), logn);
+        }
+    }
+    // step-1 table: gamma^-1 * NTT(X^e_ij), e_ij = j * g^((n-i) mod n) mod 2n, Montgomery form
+    const uint64_t ginv = mod_inverse_u64(gamma, ctx->q);
+    std::vector<uint32_t> step1((size_t)gamma * gamma * KN);
+    for (uint32_t k = 0; k < K; k++) {
+        const uint64_t q = ctx->hm[k].q, psi = ctx->hm[k].psi;
+        const uint64_t scale = mulmod64(ginv % q, (1ull << 32) % q, q);
+        for (uint32_t s = 0; s < n; s++) {
+            const uint64_t w = powmod64(psi, 2ull * bitrev(s, logn) + 1, q);  // NTT slot s evaluates at w
+            for (uint32_t i = 0; i < gamma; i++) {
+                const uint64_t gp = G.gen_pows[(n - i) % n];
+                for (uint32_t j = 0; j < gamma; j++) {
+                    const uint64_t e = ((uint64_t)j * gp) % (2ull * n);
+                    step1[(((size_t)i * gamma + j) * K + k) * n + s] = (uint32_t)mulmod64(powmod64(w, e, q), scale, q);
+                }
+            }
+        }
+    }
+    // w_mask (inspiring2.rs:816-835) -> NTT on the device -> rotations with the
+    // reference's residue handling -> Montgomery form.
+    std::vector<uint64_t> mask((size_t)ell * KN);
+    {
+        ChaChaStream rng(ctx->w_seed);
+        for (uint32_t d = 0; d < ell; d++)
+            for (uint32_t i = 0; i < n; i++) {
+                const uint64_t v = rng.next_u64() % ctx->q;
+                for (uint32_t k = 0; k < K; k++) mask[(size_t)d * KN + (size_t)k * n + i] = v % ctx->hm[k].q;
+            }
+    }
+    DevBuf tmp((size_t)ell * KN * 4);
+    upload_u64(ctx, mask.data(), (size_t)ell * KN, tmp.get<uint32_t>());
+    hip_check(launch_ntt(tmp.get<uint32_t>(), ell, ctx->dev, scale_none(), false, ctx->stream), "mask ntt");
+    std::vector<uint64_t> mask_ntt((size_t)ell * KN);
+    download_u64(ctx, tmp.get<uint32_t>(), (size_t)ell * KN, mask_ntt.data());
+    const uint32_t nrot = gamma > 0 ? gamma - 1 : 0;
+    std::vector<uint32_t> w_all((size_t)std::max(nrot, 1u) * ell * KN, 0);
+    const bool rw = (ctx->params.flags & INSPIRE_FLAG_RESIDUE_WISE_AUTOMORPHISM) != 0;
+    const uint64_t q0 = ctx->hm[0].q;
+    const uint64_t r64_q0 = pow2_64_mod(q0);
+    for (uint32_t i = 0; i < nrot; i++)
+        for (uint32_t d = 0; d < ell; d++)
+            for (uint32_t s = 0; s < n; s++) {
+                const uint32_t src = perm[(size_t)i * n + s];
+                const uint64_t v0 = mask_ntt[(size_t)d * KN + src];
+                uint32_t* dst = w_all.data() + ((size_t)i * ell + d) * KN;
+                dst[s] = (uint32_t)mulmod64(v0, (1ull << 32) % q0, q0);
+                if (K == 2) {
+                    const uint64_t q1 = ctx->hm[1].q;
+                    uint64_t v1;
+                    if (rw) v1 = mask_ntt[(size_t)d * KN + n + src];
+                    else {  // apply_automorphism_ntt: residue 1 := (v0 * 2^64 mod q0) mod q1, a 2^64 Montgomery value
+                        const uint64_t ref = mulmod64(v0, r64_q0, q0) % q1;
+                        v1 = mulmod64(ref, powmod64(pow2_64_mod(q1), q1 - 2, q1), q1);
+                    }
+                    dst[n + s] = (uint32_t)mulmod64(v1, (1ull << 32) % q1, q1);
+                }
+            }
+    T->perm.alloc(perm.size() * 4);
+    T->step1.alloc(step1.size() * 4);
+    T->w_all.alloc(w_all.size() * 4);
+    hip_check(hipMemcpy(T->perm.get(), perm.data(), perm.size() * 4, hipMemcpyHostToDevice), "perm");
+    hip_check(hipMemcpy(T->step1.get(), step1.data(), step1.size() * 4, hipMemcpyHostToDevice), "step1");
+    hip_check(hipMemcpy(T->w_all.get(), w_all.data(), w_all.size() * 4, hipMemcpyHostToDevice), "w_all");
+    T->gen_pows = G.gen_pows;
+    auto& ref = *T;
+    ctx->packs[gamma] = std::move(T);
+    return ref;
+}
+
+void pack_respond(inspire_ctx* ctx, const Database& db, const BatchItem* items, uint32_t Q, const uint32_t* rows,
+                  const int32_t* d_slot, const uint32_t* d_err, const inspire_query* qs, inspire_response* rs,
+                  int32_t* status) {
+    const uint32_t gamma = db.cols, n = ctx->N, K = ctx->K, ell = ctx->ell;
+    const size_t KN = (size_t)K * n;
+    PackTables& T = pack_tables(ctx, gamma);
+    const bool rw = (ctx->params.flags & INSPIRE_FLAG_RESIDUE_WISE_AUTOMORPHISM) != 0;
+    const uint32_t nrot = gamma - 1;
+    bool odd_gens = true;
+    for (uint32_t i = 0; i < nrot; i++) odd_gens &= (T.gen_pows[i] & 1) != 0;
+
+    // ---- client keys: y_body (rotated on the device) or explicit y_all
+    std::vector<uint64_t> y_off(Q);
+    std::vector<uint32_t> y_mode(Q), y_count(Q);
+    size_t y_words = 0;
+    for (uint32_t t = 0; t < Q; t++) {
+        const inspire_query& q = qs[items[t].idx];
+        y_off[t] = y_words;
+        if (q.y_all_len > 0) {
+            y_mode[t] = 1;
+            y_count[t] = std::min(q.y_all_len, nrot);
+            y_words += (size_t)std::max(y_count[t], 1u) * ell * KN;
+        } else if (odd_gens) {  // y_all[i] = tau_{g^i}(y_body): a permutation of NTT(y_body)
+            y_mode[t] = 0;
+            y_count[t] = nrot;
+            y_words += (size_t)ell * KN;
+        } else {  // generate_rotations with even exponents, computed as written
+            y_mode[t] = 1;
+            y_count[t] = nrot;
+            y_words += (size_t)std::max(nrot, 1u) * ell * KN;
+        }
+    }
+    std::vector<uint32_t> hy(y_words, 0);
+    for (uint32_t t = 0; t < Q; t++) {
+        const inspire_query& q = qs[items[t].idx];
+        uint32_t* dst = hy.data() + y_off[t];
+        if (q.y_all_len > 0) {
+            for (size_t w = 0; w < (size_t)y_count[t] * ell * KN; w++) dst[w] = (uint32_t)q.y_all[w];
+        } else if (y_mode[t] == 0) {
+            for (size_t w = 0; w < (size_t)ell * KN; w++) dst[w] = (uint32_t)q.y_body[w];
+        } else {
+            for (uint32_t i = 0; i < nrot; i++)
+                for (uint32_t d = 0; d < ell; d++) {
+                    std::vector<uint64_t> r = coeff_automorphism(ctx, q.y_body + (size_t)d * KN, T.gen_pows[i]);
+                    for (size_t w = 0; w < KN; w++) dst[((size_t)i * ell + d) * KN + w] = (uint32_t)r[w];
+                }
+        }
+    }
+    // ---- device workspace
+    const size_t cols_words = (size_t)Q * 2 * gamma * KN, rpre_words = (size_t)Q * gamma * KN;
+    const size_t out_words = (size_t)Q * 2 * KN, bp_words = (size_t)Q * K * gamma;
+    ctx->ws2.reserve(cols_words * 4);
+    ctx->ws3.reserve((rpre_words + out_words + bp_words) * 4);
+    ctx->ws0.reserve(std::max<size_t>(y_words, 1) * 4 + (size_t)Q * 24 + 64);
+    uint32_t* d_cols = ctx->ws2.get<uint32_t>();
+    uint32_t* d_rpre = ctx->ws3.get<uint32_t>();
+    uint32_t* d_out = d_rpre + rpre_words;
+    uint32_t* d_bp = d_out + out_words;
+    uint32_t* d_y = ctx->ws0.get<uint32_t>();
+    uint8_t* meta = reinterpret_cast<uint8_t*>(d_y + y_words);
+    uint64_t* d_yoff = reinterpret_cast<uint64_t*>(meta + ((8 - ((uintptr_t)meta & 7)) & 7));
+    uint32_t* d_ymode = reinterpret_cast<uint32_t*>(d_yoff + Q);
+    uint32_t* d_ycount = d_ymode + Q;
+    hip_check(hipMemcpyAsync(d_y, hy.data(), y_words * 4, hipMemcpyHostToDevice, ctx->stream), "y");
+    hip_check(hipMemcpyAsync(d_yoff, y_off.data(), Q * 8, hipMemcpyHostToDevice, ctx->stream), "y off");
+    hip_check(hipMemcpyAsync(d_ymode, y_mode.data(), Q * 4, hipMemcpyHostToDevice, ctx->stream), "y mode");
+    hip_check(hipMemcpyAsync(d_ycount, y_count.data(), Q * 4, hipMemcpyHostToDevice, ctx->stream), "y count");
+    uint64_t mont[kMaxK];
+    for (uint32_t k = 0; k < K; k++) mont[k] = (1ull << 32) % ctx->hm[k].q;
+    hip_check(launch_ntt(d_y, y_words / KN, ctx->dev, scale_fwd(ctx, mont), false, ctx->stream), "y ntt");
+
+    // ---- first dimension, b_poly, step 1, chain
+    hip_check(launch_first_dim(db.digits.get<uint32_t>(), rows, d_slot, d_cols, Q, gamma, 2 * gamma, 0, ctx->dev,
+                               ctx->stream),
+              "first dim");
+    hip_check(launch_pack_b0(d_cols, d_bp, Q, gamma, ctx->dev, ctx->stream), "b0");
+    const uint32_t kres = (rw || K == 1) ? K : 1;
+    hip_check(launch_pack_step1(d_cols, T.step1.get<uint32_t>(), d_rpre, Q, gamma, kres, ctx->dev, ctx->stream),
+              "step1");
+    PackArgs A{};
+    A.a_hat_cols = d_cols;
+    A.r_pre = d_rpre;
+    A.perm = T.perm.get<uint32_t>();
+    A.w_all = T.w_all.get<uint32_t>();
+    A.y = d_y;
+    A.y_off = d_yoff;
+    A.y_mode = d_ymode;
+    A.y_count = d_ycount;
+    A.b_poly = d_bp;
+    A.out = d_out;
+    A.gamma = gamma;
+    A.residue_wise = rw ? 1u : 0u;
+    {
+        const uint64_t q0 = ctx->hm[0].q;
+        A.q0r64 = (uint32_t)pow2_64_mod(q0);
+        A.q0r64p = shoup_pre(A.q0r64, (uint32_t)q0);
+        if (K == 2) {
+            const uint64_t q1 = ctx->hm[1].q;
+            A.q1r64i = (uint32_t)powmod64(pow2_64_mod(q1), q1 - 2, q1);
+            A.q1r64ip = shoup_pre(A.q1r64i, (uint32_t)q1);
+            A.one_q1p = shoup_pre(1, (uint32_t)q1);
+        }
+    }
+    hip_check(launch_pack_chain_scaled(A, Q, ctx->dev, scale_inv(ctx, nullptr), ctx->stream), "pack chain");
+    std::vector<uint32_t> hout(out_words);
+    uint32_t herr = 0;
+    hip_check(hipMemcpyAsync(hout.data(), d_out, out_words * 4, hipMemcpyDeviceToHost, ctx->stream), "out");
+    hip_check(hipMemcpyAsync(&herr, d_err, 4, hipMemcpyDeviceToHost, ctx->stream), "err");
+    hip_check(hipStreamSynchronize(ctx->stream), "pack respond");
+    for (uint32_t t = 0; t < Q; t++) {
+        const uint32_t i = items[t].idx;
+        if (herr) {
+            status[i] = INSPIRE_ERR_DEVICE;
+            set_last_error("seed expansion exhausted its draw budget");
+            continue;
+        }
+        inspire_response& r = rs[i];
+        for (size_t w = 0; w < 2 * KN; w++) r.ciphertext[w] = hout[(size_t)t * 2 * KN + w];
+        r.num_column_ciphertexts = 0;  // respond.rs:382-385
+    }
+}
+
+}  // namespace inspire

@@ -1,0 +1,54 @@
+// InspiRING packing on the device (src/inspiring/inspiring2.rs, the partial
+// packing used by respond_inspiring, src/pir/respond.rs:284-386).
+#pragma once
+#include <cstddef>
+#include <cstdint>
+
+#include "context.hpp"
+
+namespace inspire {
+
+// One runnable query of a respond batch (shared with respond.cpp).
+struct BatchItem {
+    uint32_t idx;   // position in the caller's batch
+    int32_t slot;   // shard slot in its database
+    bool packed;    // InsPIRe^1/^2
+};
+
+// Kernel-side description of a packing batch.
+struct PackArgs {
+    const uint32_t* a_hat_cols;  // [Q][2*gamma][K][N]: column c a at 2c, b at 2c+1 (NTT domain)
+    const uint32_t* r_pre;       // [Q][gamma][K][N] step-1 sums before the automorphism
+    const uint32_t* perm;        // [gamma][N] NTT-domain automorphism tables of g^i
+    const uint32_t* w_all;       // [gamma-1][ell][K][N] Montgomery form
+    const uint32_t* y;           // per query: NTT of y_body [ell][K][N] or of y_all [gamma-1][ell][K][N], Montgomery
+    const uint64_t* y_off;       // [Q] word offset of each query's y block
+    const uint32_t* y_mode;      // [Q] 0: rotate y_body by perm, 1: explicit y_all
+    const uint32_t* y_count;     // [Q] rotations available (y_all.len(), or gamma-1 for y_body)
+    const uint32_t* b_poly;      // [Q][K][gamma] b coefficients 0..gamma-1
+    uint32_t* out;               // [Q][2][K][N] packed ciphertext (a_hat, b)
+    uint32_t gamma;
+    uint32_t residue_wise;       // 1: corrected automorphism on every residue
+    uint32_t q0r64, q0r64p;      // 2^64 mod q0 (Shoup pair): plain -> reference Montgomery value
+    uint32_t q1r64i, q1r64ip;    // 2^-64 mod q1 (Shoup pair): reference value -> plain
+    uint32_t one_q1p;            // Shoup companion of 1 mod q1
+};
+
+// Builds (once per gamma) the resident PackParams/OfflinePackingKeys tables.
+PackTables& pack_tables(inspire_ctx* ctx, uint32_t gamma);
+
+// InspiRING tail of a respond batch.  rows: staged RGSW rows (NTT, Montgomery),
+// d_slot: shard slots of the items, d_err: seed-expansion flag.
+void pack_respond(inspire_ctx* ctx, const Database& db, const BatchItem* items, uint32_t Q, const uint32_t* rows,
+                  const int32_t* d_slot, const uint32_t* d_err, const inspire_query* qs, inspire_response* rs,
+                  int32_t* status);
+
+hipError_t launch_pack_b0(const uint32_t* cols, uint32_t* b_poly, uint32_t Q, uint32_t gamma, const CtxDev& C,
+                          hipStream_t st);
+hipError_t launch_pack_step1(const uint32_t* cols, const uint32_t* step1, uint32_t* r_pre, uint32_t Q,
+                             uint32_t gamma, uint32_t kres, const CtxDev& C, hipStream_t st);
+// inv: inverse-NTT scaling (plain inverse: n^-1 folded into the last stage).
+hipError_t launch_pack_chain_scaled(const PackArgs& A, uint32_t Q, const CtxDev& C, const ScaleDev& inv,
+                                    hipStream_t st);
+
+}  // namespace inspire

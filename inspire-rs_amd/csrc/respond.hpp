@@ -1,0 +1,33 @@
+// Respond-path drivers shared by the C ABI entry points.
+#pragma once
+#include <cstdint>
+
+#include "context.hpp"
+
+namespace inspire {
+
+void hip_check(hipError_t e, const char* what);
+
+ScaleDev scale_none();
+ScaleDev scale_fwd(const inspire_ctx* ctx, const uint64_t* c);
+ScaleDev scale_inv(const inspire_ctx* ctx, const uint64_t* c);
+uint64_t pow2_64_mod(uint64_t q);
+
+// u64 residues (reference layout) <-> u32 device words, through the stream.
+void upload_u64(inspire_ctx* ctx, const uint64_t* src, size_t words, uint32_t* dst);
+void download_u64(inspire_ctx* ctx, const uint32_t* src, size_t words, uint64_t* dst);
+
+void db_load(inspire_ctx* ctx, uint32_t db_id, uint32_t num_shards, const uint32_t* shard_ids,
+             uint32_t polys_per_shard, const uint64_t* polys);
+void db_load_entries(inspire_ctx* ctx, uint32_t db_id, const uint8_t* data, uint64_t len, uint32_t entry_size);
+
+void external_product_batch(inspire_ctx* ctx, uint64_t count, const uint64_t* rlwe, const uint64_t* rgsw,
+                            uint64_t* out);
+
+// Runs n queries.  variants[i] selects InsPIRe^0/1/2.  direct_inspiring: the
+// caller is respond_inspiring / respond_seeded_inspiring (no packing_mode
+// dispatch, respond.rs:284-398) rather than respond_with_variant.
+void respond_batch(inspire_ctx* ctx, uint32_t n, const inspire_query* qs, const uint32_t* variants,
+                   inspire_response* rs, int32_t* status, bool direct_inspiring);
+
+}  // namespace inspire

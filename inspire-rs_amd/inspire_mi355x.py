@@ -1,0 +1,388 @@
+"""Python mirror of the reference respond API over the gfx950 C ABI.
+
+Binds libinspire_mi355x.so (include/inspire_mi355x.h) with ctypes.  Names,
+argument meaning and errors follow the Rust crate `inspire`:
+
+    respond(ctx, db_id, query)                 src/pir/respond.rs:71-115
+    respond_seeded(ctx, db_id, query)          src/pir/respond.rs:408-415
+    respond_inspiring(ctx, db_id, query)       src/pir/respond.rs:284-386
+    respond_seeded_inspiring(ctx, db_id, query) src/pir/respond.rs:391-398
+    respond_with_variant(ctx, db_id, query, v) src/pir/respond.rs:129-183
+
+Errors raise PirError carrying the reference's message text
+(src/pir/error.rs).  There is no CPU fallback: importing this module without
+the built library raises, so every call runs on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libinspire_mi355x.so")
+
+Q0, Q1 = 268369921, 249561089  # DEFAULT_CRT_MODULI (src/params.rs:31)
+
+OK = 0
+ERR_INVALID_ARGUMENT = 1
+ERR_INVALID_PARAMS = 2
+ERR_UNSUPPORTED = 3
+ERR_SHARD_NOT_FOUND = 4
+ERR_PACKING_KEYS_MISSING = 5
+ERR_PACKING_KEYS_EMPTY = 6
+ERR_PACKING_KEYS_MISSING_TREE = 7
+ERR_UNKNOWN_DATABASE = 8
+ERR_OUT_OF_MEMORY = 9
+ERR_DEVICE = 10
+ERR_CAPACITY = 11
+
+NO_PACKING, ONE_PACKING, TWO_PACKING = 0, 1, 2  # InspireVariant (src/params.rs:78-103)
+PACKING_INSPIRING, PACKING_TREE = 0, 1  # PackingMode (src/pir/query.rs:27-40)
+FLAG_RESIDUE_WISE_AUTOMORPHISM = 0x1
+
+
+class PirError(Exception):
+    """PirError(String) (src/pir/error.rs:13-45) plus the status code."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(message)
+        self.code = code
+        self.message = message
+
+
+class _Params(C.Structure):
+    _fields_ = [
+        ("ring_dim", C.c_uint32),
+        ("crt_count", C.c_uint32),
+        ("crt_moduli", C.c_uint64 * 2),
+        ("q", C.c_uint64),
+        ("p", C.c_uint64),
+        ("gadget_base", C.c_uint64),
+        ("gadget_len", C.c_uint32),
+        ("flags", C.c_uint32),
+    ]
+
+
+class _Query(C.Structure):
+    _fields_ = [
+        ("db_id", C.c_uint32),
+        ("shard_id", C.c_uint32),
+        ("seeded", C.c_uint32),
+        ("packing_mode", C.c_uint32),
+        ("rgsw", C.POINTER(C.c_uint64)),
+        ("rgsw_seeds", C.POINTER(C.c_uint8)),
+        ("has_packing_keys", C.c_uint32),
+        ("y_all_len", C.c_uint32),
+        ("y_body", C.POINTER(C.c_uint64)),
+        ("y_all", C.POINTER(C.c_uint64)),
+    ]
+
+
+class _Response(C.Structure):
+    _fields_ = [
+        ("ciphertext", C.POINTER(C.c_uint64)),
+        ("column_ciphertexts", C.POINTER(C.c_uint64)),
+        ("column_capacity", C.c_uint32),
+        ("num_column_ciphertexts", C.c_uint32),
+    ]
+
+
+_u64p = C.POINTER(C.c_uint64)
+
+# (name, restype, argtypes) of every entry point in include/inspire_mi355x.h
+EXPORTS = [
+    ("inspire_ctx_create", C.c_int, [C.POINTER(_Params), C.POINTER(C.c_uint8), C.POINTER(C.c_void_p)]),
+    ("inspire_ctx_destroy", None, [C.c_void_p]),
+    ("inspire_last_error", C.c_char_p, []),
+    ("inspire_ctx_device_bytes", C.c_uint64, [C.c_void_p]),
+    ("inspire_db_load", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32), C.c_uint32, _u64p]),
+    ("inspire_db_load_entries", C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint8), C.c_uint64, C.c_uint32]),
+    ("inspire_db_unload", C.c_int, [C.c_void_p, C.c_uint32]),
+    ("inspire_respond", C.c_int, [C.c_void_p, C.POINTER(_Query), C.POINTER(_Response)]),
+    ("inspire_respond_seeded", C.c_int, [C.c_void_p, C.POINTER(_Query), C.POINTER(_Response)]),
+    ("inspire_respond_inspiring", C.c_int, [C.c_void_p, C.POINTER(_Query), C.POINTER(_Response)]),
+    ("inspire_respond_seeded_inspiring", C.c_int, [C.c_void_p, C.POINTER(_Query), C.POINTER(_Response)]),
+    ("inspire_respond_with_variant", C.c_int, [C.c_void_p, C.POINTER(_Query), C.c_uint32, C.POINTER(_Response)]),
+    (
+        "inspire_respond_batch",
+        C.c_int,
+        [C.c_void_p, C.c_uint32, C.POINTER(_Query), C.POINTER(C.c_uint32), C.POINTER(_Response), C.POINTER(C.c_int32)],
+    ),
+    ("inspire_ntt_forward", C.c_int, [C.c_void_p, _u64p, C.c_uint64]),
+    ("inspire_ntt_inverse", C.c_int, [C.c_void_p, _u64p, C.c_uint64]),
+    ("inspire_external_product", C.c_int, [C.c_void_p, C.c_uint64, _u64p, _u64p, _u64p]),
+]
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Loads the native library; raises when it has not been built."""
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} is missing: build it with `make -C inspire-rs_amd` (or __graft_entry__.build()); "
+            "the respond path has no CPU fallback"
+        )
+    lib = C.CDLL(path)
+    for name, res, args in EXPORTS:
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+_lib: Optional[C.CDLL] = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        _lib = load_library()
+    return _lib
+
+
+def _raise(code: int):
+    if code != OK:
+        msg = lib().inspire_last_error()
+        raise PirError(code, msg.decode() if msg else f"error {code}")
+
+
+def _ptr64(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    assert a.dtype == np.uint64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_u64p)
+
+
+@dataclass
+class InspireParams:
+    """InspireParams (src/params.rs:134-185): the fields respond reads."""
+
+    ring_dim: int = 2048
+    crt_moduli: Sequence[int] = (Q0, Q1)
+    p: int = 65536
+    gadget_base: int = 1 << 20
+    gadget_len: int = 3
+    flags: int = 0
+
+    @property
+    def q(self) -> int:
+        q = 1
+        for m in self.crt_moduli:
+            q *= m
+        return q
+
+    @property
+    def crt_count(self) -> int:
+        return len(self.crt_moduli)
+
+    def delta(self) -> int:  # params.rs:296-298
+        return self.q // self.p
+
+    def _c(self) -> _Params:
+        m = list(self.crt_moduli) + [0] * (2 - len(self.crt_moduli))
+        return _Params(self.ring_dim, len(self.crt_moduli), (C.c_uint64 * 2)(*m), self.q, self.p,
+                       self.gadget_base, self.gadget_len, self.flags)
+
+
+@dataclass
+class ClientQuery:
+    """ClientQuery / SeededClientQuery (src/pir/query.rs:119-150).
+
+    rgsw: full rows uint64[2ell, 2, K, d], or (seeded) the b polys
+    uint64[2ell, K, d] with seeds uint8[2ell, 32].
+    """
+
+    shard_id: int
+    rgsw: np.ndarray
+    seeds: Optional[np.ndarray] = None
+    packing_mode: int = PACKING_INSPIRING
+    has_packing_keys: bool = False
+    y_body: Optional[np.ndarray] = None   # uint64[ell, K, d]
+    y_all: Optional[np.ndarray] = None    # uint64[len, ell, K, d]
+
+    @property
+    def seeded(self) -> bool:
+        return self.seeds is not None
+
+
+@dataclass
+class ServerResponse:
+    """ServerResponse (src/pir/respond.rs:30-37)."""
+
+    ciphertext: np.ndarray                       # uint64[2, K, d]
+    column_ciphertexts: np.ndarray = field(default=None)  # uint64[n, 2, K, d]
+
+
+class Context:
+    """A device context: parameters, CRS seed, resident databases."""
+
+    def __init__(self, params: InspireParams, w_seed: bytes):
+        self.params = params
+        self._h = C.c_void_p()
+        seed = (C.c_uint8 * 32)(*bytes(w_seed))
+        p = params._c()
+        _raise(lib().inspire_ctx_create(C.byref(p), seed, C.byref(self._h)))
+        self.K = params.crt_count
+        self.d = params.ring_dim
+        self.ell = params.gadget_len
+        self._dbs = {}
+
+    def close(self):
+        if self._h:
+            lib().inspire_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def device_bytes(self) -> int:
+        return lib().inspire_ctx_device_bytes(self._h)
+
+    # -- databases (EncodedDatabase, src/pir/setup.rs:105-120)
+    def load_database(self, db_id: int, shard_polys: np.ndarray, shard_ids: Optional[Sequence[int]] = None):
+        """shard_polys: uint64[num_shards, polys_per_shard, K, d] (ShardData.polynomials)."""
+        sp = np.ascontiguousarray(shard_polys, dtype=np.uint64)
+        ns = sp.shape[0]
+        ids = np.ascontiguousarray(np.arange(ns) if shard_ids is None else shard_ids, dtype=np.uint32)
+        cols = sp.shape[1] if sp.ndim == 4 else 0
+        _raise(lib().inspire_db_load(self._h, db_id, ns, ids.ctypes.data_as(C.POINTER(C.c_uint32)), cols,
+                                     _ptr64(sp) if sp.size else None))
+        self._dbs[db_id] = cols
+
+    def load_entries(self, db_id: int, data: bytes, entry_size: int):
+        """encode_database (src/pir/encode_db.rs:106-164) of raw entries, then load."""
+        buf = np.frombuffer(bytes(data), dtype=np.uint8).copy()
+        _raise(lib().inspire_db_load_entries(self._h, db_id, buf.ctypes.data_as(C.POINTER(C.c_uint8)), buf.size,
+                                             entry_size))
+        self._dbs[db_id] = (entry_size * 8 + 15) // 16 if (entry_size and len(data)) else 0
+
+    def unload_database(self, db_id: int):
+        _raise(lib().inspire_db_unload(self._h, db_id))
+        self._dbs.pop(db_id, None)
+
+    # -- query marshalling
+    def _query(self, db_id: int, q: ClientQuery, keep: list) -> _Query:
+        rgsw = np.ascontiguousarray(q.rgsw, dtype=np.uint64)
+        keep.append(rgsw)
+        cq = _Query()
+        cq.db_id = db_id
+        cq.shard_id = q.shard_id
+        cq.seeded = 1 if q.seeded else 0
+        cq.packing_mode = q.packing_mode
+        cq.rgsw = _ptr64(rgsw)
+        if q.seeded:
+            seeds = np.ascontiguousarray(q.seeds, dtype=np.uint8)
+            keep.append(seeds)
+            cq.rgsw_seeds = seeds.ctypes.data_as(C.POINTER(C.c_uint8))
+        cq.has_packing_keys = 1 if q.has_packing_keys else 0
+        if q.y_body is not None:
+            yb = np.ascontiguousarray(q.y_body, dtype=np.uint64)
+            keep.append(yb)
+            cq.y_body = _ptr64(yb)
+        if q.y_all is not None and len(q.y_all):
+            ya = np.ascontiguousarray(q.y_all, dtype=np.uint64)
+            keep.append(ya)
+            cq.y_all = _ptr64(ya)
+            cq.y_all_len = ya.shape[0]
+        return cq
+
+    def _response(self, cols: int, keep: list):
+        ct = np.zeros((2, self.K, self.d), dtype=np.uint64)
+        cc = np.zeros((max(cols, 1), 2, self.K, self.d), dtype=np.uint64)
+        keep += [ct, cc]
+        r = _Response()
+        r.ciphertext = _ptr64(ct)
+        r.column_ciphertexts = _ptr64(cc)
+        r.column_capacity = cc.shape[0]
+        return r, ct, cc
+
+    def _call(self, fn, db_id: int, q: ClientQuery, *extra) -> ServerResponse:
+        keep: list = []
+        cq = self._query(db_id, q, keep)
+        r, ct, cc = self._response(self._dbs.get(db_id, 0), keep)
+        _raise(fn(self._h, C.byref(cq), *extra, C.byref(r)))
+        return ServerResponse(ct, cc[: r.num_column_ciphertexts])
+
+    def respond_batch(self, db_ids: Sequence[int], queries: Sequence[ClientQuery], variants: Sequence[int]):
+        """Many queries in one batched launch sequence -> list of ServerResponse or PirError."""
+        n = len(queries)
+        keep: list = []
+        qa = (_Query * n)()
+        ra = (_Response * n)()
+        outs = []
+        for i, (db, q) in enumerate(zip(db_ids, queries)):
+            qa[i] = self._query(db, q, keep)
+            r, ct, cc = self._response(self._dbs.get(db, 0), keep)
+            ra[i] = r
+            outs.append((ct, cc))
+        va = (C.c_uint32 * n)(*variants)
+        st = (C.c_int32 * n)()
+        _raise(lib().inspire_respond_batch(self._h, n, qa, va, ra, st))
+        res = []
+        for i in range(n):
+            if st[i] != OK:
+                res.append(PirError(st[i], f"query {i} failed with status {st[i]}"))
+            else:
+                ct, cc = outs[i]
+                res.append(ServerResponse(ct, cc[: ra[i].num_column_ciphertexts]))
+        return res
+
+    # -- math entry points
+    def ntt_forward(self, polys: np.ndarray) -> np.ndarray:
+        """NttContext::forward (src/math/ntt.rs:223-246), reference representation."""
+        a = np.ascontiguousarray(polys, dtype=np.uint64).copy()
+        _raise(lib().inspire_ntt_forward(self._h, _ptr64(a), a.size // (self.K * self.d)))
+        return a
+
+    def ntt_inverse(self, polys: np.ndarray) -> np.ndarray:
+        """NttContext::inverse (src/math/ntt.rs:308-325)."""
+        a = np.ascontiguousarray(polys, dtype=np.uint64).copy()
+        _raise(lib().inspire_ntt_inverse(self._h, _ptr64(a), a.size // (self.K * self.d)))
+        return a
+
+    def external_product(self, rlwe: np.ndarray, rgsw: np.ndarray) -> np.ndarray:
+        """external_product (src/rgsw/external_product.rs:88-150), batched over the leading axis."""
+        rl = np.ascontiguousarray(rlwe, dtype=np.uint64)
+        rg = np.ascontiguousarray(rgsw, dtype=np.uint64)
+        count = rl.size // (2 * self.K * self.d)
+        out = np.zeros((count, 2, self.K, self.d), dtype=np.uint64)
+        _raise(lib().inspire_external_product(self._h, count, _ptr64(rl), _ptr64(rg), _ptr64(out)))
+        return out
+
+
+def respond(ctx: Context, db_id: int, query: ClientQuery) -> ServerResponse:
+    return ctx._call(lib().inspire_respond, db_id, query)
+
+
+def respond_seeded(ctx: Context, db_id: int, query: ClientQuery) -> ServerResponse:
+    return ctx._call(lib().inspire_respond_seeded, db_id, query)
+
+
+def respond_inspiring(ctx: Context, db_id: int, query: ClientQuery) -> ServerResponse:
+    return ctx._call(lib().inspire_respond_inspiring, db_id, query)
+
+
+def respond_seeded_inspiring(ctx: Context, db_id: int, query: ClientQuery) -> ServerResponse:
+    return ctx._call(lib().inspire_respond_seeded_inspiring, db_id, query)
+
+
+def respond_with_variant(ctx: Context, db_id: int, query: ClientQuery, variant: int) -> ServerResponse:
+    return ctx._call(lib().inspire_respond_with_variant, db_id, query, variant)
+
+
+def respond_seeded_with_variant(ctx: Context, db_id: int, query: ClientQuery, variant: int) -> ServerResponse:
+    assert query.seeded
+    return ctx._call(lib().inspire_respond_with_variant, db_id, query, variant)
