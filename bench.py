@@ -1,0 +1,263 @@
+#!/usr/bin/env python3
+"""InsPIRe^2 respond throughput on one MI355X.
+
+Workload (the headline configuration): InsPIRe^2 (seeded RGSW query +
+InspiRING packing with y_body keys) at d=2048 over the CRT moduli
+[268369921, 249561089], p=2^16, gadget 2^20 x 3, against a 2^26-entry x 32 B
+database of uniform random bytes held resident in HBM.  A step is one batch of
+256 queries with uniform shard ids; query material (seeds, b rows, y_body) is
+uniform synthetic data of the real shape, already resident in HBM when the
+timed region starts.
+
+Prints one JSON line: queries/s over the timed steps, p50/p99 respond latency
+(a query's latency is its batch's completion time), the speed-up over the same
+path run one query per launch, the roofline of the dominant kernel (HIP events
+inside the library over the timed region), and the CPU baseline (the oracle,
+a C restatement of the reference respond, on this host's cores, on a bounded
+sample) together with a bit-exact comparison of the sampled responses.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "inspire-rs_amd"))
+
+METRIC = "PIR queries/sec + p50/p99 respond latency, InsPIRe^2, d=2048, 2^26x32B DB"
+Q0, Q1 = 268369921, 249561089
+D, K, ELL, GAMMA = 2048, 2, 3, 16
+
+# Algorithmic modular multiplications per query in each kernel (one NTT of one
+# residue = (d/2) log2 d = 11264; one pointwise product of one residue = d).
+NTT = (D // 2) * 11
+MODMUL_CHAIN = (GAMMA - 1) * (2 + 2 * ELL) * NTT + (GAMMA - 1) * 2 * ELL * K * D + 4 * NTT
+MODMUL_QUERY = (
+    2 * ELL * K * NTT            # RGSW rows ell..2ell-1, a and b
+    + GAMMA * 2 * ELL * K * D    # first dimension MAC
+    + GAMMA * GAMMA * D          # step 1 (residue 0)
+    + ELL * K * NTT              # NTT of y_body
+    + MODMUL_CHAIN
+)
+# HBM bytes per query the first dimension reads: the shard's NTT-domain digits
+FIRST_DIM_BYTES = GAMMA * ELL * K * D * 4
+# Published MI355X figures used for the roofs: HBM3E 8 TB/s; vector FP32
+# 157.3 TFLOP/s = 78.6 T lane-FMA/s; 32-bit integer multiplies issue at half
+# that rate and a Shoup/Montgomery modular product needs 3 of them.
+HBM_PEAK_GBS = 8000.0
+MODMUL_PEAK_T = 157.3 / 2 / 2 / 3
+
+
+def synthetic_queries(rng: np.random.Generator, n: int, nshards: int):
+    moduli = np.array([Q0, Q1], dtype=np.uint64)[:, None]
+    b = (rng.integers(0, 2**63, (n, ELL, K, D), dtype=np.uint64) % moduli).astype(np.uint32)
+    y = (rng.integers(0, 2**63, (n, ELL, K, D), dtype=np.uint64) % moduli).astype(np.uint32)
+    seeds = rng.integers(0, 2**32, (n, ELL, 8), dtype=np.uint64).astype(np.uint32)
+    shards = rng.integers(0, nshards, n, dtype=np.uint64).astype(np.uint32)
+    return shards, b, seeds, y
+
+
+def percentile(xs, p):
+    return float(np.percentile(np.asarray(xs), p))
+
+
+def cpu_baseline(db_bytes: np.ndarray, qset, gpu_out: np.ndarray, budget_s: float, w_seed: bytes):
+    """The oracle (C restatement of the reference respond) on this host's
+    cores: InsPIRe^2 with the query-independent tables hoisted, one query per
+    thread.  Also compares the sampled responses with the GPU's, bit for bit."""
+    sys.path.insert(0, ROOT)
+    import oracle as O
+
+    o = O.Oracle(O.baseline_params())
+    shards, b, seeds, y = qset
+    threads = O.lib.or_num_threads()
+
+    def oracle_inputs(i):
+        s = int(shards[i])
+        chunk = db_bytes[s * D * 32:(s + 1) * D * 32]
+        polys = o.encode_shard(chunk, 32, 0)
+        rg = np.zeros((2 * ELL, 2, K, D), dtype=np.uint64)
+        for r in range(ELL):
+            rg[ELL + r, 0] = o.poly_from_seed(seeds[i, r].astype("<u4").tobytes())
+            rg[ELL + r, 1] = b[i, r]
+        return polys, rg, y[i].astype(np.uint64)
+
+    o.pack(GAMMA, w_seed)  # tables built once (the hoisted variant)
+    done, t_total, mismatches = 0, 0.0, 0
+    n = len(shards)
+    while done < n and (t_total < budget_s or done == 0):
+        m = min(threads, n - done)
+        ins = [oracle_inputs(done + j) for j in range(m)]
+        t0 = time.perf_counter()
+        rc, outs = o.respond_inspiring_many(w_seed, [x[0] for x in ins], [x[1] for x in ins], [x[2] for x in ins])
+        t_total += time.perf_counter() - t0
+        for j in range(m):
+            if rc[j] != 0 or not np.array_equal(outs[j].astype(np.uint32), gpu_out[done + j]):
+                mismatches += 1
+        done += m
+    return {
+        "value": done / t_total,
+        "unit": "queries/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{done} InsPIRe^2 queries of the timed workload, {t_total:.1f} s of CPU time; "
+                  f"PackParams/OfflinePackingKeys hoisted; bit-exact vs GPU: {done - mismatches}/{done}",
+        "bit_exact_sampled": f"{done - mismatches}/{done}",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--db-log2", type=int, default=26)
+    ap.add_argument("--single-steps", type=int, default=32, help="steps of the one-query-per-launch run")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: the respond path shards by database across GPUs with no collective, "
+              f"and this build measures one GPU's share (asked for --gpus {args.gpus})", file=sys.stderr)
+        sys.exit(2)
+
+    import inspire_mi355x as M
+
+    log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)
+    w_seed = bytes(range(32))
+    rng = np.random.default_rng(2026)
+    n_entries = 1 << args.db_log2
+    nshards = n_entries // D
+    log(f"generating {n_entries} x 32 B database")
+    db = np.frombuffer(rng.bytes(n_entries * 32), dtype=np.uint8)
+    ctx = M.Context(M.InspireParams(), w_seed)
+    t0 = time.perf_counter()
+    ctx.load_entries(0, db, 32)
+    log(f"database resident: {nshards} shards, {ctx.device_bytes / 2**30:.1f} GiB on device, "
+        f"load {time.perf_counter() - t0:.1f} s")
+
+    B = args.batch
+    n_sets = 4  # distinct query batches cycled through the steps
+    qsets = [synthetic_queries(rng, B, nshards) for _ in range(n_sets)]
+    dev = [(shards, M.DeviceArray.from_numpy(b), M.DeviceArray.from_numpy(seeds), M.DeviceArray.from_numpy(y))
+           for shards, b, seeds, y in qsets]
+    dout = M.DeviceArray(B * 2 * K * D * 4)
+
+    def step(i, count=B):
+        shards, db_, ds_, dy_ = dev[i % n_sets]
+        st = ctx.respond_batch_device(0, M.TWO_PACKING, shards[:count], db_.ptr, dout.ptr, seeds_ptr=ds_.ptr,
+                                      y_body_ptr=dy_.ptr)
+        if (st != 0).any():
+            raise RuntimeError(f"status {st[st != 0][:4]}")
+
+    for i in range(args.warmup):
+        step(i)
+    ctx.set_profiling(True)
+    for name in ("pack_chain", "first_dim", "pack_step1", "rows_ntt", "y_ntt", "expand_seeds", "pack_b0"):
+        ctx.profile(name, reset=True)
+    lat = []
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        step(i)
+        lat.append((time.perf_counter() - t0) * 1e3)
+    elapsed = time.perf_counter() - t_start
+    prof = {name: ctx.profile(name, reset=True) for name in
+            ("pack_chain", "first_dim", "pack_step1", "rows_ntt", "y_ntt", "expand_seeds", "pack_b0")}
+    ctx.set_profiling(False)
+    qps = B * args.steps / elapsed
+    log(f"batched: {qps:.0f} queries/s, {elapsed / args.steps * 1e3:.2f} ms/step")
+    for k_, (ms, n) in prof.items():
+        log(f"  {k_:13s} {ms / max(n, 1):8.3f} ms/launch x {n}")
+
+    # the same path, one query per launch
+    single = []
+    for i in range(args.warmup):
+        step(i, 1)
+    t0 = time.perf_counter()
+    for i in range(args.single_steps):
+        t1 = time.perf_counter()
+        step(i, 1)
+        single.append((time.perf_counter() - t1) * 1e3)
+    single_qps = args.single_steps / (time.perf_counter() - t0)
+    log(f"one query per launch: {single_qps:.0f} queries/s, p50 {percentile(single, 50):.2f} ms")
+
+    chain_ms, chain_n = prof["pack_chain"]
+    chain_avg_s = chain_ms / max(chain_n, 1) / 1e3
+    fd_ms, fd_n = prof["first_dim"]
+    fd_avg_s = fd_ms / max(fd_n, 1) / 1e3
+    chain_tmodmul = MODMUL_CHAIN * B / chain_avg_s / 1e12
+    roofline = {
+        "bound": "valu",
+        "kernel": "k_pack_chain",
+        "achieved": round(chain_tmodmul, 3),
+        "peak": round(MODMUL_PEAK_T, 2),
+        "unit": "Tmodmul/s",
+        "frac": round(chain_tmodmul / MODMUL_PEAK_T, 4),
+        "traffic": None,
+        "algorithmic_per_launch": f"{MODMUL_CHAIN} modmul/query x {B} queries",
+        "avg_launch_ms": round(chain_avg_s * 1e3, 4),
+    }
+    roofline_hbm = {
+        "bound": "hbm",
+        "kernel": "k_first_dim",
+        "achieved": round(FIRST_DIM_BYTES * B / fd_avg_s / 1e9, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(FIRST_DIM_BYTES * B / fd_avg_s / 1e9 / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "avg_launch_ms": round(fd_avg_s * 1e3, 4),
+    }
+    line = {
+        "metric": METRIC,
+        "value": round(qps, 1),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {
+            "workload": "InsPIRe^2 batched respond: 256 seeded queries + y_body keys per step, "
+                        "d=2048, CRT [268369921, 249561089], 2^26 x 32 B DB resident in HBM",
+            "batch": B,
+            "db_entries": n_entries,
+            "entry_bytes": 32,
+            "ring_dim": D,
+            "gadget": "2^20 x 3",
+        },
+        "p50_ms": round(percentile(lat, 50), 3),
+        "p99_ms": round(percentile(lat, 99), 3),
+        "single_query_qps": round(single_qps, 1),
+        "single_query_p50_ms": round(percentile(single, 50), 3),
+        "speedup_vs_single_query_launch": round(qps / single_qps, 2),
+        "modmul_per_query": MODMUL_QUERY,
+        "roofline": roofline,
+        "roofline_hbm_first_dim": roofline_hbm,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        shards, b, seeds, y = qsets[(args.steps - 1) % n_sets]
+        step(args.steps - 1)  # the batch of this query set, responses in pout
+        gout = dout.download(np.zeros((B, 2, K, D), dtype=np.uint32))
+        log("cpu baseline (oracle) ...")
+        line["cpu_baseline"] = cpu_baseline(db, (shards, b, seeds, y), gout, args.cpu_seconds, w_seed)
+    for _, db_, ds_, dy_ in dev:
+        db_.free(), ds_.free(), dy_.free()
+    dout.free()
+    ctx.close()
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

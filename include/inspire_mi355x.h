@@ -20,8 +20,8 @@
  *    carry the same text.
  *  - All inputs are borrowed and read-only; outputs are caller-owned buffers.
  *    A context may be shared by several threads: calls are serialised inside.
- *  - The context keeps the preprocessed databases, CRS-derived packing tables
- *    and registered client keys resident in device memory (HBM).
+ *  - The context keeps the preprocessed databases and the CRS-derived
+ *    packing tables resident in device memory (HBM).
  */
 #ifndef INSPIRE_MI355X_H
 #define INSPIRE_MI355X_H
@@ -69,11 +69,11 @@ typedef struct inspire_ctx inspire_ctx;
  * sigma and security_level only matter to setup and clients. */
 typedef struct {
     uint32_t ring_dim;       /* d: 256, 512, 1024, 2048 or 4096 on this path */
-    uint32_t crt_count;      /* 1 or 2; each modulus below 2^30 on this path */
+    uint32_t crt_count;      /* 1 or 2; each modulus below 2^29 on this path */
     uint64_t crt_moduli[2];
     uint64_t q;              /* product of crt_moduli */
     uint64_t p;              /* plaintext modulus */
-    uint64_t gadget_base;    /* power of two */
+    uint64_t gadget_base;    /* power of two, at most 2^32 on this path */
     uint32_t gadget_len;     /* ell */
     uint32_t flags;          /* INSPIRE_FLAG_* */
 } inspire_params;
@@ -149,6 +149,48 @@ int inspire_respond_with_variant(inspire_ctx *ctx, const inspire_query *q, uint3
  * INSPIRE_OK when the batch ran (individual queries may still have failed). */
 int inspire_respond_batch(inspire_ctx *ctx, uint32_t n, const inspire_query *qs, const uint32_t *variants,
                           inspire_response *rs, int32_t *status);
+
+/* ---- device-resident batches ---- */
+
+/* A batch whose inputs already sit in device memory (HBM), all of one
+ * variant and one database; the semantics are those of respond_with_variant
+ * / respond_seeded_with_variant (src/pir/respond.rs:129-183) per query, with
+ * InspiRING keys given as y_body (the wire form, inspiring2.rs:603-628).
+ * Device layouts are u32 canonical residues:
+ *   rows   seeded: b polys of RGSW rows ell..2ell-1, [count][ell][K][d]
+ *          full:   rows ell..2ell-1 (a, b),           [count][ell][2][K][d]
+ *          (rows 0..ell-1 multiply the zero a-digits of the database's
+ *          trivial encryption and do not affect the response)
+ *   seeds  seeded: [count][ell][8] (32-byte seeds of rows ell..2ell-1)
+ *   y_body packed variants: [count][ell][K][d]
+ *   out    packed variants: [count][2][K][d];
+ *          InsPIRe^0: [count][1 + cols][2][K][d] (ciphertext, then columns)
+ * shard_ids and status are host arrays.  status[i] is INSPIRE_OK or
+ * INSPIRE_ERR_SHARD_NOT_FOUND.  The call returns when the batch has finished. */
+typedef struct {
+    uint32_t count;
+    uint32_t db_id;
+    uint32_t variant;
+    uint32_t seeded;
+    const uint32_t *shard_ids;
+    const uint32_t *rows;
+    const uint32_t *seeds;
+    const uint32_t *y_body;
+    uint32_t *out;
+    int32_t *status;
+} inspire_device_batch;
+
+int inspire_respond_batch_device(inspire_ctx *ctx, const inspire_device_batch *batch);
+
+/* ---- profiling ---- */
+
+/* When enabled, every kernel launch of the respond path is bracketed by HIP
+ * events on the context's stream.  inspire_ctx_profile reads the summed
+ * duration (ms) and launch count of one kernel by name (rows_ntt,
+ * expand_seeds, first_dim, cols_intt, fold, pack_b0, pack_step1, y_ntt,
+ * pack_chain, db_load) and, with reset != 0, clears it. */
+int inspire_ctx_set_profiling(inspire_ctx *ctx, int enable);
+int inspire_ctx_profile(inspire_ctx *ctx, const char *kernel, double *total_ms, uint64_t *launches, int reset);
 
 /* ---- math entry points (NttContext, external_product) ---- */
 

@@ -68,6 +68,21 @@ struct PackTables {
     std::vector<uint32_t> gen_pows;  // g^i mod 2n, i < n
 };
 
+// HIP-event timing of each kernel category on the context's stream.
+enum ProfId {
+    P_ROWS_NTT, P_EXPAND, P_FIRST_DIM, P_COLS_INTT, P_FOLD, P_B0, P_STEP1, P_Y_NTT, P_CHAIN, P_DB, P_COUNT
+};
+extern const char* const kProfNames[P_COUNT];
+
+struct Profiler {
+    bool on = false;
+    struct Span { int id; hipEvent_t a, b; };
+    std::vector<Span> spans;
+    std::vector<hipEvent_t> pool;
+    double total_ms[P_COUNT] = {};
+    uint64_t launches[P_COUNT] = {};
+};
+
 struct HostModulus {
     uint64_t q = 0, psi = 0;
     std::vector<uint32_t> tw, twp, itw, itwp;
@@ -88,6 +103,23 @@ struct inspire_ctx {
     std::map<uint32_t, inspire::Database> dbs;
     std::map<uint32_t, std::unique_ptr<inspire::PackTables>> packs;
     // scratch reused across calls
-    inspire::DevBuf ws0, ws1, ws2, ws3, ws4;
+    inspire::DevBuf ws0, ws1, ws2, ws3, ws4, ws5, ws6;
     std::vector<uint32_t> hbuf;
+    inspire::Profiler prof;
 };
+
+namespace inspire {
+// Records HIP events around the launches in its scope when profiling is on.
+class ProfScope {
+   public:
+    ProfScope(inspire_ctx* ctx, int id);
+    ~ProfScope();
+
+   private:
+    inspire_ctx* ctx_;
+    int id_;
+    hipEvent_t a_ = nullptr;
+};
+// Folds finished spans into the totals (synchronises the stream).
+void prof_collect(inspire_ctx* ctx);
+}  // namespace inspire

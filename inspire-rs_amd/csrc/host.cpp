@@ -262,20 +262,61 @@ int inspire_ctx_create(const inspire_params* params, const uint8_t w_seed[32], i
 
 void inspire_ctx_destroy(inspire_ctx* ctx) {
     if (!ctx) return;
-    if (ctx->stream) {
-        (void)hipStreamSynchronize(ctx->stream);
-        (void)hipStreamDestroy(ctx->stream);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto& s : ctx->prof.spans) {
+        (void)hipEventDestroy(s.a);
+        (void)hipEventDestroy(s.b);
     }
+    for (auto e : ctx->prof.pool) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
 
 uint64_t inspire_ctx_device_bytes(const inspire_ctx* ctx) {
     if (!ctx) return 0;
     uint64_t b = ctx->tables.bytes() + ctx->ws0.bytes() + ctx->ws1.bytes() + ctx->ws2.bytes() + ctx->ws3.bytes() +
-                 ctx->ws4.bytes();
+                 ctx->ws4.bytes() + ctx->ws5.bytes() + ctx->ws6.bytes();
     for (auto& kv : ctx->dbs) b += kv.second.digits.bytes();
     for (auto& kv : ctx->packs) b += kv.second->step1.bytes() + kv.second->perm.bytes() + kv.second->w_all.bytes();
     return b;
+}
+
+int inspire_respond_batch_device(inspire_ctx* ctx, const inspire_device_batch* batch) {
+    INSPIRE_TRY({
+        if (!ctx || !batch) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "null argument"};
+        std::lock_guard<std::recursive_mutex> g(ctx->mu);
+        respond_batch_device(ctx, *batch);
+        return INSPIRE_OK;
+    })
+}
+
+int inspire_ctx_set_profiling(inspire_ctx* ctx, int enable) {
+    INSPIRE_TRY({
+        if (!ctx) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "null argument"};
+        std::lock_guard<std::recursive_mutex> g(ctx->mu);
+        prof_collect(ctx);
+        ctx->prof.on = enable != 0;
+        return INSPIRE_OK;
+    })
+}
+
+int inspire_ctx_profile(inspire_ctx* ctx, const char* kernel, double* total_ms, uint64_t* launches, int reset) {
+    INSPIRE_TRY({
+        if (!ctx || !kernel) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "null argument"};
+        std::lock_guard<std::recursive_mutex> g(ctx->mu);
+        prof_collect(ctx);
+        for (int i = 0; i < P_COUNT; i++) {
+            if (std::strcmp(kProfNames[i], kernel) != 0) continue;
+            if (total_ms) *total_ms = ctx->prof.total_ms[i];
+            if (launches) *launches = ctx->prof.launches[i];
+            if (reset) {
+                ctx->prof.total_ms[i] = 0;
+                ctx->prof.launches[i] = 0;
+            }
+            return INSPIRE_OK;
+        }
+        throw Error{INSPIRE_ERR_INVALID_ARGUMENT, std::string("unknown kernel name ") + kernel};
+    })
 }
 
 int inspire_ntt_forward(inspire_ctx* ctx, uint64_t* polys, uint64_t count) {

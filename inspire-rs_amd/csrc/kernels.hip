@@ -27,13 +27,15 @@ struct BlockSync {
 // ---------------------------------------------------------------------------
 template <int LOGN>
 __global__ void __launch_bounds__(NttShape<LOGN>::T)
-k_ntt_fwd(uint32_t* __restrict__ data, CtxDev C, ScaleDev post) {
+k_ntt_fwd(const uint32_t* src, size_t src_stride, uint32_t* dst, size_t dst_stride, CtxDev C, ScaleDev post) {
     using S = NttShape<LOGN>;
     __shared__ __align__(16) uint32_t s[S::LDS];
     const int tid = threadIdx.x;
     const int k = blockIdx.x % C.K;
+    const size_t poly = blockIdx.x / C.K;
     const ModDev& M = C.m[k];
-    uint32_t* p = data + (size_t)blockIdx.x * S::N;
+    const uint32_t* p = src + poly * src_stride + (size_t)k * S::N;
+    uint32_t* o = dst + poly * dst_stride + (size_t)k * S::N;
 #pragma unroll
     for (int j = 0; j < 8; j++) s[lds_ix(tid + j * S::T)] = p[tid + j * S::T];
     __syncthreads();
@@ -43,25 +45,27 @@ k_ntt_fwd(uint32_t* __restrict__ data, CtxDev C, ScaleDev post) {
     for (int j = 0; j < 8; j++) {
         uint32_t x = s[lds_ix(tid + j * S::T)];
         x = post.enable ? shoup_mul(x, c, cp, M.q) : csub(x, M.q2);
-        p[tid + j * S::T] = csub(x, M.q);
+        o[tid + j * S::T] = csub(x, M.q);
     }
 }
 
 template <int LOGN>
 __global__ void __launch_bounds__(NttShape<LOGN>::T)
-k_ntt_inv(uint32_t* __restrict__ data, CtxDev C, ScaleDev fin) {
+k_ntt_inv(const uint32_t* src, size_t src_stride, uint32_t* dst, size_t dst_stride, CtxDev C, ScaleDev fin) {
     using S = NttShape<LOGN>;
     __shared__ __align__(16) uint32_t s[S::LDS];
     const int tid = threadIdx.x;
     const int k = blockIdx.x % C.K;
+    const size_t poly = blockIdx.x / C.K;
     const ModDev& M = C.m[k];
-    uint32_t* p = data + (size_t)blockIdx.x * S::N;
+    const uint32_t* p = src + poly * src_stride + (size_t)k * S::N;
+    uint32_t* o = dst + poly * dst_stride + (size_t)k * S::N;
 #pragma unroll
     for (int j = 0; j < 8; j++) s[lds_ix(tid + j * S::T)] = p[tid + j * S::T];
     __syncthreads();
     ntt_inv_lds<LOGN>(s, tid, M, BlockSync{}, fin.c[k], fin.cp[k], fin.cw[k], fin.cwp[k]);
 #pragma unroll
-    for (int j = 0; j < 8; j++) p[tid + j * S::T] = csub(s[lds_ix(tid + j * S::T)], M.q);
+    for (int j = 0; j < 8; j++) o[tid + j * S::T] = csub(s[lds_ix(tid + j * S::T)], M.q);
 }
 
 // ---------------------------------------------------------------------------
@@ -344,30 +348,69 @@ k_expand_seeds(const uint32_t* __restrict__ seeds, uint32_t* __restrict__ out, s
 }
 
 // ---------------------------------------------------------------------------
+// encode_database (encode_db.rs:106-164) on the device: entry e of the byte
+// stream lands in shard e / N, local index e % N; column c is the LE u16 of
+// bytes (2c, 2c+1), a lone last byte, or 0 (extract_column_value,
+// encode_db.rs:169-181).  out: [S][cols][K][N] residues, zero past the last
+// entry.  `bytes` holds the entries of shards first_shard.. of this slice.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_encode_entries(const uint8_t* __restrict__ bytes, uint64_t num_entries_in_slice, uint32_t entry_size,
+                 uint32_t cols, uint32_t shards, uint32_t* __restrict__ out, CtxDev C) {
+    const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (size_t)shards * cols * C.n) return;
+    const uint32_t j = (uint32_t)(gid % C.n);
+    const size_t sc = gid / C.n;                    // shard * cols + c
+    const uint32_t c = (uint32_t)(sc % cols);
+    const uint64_t e = (sc / cols) * (uint64_t)C.n + j;
+    uint32_t v = 0;
+    if (e < num_entries_in_slice) {
+        const uint8_t* p = bytes + e * entry_size;
+        const uint32_t bo = 2 * c;
+        if (bo + 1 < entry_size) v = (uint32_t)p[bo] | ((uint32_t)p[bo + 1] << 8);
+        else if (bo < entry_size) v = p[bo];
+    }
+    uint32_t* o = out + sc * C.K * C.n;
+    for (uint32_t k = 0; k < C.K; k++) {
+        const uint32_t q = C.m[k].q;
+        o[(size_t)k * C.n + j] = v < q ? v : v % q;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
 template <int LOGN>
-static hipError_t launch_ntt_t(uint32_t* data, size_t count, const CtxDev& C, const ScaleDev& sc,
-                               bool inverse, hipStream_t st) {
+static hipError_t launch_ntt_t(const uint32_t* src, size_t ss, uint32_t* dst, size_t ds, size_t count,
+                               const CtxDev& C, const ScaleDev& sc, bool inverse, hipStream_t st) {
     const size_t blocks = count * C.K;
     if (blocks == 0) return hipSuccess;
+    if (blocks > 0x7fffffffu) return hipErrorInvalidValue;
     if (inverse)
-        hipLaunchKernelGGL(k_ntt_inv<LOGN>, dim3((unsigned)blocks), dim3(NttShape<LOGN>::T), 0, st, data, C, sc);
+        hipLaunchKernelGGL(k_ntt_inv<LOGN>, dim3((unsigned)blocks), dim3(NttShape<LOGN>::T), 0, st, src, ss, dst, ds, C,
+                           sc);
     else
-        hipLaunchKernelGGL(k_ntt_fwd<LOGN>, dim3((unsigned)blocks), dim3(NttShape<LOGN>::T), 0, st, data, C, sc);
+        hipLaunchKernelGGL(k_ntt_fwd<LOGN>, dim3((unsigned)blocks), dim3(NttShape<LOGN>::T), 0, st, src, ss, dst, ds, C,
+                           sc);
     return hipGetLastError();
+}
+
+hipError_t launch_ntt_strided(const uint32_t* src, size_t src_stride, uint32_t* dst, size_t dst_stride, size_t count,
+                              const CtxDev& C, const ScaleDev& sc, bool inverse, hipStream_t st) {
+    switch (C.logn) {
+        case 8: return launch_ntt_t<8>(src, src_stride, dst, dst_stride, count, C, sc, inverse, st);
+        case 9: return launch_ntt_t<9>(src, src_stride, dst, dst_stride, count, C, sc, inverse, st);
+        case 10: return launch_ntt_t<10>(src, src_stride, dst, dst_stride, count, C, sc, inverse, st);
+        case 11: return launch_ntt_t<11>(src, src_stride, dst, dst_stride, count, C, sc, inverse, st);
+        case 12: return launch_ntt_t<12>(src, src_stride, dst, dst_stride, count, C, sc, inverse, st);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_ntt(uint32_t* data, size_t count, const CtxDev& C, const ScaleDev& sc, bool inverse,
                       hipStream_t st) {
-    switch (C.logn) {
-        case 8: return launch_ntt_t<8>(data, count, C, sc, inverse, st);
-        case 9: return launch_ntt_t<9>(data, count, C, sc, inverse, st);
-        case 10: return launch_ntt_t<10>(data, count, C, sc, inverse, st);
-        case 11: return launch_ntt_t<11>(data, count, C, sc, inverse, st);
-        case 12: return launch_ntt_t<12>(data, count, C, sc, inverse, st);
-        default: return hipErrorInvalidValue;
-    }
+    const size_t KN = (size_t)C.K * C.n;
+    return launch_ntt_strided(data, KN, data, KN, count, C, sc, inverse, st);
 }
 
 static inline unsigned blocks_for(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
@@ -413,6 +456,15 @@ hipError_t launch_expand_seeds(const uint32_t* seeds, uint32_t* out, size_t coun
                                const CtxDev& C, hipStream_t st) {
     if (!count) return hipSuccess;
     hipLaunchKernelGGL(k_expand_seeds, dim3((unsigned)count), dim3(256), 0, st, seeds, out, out_stride, err, C);
+    return hipGetLastError();
+}
+
+hipError_t launch_encode_entries(const uint8_t* bytes, uint64_t num_entries, uint32_t entry_size, uint32_t cols,
+                                 uint32_t shards, uint32_t* out, const CtxDev& C, hipStream_t st) {
+    const size_t total = (size_t)shards * cols * C.n;
+    if (!total) return hipSuccess;
+    hipLaunchKernelGGL(k_encode_entries, dim3(blocks_for(total, 256)), dim3(256), 0, st, bytes, num_entries, entry_size,
+                       cols, shards, out, C);
     return hipGetLastError();
 }
 

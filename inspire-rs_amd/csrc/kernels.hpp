@@ -32,8 +32,14 @@ struct ScaleDev {
     uint32_t c[kMaxK], cp[kMaxK], cw[kMaxK], cwp[kMaxK];
 };
 
+// In-place transform of count contiguous [K][N] polys.
 hipError_t launch_ntt(uint32_t* data, size_t count, const CtxDev& C, const ScaleDev& sc, bool inverse,
                       hipStream_t st);
+// Transform poly p from src + p*src_stride into dst + p*dst_stride (strides in words).
+hipError_t launch_ntt_strided(const uint32_t* src, size_t src_stride, uint32_t* dst, size_t dst_stride, size_t count,
+                              const CtxDev& C, const ScaleDev& sc, bool inverse, hipStream_t st);
+hipError_t launch_encode_entries(const uint8_t* bytes, uint64_t num_entries, uint32_t entry_size, uint32_t cols,
+                                 uint32_t shards, uint32_t* out, const CtxDev& C, hipStream_t st);
 hipError_t launch_db_digits(const uint32_t* in, uint32_t* out, size_t P, const CtxDev& C, hipStream_t st);
 hipError_t launch_rlwe_digits(const uint32_t* in, uint32_t* out, size_t P, const CtxDev& C, hipStream_t st);
 hipError_t launch_first_dim(const uint32_t* digits, const uint32_t* rows, const int32_t* slot, uint32_t* out,

@@ -214,13 +214,110 @@ PackTables& pack_tables(inspire_ctx* ctx, uint32_t gamma) {
     return ref;
 }
 
-void pack_respond(inspire_ctx* ctx, const Database& db, const BatchItem* items, uint32_t Q, const uint32_t* rows,
-                  const int32_t* d_slot, const uint32_t* d_err, const inspire_query* qs, inspire_response* rs,
-                  int32_t* status) {
-    const uint32_t gamma = db.cols, n = ctx->N, K = ctx->K, ell = ctx->ell;
-    const size_t KN = (size_t)K * n;
+// Device-only packing of a staged group: first dimension, b_poly, step 1 and
+// the chain.  y: NTT-domain (Montgomery) client keys, per query y_off/y_mode/
+// y_count (device arrays).  out: [Q][2][K][N].
+void pack_run(inspire_ctx* ctx, const Database& db, uint32_t Q, const uint32_t* rows, const int32_t* d_slot,
+              const uint32_t* d_y, const uint64_t* d_yoff, const uint32_t* d_ymode, const uint32_t* d_ycount,
+              uint32_t* d_out) {
+    const uint32_t gamma = db.cols, K = ctx->K;
+    const size_t KN = (size_t)K * ctx->N;
     PackTables& T = pack_tables(ctx, gamma);
     const bool rw = (ctx->params.flags & INSPIRE_FLAG_RESIDUE_WISE_AUTOMORPHISM) != 0;
+    const size_t cols_words = (size_t)Q * 2 * gamma * KN, rpre_words = (size_t)Q * gamma * KN;
+    const size_t bp_words = (size_t)Q * K * gamma;
+    ctx->ws2.reserve(cols_words * 4);
+    ctx->ws3.reserve((rpre_words + bp_words) * 4);
+    uint32_t* d_cols = ctx->ws2.get<uint32_t>();
+    uint32_t* d_rpre = ctx->ws3.get<uint32_t>();
+    uint32_t* d_bp = d_rpre + rpre_words;
+    {
+        ProfScope ps(ctx, P_FIRST_DIM);
+        hip_check(launch_first_dim(db.digits.get<uint32_t>(), rows, d_slot, d_cols, Q, gamma, 2 * gamma, 0, ctx->dev,
+                                   ctx->stream),
+                  "first dim");
+    }
+    {
+        ProfScope ps(ctx, P_B0);
+        hip_check(launch_pack_b0(d_cols, d_bp, Q, gamma, ctx->dev, ctx->stream), "b0");
+    }
+    {
+        ProfScope ps(ctx, P_STEP1);
+        const uint32_t kres = (rw || K == 1) ? K : 1;  // residue 1 is rebuilt from residue 0 otherwise
+        hip_check(launch_pack_step1(d_cols, T.step1.get<uint32_t>(), d_rpre, Q, gamma, kres, ctx->dev, ctx->stream),
+                  "step1");
+    }
+    PackArgs A{};
+    A.a_hat_cols = d_cols;
+    A.r_pre = d_rpre;
+    A.perm = T.perm.get<uint32_t>();
+    A.w_all = T.w_all.get<uint32_t>();
+    A.y = d_y;
+    A.y_off = d_yoff;
+    A.y_mode = d_ymode;
+    A.y_count = d_ycount;
+    A.b_poly = d_bp;
+    A.out = d_out;
+    A.gamma = gamma;
+    A.residue_wise = rw ? 1u : 0u;
+    const uint64_t q0 = ctx->hm[0].q;
+    A.q0r64 = (uint32_t)pow2_64_mod(q0);
+    A.q0r64p = shoup_pre(A.q0r64, (uint32_t)q0);
+    if (K == 2) {
+        const uint64_t q1 = ctx->hm[1].q;
+        A.q1r64i = (uint32_t)powmod64(pow2_64_mod(q1), q1 - 2, q1);
+        A.q1r64ip = shoup_pre(A.q1r64i, (uint32_t)q1);
+        A.one_q1p = shoup_pre(1, (uint32_t)q1);
+    }
+    ProfScope ps(ctx, P_CHAIN);
+    hip_check(launch_pack_chain_scaled(A, Q, ctx->dev, scale_inv(ctx, nullptr), ctx->stream), "pack chain");
+}
+
+static void upload_meta(inspire_ctx* ctx, uint32_t Q, const std::vector<uint64_t>& y_off,
+                        const std::vector<uint32_t>& y_mode, const std::vector<uint32_t>& y_count, uint64_t** d_yoff,
+                        uint32_t** d_ymode, uint32_t** d_ycount) {
+    ctx->ws5.reserve((size_t)Q * 16 + 64);
+    *d_yoff = ctx->ws5.get<uint64_t>();
+    *d_ymode = reinterpret_cast<uint32_t*>(*d_yoff + Q);
+    *d_ycount = *d_ymode + Q;
+    hip_check(hipMemcpyAsync(*d_yoff, y_off.data(), Q * 8, hipMemcpyHostToDevice, ctx->stream), "y off");
+    hip_check(hipMemcpyAsync(*d_ymode, y_mode.data(), Q * 4, hipMemcpyHostToDevice, ctx->stream), "y mode");
+    hip_check(hipMemcpyAsync(*d_ycount, y_count.data(), Q * 4, hipMemcpyHostToDevice, ctx->stream), "y count");
+}
+
+void pack_run_device_ybody(inspire_ctx* ctx, const Database& db, uint32_t Q, const uint32_t* rows,
+                           const int32_t* d_slot, const uint32_t* y_body, uint32_t* d_out) {
+    const uint32_t gamma = db.cols, ell = ctx->ell;
+    const size_t KN = (size_t)ctx->K * ctx->N;
+    PackTables& T = pack_tables(ctx, gamma);
+    for (uint32_t i = 0; i + 1 < gamma; i++)
+        if (!(T.gen_pows[i] & 1))
+            throw Error{INSPIRE_ERR_UNSUPPORTED, "device batches need an odd InspiRING generator (gamma | 2d)"};
+    std::vector<uint64_t> y_off(Q);
+    std::vector<uint32_t> y_mode(Q, 0), y_count(Q, gamma - 1);
+    for (uint32_t t = 0; t < Q; t++) y_off[t] = (uint64_t)t * ell * KN;
+    ctx->ws0.reserve((size_t)Q * ell * KN * 4);
+    uint32_t* d_y = ctx->ws0.get<uint32_t>();
+    uint64_t mont[kMaxK];
+    for (uint32_t k = 0; k < ctx->K; k++) mont[k] = (1ull << 32) % ctx->hm[k].q;
+    {
+        ProfScope ps(ctx, P_Y_NTT);
+        hip_check(launch_ntt_strided(y_body, KN, d_y, KN, (size_t)Q * ell, ctx->dev, scale_fwd(ctx, mont), false,
+                                     ctx->stream),
+                  "y ntt");
+    }
+    uint64_t* d_yoff;
+    uint32_t *d_ymode, *d_ycount;
+    upload_meta(ctx, Q, y_off, y_mode, y_count, &d_yoff, &d_ymode, &d_ycount);
+    pack_run(ctx, db, Q, rows, d_slot, d_y, d_yoff, d_ymode, d_ycount, d_out);
+}
+
+void pack_respond_host(inspire_ctx* ctx, const Database& db, const BatchItem* items, uint32_t Q, const uint32_t* rows,
+                       const int32_t* d_slot, const uint32_t* d_err, const inspire_query* qs, inspire_response* rs,
+                       int32_t* status) {
+    const uint32_t gamma = db.cols, K = ctx->K, ell = ctx->ell;
+    const size_t KN = (size_t)K * ctx->N;
+    PackTables& T = pack_tables(ctx, gamma);
     const uint32_t nrot = gamma - 1;
     bool odd_gens = true;
     for (uint32_t i = 0; i < nrot; i++) odd_gens &= (T.gen_pows[i] & 1) != 0;
@@ -262,65 +359,25 @@ void pack_respond(inspire_ctx* ctx, const Database& db, const BatchItem* items, 
                 }
         }
     }
-    // ---- device workspace
-    const size_t cols_words = (size_t)Q * 2 * gamma * KN, rpre_words = (size_t)Q * gamma * KN;
-    const size_t out_words = (size_t)Q * 2 * KN, bp_words = (size_t)Q * K * gamma;
-    ctx->ws2.reserve(cols_words * 4);
-    ctx->ws3.reserve((rpre_words + out_words + bp_words) * 4);
-    ctx->ws0.reserve(std::max<size_t>(y_words, 1) * 4 + (size_t)Q * 24 + 64);
-    uint32_t* d_cols = ctx->ws2.get<uint32_t>();
-    uint32_t* d_rpre = ctx->ws3.get<uint32_t>();
-    uint32_t* d_out = d_rpre + rpre_words;
-    uint32_t* d_bp = d_out + out_words;
+    ctx->ws0.reserve(std::max<size_t>(y_words, 1) * 4);
     uint32_t* d_y = ctx->ws0.get<uint32_t>();
-    uint8_t* meta = reinterpret_cast<uint8_t*>(d_y + y_words);
-    uint64_t* d_yoff = reinterpret_cast<uint64_t*>(meta + ((8 - ((uintptr_t)meta & 7)) & 7));
-    uint32_t* d_ymode = reinterpret_cast<uint32_t*>(d_yoff + Q);
-    uint32_t* d_ycount = d_ymode + Q;
     hip_check(hipMemcpyAsync(d_y, hy.data(), y_words * 4, hipMemcpyHostToDevice, ctx->stream), "y");
-    hip_check(hipMemcpyAsync(d_yoff, y_off.data(), Q * 8, hipMemcpyHostToDevice, ctx->stream), "y off");
-    hip_check(hipMemcpyAsync(d_ymode, y_mode.data(), Q * 4, hipMemcpyHostToDevice, ctx->stream), "y mode");
-    hip_check(hipMemcpyAsync(d_ycount, y_count.data(), Q * 4, hipMemcpyHostToDevice, ctx->stream), "y count");
     uint64_t mont[kMaxK];
     for (uint32_t k = 0; k < K; k++) mont[k] = (1ull << 32) % ctx->hm[k].q;
-    hip_check(launch_ntt(d_y, y_words / KN, ctx->dev, scale_fwd(ctx, mont), false, ctx->stream), "y ntt");
-
-    // ---- first dimension, b_poly, step 1, chain
-    hip_check(launch_first_dim(db.digits.get<uint32_t>(), rows, d_slot, d_cols, Q, gamma, 2 * gamma, 0, ctx->dev,
-                               ctx->stream),
-              "first dim");
-    hip_check(launch_pack_b0(d_cols, d_bp, Q, gamma, ctx->dev, ctx->stream), "b0");
-    const uint32_t kres = (rw || K == 1) ? K : 1;
-    hip_check(launch_pack_step1(d_cols, T.step1.get<uint32_t>(), d_rpre, Q, gamma, kres, ctx->dev, ctx->stream),
-              "step1");
-    PackArgs A{};
-    A.a_hat_cols = d_cols;
-    A.r_pre = d_rpre;
-    A.perm = T.perm.get<uint32_t>();
-    A.w_all = T.w_all.get<uint32_t>();
-    A.y = d_y;
-    A.y_off = d_yoff;
-    A.y_mode = d_ymode;
-    A.y_count = d_ycount;
-    A.b_poly = d_bp;
-    A.out = d_out;
-    A.gamma = gamma;
-    A.residue_wise = rw ? 1u : 0u;
     {
-        const uint64_t q0 = ctx->hm[0].q;
-        A.q0r64 = (uint32_t)pow2_64_mod(q0);
-        A.q0r64p = shoup_pre(A.q0r64, (uint32_t)q0);
-        if (K == 2) {
-            const uint64_t q1 = ctx->hm[1].q;
-            A.q1r64i = (uint32_t)powmod64(pow2_64_mod(q1), q1 - 2, q1);
-            A.q1r64ip = shoup_pre(A.q1r64i, (uint32_t)q1);
-            A.one_q1p = shoup_pre(1, (uint32_t)q1);
-        }
+        ProfScope ps(ctx, P_Y_NTT);
+        hip_check(launch_ntt(d_y, y_words / KN, ctx->dev, scale_fwd(ctx, mont), false, ctx->stream), "y ntt");
     }
-    hip_check(launch_pack_chain_scaled(A, Q, ctx->dev, scale_inv(ctx, nullptr), ctx->stream), "pack chain");
+    uint64_t* d_yoff;
+    uint32_t *d_ymode, *d_ycount;
+    upload_meta(ctx, Q, y_off, y_mode, y_count, &d_yoff, &d_ymode, &d_ycount);
+    const size_t out_words = (size_t)Q * 2 * KN;
+    DevBuf& outb = ctx->ws6;
+    outb.reserve(out_words * 4);
+    pack_run(ctx, db, Q, rows, d_slot, d_y, d_yoff, d_ymode, d_ycount, outb.get<uint32_t>());
     std::vector<uint32_t> hout(out_words);
     uint32_t herr = 0;
-    hip_check(hipMemcpyAsync(hout.data(), d_out, out_words * 4, hipMemcpyDeviceToHost, ctx->stream), "out");
+    hip_check(hipMemcpyAsync(hout.data(), outb.get(), out_words * 4, hipMemcpyDeviceToHost, ctx->stream), "out");
     hip_check(hipMemcpyAsync(&herr, d_err, 4, hipMemcpyDeviceToHost, ctx->stream), "err");
     hip_check(hipStreamSynchronize(ctx->stream), "pack respond");
     for (uint32_t t = 0; t < Q; t++) {

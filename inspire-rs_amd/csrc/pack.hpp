@@ -37,11 +37,19 @@ struct PackArgs {
 // Builds (once per gamma) the resident PackParams/OfflinePackingKeys tables.
 PackTables& pack_tables(inspire_ctx* ctx, uint32_t gamma);
 
-// InspiRING tail of a respond batch.  rows: staged RGSW rows (NTT, Montgomery),
-// d_slot: shard slots of the items, d_err: seed-expansion flag.
-void pack_respond(inspire_ctx* ctx, const Database& db, const BatchItem* items, uint32_t Q, const uint32_t* rows,
-                  const int32_t* d_slot, const uint32_t* d_err, const inspire_query* qs, inspire_response* rs,
-                  int32_t* status);
+// Device-only packing of a staged group (rows in NTT/Montgomery form):
+// out [Q][2][K][N].  y: NTT-domain Montgomery client keys with per-query
+// offsets, modes (0: y_body rotated on the fly, 1: explicit y_all) and counts.
+void pack_run(inspire_ctx* ctx, const Database& db, uint32_t Q, const uint32_t* rows, const int32_t* d_slot,
+              const uint32_t* d_y, const uint64_t* d_yoff, const uint32_t* d_ymode, const uint32_t* d_ycount,
+              uint32_t* d_out);
+// Device-resident y_body [Q][ell][K][N] (coefficient domain).
+void pack_run_device_ybody(inspire_ctx* ctx, const Database& db, uint32_t Q, const uint32_t* rows,
+                           const int32_t* d_slot, const uint32_t* y_body, uint32_t* d_out);
+// Host-side keys of the queries `items` (y_body or y_all), responses copied back.
+void pack_respond_host(inspire_ctx* ctx, const Database& db, const BatchItem* items, uint32_t Q, const uint32_t* rows,
+                       const int32_t* d_slot, const uint32_t* d_err, const inspire_query* qs, inspire_response* rs,
+                       int32_t* status);
 
 hipError_t launch_pack_b0(const uint32_t* cols, uint32_t* b_poly, uint32_t Q, uint32_t gamma, const CtxDev& C,
                           hipStream_t st);

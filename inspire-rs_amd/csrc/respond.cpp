@@ -1,5 +1,13 @@
 // Batched respond driver (src/pir/respond.rs) and database residency
 // (src/pir/setup.rs:105-120, src/pir/encode_db.rs:106-164).
+//
+// A respond batch is split by database and by variant class (InsPIRe^0 or
+// InspiRING); each group runs the same device stages whether its inputs came
+// from host buffers (the reference-shaped API) or are already resident in
+// HBM (inspire_respond_batch_device):
+//   rows   RGSW rows ell..2ell-1 -> (seed expansion) -> NTT, Montgomery form
+//   ^0     first dimension -> INTT of every column -> column fold
+//   ^1/^2  first dimension -> pack (b_poly, step 1, chain): pack.cpp
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -14,6 +22,54 @@
 #include "respond.hpp"
 
 namespace inspire {
+
+const char* const kProfNames[P_COUNT] = {"rows_ntt", "expand_seeds", "first_dim", "cols_intt", "fold",
+                                         "pack_b0",  "pack_step1",   "y_ntt",     "pack_chain", "db_load"};
+
+ProfScope::ProfScope(inspire_ctx* ctx, int id) : ctx_(ctx), id_(id) {
+    if (!ctx_->prof.on) return;
+    auto& P = ctx_->prof;
+    if (P.pool.empty()) {
+        hipEvent_t e;
+        hip_check(hipEventCreate(&e), "event");
+        P.pool.push_back(e);
+    }
+    a_ = P.pool.back();
+    P.pool.pop_back();
+    hip_check(hipEventRecord(a_, ctx_->stream), "event record");
+}
+
+ProfScope::~ProfScope() {
+    if (!a_) return;
+    auto& P = ctx_->prof;
+    hipEvent_t b;
+    if (P.pool.empty()) {
+        if (hipEventCreate(&b) != hipSuccess) { (void)hipGetLastError(); return; }
+    } else {
+        b = P.pool.back();
+        P.pool.pop_back();
+    }
+    (void)hipEventRecord(b, ctx_->stream);
+    P.spans.push_back({id_, a_, b});
+}
+
+void prof_collect(inspire_ctx* ctx) {
+    auto& P = ctx->prof;
+    if (P.spans.empty()) return;
+    hip_check(hipStreamSynchronize(ctx->stream), "profile sync");
+    for (auto& s : P.spans) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, s.a, s.b) == hipSuccess) {
+            P.total_ms[s.id] += ms;
+            P.launches[s.id] += 1;
+        } else {
+            (void)hipGetLastError();
+        }
+        P.pool.push_back(s.a);
+        P.pool.push_back(s.b);
+    }
+    P.spans.clear();
+}
 
 void upload_u64(inspire_ctx* ctx, const uint64_t* src, size_t words, uint32_t* dst) {
     if (!words) return;
@@ -31,83 +87,101 @@ void download_u64(inspire_ctx* ctx, const uint32_t* src, size_t words, uint64_t*
     for (size_t i = 0; i < words; i++) dst[i] = ctx->hbuf[i];
 }
 
+static uint64_t mont_of(const inspire_ctx* ctx, uint32_t k) { return (1ull << 32) % ctx->hm[k].q; }
+
 // ---------------------------------------------------------------------------
 // Databases
 // ---------------------------------------------------------------------------
 
+// Preprocesses `np` shard-column polys of residues (device, [np][K][N]) into
+// NTT-domain gadget digits of Delta*h at `dig` ([np][ell][K][N]).
+static void db_digits(inspire_ctx* ctx, const uint32_t* h, size_t np, uint32_t* dig) {
+    ProfScope ps(ctx, P_DB);
+    hip_check(launch_db_digits(h, dig, np, ctx->dev, ctx->stream), "db digits");
+    hip_check(launch_ntt(dig, np * ctx->ell, ctx->dev, scale_none(), false, ctx->stream), "db ntt");
+}
+
+static void db_begin(Database& db, uint32_t num_shards, const uint32_t* shard_ids, uint32_t cols) {
+    db.num_shards = num_shards;
+    db.cols = cols;
+    for (uint32_t s = 0; s < num_shards; s++) db.slot_of.emplace(shard_ids ? shard_ids[s] : s, s);  // first id wins
+}
+
 void db_load(inspire_ctx* ctx, uint32_t db_id, uint32_t num_shards, const uint32_t* shard_ids,
              uint32_t polys_per_shard, const uint64_t* polys) {
     Database db;
-    db.num_shards = num_shards;
-    db.cols = polys_per_shard;
-    for (uint32_t s = 0; s < num_shards; s++) db.slot_of.emplace(shard_ids[s], s);  // keeps the first id
+    db_begin(db, num_shards, shard_ids, polys_per_shard);
     const size_t KN = (size_t)ctx->K * ctx->N;
     const size_t P = (size_t)num_shards * polys_per_shard;
+    ctx->dbs.erase(db_id);  // release the old copy before allocating the new one
     if (P) {
         db.digits.alloc(P * ctx->ell * KN * 4);
-        // Stage in slices so the raw staging buffer stays small.
         const size_t slice = std::max<size_t>(1, (size_t)(64u << 20) / (KN * 4));
         ctx->ws0.reserve(std::min(P, slice) * KN * 4);
         for (size_t p0 = 0; p0 < P; p0 += slice) {
             const size_t np = std::min(slice, P - p0);
             upload_u64(ctx, polys + p0 * KN, np * KN, ctx->ws0.get<uint32_t>());
-            uint32_t* dig = db.digits.get<uint32_t>() + p0 * ctx->ell * KN;
-            hip_check(launch_db_digits(ctx->ws0.get<uint32_t>(), dig, np, ctx->dev, ctx->stream), "db digits");
-            hip_check(launch_ntt(dig, np * ctx->ell, ctx->dev, scale_none(), false, ctx->stream), "db ntt");
+            db_digits(ctx, ctx->ws0.get<uint32_t>(), np, db.digits.get<uint32_t>() + p0 * ctx->ell * KN);
         }
         hip_check(hipStreamSynchronize(ctx->stream), "db load");
     }
+    prof_collect(ctx);
     ctx->dbs[db_id] = std::move(db);
 }
 
-// encode_database (encode_db.rs:106-164) with shard_size = d entries
-// (setup.rs:149-153): column c of local entry k is the LE u16 of bytes
-// (2c, 2c+1) (extract_column_value, encode_db.rs:169-181).
+// encode_database (encode_db.rs:106-164) with shards of d entries
+// (setup.rs:149-153), on the device, in slices of whole shards.
 void db_load_entries(inspire_ctx* ctx, uint32_t db_id, const uint8_t* data, uint64_t len, uint32_t entry_size) {
     if (entry_size == 0 || len == 0) {  // encode_db.rs:112-114: no shards
         db_load(ctx, db_id, 0, nullptr, 0, nullptr);
         return;
     }
     const uint64_t total = len / entry_size, d = ctx->N;
-    const uint32_t npolys = (entry_size * 8 + 15) / 16;
-    const uint32_t nshards = (uint32_t)((total + d - 1) / d);
+    const uint32_t cols = (entry_size * 8 + 15) / 16;
+    const uint64_t nshards64 = (total + d - 1) / d;
+    if (nshards64 > 0x7fffffffull) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "too many shards"};
+    const uint32_t nshards = (uint32_t)nshards64;
     const size_t KN = (size_t)ctx->K * d;
-    std::vector<uint32_t> ids(nshards);
-    std::vector<uint64_t> polys((size_t)nshards * npolys * KN, 0);
-    for (uint32_t s = 0; s < nshards; s++) {
-        ids[s] = s;
-        const uint64_t off = (uint64_t)s * d, actual = std::min<uint64_t>(d, total - off);
-        for (uint32_t c = 0; c < npolys; c++) {
-            uint64_t* poly = polys.data() + ((size_t)s * npolys + c) * KN;
-            const uint32_t bo = 2 * c;
-            for (uint64_t li = 0; li < actual; li++) {
-                const uint8_t* e = data + (off + li) * entry_size;
-                uint64_t v = 0;
-                if (bo + 1 < entry_size) v = (uint64_t)e[bo] | ((uint64_t)e[bo + 1] << 8);
-                else if (bo < entry_size) v = e[bo];
-                v %= ctx->q;
-                for (uint32_t k = 0; k < ctx->K; k++) poly[(size_t)k * d + li] = v % ctx->hm[k].q;
-            }
-        }
+    Database db;
+    db_begin(db, nshards, nullptr, cols);
+    ctx->dbs.erase(db_id);
+    db.digits.alloc((size_t)nshards * cols * ctx->ell * KN * 4);
+    // slices of whole shards, bounded by a 256 MiB residue staging buffer
+    const uint32_t slice = (uint32_t)std::max<size_t>(1, (size_t)(256u << 20) / ((size_t)cols * KN * 4));
+    ctx->ws0.reserve((size_t)std::min(slice, nshards) * cols * KN * 4);
+    ctx->ws5.reserve((size_t)std::min(slice, nshards) * d * entry_size);
+    for (uint32_t s0 = 0; s0 < nshards; s0 += slice) {
+        const uint32_t ns = std::min(slice, nshards - s0);
+        const uint64_t e0 = (uint64_t)s0 * d;
+        const uint64_t ne = std::min<uint64_t>((uint64_t)ns * d, total - e0);
+        hip_check(hipMemcpyAsync(ctx->ws5.get(), data + e0 * entry_size, ne * entry_size, hipMemcpyHostToDevice,
+                                 ctx->stream),
+                  "entries upload");
+        hip_check(launch_encode_entries(ctx->ws5.get<uint8_t>(), ne, entry_size, cols, ns, ctx->ws0.get<uint32_t>(),
+                                        ctx->dev, ctx->stream),
+                  "encode entries");
+        db_digits(ctx, ctx->ws0.get<uint32_t>(), (size_t)ns * cols,
+                  db.digits.get<uint32_t>() + (size_t)s0 * cols * ctx->ell * KN);
+        hip_check(hipStreamSynchronize(ctx->stream), "db slice");
     }
-    db_load(ctx, db_id, nshards, ids.data(), npolys, polys.data());
+    prof_collect(ctx);
+    ctx->dbs[db_id] = std::move(db);
 }
 
 // ---------------------------------------------------------------------------
-// General external product (config: RLWE with a != 0)
+// General external product
 // ---------------------------------------------------------------------------
 
 void external_product_batch(inspire_ctx* ctx, uint64_t count, const uint64_t* rlwe, const uint64_t* rgsw,
                             uint64_t* out) {
     if (!count) return;
     const size_t KN = (size_t)ctx->K * ctx->N, rows = 2 * ctx->ell;
-    // workspace: rlwe -> digits [P][2ell][K][N]; rgsw [P][2ell][2][K][N]; out [P][2][K][N]
     const size_t chunk = std::max<size_t>(1, std::min<size_t>(count, 1024));
     ctx->ws0.reserve(chunk * 2 * KN * 4);
     ctx->ws1.reserve(chunk * rows * KN * 4);
     ctx->ws2.reserve(chunk * rows * 2 * KN * 4);
     uint64_t mont[kMaxK];
-    for (uint32_t k = 0; k < ctx->K; k++) mont[k] = (1ull << 32) % ctx->hm[k].q;
+    for (uint32_t k = 0; k < ctx->K; k++) mont[k] = mont_of(ctx, k);
     for (uint64_t c0 = 0; c0 < count; c0 += chunk) {
         const size_t P = std::min<uint64_t>(chunk, count - c0);
         uint32_t* in = ctx->ws0.get<uint32_t>();
@@ -125,12 +199,57 @@ void external_product_batch(inspire_ctx* ctx, uint64_t count, const uint64_t* rl
 }
 
 // ---------------------------------------------------------------------------
-// Respond
+// Device stages
+// ---------------------------------------------------------------------------
+
+// rows_ws: [Q][ell][2][K][N] staged coefficient rows (a-slots of seeded
+// queries are filled here from their seeds).  seeds: [Q][ell][8] (device).
+// seeded_runs: [first, end) ranges of seeded queries.  -> NTT, Montgomery.
+void stage_rows(inspire_ctx* ctx, uint32_t Q, uint32_t* rows_ws, const uint32_t* seeds,
+                const std::vector<std::pair<uint32_t, uint32_t>>& seeded_runs, uint32_t* d_err) {
+    const size_t KN = (size_t)ctx->K * ctx->N, ell = ctx->ell;
+    {
+        ProfScope ps(ctx, P_EXPAND);
+        for (auto& r : seeded_runs)
+            hip_check(launch_expand_seeds(seeds + (size_t)r.first * ell * 8, rows_ws + (size_t)r.first * ell * 2 * KN,
+                                          (size_t)(r.second - r.first) * ell, 2 * KN, d_err, ctx->dev, ctx->stream),
+                      "expand seeds");
+    }
+    uint64_t mont[kMaxK];
+    for (uint32_t k = 0; k < ctx->K; k++) mont[k] = mont_of(ctx, k);
+    ProfScope ps(ctx, P_ROWS_NTT);
+    hip_check(launch_ntt(rows_ws, (size_t)Q * ell * 2, ctx->dev, scale_fwd(ctx, mont), false, ctx->stream),
+              "rows ntt");
+}
+
+// InsPIRe^0 on staged rows: out [Q][1 + cols][2][K][N] (combined, then columns).
+void run_respond0(inspire_ctx* ctx, const Database& db, uint32_t Q, const uint32_t* rows, const int32_t* d_slot,
+                  uint32_t* out) {
+    const uint32_t cols = db.cols;
+    {
+        ProfScope ps(ctx, P_FIRST_DIM);
+        // polys per query: 2 (combined) + 2 per column; column c at polys 2 + 2c, 3 + 2c
+        hip_check(launch_first_dim(db.digits.get<uint32_t>(), rows, d_slot, out, Q, cols, 2 * (1 + cols), 2, ctx->dev,
+                                   ctx->stream),
+                  "first dim");
+    }
+    {
+        ProfScope ps(ctx, P_COLS_INTT);
+        // One launch over every poly of the response blocks.  The combined slot
+        // (polys 0, 1 of each block) holds scratch here; its transform is
+        // discarded because the fold below overwrites it.
+        hip_check(launch_ntt(out, (size_t)Q * 2 * (1 + cols), ctx->dev, scale_inv(ctx, nullptr), true, ctx->stream),
+                  "cols intt");
+    }
+    ProfScope ps(ctx, P_FOLD);
+    hip_check(launch_sum_cols(out, Q, cols, ctx->dev, ctx->stream), "fold");
+}
+
+// ---------------------------------------------------------------------------
+// Respond (host API)
 // ---------------------------------------------------------------------------
 
 namespace {
-
-using Item = BatchItem;
 
 void fail(int32_t* status, uint32_t i, int code, const std::string& msg) {
     status[i] = code;
@@ -139,81 +258,90 @@ void fail(int32_t* status, uint32_t i, int code, const std::string& msg) {
 
 }  // namespace
 
+// The reference's checks, in its order, for one query.  Returns the database
+// when the query must be computed; nullptr when it finished here (an error,
+// or an empty shard answered with zeros).
+static const Database* admit(inspire_ctx* ctx, const inspire_query& q, uint32_t v, inspire_response* r,
+                             int32_t* status, uint32_t i, bool direct_inspiring, int32_t* slot_out) {
+    const size_t KN = (size_t)ctx->K * ctx->N;
+    r->num_column_ciphertexts = 0;
+    if (v > INSPIRE_VARIANT_TWO_PACKING) { fail(status, i, INSPIRE_ERR_INVALID_ARGUMENT, "unknown variant"); return nullptr; }
+    const bool packed = v != INSPIRE_VARIANT_NO_PACKING;
+    if (packed && !direct_inspiring) {  // respond_with_variant (respond.rs:135-149)
+        if (q.packing_mode == INSPIRE_PACKING_TREE) {
+            fail(status, i, INSPIRE_ERR_UNSUPPORTED, "tree packing (packing_mode=tree) is not on the GPU path");
+            return nullptr;
+        }
+        if (!q.has_packing_keys) {
+            fail(status, i, INSPIRE_ERR_PACKING_KEYS_MISSING_TREE,
+                 "InspiRING packing keys missing (set packing_mode=tree to use tree packing)");
+            return nullptr;
+        }
+    }
+    if (packed && !q.has_packing_keys) {  // respond.rs:296-299
+        fail(status, i, INSPIRE_ERR_PACKING_KEYS_MISSING, "InspiRING client packing keys missing from query");
+        return nullptr;
+    }
+    auto it = ctx->dbs.find(q.db_id);
+    if (it == ctx->dbs.end()) {
+        fail(status, i, INSPIRE_ERR_UNKNOWN_DATABASE, "unknown database " + std::to_string(q.db_id));
+        return nullptr;
+    }
+    const Database& db = it->second;
+    auto sit = db.slot_of.find(q.shard_id);
+    if (sit == db.slot_of.end()) {  // respond.rs:78-82
+        fail(status, i, INSPIRE_ERR_SHARD_NOT_FOUND, "Shard " + std::to_string(q.shard_id) + " not found");
+        return nullptr;
+    }
+    if (db.cols == 0) {  // empty shard (respond.rs:84-90, 307-313): zero ciphertext, [zero]
+        std::memset(r->ciphertext, 0, 2 * KN * 8);
+        r->num_column_ciphertexts = 1;
+        if (r->column_ciphertexts) {
+            if (r->column_capacity < 1) {
+                fail(status, i, INSPIRE_ERR_CAPACITY, "column_ciphertexts capacity too small");
+                return nullptr;
+            }
+            std::memset(r->column_ciphertexts, 0, 2 * KN * 8);
+        }
+        return nullptr;
+    }
+    if (packed && q.y_all_len == 0 && !q.y_body) {  // respond.rs:365-369
+        fail(status, i, INSPIRE_ERR_PACKING_KEYS_EMPTY, "InspiRING packing keys invalid: y_all and y_body are both empty");
+        return nullptr;
+    }
+    if (!packed && r->column_ciphertexts && r->column_capacity < db.cols) {
+        fail(status, i, INSPIRE_ERR_CAPACITY, "column_ciphertexts capacity too small");
+        return nullptr;
+    }
+    *slot_out = (int32_t)sit->second;
+    return &db;
+}
+
 void respond_batch(inspire_ctx* ctx, uint32_t n, const inspire_query* qs, const uint32_t* variants,
                    inspire_response* rs, int32_t* status, bool direct_inspiring) {
     const size_t KN = (size_t)ctx->K * ctx->N, ell = ctx->ell;
-    // Group runnable queries by database.
-    std::map<uint32_t, std::vector<Item>> groups;
+    std::map<uint32_t, std::vector<BatchItem>> groups;
     for (uint32_t i = 0; i < n; i++) {
         status[i] = INSPIRE_OK;
         const inspire_query& q = qs[i];
         inspire_response& r = rs[i];
-        r.num_column_ciphertexts = 0;
         if (!r.ciphertext) { fail(status, i, INSPIRE_ERR_INVALID_ARGUMENT, "response ciphertext buffer is null"); continue; }
         if (!q.rgsw || (q.seeded && !q.rgsw_seeds)) { fail(status, i, INSPIRE_ERR_INVALID_ARGUMENT, "query rgsw is null"); continue; }
-        const uint32_t v = variants[i];
-        if (v > INSPIRE_VARIANT_TWO_PACKING) { fail(status, i, INSPIRE_ERR_INVALID_ARGUMENT, "unknown variant"); continue; }
-        const bool packed = v != INSPIRE_VARIANT_NO_PACKING;
-        if (packed && !direct_inspiring) {  // respond_with_variant (respond.rs:135-149)
-            if (q.packing_mode == INSPIRE_PACKING_TREE) {
-                fail(status, i, INSPIRE_ERR_UNSUPPORTED, "tree packing (packing_mode=tree) is not on the GPU path");
-                continue;
-            }
-            if (!q.has_packing_keys) {
-                fail(status, i, INSPIRE_ERR_PACKING_KEYS_MISSING_TREE,
-                     "InspiRING packing keys missing (set packing_mode=tree to use tree packing)");
-                continue;
-            }
-        }
-        if (packed && !q.has_packing_keys) {  // respond.rs:296-299
-            fail(status, i, INSPIRE_ERR_PACKING_KEYS_MISSING, "InspiRING client packing keys missing from query");
-            continue;
-        }
-        auto it = ctx->dbs.find(q.db_id);
-        if (it == ctx->dbs.end()) {
-            fail(status, i, INSPIRE_ERR_UNKNOWN_DATABASE, "unknown database " + std::to_string(q.db_id));
-            continue;
-        }
-        const Database& db = it->second;
-        auto sit = db.slot_of.find(q.shard_id);
-        if (sit == db.slot_of.end()) {  // respond.rs:78-82
-            fail(status, i, INSPIRE_ERR_SHARD_NOT_FOUND, "Shard " + std::to_string(q.shard_id) + " not found");
-            continue;
-        }
-        if (db.cols == 0) {  // empty shard (respond.rs:84-90, 307-313): zero ciphertext, [zero]
-            std::memset(r.ciphertext, 0, 2 * KN * 8);
-            r.num_column_ciphertexts = 1;
-            if (r.column_ciphertexts) {
-                if (r.column_capacity < 1) { fail(status, i, INSPIRE_ERR_CAPACITY, "column_ciphertexts capacity too small"); continue; }
-                std::memset(r.column_ciphertexts, 0, 2 * KN * 8);
-            }
-            continue;
-        }
-        if (packed && q.y_all_len == 0 && !q.y_body) {  // respond.rs:365-369
-            fail(status, i, INSPIRE_ERR_PACKING_KEYS_EMPTY,
-                 "InspiRING packing keys invalid: y_all and y_body are both empty");
-            continue;
-        }
-        if (!packed && r.column_ciphertexts && r.column_capacity < db.cols) {
-            fail(status, i, INSPIRE_ERR_CAPACITY, "column_ciphertexts capacity too small");
-            continue;
-        }
-        groups[q.db_id].push_back(Item{i, (int32_t)sit->second, packed});
+        int32_t slot = -1;
+        if (!admit(ctx, q, variants[i], &r, status, i, direct_inspiring, &slot)) continue;
+        groups[q.db_id].push_back(BatchItem{i, slot, variants[i] != INSPIRE_VARIANT_NO_PACKING});
     }
-
-    uint64_t mont[kMaxK];
-    for (uint32_t k = 0; k < ctx->K; k++) mont[k] = (1ull << 32) % ctx->hm[k].q;
 
     for (auto& kv : groups) {
         const Database& db = ctx->dbs.at(kv.first);
         const uint32_t cols = db.cols;
         for (int pass = 0; pass < 2; pass++) {  // 0: InsPIRe^0, 1: InspiRING
-            std::vector<Item> items;
-            for (const Item& it : kv.second)
+            std::vector<BatchItem> items;
+            for (const BatchItem& it : kv.second)
                 if (it.packed == (pass == 1)) items.push_back(it);
             if (items.empty()) continue;
             const uint32_t Q = (uint32_t)items.size();
-            // ---- stage rows ell..2ell-1: [Q][ell][2][K][N], seeds, slots
+            // ---- stage rows ell..2ell-1 (the a-digits of the trivial encryption are 0)
             ctx->ws1.reserve((size_t)Q * ell * 2 * KN * 4);
             ctx->ws4.reserve((size_t)Q * (ell * 8 + 1) * 4 + 64);
             uint32_t* rows = ctx->ws1.get<uint32_t>();
@@ -223,15 +351,18 @@ void respond_batch(inspire_ctx* ctx, uint32_t n, const inspire_query* qs, const 
             std::vector<uint32_t> hrows((size_t)Q * ell * 2 * KN, 0);
             std::vector<uint32_t> hseeds((size_t)Q * ell * 8, 0);
             std::vector<int32_t> hslot(Q);
-            bool any_seeded = false;
+            std::vector<std::pair<uint32_t, uint32_t>> runs;
             for (uint32_t t = 0; t < Q; t++) {
                 const inspire_query& q = qs[items[t].idx];
                 hslot[t] = items[t].slot;
+                if (q.seeded) {
+                    if (!runs.empty() && runs.back().second == t) runs.back().second = t + 1;
+                    else runs.push_back({t, t + 1});
+                }
                 for (size_t i = 0; i < ell; i++) {
-                    const size_t row = ell + i;  // rows ell..2ell-1 only: the a-digits are 0
+                    const size_t row = ell + i;
                     uint32_t* dst = hrows.data() + ((size_t)t * ell + i) * 2 * KN;
                     if (q.seeded) {
-                        any_seeded = true;
                         const uint64_t* b = q.rgsw + row * KN;
                         for (size_t w = 0; w < KN; w++) dst[KN + w] = (uint32_t)b[w];
                         std::memcpy(&hseeds[((size_t)t * ell + i) * 8], q.rgsw_seeds + 32 * row, 32);
@@ -243,39 +374,15 @@ void respond_batch(inspire_ctx* ctx, uint32_t n, const inspire_query* qs, const 
             }
             hip_check(hipMemcpyAsync(rows, hrows.data(), hrows.size() * 4, hipMemcpyHostToDevice, ctx->stream), "rows");
             hip_check(hipMemcpyAsync(d_slot, hslot.data(), Q * 4, hipMemcpyHostToDevice, ctx->stream), "slots");
+            hip_check(hipMemcpyAsync(d_seeds, hseeds.data(), hseeds.size() * 4, hipMemcpyHostToDevice, ctx->stream),
+                      "seeds");
             hip_check(hipMemsetAsync(d_err, 0, 4, ctx->stream), "err");
-            if (any_seeded) {
-                hip_check(hipMemcpyAsync(d_seeds, hseeds.data(), hseeds.size() * 4, hipMemcpyHostToDevice, ctx->stream),
-                          "seeds");
-                // The a-polys of seeded rows come from their seeds: one launch over
-                // every row of each run of consecutive seeded queries.
-                for (uint32_t t = 0; t < Q;) {
-                    if (!qs[items[t].idx].seeded) { t++; continue; }
-                    uint32_t e = t;
-                    while (e < Q && qs[items[e].idx].seeded) e++;
-                    hip_check(launch_expand_seeds(d_seeds + (size_t)t * ell * 8, rows + (size_t)t * ell * 2 * KN,
-                                                  (size_t)(e - t) * ell, 2 * KN, d_err, ctx->dev, ctx->stream),
-                              "expand seeds");
-                    t = e;
-                }
-            }
-            hip_check(launch_ntt(rows, (size_t)Q * ell * 2, ctx->dev, scale_fwd(ctx, mont), false, ctx->stream),
-                      "rows ntt");
+            stage_rows(ctx, Q, rows, d_seeds, runs, d_err);
             if (pass == 0) {
-                // ---- InsPIRe^0: [Q][1 + cols][2][K][N]
                 const size_t per = (size_t)(1 + cols) * 2 * KN;
                 ctx->ws2.reserve(Q * per * 4);
                 uint32_t* out = ctx->ws2.get<uint32_t>();
-                // polys per query: 2 (combined) + 2 per column; column c at polys 2 + 2c, 3 + 2c
-                hip_check(launch_first_dim(db.digits.get<uint32_t>(), rows, d_slot, out, Q, cols, 2 * (1 + cols), 2,
-                                           ctx->dev, ctx->stream),
-                          "first dim");
-                // INTT every poly of the response; the combined slot (polys 0, 1) is
-                // overwritten by the fold afterwards.
-                hip_check(launch_ntt(out, (size_t)Q * (1 + cols) * 2, ctx->dev, scale_inv(ctx, nullptr), true,
-                                     ctx->stream),
-                          "cols intt");
-                hip_check(launch_sum_cols(out, Q, cols, ctx->dev, ctx->stream), "fold");
+                run_respond0(ctx, db, Q, rows, d_slot, out);
                 std::vector<uint32_t> hout(Q * per);
                 uint32_t herr = 0;
                 hip_check(hipMemcpyAsync(hout.data(), out, hout.size() * 4, hipMemcpyDeviceToHost, ctx->stream), "out");
@@ -292,10 +399,64 @@ void respond_batch(inspire_ctx* ctx, uint32_t n, const inspire_query* qs, const 
                         for (size_t w = 0; w < (size_t)cols * 2 * KN; w++) r.column_ciphertexts[w] = src[2 * KN + w];
                 }
             } else {
-                pack_respond(ctx, db, items.data(), Q, rows, d_slot, d_err, qs, rs, status);
+                pack_respond_host(ctx, db, items.data(), Q, rows, d_slot, d_err, qs, rs, status);
             }
         }
     }
+    prof_collect(ctx);
+}
+
+// ---------------------------------------------------------------------------
+// Respond (device-resident batch)
+// ---------------------------------------------------------------------------
+
+void respond_batch_device(inspire_ctx* ctx, const inspire_device_batch& b) {
+    const size_t KN = (size_t)ctx->K * ctx->N, ell = ctx->ell;
+    const uint32_t Q = b.count;
+    if (!Q) return;
+    if (!b.shard_ids || !b.rows || !b.out || !b.status || (b.seeded && !b.seeds))
+        throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "null argument in device batch"};
+    if (b.variant > INSPIRE_VARIANT_TWO_PACKING) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "unknown variant"};
+    const bool packed = b.variant != INSPIRE_VARIANT_NO_PACKING;
+    if (packed && !b.y_body)
+        throw Error{INSPIRE_ERR_PACKING_KEYS_MISSING, "InspiRING client packing keys missing from query"};
+    auto it = ctx->dbs.find(b.db_id);
+    if (it == ctx->dbs.end()) throw Error{INSPIRE_ERR_UNKNOWN_DATABASE, "unknown database " + std::to_string(b.db_id)};
+    const Database& db = it->second;
+    if (db.cols == 0) throw Error{INSPIRE_ERR_UNSUPPORTED, "device batches need a database with columns"};
+    // slots (a query whose shard is missing gets slot -1 and status SHARD_NOT_FOUND)
+    ctx->ws4.reserve((size_t)Q * 4 + 64);
+    int32_t* d_slot = ctx->ws4.get<int32_t>();
+    uint32_t* d_err = reinterpret_cast<uint32_t*>(d_slot + Q);
+    std::vector<int32_t> hslot(Q);
+    for (uint32_t t = 0; t < Q; t++) {
+        auto sit = db.slot_of.find(b.shard_ids[t]);
+        hslot[t] = sit == db.slot_of.end() ? -1 : (int32_t)sit->second;
+        b.status[t] = hslot[t] < 0 ? INSPIRE_ERR_SHARD_NOT_FOUND : INSPIRE_OK;
+    }
+    hip_check(hipMemcpyAsync(d_slot, hslot.data(), Q * 4, hipMemcpyHostToDevice, ctx->stream), "slots");
+    hip_check(hipMemsetAsync(d_err, 0, 4, ctx->stream), "err");
+    // rows: the caller's rows into the workspace layout
+    ctx->ws1.reserve((size_t)Q * ell * 2 * KN * 4);
+    uint32_t* rows = ctx->ws1.get<uint32_t>();
+    std::vector<std::pair<uint32_t, uint32_t>> runs;
+    if (b.seeded) {
+        hip_check(hipMemcpy2DAsync(rows + KN, 2 * KN * 4, b.rows, KN * 4, KN * 4, (size_t)Q * ell,
+                                   hipMemcpyDeviceToDevice, ctx->stream),
+                  "rows b");
+        runs.push_back({0, Q});
+    } else {
+        hip_check(hipMemcpyAsync(rows, b.rows, (size_t)Q * ell * 2 * KN * 4, hipMemcpyDeviceToDevice, ctx->stream),
+                  "rows");
+    }
+    stage_rows(ctx, Q, rows, b.seeds, runs, d_err);
+    if (!packed) run_respond0(ctx, db, Q, rows, d_slot, b.out);
+    else pack_run_device_ybody(ctx, db, Q, rows, d_slot, b.y_body, b.out);
+    uint32_t herr = 0;
+    hip_check(hipMemcpyAsync(&herr, d_err, 4, hipMemcpyDeviceToHost, ctx->stream), "err");
+    hip_check(hipStreamSynchronize(ctx->stream), "device batch");
+    prof_collect(ctx);
+    if (herr) throw Error{INSPIRE_ERR_DEVICE, "seed expansion exhausted its draw budget"};
 }
 
 }  // namespace inspire

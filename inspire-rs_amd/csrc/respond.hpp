@@ -30,4 +30,7 @@ void external_product_batch(inspire_ctx* ctx, uint64_t count, const uint64_t* rl
 void respond_batch(inspire_ctx* ctx, uint32_t n, const inspire_query* qs, const uint32_t* variants,
                    inspire_response* rs, int32_t* status, bool direct_inspiring);
 
+// Same work for a batch whose inputs are resident in device memory.
+void respond_batch_device(inspire_ctx* ctx, const inspire_device_batch& b);
+
 }  // namespace inspire

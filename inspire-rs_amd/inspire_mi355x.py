@@ -82,6 +82,21 @@ class _Query(C.Structure):
     ]
 
 
+class _DeviceBatch(C.Structure):
+    _fields_ = [
+        ("count", C.c_uint32),
+        ("db_id", C.c_uint32),
+        ("variant", C.c_uint32),
+        ("seeded", C.c_uint32),
+        ("shard_ids", C.POINTER(C.c_uint32)),
+        ("rows", C.c_void_p),
+        ("seeds", C.c_void_p),
+        ("y_body", C.c_void_p),
+        ("out", C.c_void_p),
+        ("status", C.POINTER(C.c_int32)),
+    ]
+
+
 class _Response(C.Structure):
     _fields_ = [
         ("ciphertext", C.POINTER(C.c_uint64)),
@@ -111,6 +126,13 @@ EXPORTS = [
         "inspire_respond_batch",
         C.c_int,
         [C.c_void_p, C.c_uint32, C.POINTER(_Query), C.POINTER(C.c_uint32), C.POINTER(_Response), C.POINTER(C.c_int32)],
+    ),
+    ("inspire_respond_batch_device", C.c_int, [C.c_void_p, C.POINTER(_DeviceBatch)]),
+    ("inspire_ctx_set_profiling", C.c_int, [C.c_void_p, C.c_int]),
+    (
+        "inspire_ctx_profile",
+        C.c_int,
+        [C.c_void_p, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.c_int],
     ),
     ("inspire_ntt_forward", C.c_int, [C.c_void_p, _u64p, C.c_uint64]),
     ("inspire_ntt_inverse", C.c_int, [C.c_void_p, _u64p, C.c_uint64]),
@@ -262,9 +284,13 @@ class Context:
                                      _ptr64(sp) if sp.size else None))
         self._dbs[db_id] = cols
 
-    def load_entries(self, db_id: int, data: bytes, entry_size: int):
-        """encode_database (src/pir/encode_db.rs:106-164) of raw entries, then load."""
-        buf = np.frombuffer(bytes(data), dtype=np.uint8).copy()
+    def load_entries(self, db_id: int, data, entry_size: int):
+        """encode_database (src/pir/encode_db.rs:106-164) of raw entries, then load.
+        data: bytes or a uint8 numpy array (used in place, not copied)."""
+        if isinstance(data, np.ndarray):
+            buf = np.ascontiguousarray(data.reshape(-1), dtype=np.uint8)
+        else:
+            buf = np.frombuffer(bytes(data), dtype=np.uint8)
         _raise(lib().inspire_db_load_entries(self._h, db_id, buf.ctypes.data_as(C.POINTER(C.c_uint8)), buf.size,
                                              entry_size))
         self._dbs[db_id] = (entry_size * 8 + 15) // 16 if (entry_size and len(data)) else 0
@@ -340,6 +366,30 @@ class Context:
                 res.append(ServerResponse(ct, cc[: ra[i].num_column_ciphertexts]))
         return res
 
+    # -- device-resident batches and profiling
+    def respond_batch_device(self, db_id: int, variant: int, shard_ids, rows_ptr: int, out_ptr: int,
+                             seeds_ptr: int = 0, y_body_ptr: int = 0, seeded: bool = True):
+        """Runs one batch whose inputs live in device memory (pointers from e.g.
+        torch tensors; layouts in include/inspire_mi355x.h).  Returns the
+        per-query status array."""
+        ids = np.ascontiguousarray(shard_ids, dtype=np.uint32)
+        st = np.zeros(ids.size, dtype=np.int32)
+        b = _DeviceBatch(ids.size, db_id, variant, 1 if seeded else 0, ids.ctypes.data_as(C.POINTER(C.c_uint32)),
+                         rows_ptr, seeds_ptr or None, y_body_ptr or None, out_ptr,
+                         st.ctypes.data_as(C.POINTER(C.c_int32)))
+        _raise(lib().inspire_respond_batch_device(self._h, C.byref(b)))
+        return st
+
+    def set_profiling(self, on: bool):
+        _raise(lib().inspire_ctx_set_profiling(self._h, 1 if on else 0))
+
+    def profile(self, kernel: str, reset: bool = False):
+        """(total milliseconds, launches) of one kernel since the last reset."""
+        ms = C.c_double()
+        n = C.c_uint64()
+        _raise(lib().inspire_ctx_profile(self._h, kernel.encode(), C.byref(ms), C.byref(n), 1 if reset else 0))
+        return ms.value, n.value
+
     # -- math entry points
     def ntt_forward(self, polys: np.ndarray) -> np.ndarray:
         """NttContext::forward (src/math/ntt.rs:223-246), reference representation."""
@@ -361,6 +411,64 @@ class Context:
         out = np.zeros((count, 2, self.K, self.d), dtype=np.uint64)
         _raise(lib().inspire_external_product(self._h, count, _ptr64(rl), _ptr64(rg), _ptr64(out)))
         return out
+
+
+class DeviceArray:
+    """A device allocation made through the HIP runtime the library uses
+    (for inspire_respond_batch_device inputs and outputs)."""
+
+    _hip = None
+
+    @classmethod
+    def hip(cls):
+        if cls._hip is None:
+            lib()  # the library's runtime is the one loaded
+            h = C.CDLL("libamdhip64.so.7")
+            h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+            h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+            h.hipFree.argtypes = [C.c_void_p]
+            cls._hip = h
+        return cls._hip
+
+    def __init__(self, nbytes: int):
+        p = C.c_void_p()
+        rc = self.hip().hipMalloc(C.byref(p), max(int(nbytes), 1))
+        if rc != 0:
+            raise PirError(ERR_OUT_OF_MEMORY, f"hipMalloc({nbytes}) failed: {rc}")
+        self.ptr = p.value
+        self.nbytes = int(nbytes)
+
+    @classmethod
+    def from_numpy(cls, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.nbytes)
+        d.upload(a)
+        return d
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        rc = self.hip().hipMemcpy(self.ptr, a.ctypes.data, a.nbytes, 1)
+        if rc != 0:
+            raise PirError(ERR_DEVICE, f"hipMemcpy H2D failed: {rc}")
+
+    def download(self, a: np.ndarray) -> np.ndarray:
+        assert a.flags["C_CONTIGUOUS"] and a.nbytes <= self.nbytes
+        rc = self.hip().hipMemcpy(a.ctypes.data, self.ptr, a.nbytes, 2)
+        if rc != 0:
+            raise PirError(ERR_DEVICE, f"hipMemcpy D2H failed: {rc}")
+        return a
+
+    def free(self):
+        if self.ptr:
+            self.hip().hipFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def respond(ctx: Context, db_id: int, query: ClientQuery) -> ServerResponse:

@@ -98,6 +98,12 @@ def _load():
             C.c_int,
             [P, C.c_void_p, C.c_void_p, _u64p, C.c_uint32, _u64p, _u64p, _u64p, C.c_uint32, _u64p],
         ),
+        "or_respond_inspiring_many": (
+            None,
+            [P, C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(_u64p), C.c_uint32, C.POINTER(_u64p),
+             C.POINTER(_u64p), C.POINTER(_u64p), C.POINTER(C.c_int)],
+        ),
+        "or_num_threads": (C.c_int, []),
         "or_chacha20_block": (None, [_u32p, C.c_uint64, C.c_uint64, _u32p]),
         "or_chacha_seed": (None, [C.POINTER(OrChacha), _u8p]),
         "or_chacha_next_u32": (C.c_uint32, [C.POINTER(OrChacha)]),
@@ -327,6 +333,19 @@ class Oracle:
         rc = lib.or_respond_inspiring(C.byref(self.cp), self.ntt, pk, _p64(shard_polys) if ncols else None, ncols,
                                       _p64(np.ascontiguousarray(rgsw)), yb, ya, ylen, _p64(ct))
         return rc, ct
+
+    def respond_inspiring_many(self, w_seed, shards, rgsws, y_bodies):
+        """Many y_body respond_inspiring calls, one per OpenMP thread at a time."""
+        n = len(shards)
+        ncols = shards[0].shape[0]
+        pk = self.pack(ncols, w_seed)
+        keep = [np.ascontiguousarray(x, dtype=np.uint64) for x in list(shards) + list(rgsws) + list(y_bodies)]
+        outs = [self.zeros(2) for _ in range(n)]
+        arr = lambda xs: (_u64p * n)(*[_p64(x) for x in xs])
+        rc = (C.c_int * n)()
+        lib.or_respond_inspiring_many(C.byref(self.cp), self.ntt, pk, n, arr(keep[:n]), ncols, arr(keep[n:2 * n]),
+                                      arr(keep[2 * n:]), arr(outs), rc)
+        return list(rc), outs
 
     # -- automorphisms
     def automorph_table(self, t):

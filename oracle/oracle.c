@@ -27,6 +27,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 typedef unsigned __int128 u128;
 
@@ -705,6 +708,7 @@ struct or_pack {
     uint64_t mod_inv_gamma;
     uint64_t *mod_inv_ntt;  /* [K][d] */
     uint32_t *tables;       /* [gamma][n]: table for gen_pows[i] */
+    uint64_t *mono;         /* [gamma][gamma][K][d]: NTT(X^e_ij), e_ij = j g^((n-i) mod n) mod 2n */
     uint64_t *w_mask;       /* [ell][K][d], coefficient form */
     uint64_t *w_all_ntt;    /* [(gamma-1)][ell][K][d] */
 };
@@ -772,6 +776,14 @@ or_pack *or_pack_new_mode(const or_params *pr, const or_ntt *c, uint32_t gamma, 
     pk->tables = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)gamma * n);
     for (uint32_t i = 0; i < gamma; i++)
         or_automorph_table(n, 2 * table_index(pk->gen_pows[i]) + 1, pk->tables + (size_t)i * n);
+    /* the NTT-form monomials packing_offline step 1 reads (PackParams.monomials_ntt,
+     * inspiring2.rs:143-158), kept per (i, j) */
+    pk->mono = palloc(pr, (size_t)gamma * gamma);
+    for (uint32_t i = 0; i < gamma; i++)
+        for (uint32_t j = 0; j < gamma; j++) {
+            uint32_t e = (uint32_t)(((uint64_t)j * pk->gen_pows[(n - i) % n]) % (2ull * n));
+            monomial_ntt(pk, e, pk->mono + ((size_t)i * gamma + j) * P);
+        }
     /* OfflinePackingKeys::generate (inspiring2.rs:448-506) */
     pk->w_mask = palloc(pr, ell);
     or_generate_mask(pr, w_seed, pk->w_mask);
@@ -793,6 +805,7 @@ void or_pack_free(or_pack *pk) {
     free(pk->gen_pows);
     free(pk->mod_inv_ntt);
     free(pk->tables);
+    free(pk->mono);
     free(pk->w_all_ntt);
     free(pk->w_mask);
     free(pk);
@@ -812,12 +825,8 @@ void or_packing_offline(const or_pack *pk, const uint64_t *a_ct, uint64_t *a_hat
     for (uint32_t j = 0; j < gamma; j++) or_ntt_forward(c, a_ntt + (size_t)j * P);
     for (uint32_t i = 0; i < gamma; i++) {                            /* step 1, :952-982 */
         memset(r, 0, P * 8);
-        for (uint32_t j = 0; j < gamma; j++) {
-            uint32_t exp_index = (n - i) % n;
-            uint32_t index = (uint32_t)(((uint64_t)j * pk->gen_pows[exp_index]) % (2ull * n));
-            monomial_ntt(pk, index, mono);
-            poly_mul_acc(pr, c, r, mono, a_ntt + (size_t)j * P);
-        }
+        for (uint32_t j = 0; j < gamma; j++)  /* index = j * gen_pows[(n - i) % n] mod 2n */
+            poly_mul_acc(pr, c, r, pk->mono + ((size_t)i * gamma + j) * P, a_ntt + (size_t)j * P);
         poly_pointwise(pr, c, r, pk->mod_inv_ntt, tmp);
         pack_automorph_ntt(pk, tmp, pk->tables + (size_t)i * n, r_all + (size_t)i * P);
     }
@@ -915,6 +924,24 @@ int or_respond_inspiring(const or_params *pr, const or_ntt *c, const or_pack *pk
     }
     free(cols); free(a_ct); free(bcf); free(b_poly);
     return rc;
+}
+
+/* n independent respond_inspiring calls, one query per thread (the CPU
+ * baseline's throughput form: rayon over queries instead of over columns). */
+void or_respond_inspiring_many(const or_params *pr, const or_ntt *c, const or_pack *pk, uint32_t n,
+                               const uint64_t *const *shards, uint32_t ncols, const uint64_t *const *rgsw,
+                               const uint64_t *const *y_body, uint64_t *const *ct_out, int *rc) {
+#pragma omp parallel for schedule(dynamic, 1)
+    for (uint32_t i = 0; i < n; i++)
+        rc[i] = or_respond_inspiring(pr, c, pk, shards[i], ncols, rgsw[i], y_body[i], NULL, 0, ct_out[i]);
+}
+
+int or_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
 }
 
 /* ------------------------------------------------------------------ */

@@ -112,6 +112,12 @@ int or_respond_inspiring(const or_params *pr, const or_ntt *c, const or_pack *pk
                          const uint64_t *y_body, const uint64_t *y_all, uint32_t y_all_len,
                          uint64_t *ct_out);
 
+/* n respond_inspiring calls (y_body keys) spread over the OpenMP threads. */
+void or_respond_inspiring_many(const or_params *pr, const or_ntt *c, const or_pack *pk, uint32_t n,
+                               const uint64_t *const *shards, uint32_t ncols, const uint64_t *const *rgsw,
+                               const uint64_t *const *y_body, uint64_t *const *ct_out, int *rc);
+int or_num_threads(void);
+
 /* ---- ChaCha20Rng (rand_chacha 0.3) + rand 0.8 gen_range ---- */
 typedef struct {
     uint32_t key[8];

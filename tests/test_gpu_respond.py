@@ -222,3 +222,55 @@ def test_batch_mixes_variants_and_databases(M, inp):
             else:
                 _, ct = crt.respond_inspiring(W_SEED, shards[q.shard_id], rows, y_body=q.y_body, y_all=q.y_all)
             assert (res[i].ciphertext == ct).all(), i
+
+
+@pytest.mark.parametrize("variant", ["two", "zero", "one_full"])
+def test_device_batch_matches_host_api(M, variant):
+    """inspire_respond_batch_device (inputs resident in HBM) == the host API."""
+    say("device batch", variant)
+    o = O.Oracle(O.baseline_params())
+    d, K = o.params.d, o.params.K
+    db = random_db(3 * d, 32, seed=51)
+    shards = encode_db(o, db, 32)
+    cl = Client(o, 52)
+    yb, _ = cl.packing_keys(16, W_SEED, with_y_all=False)
+    g = np.random.default_rng(53)
+    n = 7
+    sids = g.integers(0, 3, n).astype(np.uint32)
+    sids[3] = 99  # unknown shard: reported per query, the others still run
+    locals_ = g.integers(0, d, n)
+    qs, rows, seeds, ys = [], [], [], []
+    for i in range(n):
+        if variant == "one_full":
+            full = cl.query_rows(int(locals_[i]))
+            rows.append(full[3:6].astype(np.uint32))
+            qs.append(M.ClientQuery(shard_id=int(sids[i]), rgsw=full, has_packing_keys=True, y_body=yb))
+        else:
+            s, b = cl.seeded_query(int(locals_[i]))
+            rows.append(b[3:6].astype(np.uint32))
+            seeds.append(np.frombuffer(s[3:6].tobytes(), dtype="<u4").reshape(3, 8))
+            qs.append(M.ClientQuery(shard_id=int(sids[i]), rgsw=b, seeds=s, has_packing_keys=True, y_body=yb))
+        ys.append(yb.astype(np.uint32))
+    v = {"two": M.TWO_PACKING, "zero": M.NO_PACKING, "one_full": M.ONE_PACKING}[variant]
+    cols = 16
+    per = (2 if v != M.NO_PACKING else 2 * (1 + cols)) * K * d
+    with make_ctx(M, o.params) as ctx:
+        ctx.load_database(0, shards)
+        drows = M.DeviceArray.from_numpy(np.stack(rows))
+        dseeds = M.DeviceArray.from_numpy(np.stack(seeds)) if seeds else None
+        dy = M.DeviceArray.from_numpy(np.stack(ys))
+        dout = M.DeviceArray(n * per * 4)
+        st = ctx.respond_batch_device(0, v, sids, drows.ptr, dout.ptr, seeds_ptr=dseeds.ptr if dseeds else 0,
+                                      y_body_ptr=dy.ptr if v != M.NO_PACKING else 0, seeded=bool(seeds))
+        out = dout.download(np.zeros((n, per), dtype=np.uint32))
+        for i in range(n):
+            if sids[i] == 99:
+                assert st[i] == M.ERR_SHARD_NOT_FOUND
+                continue
+            assert st[i] == M.OK
+            ref = M.respond_with_variant(ctx, 0, qs[i], v)
+            if v == M.NO_PACKING:
+                want = np.concatenate([ref.ciphertext.reshape(-1), ref.column_ciphertexts.reshape(-1)])
+            else:
+                want = ref.ciphertext.reshape(-1)
+            assert np.array_equal(out[i].astype(np.uint64), want), i
