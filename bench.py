@@ -121,6 +121,9 @@ def main():
     ap.add_argument("--single-steps", type=int, default=32, help="steps of the one-query-per-launch run")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (a known-size kernel for "
+                         "calibrating rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the respond path shards by database across GPUs with no collective, "
@@ -156,10 +159,19 @@ def main():
         if (st != 0).any():
             raise RuntimeError(f"status {st[st != 0][:4]}")
 
+    if args.calibrate_copy_mib:
+        nb = args.calibrate_copy_mib << 20
+        src, dst = M.DeviceArray(nb), M.DeviceArray(nb)
+        M.DeviceArray.hip().hipMemcpy(dst.ptr, src.ptr, nb, 3)
+        M.DeviceArray.hip().hipDeviceSynchronize()
+        src.free(), dst.free()
+        log(f"calibration copy: {args.calibrate_copy_mib} MiB device to device")
+
     for i in range(args.warmup):
         step(i)
     ctx.set_profiling(True)
-    for name in ("pack_chain", "first_dim", "pack_step1", "rows_ntt", "y_ntt", "expand_seeds", "pack_b0"):
+    kernels = ("pack_chain", "pack_front", "first_dim", "pack_step1", "pack_b0", "rows_ntt", "y_ntt", "expand_seeds")
+    for name in kernels:
         ctx.profile(name, reset=True)
     lat = []
     t_start = time.perf_counter()
@@ -168,13 +180,13 @@ def main():
         step(i)
         lat.append((time.perf_counter() - t0) * 1e3)
     elapsed = time.perf_counter() - t_start
-    prof = {name: ctx.profile(name, reset=True) for name in
-            ("pack_chain", "first_dim", "pack_step1", "rows_ntt", "y_ntt", "expand_seeds", "pack_b0")}
+    prof = {name: ctx.profile(name, reset=True) for name in kernels}
     ctx.set_profiling(False)
     qps = B * args.steps / elapsed
     log(f"batched: {qps:.0f} queries/s, {elapsed / args.steps * 1e3:.2f} ms/step")
     for k_, (ms, n) in prof.items():
-        log(f"  {k_:13s} {ms / max(n, 1):8.3f} ms/launch x {n}")
+        if n:
+            log(f"  {k_:13s} {ms / n:8.3f} ms/launch x {n}")
 
     # the same path, one query per launch
     single = []
@@ -190,7 +202,8 @@ def main():
 
     chain_ms, chain_n = prof["pack_chain"]
     chain_avg_s = chain_ms / max(chain_n, 1) / 1e3
-    fd_ms, fd_n = prof["first_dim"]
+    front = "pack_front" if prof["pack_front"][1] else "first_dim"
+    fd_ms, fd_n = prof[front]
     fd_avg_s = fd_ms / max(fd_n, 1) / 1e3
     chain_tmodmul = MODMUL_CHAIN * B / chain_avg_s / 1e12
     roofline = {
@@ -206,7 +219,7 @@ def main():
     }
     roofline_hbm = {
         "bound": "hbm",
-        "kernel": "k_first_dim",
+        "kernel": "k_pack_front" if front == "pack_front" else "k_first_dim",
         "achieved": round(FIRST_DIM_BYTES * B / fd_avg_s / 1e9, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
@@ -243,7 +256,8 @@ def main():
         "speedup_vs_single_query_launch": round(qps / single_qps, 2),
         "modmul_per_query": MODMUL_QUERY,
         "roofline": roofline,
-        "roofline_hbm_first_dim": roofline_hbm,
+        "roofline_hbm_front": roofline_hbm,
+        "kernel_ms_per_step": {k_: round(ms / n, 4) for k_, (ms, n) in prof.items() if n},
         "cpu_baseline": None,
     }
     if not args.no_cpu:

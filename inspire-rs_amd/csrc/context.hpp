@@ -66,11 +66,14 @@ struct PackTables {
     DevBuf perm;       // [gamma][N] u32: automorphism table for g^i
     DevBuf w_all;      // [(gamma-1)][ell][K][N] u32 NTT domain, Montgomery form
     std::vector<uint32_t> gen_pows;  // g^i mod 2n, i < n
+    bool fast_front = false;   // gamma = 2^k, gamma^2 | 2n: step 1 as a gamma-point DFT
+    DevBuf twist;              // [K][gamma][N] gamma^-1 w_s^j, Montgomery form
+    DevBuf dftw;               // [K][gamma/2][gamma/2] (W_c^m, companion), W_c = zeta^-1 of class c
 };
 
 // HIP-event timing of each kernel category on the context's stream.
 enum ProfId {
-    P_ROWS_NTT, P_EXPAND, P_FIRST_DIM, P_COLS_INTT, P_FOLD, P_B0, P_STEP1, P_Y_NTT, P_CHAIN, P_DB, P_COUNT
+    P_ROWS_NTT, P_EXPAND, P_FIRST_DIM, P_COLS_INTT, P_FOLD, P_B0, P_STEP1, P_Y_NTT, P_CHAIN, P_DB, P_FRONT, P_COUNT
 };
 extern const char* const kProfNames[P_COUNT];
 

@@ -42,6 +42,22 @@ struct NttShape {
 
 __device__ __forceinline__ int lds_ix(int x) { return x + 4 * (x >> 5); }
 
+// Twiddle sources: tw(i) -> (psi^br(i), Shoup companion), itw(i) likewise for
+// psi^-1.  GlobalTw reads the tables of a ModDev; LdsTw reads (w, w') pairs
+// staged in LDS (one 64-bit read per twiddle).
+struct GlobalTw {
+    const ModDev& M;
+    __device__ __forceinline__ uint2 tw(int i) const { return make_uint2(M.tw[i], M.twp[i]); }
+    __device__ __forceinline__ uint2 itw(int i) const { return make_uint2(M.itw[i], M.itwp[i]); }
+};
+
+struct LdsTw {
+    const uint2* fwd;  // [N] (psi^br(i), companion)
+    const uint2* inv;  // [N] (psi^-br(i), companion)
+    __device__ __forceinline__ uint2 tw(int i) const { return fwd[i]; }
+    __device__ __forceinline__ uint2 itw(int i) const { return inv[i]; }
+};
+
 // Harvey CT butterfly: X, Y < 4q  ->  X + wY, X - wY  (both < 4q).
 __device__ __forceinline__ void ct_bf(uint32_t& X, uint32_t& Y, uint32_t w, uint32_t wp, uint32_t q,
                                       uint32_t q2) {
@@ -61,8 +77,8 @@ __device__ __forceinline__ void gs_bf(uint32_t& X, uint32_t& Y, uint32_t w, uint
 }
 
 // One forward radix-8 pass over stages m0, 2m0, 4m0 (t0 = N / (2 m0)).
-template <int LOGN>
-__device__ __forceinline__ void fwd_pass8(uint32_t* s, int tid, int m0, const ModDev& M) {
+template <int LOGN, typename TW>
+__device__ __forceinline__ void fwd_pass8(uint32_t* s, int tid, int m0, const ModDev& M, const TW& T) {
     constexpr int N = 1 << LOGN;
     const int t0 = N / (2 * m0), st = t0 >> 2;
     const int i = tid / st, r = tid - i * st;
@@ -71,28 +87,28 @@ __device__ __forceinline__ void fwd_pass8(uint32_t* s, int tid, int m0, const Mo
 #pragma unroll
     for (int k = 0; k < 8; k++) x[k] = s[lds_ix(base + k * st)];
     {
-        uint32_t w = M.tw[m0 + i], wp = M.twp[m0 + i];
+        const uint2 w = T.tw(m0 + i);
 #pragma unroll
-        for (int k = 0; k < 4; k++) ct_bf(x[k], x[k + 4], w, wp, M.q, M.q2);
+        for (int k = 0; k < 4; k++) ct_bf(x[k], x[k + 4], w.x, w.y, M.q, M.q2);
     }
 #pragma unroll
     for (int h = 0; h < 2; h++) {
-        uint32_t w = M.tw[2 * m0 + 2 * i + h], wp = M.twp[2 * m0 + 2 * i + h];
-        ct_bf(x[4 * h + 0], x[4 * h + 2], w, wp, M.q, M.q2);
-        ct_bf(x[4 * h + 1], x[4 * h + 3], w, wp, M.q, M.q2);
+        const uint2 w = T.tw(2 * m0 + 2 * i + h);
+        ct_bf(x[4 * h + 0], x[4 * h + 2], w.x, w.y, M.q, M.q2);
+        ct_bf(x[4 * h + 1], x[4 * h + 3], w.x, w.y, M.q, M.q2);
     }
 #pragma unroll
     for (int h = 0; h < 4; h++) {
-        uint32_t w = M.tw[4 * m0 + 4 * i + h], wp = M.twp[4 * m0 + 4 * i + h];
-        ct_bf(x[2 * h], x[2 * h + 1], w, wp, M.q, M.q2);
+        const uint2 w = T.tw(4 * m0 + 4 * i + h);
+        ct_bf(x[2 * h], x[2 * h + 1], w.x, w.y, M.q, M.q2);
     }
 #pragma unroll
     for (int k = 0; k < 8; k++) s[lds_ix(base + k * st)] = x[k];
 }
 
 // Forward leftover pass: 2 stages (radix-4, t = 2, 1) or 1 stage (t = 1).
-template <int LOGN, int REM>
-__device__ __forceinline__ void fwd_tail(uint32_t* s, int tid, const ModDev& M) {
+template <int LOGN, int REM, typename TW>
+__device__ __forceinline__ void fwd_tail(uint32_t* s, int tid, const ModDev& M, const TW& T) {
     constexpr int N = 1 << LOGN;
     if constexpr (REM == 2) {
         constexpr int m0 = N / 4;
@@ -101,13 +117,13 @@ __device__ __forceinline__ void fwd_tail(uint32_t* s, int tid, const ModDev& M) 
             const int i = tid + g * (N / 8);
             uint4 v = *reinterpret_cast<uint4*>(&s[lds_ix(4 * i)]);
             uint32_t x0 = v.x, x1 = v.y, x2 = v.z, x3 = v.w;
-            uint32_t w = M.tw[m0 + i], wp = M.twp[m0 + i];
-            ct_bf(x0, x2, w, wp, M.q, M.q2);
-            ct_bf(x1, x3, w, wp, M.q, M.q2);
-            w = M.tw[2 * m0 + 2 * i]; wp = M.twp[2 * m0 + 2 * i];
-            ct_bf(x0, x1, w, wp, M.q, M.q2);
-            w = M.tw[2 * m0 + 2 * i + 1]; wp = M.twp[2 * m0 + 2 * i + 1];
-            ct_bf(x2, x3, w, wp, M.q, M.q2);
+            uint2 w = T.tw(m0 + i);
+            ct_bf(x0, x2, w.x, w.y, M.q, M.q2);
+            ct_bf(x1, x3, w.x, w.y, M.q, M.q2);
+            w = T.tw(2 * m0 + 2 * i);
+            ct_bf(x0, x1, w.x, w.y, M.q, M.q2);
+            w = T.tw(2 * m0 + 2 * i + 1);
+            ct_bf(x2, x3, w.x, w.y, M.q, M.q2);
             *reinterpret_cast<uint4*>(&s[lds_ix(4 * i)]) = make_uint4(x0, x1, x2, x3);
         }
     } else if constexpr (REM == 1) {
@@ -117,10 +133,10 @@ __device__ __forceinline__ void fwd_tail(uint32_t* s, int tid, const ModDev& M) 
             const int i2 = tid + g * (N / 8);  // pair of butterflies 2*i2, 2*i2+1
             uint4 v = *reinterpret_cast<uint4*>(&s[lds_ix(4 * i2)]);
             uint32_t x0 = v.x, x1 = v.y, x2 = v.z, x3 = v.w;
-            uint32_t w = M.tw[m0 + 2 * i2], wp = M.twp[m0 + 2 * i2];
-            ct_bf(x0, x1, w, wp, M.q, M.q2);
-            w = M.tw[m0 + 2 * i2 + 1]; wp = M.twp[m0 + 2 * i2 + 1];
-            ct_bf(x2, x3, w, wp, M.q, M.q2);
+            uint2 w = T.tw(m0 + 2 * i2);
+            ct_bf(x0, x1, w.x, w.y, M.q, M.q2);
+            w = T.tw(m0 + 2 * i2 + 1);
+            ct_bf(x2, x3, w.x, w.y, M.q, M.q2);
             *reinterpret_cast<uint4*>(&s[lds_ix(4 * i2)]) = make_uint4(x0, x1, x2, x3);
         }
     }
@@ -129,25 +145,30 @@ __device__ __forceinline__ void fwd_tail(uint32_t* s, int tid, const ModDev& M) 
 // In-place forward NTT of the polynomial in LDS `s` by T = N/8 threads
 // (tid in [0, T)).  Input values < 4q; output values < 4q (callers reduce).
 // `sync` must be a barrier over every thread that shares `s`.
-template <int LOGN, typename Sync>
-__device__ __forceinline__ void ntt_fwd_lds(uint32_t* s, int tid, const ModDev& M, Sync sync) {
+template <int LOGN, typename Sync, typename TW>
+__device__ __forceinline__ void ntt_fwd_lds(uint32_t* s, int tid, const ModDev& M, Sync sync, const TW& T) {
     using S = NttShape<LOGN>;
     int m0 = 1;
 #pragma unroll
     for (int p = 0; p < S::R8; p++) {
-        fwd_pass8<LOGN>(s, tid, m0, M);
+        fwd_pass8<LOGN>(s, tid, m0, M, T);
         sync();
         m0 <<= 3;
     }
     if constexpr (S::REM) {
-        fwd_tail<LOGN, S::REM>(s, tid, M);
+        fwd_tail<LOGN, S::REM>(s, tid, M, T);
         sync();
     }
 }
 
+template <int LOGN, typename Sync>
+__device__ __forceinline__ void ntt_fwd_lds(uint32_t* s, int tid, const ModDev& M, Sync sync) {
+    ntt_fwd_lds<LOGN>(s, tid, M, sync, GlobalTw{M});
+}
+
 // One inverse radix-8 pass over stages t0, 2t0, 4t0.
-template <int LOGN>
-__device__ __forceinline__ void inv_pass8(uint32_t* s, int tid, int t0, const ModDev& M,
+template <int LOGN, typename TW>
+__device__ __forceinline__ void inv_pass8(uint32_t* s, int tid, int t0, const ModDev& M, const TW& T,
                                           bool last, uint32_t c, uint32_t cp, uint32_t cw, uint32_t cwp) {
     constexpr int N = 1 << LOGN;
     const int i3 = tid / t0, r = tid - i3 * t0;
@@ -158,19 +179,19 @@ __device__ __forceinline__ void inv_pass8(uint32_t* s, int tid, int t0, const Mo
     const int mA = N / (2 * t0), mB = N / (4 * t0), mC = N / (8 * t0);
 #pragma unroll
     for (int h = 0; h < 4; h++) {
-        uint32_t w = M.itw[mA + 4 * i3 + h], wp = M.itwp[mA + 4 * i3 + h];
-        gs_bf(x[2 * h], x[2 * h + 1], w, wp, M.q, M.q2);
+        const uint2 w = T.itw(mA + 4 * i3 + h);
+        gs_bf(x[2 * h], x[2 * h + 1], w.x, w.y, M.q, M.q2);
     }
 #pragma unroll
     for (int h = 0; h < 2; h++) {
-        uint32_t w = M.itw[mB + 2 * i3 + h], wp = M.itwp[mB + 2 * i3 + h];
-        gs_bf(x[4 * h + 0], x[4 * h + 2], w, wp, M.q, M.q2);
-        gs_bf(x[4 * h + 1], x[4 * h + 3], w, wp, M.q, M.q2);
+        const uint2 w = T.itw(mB + 2 * i3 + h);
+        gs_bf(x[4 * h + 0], x[4 * h + 2], w.x, w.y, M.q, M.q2);
+        gs_bf(x[4 * h + 1], x[4 * h + 3], w.x, w.y, M.q, M.q2);
     }
     if (!last) {
-        uint32_t w = M.itw[mC + i3], wp = M.itwp[mC + i3];
+        const uint2 w = T.itw(mC + i3);
 #pragma unroll
-        for (int k = 0; k < 4; k++) gs_bf(x[k], x[k + 4], w, wp, M.q, M.q2);
+        for (int k = 0; k < 4; k++) gs_bf(x[k], x[k + 4], w.x, w.y, M.q, M.q2);
     } else {
         // final stage (m = 1) with the output scale folded in:
         // X' = c (X + Y), Y' = (c psi^-br(1)) (X - Y)
@@ -185,8 +206,8 @@ __device__ __forceinline__ void inv_pass8(uint32_t* s, int tid, int t0, const Mo
     for (int k = 0; k < 8; k++) s[lds_ix(base + k * t0)] = x[k];
 }
 
-template <int LOGN, int REM>
-__device__ __forceinline__ void inv_head(uint32_t* s, int tid, const ModDev& M) {
+template <int LOGN, int REM, typename TW>
+__device__ __forceinline__ void inv_head(uint32_t* s, int tid, const ModDev& M, const TW& T) {
     constexpr int N = 1 << LOGN;
     if constexpr (REM == 2) {  // stages t = 1 (m = N/2) then t = 2 (m = N/4)
 #pragma unroll
@@ -194,13 +215,13 @@ __device__ __forceinline__ void inv_head(uint32_t* s, int tid, const ModDev& M) 
             const int i = tid + g * (N / 8);
             uint4 v = *reinterpret_cast<uint4*>(&s[lds_ix(4 * i)]);
             uint32_t x0 = v.x, x1 = v.y, x2 = v.z, x3 = v.w;
-            uint32_t w = M.itw[N / 2 + 2 * i], wp = M.itwp[N / 2 + 2 * i];
-            gs_bf(x0, x1, w, wp, M.q, M.q2);
-            w = M.itw[N / 2 + 2 * i + 1]; wp = M.itwp[N / 2 + 2 * i + 1];
-            gs_bf(x2, x3, w, wp, M.q, M.q2);
-            w = M.itw[N / 4 + i]; wp = M.itwp[N / 4 + i];
-            gs_bf(x0, x2, w, wp, M.q, M.q2);
-            gs_bf(x1, x3, w, wp, M.q, M.q2);
+            uint2 w = T.itw(N / 2 + 2 * i);
+            gs_bf(x0, x1, w.x, w.y, M.q, M.q2);
+            w = T.itw(N / 2 + 2 * i + 1);
+            gs_bf(x2, x3, w.x, w.y, M.q, M.q2);
+            w = T.itw(N / 4 + i);
+            gs_bf(x0, x2, w.x, w.y, M.q, M.q2);
+            gs_bf(x1, x3, w.x, w.y, M.q, M.q2);
             *reinterpret_cast<uint4*>(&s[lds_ix(4 * i)]) = make_uint4(x0, x1, x2, x3);
         }
     } else if constexpr (REM == 1) {  // stage t = 1
@@ -209,10 +230,10 @@ __device__ __forceinline__ void inv_head(uint32_t* s, int tid, const ModDev& M) 
             const int i2 = tid + g * (N / 8);
             uint4 v = *reinterpret_cast<uint4*>(&s[lds_ix(4 * i2)]);
             uint32_t x0 = v.x, x1 = v.y, x2 = v.z, x3 = v.w;
-            uint32_t w = M.itw[N / 2 + 2 * i2], wp = M.itwp[N / 2 + 2 * i2];
-            gs_bf(x0, x1, w, wp, M.q, M.q2);
-            w = M.itw[N / 2 + 2 * i2 + 1]; wp = M.itwp[N / 2 + 2 * i2 + 1];
-            gs_bf(x2, x3, w, wp, M.q, M.q2);
+            uint2 w = T.itw(N / 2 + 2 * i2);
+            gs_bf(x0, x1, w.x, w.y, M.q, M.q2);
+            w = T.itw(N / 2 + 2 * i2 + 1);
+            gs_bf(x2, x3, w.x, w.y, M.q, M.q2);
             *reinterpret_cast<uint4*>(&s[lds_ix(4 * i2)]) = make_uint4(x0, x1, x2, x3);
         }
     }
@@ -222,22 +243,28 @@ __device__ __forceinline__ void inv_head(uint32_t* s, int tid, const ModDev& M) 
 // result is multiplied by the constant c: pass c = n^-1 (with c*psi^-br(1) as
 // cw) for the plain inverse, or n^-1 times any extra factor to fold a scaling
 // into the last stage.  Output values < 2q.
-template <int LOGN, typename Sync>
-__device__ __forceinline__ void ntt_inv_lds(uint32_t* s, int tid, const ModDev& M, Sync sync,
+template <int LOGN, typename Sync, typename TW>
+__device__ __forceinline__ void ntt_inv_lds(uint32_t* s, int tid, const ModDev& M, Sync sync, const TW& T,
                                             uint32_t c, uint32_t cp, uint32_t cw, uint32_t cwp) {
     using S = NttShape<LOGN>;
     static_assert(S::R8 >= 1, "N >= 8");
     if constexpr (S::REM) {
-        inv_head<LOGN, S::REM>(s, tid, M);
+        inv_head<LOGN, S::REM>(s, tid, M, T);
         sync();
     }
     int t0 = 1 << S::REM;
 #pragma unroll
     for (int p = 0; p < S::R8; p++) {
-        inv_pass8<LOGN>(s, tid, t0, M, p == S::R8 - 1, c, cp, cw, cwp);
+        inv_pass8<LOGN>(s, tid, t0, M, T, p == S::R8 - 1, c, cp, cw, cwp);
         sync();
         t0 <<= 3;
     }
+}
+
+template <int LOGN, typename Sync>
+__device__ __forceinline__ void ntt_inv_lds(uint32_t* s, int tid, const ModDev& M, Sync sync,
+                                            uint32_t c, uint32_t cp, uint32_t cw, uint32_t cwp) {
+    ntt_inv_lds<LOGN>(s, tid, M, sync, GlobalTw{M}, c, cp, cw, cwp);
 }
 
 }  // namespace inspire

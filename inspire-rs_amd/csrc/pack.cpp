@@ -202,6 +202,46 @@ PackTables& pack_tables(inspire_ctx* ctx, uint32_t gamma) {
                     dst[n + s] = (uint32_t)mulmod64(v1, (1ull << 32) % q1, q1);
                 }
             }
+    // fused front end: gamma = 2^LG with gamma^2 | 2n (then g^i = 1 + (2n/gamma) i mod 2n)
+    {
+        uint32_t lg = 0;
+        while ((1u << lg) < gamma) lg++;
+        const bool pow2 = (1u << lg) == gamma && gamma <= 64;
+        T->fast_front = pow2 && ((2ull * n) % ((uint64_t)gamma * gamma) == 0) && n >= 256;
+        if (T->fast_front) {
+            const uint32_t G = gamma, H = std::max(G / 2, 1u);
+            std::vector<uint32_t> twist((size_t)K * G * n);
+            std::vector<uint32_t> dftw((size_t)K * H * H * 2, 0);
+            for (uint32_t k = 0; k < K; k++) {
+                const uint64_t q = ctx->hm[k].q, psi = ctx->hm[k].psi;
+                const uint64_t scale = mulmod64(ginv % q, (1ull << 32) % q, q);  // gamma^-1, Montgomery form
+                for (uint32_t s = 0; s < n; s++) {
+                    const uint64_t w = powmod64(psi, 2ull * bitrev(s, logn) + 1, q);
+                    uint64_t wj = scale;
+                    for (uint32_t j = 0; j < G; j++) {
+                        twist[((size_t)k * G + j) * n + s] = (uint32_t)wj;
+                        wj = mulmod64(wj, w, q);
+                    }
+                }
+                // class c = 2*ci + 1: W_c = psi^(-c * 2n/G) (exponent mod 2n), powers W_c^m for m < G/2
+                for (uint32_t ci = 0; ci < H; ci++) {
+                    const uint64_t e = (2ull * n - ((2ull * ci + 1) * (2ull * n / G)) % (2ull * n)) % (2ull * n);
+                    const uint64_t Wc = powmod64(psi, e, q);
+                    uint64_t wm = 1;
+                    for (uint32_t m = 0; m < H; m++) {
+                        const size_t o = (((size_t)k * H + ci) * H + m) * 2;
+                        dftw[o] = (uint32_t)wm;
+                        dftw[o + 1] = shoup_pre((uint32_t)wm, (uint32_t)q);
+                        wm = mulmod64(wm, Wc, q);
+                    }
+                }
+            }
+            T->twist.alloc(twist.size() * 4);
+            T->dftw.alloc(dftw.size() * 4);
+            hip_check(hipMemcpy(T->twist.get(), twist.data(), twist.size() * 4, hipMemcpyHostToDevice), "twist");
+            hip_check(hipMemcpy(T->dftw.get(), dftw.data(), dftw.size() * 4, hipMemcpyHostToDevice), "dftw");
+        }
+    }
     T->perm.alloc(perm.size() * 4);
     T->step1.alloc(step1.size() * 4);
     T->w_all.alloc(w_all.size() * 4);
@@ -224,31 +264,37 @@ void pack_run(inspire_ctx* ctx, const Database& db, uint32_t Q, const uint32_t* 
     const size_t KN = (size_t)K * ctx->N;
     PackTables& T = pack_tables(ctx, gamma);
     const bool rw = (ctx->params.flags & INSPIRE_FLAG_RESIDUE_WISE_AUTOMORPHISM) != 0;
-    const size_t cols_words = (size_t)Q * 2 * gamma * KN, rpre_words = (size_t)Q * gamma * KN;
-    const size_t bp_words = (size_t)Q * K * gamma;
-    ctx->ws2.reserve(cols_words * 4);
-    ctx->ws3.reserve((rpre_words + bp_words) * 4);
-    uint32_t* d_cols = ctx->ws2.get<uint32_t>();
+    const size_t rpre_words = (size_t)Q * gamma * KN;
+    const uint32_t nparts = T.fast_front ? ctx->N / 256 : 1;
+    const size_t bpart_words = (size_t)Q * nparts * gamma * K * 2;  // u64
+    ctx->ws3.reserve((rpre_words + bpart_words) * 4);
     uint32_t* d_rpre = ctx->ws3.get<uint32_t>();
-    uint32_t* d_bp = d_rpre + rpre_words;
-    {
-        ProfScope ps(ctx, P_FIRST_DIM);
-        hip_check(launch_first_dim(db.digits.get<uint32_t>(), rows, d_slot, d_cols, Q, gamma, 2 * gamma, 0, ctx->dev,
-                                   ctx->stream),
-                  "first dim");
-    }
-    {
-        ProfScope ps(ctx, P_B0);
-        hip_check(launch_pack_b0(d_cols, d_bp, Q, gamma, ctx->dev, ctx->stream), "b0");
-    }
-    {
+    uint64_t* d_bpart = reinterpret_cast<uint64_t*>(d_rpre + rpre_words + (rpre_words & 1));
+    if (T.fast_front) {
+        ProfScope ps(ctx, P_FRONT);
+        hip_check(launch_pack_front(db.digits.get<uint32_t>(), rows, d_slot, d_rpre, d_bpart, T.twist.get<uint32_t>(),
+                                    T.dftw.get<uint2>(), Q, gamma, rw || K == 1, ctx->dev, ctx->stream),
+                  "pack front");
+    } else {
+        const size_t cols_words = (size_t)Q * 2 * gamma * KN;
+        ctx->ws2.reserve(cols_words * 4);
+        uint32_t* d_cols = ctx->ws2.get<uint32_t>();
+        {
+            ProfScope ps(ctx, P_FIRST_DIM);
+            hip_check(launch_first_dim(db.digits.get<uint32_t>(), rows, d_slot, d_cols, Q, gamma, 2 * gamma, 0,
+                                       ctx->dev, ctx->stream),
+                      "first dim");
+        }
+        {
+            ProfScope ps(ctx, P_B0);
+            hip_check(launch_pack_b0(d_cols, d_bpart, Q, gamma, ctx->dev, ctx->stream), "b0");
+        }
         ProfScope ps(ctx, P_STEP1);
         const uint32_t kres = (rw || K == 1) ? K : 1;  // residue 1 is rebuilt from residue 0 otherwise
         hip_check(launch_pack_step1(d_cols, T.step1.get<uint32_t>(), d_rpre, Q, gamma, kres, ctx->dev, ctx->stream),
                   "step1");
     }
     PackArgs A{};
-    A.a_hat_cols = d_cols;
     A.r_pre = d_rpre;
     A.perm = T.perm.get<uint32_t>();
     A.w_all = T.w_all.get<uint32_t>();
@@ -256,10 +302,15 @@ void pack_run(inspire_ctx* ctx, const Database& db, uint32_t Q, const uint32_t* 
     A.y_off = d_yoff;
     A.y_mode = d_ymode;
     A.y_count = d_ycount;
-    A.b_poly = d_bp;
+    A.bpart = d_bpart;
+    A.nparts = nparts;
     A.out = d_out;
     A.gamma = gamma;
     A.residue_wise = rw ? 1u : 0u;
+    for (uint32_t k = 0; k < K; k++) {
+        const uint64_t q = ctx->hm[k].q;
+        (k ? A.ninv1 : A.ninv0) = (uint32_t)powmod64(ctx->N, q - 2, q);
+    }
     const uint64_t q0 = ctx->hm[0].q;
     A.q0r64 = (uint32_t)pow2_64_mod(q0);
     A.q0r64p = shoup_pre(A.q0r64, (uint32_t)q0);
@@ -293,9 +344,6 @@ void pack_run_device_ybody(inspire_ctx* ctx, const Database& db, uint32_t Q, con
     for (uint32_t i = 0; i + 1 < gamma; i++)
         if (!(T.gen_pows[i] & 1))
             throw Error{INSPIRE_ERR_UNSUPPORTED, "device batches need an odd InspiRING generator (gamma | 2d)"};
-    std::vector<uint64_t> y_off(Q);
-    std::vector<uint32_t> y_mode(Q, 0), y_count(Q, gamma - 1);
-    for (uint32_t t = 0; t < Q; t++) y_off[t] = (uint64_t)t * ell * KN;
     ctx->ws0.reserve((size_t)Q * ell * KN * 4);
     uint32_t* d_y = ctx->ws0.get<uint32_t>();
     uint64_t mont[kMaxK];
@@ -306,10 +354,8 @@ void pack_run_device_ybody(inspire_ctx* ctx, const Database& db, uint32_t Q, con
                                      ctx->stream),
                   "y ntt");
     }
-    uint64_t* d_yoff;
-    uint32_t *d_ymode, *d_ycount;
-    upload_meta(ctx, Q, y_off, y_mode, y_count, &d_yoff, &d_ymode, &d_ycount);
-    pack_run(ctx, db, Q, rows, d_slot, d_y, d_yoff, d_ymode, d_ycount, d_out);
+    // y layout: query t at t*ell*K*N, rotated from y_body, gamma-1 rotations (the kernel's default)
+    pack_run(ctx, db, Q, rows, d_slot, d_y, nullptr, nullptr, nullptr, d_out);
 }
 
 void pack_respond_host(inspire_ctx* ctx, const Database& db, const BatchItem* items, uint32_t Q, const uint32_t* rows,

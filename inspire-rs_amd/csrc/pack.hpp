@@ -17,7 +17,6 @@ struct BatchItem {
 
 // Kernel-side description of a packing batch.
 struct PackArgs {
-    const uint32_t* a_hat_cols;  // [Q][2*gamma][K][N]: column c a at 2c, b at 2c+1 (NTT domain)
     const uint32_t* r_pre;       // [Q][gamma][K][N] step-1 sums before the automorphism
     const uint32_t* perm;        // [gamma][N] NTT-domain automorphism tables of g^i
     const uint32_t* w_all;       // [gamma-1][ell][K][N] Montgomery form
@@ -25,7 +24,9 @@ struct PackArgs {
     const uint64_t* y_off;       // [Q] word offset of each query's y block
     const uint32_t* y_mode;      // [Q] 0: rotate y_body by perm, 1: explicit y_all
     const uint32_t* y_count;     // [Q] rotations available (y_all.len(), or gamma-1 for y_body)
-    const uint32_t* b_poly;      // [Q][K][gamma] b coefficients 0..gamma-1
+    const uint64_t* bpart;       // [Q][nparts][gamma][K] partial sums of bhat (b_poly = n^-1 * sum)
+    uint32_t nparts;
+    uint32_t ninv0, ninv1;       // n^-1 mod q_k
     uint32_t* out;               // [Q][2][K][N] packed ciphertext (a_hat, b)
     uint32_t gamma;
     uint32_t residue_wise;       // 1: corrected automorphism on every residue
@@ -51,8 +52,12 @@ void pack_respond_host(inspire_ctx* ctx, const Database& db, const BatchItem* it
                        const int32_t* d_slot, const uint32_t* d_err, const inspire_query* qs, inspire_response* rs,
                        int32_t* status);
 
-hipError_t launch_pack_b0(const uint32_t* cols, uint32_t* b_poly, uint32_t Q, uint32_t gamma, const CtxDev& C,
+hipError_t launch_pack_b0(const uint32_t* cols, uint64_t* bpart, uint32_t Q, uint32_t gamma, const CtxDev& C,
                           hipStream_t st);
+// Fused first dimension + step 1 + b sums for gamma in {1, 2, ..., 64} with gamma^2 | 2n.
+hipError_t launch_pack_front(const uint32_t* digits, const uint32_t* rows, const int32_t* slot, uint32_t* r_pre,
+                             uint64_t* bpart, const uint32_t* twist, const uint2* dftw, uint32_t Q, uint32_t gamma,
+                             bool rw, const CtxDev& C, hipStream_t st);
 hipError_t launch_pack_step1(const uint32_t* cols, const uint32_t* step1, uint32_t* r_pre, uint32_t Q,
                              uint32_t gamma, uint32_t kres, const CtxDev& C, hipStream_t st);
 // inv: inverse-NTT scaling (plain inverse: n^-1 folded into the last stage).

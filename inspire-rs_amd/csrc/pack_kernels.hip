@@ -29,10 +29,11 @@ struct PackSync {
 };
 
 // b_poly (respond.rs:350-356): coefficient 0 of column c's b is
-// n^-1 * sum_s bhat_c[s].  One block per (query, column, residue).
+// n^-1 * sum_s bhat_c[s].  One block per (query, column, residue) writes the
+// raw sum (one part); the chain kernel reduces it mod q_k and scales by n^-1.
+// bpart: [Q][1][gamma][K] u64.
 __global__ void __launch_bounds__(256)
-k_pack_b0(const uint32_t* __restrict__ cols, uint32_t* __restrict__ b_poly, uint32_t gamma, CtxDev C,
-          uint32_t ninv0, uint32_t ninv1) {
+k_pack_b0(const uint32_t* __restrict__ cols, uint64_t* __restrict__ bpart, uint32_t gamma, CtxDev C) {
     __shared__ uint64_t sh[4];
     const uint32_t blk = blockIdx.x;
     const uint32_t k = blk % C.K;
@@ -45,12 +46,111 @@ k_pack_b0(const uint32_t* __restrict__ cols, uint32_t* __restrict__ b_poly, uint
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        const uint64_t q = C.m[k].q;
-        uint64_t t = (sh[0] + sh[1] + sh[2] + sh[3]) % q;
-        t = (t * (k ? ninv1 : ninv0)) % q;
-        b_poly[((size_t)qi * C.K + k) * gamma + c] = (uint32_t)t;
+    if (threadIdx.x == 0) bpart[((size_t)qi * gamma + c) * C.K + k] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+// ---------------------------------------------------------------------------
+// Fused front end for gamma = 2^LG columns with gamma^2 | 2n: first dimension,
+// step 1 and the b_poly partial sums, one thread per (query, NTT slot s).
+// With g = 2n/gamma + 1, g^i = 1 + (2n/gamma) i mod 2n, so the step-1
+// monomial exponent j g^(n-i) = j - (2n/gamma) i j and, with w_s the slot's
+// evaluation point (w_s^(2n) = 1) and zeta_s = w_s^(2n/gamma),
+//   r_pre[i][s] = sum_j (gamma^-1 w_s^j ahat_j[s]) zeta_s^(-ij),
+// a gamma-point DFT of the twisted column values, done here in registers.
+// zeta_s^-1 = psi^(-(2br(s)+1) 2n/gamma) depends only on c = (2br(s)+1) mod
+// gamma.  In the reference's residue handling only residue 0 of r_pre is
+// read later, so residue 1 (a-part) is computed in the corrected mode only.
+// Column values never leave registers: ahat feeds the DFT, bhat feeds the
+// per-block b sums (bpart [Q][n/256][gamma][K] u64).
+// ---------------------------------------------------------------------------
+template <int LG, bool RW>
+__global__ void __launch_bounds__(256)
+k_pack_front(const uint32_t* __restrict__ digits, const uint32_t* __restrict__ rows, const int32_t* __restrict__ slot,
+             uint32_t* __restrict__ r_pre, uint64_t* __restrict__ bpart, const uint32_t* __restrict__ twist,
+             const uint2* __restrict__ dftw, CtxDev C) {
+    constexpr int G = 1 << LG;
+    __shared__ uint32_t red[2 * G][257];
+    const int n = (int)C.n, nb = n / 256, tid = threadIdx.x;
+    const uint32_t qi = blockIdx.x / nb, blk = blockIdx.x % nb;
+    const int s = blk * 256 + tid;
+    const int sl = slot[qi] < 0 ? 0 : slot[qi];  // a missing shard is reported by the host; read shard 0
+    const size_t KN = (size_t)C.K * n;
+    const uint32_t* R = rows + (size_t)qi * C.ell * 2 * KN + s;
+    const uint32_t* Dq = digits + (size_t)sl * G * C.ell * KN + s;
+    const uint32_t cls = ((2u * (__brev((uint32_t)s) >> (32 - C.logn)) + 1u) & (G - 1)) >> 1;
+    for (int k = 0; k < (int)C.K; k++) {
+        const uint32_t q = C.m[k].q, q2 = 2 * q, qinv = C.m[k].qinv;
+        const bool want_a = k == 0 || RW;
+        uint32_t ca[kMaxEll], cb[kMaxEll];
+        for (uint32_t i = 0; i < C.ell; i++) {
+            ca[i] = want_a ? R[((size_t)(2 * i) * C.K + k) * n] : 0u;
+            cb[i] = R[((size_t)(2 * i + 1) * C.K + k) * n];
+        }
+        uint32_t v[G];
+#pragma unroll
+        for (int j = 0; j < G; j++) {
+            const uint32_t* D = Dq + ((size_t)j * C.ell * C.K + k) * n;
+            uint64_t aa = 0, bb = 0;
+            for (uint32_t i = 0; i < C.ell; i++) {
+                const uint64_t d = D[(size_t)i * KN];
+                aa += d * ca[i];
+                bb += d * cb[i];
+            }
+            red[k * G + j][tid] = csub(mont_reduce(bb, q, qinv), q);
+            if (want_a) {
+                const uint32_t a = csub(mont_reduce(aa, q, qinv), q);
+                // twist by gamma^-1 w_s^j (Montgomery form) and place at the bit-reversed index
+                const uint32_t tw = twist[((size_t)k * G + j) * n + s];
+                int p = 0;
+#pragma unroll
+                for (int b = 0; b < LG; b++) p |= ((j >> b) & 1) << (LG - 1 - b);
+                v[p] = csub(mont_reduce((uint64_t)a * tw, q, qinv), q);
+            }
+        }
+        if (want_a) {
+            // radix-2 DIT over the bit-reversed input: X_i = sum_j v_j W^(ij), W = zeta_s^-1
+            const uint2* W = dftw + ((size_t)k * (G / 2 > 0 ? G / 2 : 1) + cls) * (G / 2 > 0 ? G / 2 : 1);
+#pragma unroll
+            for (int len = 2; len <= G; len <<= 1) {
+                const int half = len / 2, step = G / len;
+#pragma unroll
+                for (int b = 0; b < G; b += len)
+#pragma unroll
+                    for (int kk = 0; kk < half; kk++) {
+                        uint32_t& X = v[b + kk];
+                        uint32_t& Y = v[b + kk + half];
+                        if (kk == 0) {
+                            const uint32_t x = csub(X, q2), y = csub(Y, q2);
+                            X = x + y;
+                            Y = x - y + q2;
+                        } else {
+                            const uint2 w = W[kk * step];
+                            ct_bf(X, Y, w.x, w.y, q, q2);
+                        }
+                    }
+            }
+            uint32_t* out = r_pre + (size_t)qi * G * KN + (size_t)k * n + s;
+#pragma unroll
+            for (int i = 0; i < G; i++) out[(size_t)i * KN] = csub(csub(v[i], q2), q);
+        }
     }
+    __syncthreads();
+    // per-block sums of bhat_j over this block's 256 slots, rows (k, j)
+    constexpr int ROWS_MAX = 2 * G;
+    const int rows_n = (int)C.K * G;
+    const int tpr = 256 / rows_n < 64 ? 256 / rows_n : 64;  // threads per row, within one wavefront
+    const int row = tid / tpr, part = tid % tpr;
+    uint64_t acc = 0;
+    if (row < rows_n) {
+        const int per = 256 / tpr;
+        for (int t = part * per; t < (part + 1) * per; t++) acc += red[row][t];
+    }
+    for (int o = 1; o < tpr; o <<= 1) acc += __shfl_down(acc, o, 64);
+    if (row < rows_n && part == 0) {
+        const int k = row / G, j = row % G;
+        bpart[(((size_t)qi * nb + blk) * G + j) * C.K + k] = acc;
+    }
+    (void)ROWS_MAX;
 }
 
 // Step 1 before the automorphism:
@@ -82,22 +182,55 @@ k_pack_step1(const uint32_t* __restrict__ cols, const uint32_t* __restrict__ ste
     }
 }
 
+// LDS budget of the chain kernel: twiddle pairs (when they fit), the ell*K
+// work polys, the online accumulator and b_poly.
 template <int LOGN, int KK>
-__global__ void __launch_bounds__(KK * NttShape<LOGN>::T)
+struct ChainShape {
+    static constexpr int N = 1 << LOGN, LP = NttShape<LOGN>::LDS;
+    static constexpr bool kLdsTw = (size_t)KK * N * 16 + (size_t)kMaxEll * KK * LP * 4 + (size_t)KK * N * 4 <= 150 * 1024;
+    static size_t bytes(uint32_t ell, uint32_t gamma) {
+        return (kLdsTw ? (size_t)KK * N * 16 : 0) + (size_t)ell * KK * LP * 4 + (size_t)KK * N * 4 + (size_t)KK * gamma * 4 + 16;
+    }
+};
+
+template <int LOGN, int KK>
+__global__ void __launch_bounds__(KK * NttShape<LOGN>::T, 1)
 k_pack_chain(PackArgs A, CtxDev C, ScaleDev inv) {
+    using CS = ChainShape<LOGN, KK>;
     constexpr int N = 1 << LOGN, T = NttShape<LOGN>::T, TT = KK * T, LP = NttShape<LOGN>::LDS;
     extern __shared__ __align__(16) uint32_t lds[];
     const int tid = threadIdx.x;
     const int half = tid / T, ht = tid - half * T;
     const uint32_t qi = blockIdx.x, gamma = A.gamma, ell = C.ell;
     const size_t KN = (size_t)KK * N;
+    uint2* twf = reinterpret_cast<uint2*>(lds);                    // [KK][N] when kLdsTw
+    uint2* twi = twf + (CS::kLdsTw ? KK * N : 0);
+    uint32_t* work = reinterpret_cast<uint32_t*>(twi + (CS::kLdsTw ? KK * N : 0));  // [ell][KK][LP]
+    uint32_t* sumb = work + (size_t)ell * KK * LP;                 // [KK][N] online accumulator
+    uint32_t* bp = sumb + KN;                                       // [KK][gamma] b_poly
+    auto buf = [&](uint32_t d, int k) { return work + ((size_t)d * KK + k) * LP; };
     const uint32_t* rpre = A.r_pre + (size_t)qi * gamma * KN;
-    const uint32_t* Y = A.y + A.y_off[qi];
-    const uint32_t ymode = A.y_mode[qi], ycount = A.y_count[qi];
-    auto buf = [&](uint32_t i, int k) { return lds + ((size_t)i * KK + k) * LP; };
+    const uint32_t* Y = A.y + (A.y_off ? A.y_off[qi] : (uint64_t)qi * ell * KN);
+    const uint32_t ymode = A.y_mode ? A.y_mode[qi] : 0u;
+    const uint32_t ycount = A.y_count ? A.y_count[qi] : gamma - 1;
     const ModDev& Mh = C.m[half];
     const uint32_t q0 = C.m[0].q;
 
+    if constexpr (CS::kLdsTw) {
+        for (int e = tid; e < KK * N; e += TT) {
+            const int k = e / N, i = e - k * N;
+            twf[e] = make_uint2(C.m[k].tw[i], C.m[k].twp[i]);
+            twi[e] = make_uint2(C.m[k].itw[i], C.m[k].itwp[i]);
+        }
+    }
+    // b_poly[k][c] = n^-1 * (sum of the column's bhat over all slots) mod q_k
+    for (uint32_t e = tid; e < KK * gamma; e += TT) {
+        const uint32_t k = e / gamma, c = e - k * gamma;
+        const uint64_t q = C.m[k].q;
+        uint64_t s = 0;
+        for (uint32_t pb = 0; pb < A.nparts; pb++) s += A.bpart[(((size_t)qi * A.nparts + pb) * gamma + c) * KK + k] % q;
+        bp[e] = (uint32_t)(((s % q) * (k ? A.ninv1 : A.ninv0)) % q);
+    }
     // R_init(i) at (k, s): the reference's apply_automorphism_ntt of r_pre[i].
     auto r_init = [&](uint32_t i, int k, int s) -> uint32_t {
         const uint32_t src = A.perm[(size_t)i * N + s];
@@ -109,25 +242,21 @@ k_pack_chain(PackArgs A, CtxDev C, ScaleDev inv) {
         const uint32_t v1 = csub(shoup_mul(ref, 1u, A.one_q1p, q1), q1);
         return csub(shoup_mul(v1, A.q1r64i, A.q1r64ip, q1), q1);
     };
-
-    uint32_t cur[8], sumb[8];
-#pragma unroll
     for (int m = 0; m < 8; m++) {
         const int e = tid + m * TT, k = e / N, s = e - k * N;
-        cur[m] = r_init(gamma - 1, k, s);
-        sumb[m] = 0;
+        buf(0, k)[lds_ix(s)] = r_init(gamma - 1, k, s);
+        sumb[e] = 0;
     }
+    __syncthreads();
+    const LdsTw Tl{twf + half * N, twi + half * N};
+    const GlobalTw Tg{Mh};
 
     for (int i = (int)gamma - 2; i >= 0; i--) {
-        // T[i] = G^-1(INTT(R[i+1]))
-#pragma unroll
-        for (int m = 0; m < 8; m++) {
-            const int e = tid + m * TT, k = e / N, s = e - k * N;
-            buf(0, k)[lds_ix(s)] = cur[m];
-        }
-        __syncthreads();
-        ntt_inv_lds<LOGN>(buf(0, half), ht, Mh, PackSync{}, inv.c[half], inv.cp[half], inv.cw[half],
-                          inv.cwp[half]);
+        // T[i] = G^-1(INTT(R[i+1])); R[i+1] sits in buf(0, .)
+        if constexpr (CS::kLdsTw)
+            ntt_inv_lds<LOGN>(buf(0, half), ht, Mh, PackSync{}, Tl, inv.c[half], inv.cp[half], inv.cw[half], inv.cwp[half]);
+        else
+            ntt_inv_lds<LOGN>(buf(0, half), ht, Mh, PackSync{}, Tg, inv.c[half], inv.cp[half], inv.cw[half], inv.cwp[half]);
         uint32_t dig[8 / KK][kMaxEll];
 #pragma unroll
         for (int m = 0; m < 8 / KK; m++) {
@@ -141,9 +270,10 @@ k_pack_chain(PackArgs A, CtxDev C, ScaleDev inv) {
                 const uint32_t t = csub(shoup_mul(c1 + q1 - a0m, C.q0_inv_q1, C.q0_inv_q1p, q1), q1);
                 x = (uint64_t)c0 + (uint64_t)q0 * t;
             }
-            for (uint32_t d = 0; d < ell; d++) {  // digit_d = (x / base^d) mod base
+#pragma unroll
+            for (int d = 0; d < kMaxEll; d++) {  // digit_d = (x / base^d) mod base
                 const uint32_t sh = d * C.gadget_log;
-                dig[m][d] = sh < 64 ? (uint32_t)((x >> sh) & C.gadget_mask) : 0u;
+                dig[m][d] = ((uint32_t)d < ell && sh < 64) ? (uint32_t)((x >> sh) & C.gadget_mask) : 0u;
             }
         }
         __syncthreads();
@@ -159,11 +289,14 @@ k_pack_chain(PackArgs A, CtxDev C, ScaleDev inv) {
                 }
         }
         __syncthreads();
-        for (uint32_t d = 0; d < ell; d++) ntt_fwd_lds<LOGN>(buf(d, half), ht, Mh, PackSync{});
-        // R[i] = R_init(i) + sum_d T^[d] * w_all[i][d];  online: += T^[d] * y^[i][d]
+        for (uint32_t d = 0; d < ell; d++) {
+            if constexpr (CS::kLdsTw) ntt_fwd_lds<LOGN>(buf(d, half), ht, Mh, PackSync{}, Tl);
+            else ntt_fwd_lds<LOGN>(buf(d, half), ht, Mh, PackSync{}, Tg);
+        }
+        // R[i] = R_init(i) + sum_d T^[d] * w_all[i][d]  -> buf(0, .);  online: += T^[d] * y^[i][d]
         const uint32_t* W = A.w_all + (size_t)i * ell * KN;
         const bool use_y = (uint32_t)i < ycount;
-#pragma unroll
+#pragma unroll 2
         for (int m = 0; m < 8; m++) {
             const int e = tid + m * TT, k = e / N, s = e - k * N;
             const uint32_t q = C.m[k].q, q2 = 2 * q, qinv = C.m[k].qinv;
@@ -178,54 +311,74 @@ k_pack_chain(PackArgs A, CtxDev C, ScaleDev inv) {
                     ab += (uint64_t)t * Y[yo];
                 }
             }
-            cur[m] = csub(r_init((uint32_t)i, k, s) + csub(mont_reduce(ar, q, qinv), q), q);
-            if (use_y) sumb[m] = csub(sumb[m] + csub(mont_reduce(ab, q, qinv), q), q);
+            buf(0, k)[lds_ix(s)] = csub(r_init((uint32_t)i, k, s) + csub(mont_reduce(ar, q, qinv), q), q);
+            if (use_y) sumb[e] = csub(sumb[e] + csub(mont_reduce(ab, q, qinv), q), q);
         }
         __syncthreads();
     }
 
     // a_hat = INTT(R[0]);  b = b_poly + INTT(sum)
     uint32_t* out = A.out + (size_t)qi * 2 * KN;
-    const uint32_t* bp = A.b_poly + (size_t)qi * KK * gamma;
-#pragma unroll
+    if constexpr (CS::kLdsTw)
+        ntt_inv_lds<LOGN>(buf(0, half), ht, Mh, PackSync{}, Tl, inv.c[half], inv.cp[half], inv.cw[half], inv.cwp[half]);
+    else
+        ntt_inv_lds<LOGN>(buf(0, half), ht, Mh, PackSync{}, Tg, inv.c[half], inv.cp[half], inv.cw[half], inv.cwp[half]);
     for (int m = 0; m < 8; m++) {
         const int e = tid + m * TT, k = e / N, s = e - k * N;
-        buf(0, k)[lds_ix(s)] = cur[m];
+        out[e] = csub(buf(0, k)[lds_ix(s)], C.m[k].q);
     }
     __syncthreads();
-    ntt_inv_lds<LOGN>(buf(0, half), ht, Mh, PackSync{}, inv.c[half], inv.cp[half], inv.cw[half], inv.cwp[half]);
-#pragma unroll
     for (int m = 0; m < 8; m++) {
         const int e = tid + m * TT, k = e / N, s = e - k * N;
-        out[(size_t)k * N + s] = csub(buf(0, k)[lds_ix(s)], C.m[k].q);
+        buf(0, k)[lds_ix(s)] = sumb[e];
     }
     __syncthreads();
-#pragma unroll
-    for (int m = 0; m < 8; m++) {
-        const int e = tid + m * TT, k = e / N, s = e - k * N;
-        buf(0, k)[lds_ix(s)] = sumb[m];
-    }
-    __syncthreads();
-    ntt_inv_lds<LOGN>(buf(0, half), ht, Mh, PackSync{}, inv.c[half], inv.cp[half], inv.cw[half], inv.cwp[half]);
-#pragma unroll
+    if constexpr (CS::kLdsTw)
+        ntt_inv_lds<LOGN>(buf(0, half), ht, Mh, PackSync{}, Tl, inv.c[half], inv.cp[half], inv.cw[half], inv.cwp[half]);
+    else
+        ntt_inv_lds<LOGN>(buf(0, half), ht, Mh, PackSync{}, Tg, inv.c[half], inv.cp[half], inv.cw[half], inv.cwp[half]);
     for (int m = 0; m < 8; m++) {
         const int e = tid + m * TT, k = e / N, s = e - k * N;
         const uint32_t q = C.m[k].q;
         uint32_t v = csub(buf(0, k)[lds_ix(s)], q);
         if ((uint32_t)s < gamma) v = csub(v + bp[(size_t)k * gamma + s], q);
-        out[KN + (size_t)k * N + s] = v;
+        out[KN + e] = v;
     }
 }
 
 static inline unsigned nblk(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
-hipError_t launch_pack_b0(const uint32_t* cols, uint32_t* b_poly, uint32_t Q, uint32_t gamma, const CtxDev& C,
+hipError_t launch_pack_b0(const uint32_t* cols, uint64_t* bpart, uint32_t Q, uint32_t gamma, const CtxDev& C,
                           hipStream_t st) {
     if (!Q) return hipSuccess;
-    uint32_t ninv[kMaxK] = {0, 0};
-    for (uint32_t k = 0; k < C.K; k++) ninv[k] = (uint32_t)powmod64(C.n, C.m[k].q - 2, C.m[k].q);
-    hipLaunchKernelGGL(k_pack_b0, dim3(Q * gamma * C.K), dim3(256), 0, st, cols, b_poly, gamma, C, ninv[0], ninv[1]);
+    hipLaunchKernelGGL(k_pack_b0, dim3(Q * gamma * C.K), dim3(256), 0, st, cols, bpart, gamma, C);
     return hipGetLastError();
+}
+
+template <int LG>
+static hipError_t front_lg(const uint32_t* digits, const uint32_t* rows, const int32_t* slot, uint32_t* r_pre,
+                           uint64_t* bpart, const uint32_t* twist, const uint2* dftw, uint32_t Q, bool rw,
+                           const CtxDev& C, hipStream_t st) {
+    const dim3 grid(Q * (C.n / 256));
+    if (rw) hipLaunchKernelGGL((k_pack_front<LG, true>), grid, dim3(256), 0, st, digits, rows, slot, r_pre, bpart, twist, dftw, C);
+    else hipLaunchKernelGGL((k_pack_front<LG, false>), grid, dim3(256), 0, st, digits, rows, slot, r_pre, bpart, twist, dftw, C);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_front(const uint32_t* digits, const uint32_t* rows, const int32_t* slot, uint32_t* r_pre,
+                             uint64_t* bpart, const uint32_t* twist, const uint2* dftw, uint32_t Q, uint32_t gamma,
+                             bool rw, const CtxDev& C, hipStream_t st) {
+    if (!Q) return hipSuccess;
+    switch (gamma) {
+        case 1: return front_lg<0>(digits, rows, slot, r_pre, bpart, twist, dftw, Q, rw, C, st);
+        case 2: return front_lg<1>(digits, rows, slot, r_pre, bpart, twist, dftw, Q, rw, C, st);
+        case 4: return front_lg<2>(digits, rows, slot, r_pre, bpart, twist, dftw, Q, rw, C, st);
+        case 8: return front_lg<3>(digits, rows, slot, r_pre, bpart, twist, dftw, Q, rw, C, st);
+        case 16: return front_lg<4>(digits, rows, slot, r_pre, bpart, twist, dftw, Q, rw, C, st);
+        case 32: return front_lg<5>(digits, rows, slot, r_pre, bpart, twist, dftw, Q, rw, C, st);
+        case 64: return front_lg<6>(digits, rows, slot, r_pre, bpart, twist, dftw, Q, rw, C, st);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_pack_step1(const uint32_t* cols, const uint32_t* step1, uint32_t* r_pre, uint32_t Q,
@@ -238,7 +391,7 @@ hipError_t launch_pack_step1(const uint32_t* cols, const uint32_t* step1, uint32
 
 template <int LOGN, int KK>
 static hipError_t chain_t(const PackArgs& A, uint32_t Q, const CtxDev& C, const ScaleDev& inv, hipStream_t st) {
-    const size_t shmem = (size_t)C.ell * KK * NttShape<LOGN>::LDS * 4;
+    const size_t shmem = ChainShape<LOGN, KK>::bytes(C.ell, A.gamma);
     auto kern = k_pack_chain<LOGN, KK>;
     if (shmem > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);

@@ -24,7 +24,8 @@
 namespace inspire {
 
 const char* const kProfNames[P_COUNT] = {"rows_ntt", "expand_seeds", "first_dim", "cols_intt", "fold",
-                                         "pack_b0",  "pack_step1",   "y_ntt",     "pack_chain", "db_load"};
+                                         "pack_b0",  "pack_step1",   "y_ntt",     "pack_chain", "db_load",
+                                         "pack_front"};
 
 ProfScope::ProfScope(inspire_ctx* ctx, int id) : ctx_(ctx), id_(id) {
     if (!ctx_->prof.on) return;
