@@ -44,8 +44,12 @@ MODMUL_QUERY = (
     + ELL * K * NTT              # NTT of y_body
     + MODMUL_CHAIN
 )
-# HBM bytes per query the first dimension reads: the shard's NTT-domain digits
-FIRST_DIM_BYTES = GAMMA * ELL * K * D * 4
+# Algorithmic HBM bytes per query of the fused front end (first dimension +
+# step 1 + b sums): the shard's NTT-domain digits, the RGSW rows it reads (b in
+# both residues, a in residue 0, which is all the reference's residue handling
+# reads), r_pre written for residue 0 and the per-block b sums.
+FRONT_BYTES = (GAMMA * ELL * K * D * 4 + ELL * (K + 1) * D * 4 + GAMMA * D * 4
+               + (D // 256) * GAMMA * K * 8)
 # Published MI355X figures used for the roofs: HBM3E 8 TB/s; vector FP32
 # 157.3 TFLOP/s = 78.6 T lane-FMA/s; 32-bit integer multiplies issue at half
 # that rate and a Shoup/Montgomery modular product needs 3 of them.
@@ -60,6 +64,25 @@ def synthetic_queries(rng: np.random.Generator, n: int, nshards: int):
     seeds = rng.integers(0, 2**32, (n, ELL, 8), dtype=np.uint64).astype(np.uint32)
     shards = rng.integers(0, nshards, n, dtype=np.uint64).astype(np.uint32)
     return shards, b, seeds, y
+
+
+def measured_traffic(kernel: str, batch: int):
+    """HBM bytes per launch of `kernel` at this batch size, from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs summarised under profiles/ (lines
+    `TRAFFIC <kernel> <batch> <fetch bytes> <write bytes>`, already scaled by
+    the copy-kernel calibration), or None when no such run is recorded."""
+    pdir = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(pdir):
+        return None
+    for name in sorted(os.listdir(pdir)):
+        if not name.endswith((".md", ".txt")):
+            continue
+        with open(os.path.join(pdir, name)) as f:
+            for line in f:
+                tok = line.split()
+                if len(tok) == 5 and tok[0] == "TRAFFIC" and tok[1] == kernel and int(tok[2]) == batch:
+                    return int(tok[3]) + int(tok[4])
+    return None
 
 
 def percentile(xs, p):
@@ -119,6 +142,7 @@ def main():
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--db-log2", type=int, default=26)
     ap.add_argument("--single-steps", type=int, default=32, help="steps of the one-query-per-launch run")
+    ap.add_argument("--host-steps", type=int, default=3, help="steps of the host-buffer (PCIe-inclusive) run")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -189,16 +213,39 @@ def main():
             log(f"  {k_:13s} {ms / n:8.3f} ms/launch x {n}")
 
     # the same path, one query per launch
-    single = []
-    for i in range(args.warmup):
-        step(i, 1)
-    t0 = time.perf_counter()
-    for i in range(args.single_steps):
-        t1 = time.perf_counter()
-        step(i, 1)
-        single.append((time.perf_counter() - t1) * 1e3)
-    single_qps = args.single_steps / (time.perf_counter() - t0)
-    log(f"one query per launch: {single_qps:.0f} queries/s, p50 {percentile(single, 50):.2f} ms")
+    single, single_qps = [], None
+    if args.single_steps:
+        for i in range(args.warmup):
+            step(i, 1)
+        t0 = time.perf_counter()
+        for i in range(args.single_steps):
+            t1 = time.perf_counter()
+            step(i, 1)
+            single.append((time.perf_counter() - t1) * 1e3)
+        single_qps = args.single_steps / (time.perf_counter() - t0)
+        log(f"one query per launch: {single_qps:.0f} queries/s, p50 {percentile(single, 50):.2f} ms")
+
+    # the same batches through the host-buffer API (queries in the reference's
+    # u64 layout in host memory, responses copied back): PCIe-inclusive rate
+    host_qps = None
+    if args.host_steps:
+        shards, b, seeds, y = qsets[0]
+        hq = []
+        for i in range(B):
+            rg = np.zeros((2 * ELL, K, D), dtype=np.uint64)
+            rg[ELL:] = b[i]
+            sd = np.zeros((2 * ELL, 32), dtype=np.uint8)
+            sd[ELL:] = seeds[i].astype("<u4").view(np.uint8).reshape(ELL, 32)
+            hq.append(M.ClientQuery(shard_id=int(shards[i]), rgsw=rg, seeds=sd, has_packing_keys=True,
+                                    y_body=y[i].astype(np.uint64)))
+        ids, vs = [0] * B, [M.TWO_PACKING] * B
+        ctx.respond_batch(ids, hq, vs)
+        t0 = time.perf_counter()
+        for _ in range(args.host_steps):
+            res = ctx.respond_batch(ids, hq, vs)
+        host_qps = B * args.host_steps / (time.perf_counter() - t0)
+        assert all(not isinstance(r, M.PirError) for r in res)
+        log(f"host-buffer API (PCIe-inclusive): {host_qps:.0f} queries/s")
 
     chain_ms, chain_n = prof["pack_chain"]
     chain_avg_s = chain_ms / max(chain_n, 1) / 1e3
@@ -213,18 +260,19 @@ def main():
         "peak": round(MODMUL_PEAK_T, 2),
         "unit": "Tmodmul/s",
         "frac": round(chain_tmodmul / MODMUL_PEAK_T, 4),
-        "traffic": None,
+        "traffic": measured_traffic("k_pack_chain", B),
         "algorithmic_per_launch": f"{MODMUL_CHAIN} modmul/query x {B} queries",
         "avg_launch_ms": round(chain_avg_s * 1e3, 4),
     }
     roofline_hbm = {
         "bound": "hbm",
         "kernel": "k_pack_front" if front == "pack_front" else "k_first_dim",
-        "achieved": round(FIRST_DIM_BYTES * B / fd_avg_s / 1e9, 1),
+        "achieved": round(FRONT_BYTES * B / fd_avg_s / 1e9, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
-        "frac": round(FIRST_DIM_BYTES * B / fd_avg_s / 1e9 / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "frac": round(FRONT_BYTES * B / fd_avg_s / 1e9 / HBM_PEAK_GBS, 4),
+        "traffic": measured_traffic("k_pack_front", B),
+        "algorithmic_per_launch": f"{FRONT_BYTES} B/query x {B} queries",
         "avg_launch_ms": round(fd_avg_s * 1e3, 4),
     }
     line = {
@@ -251,9 +299,10 @@ def main():
         },
         "p50_ms": round(percentile(lat, 50), 3),
         "p99_ms": round(percentile(lat, 99), 3),
-        "single_query_qps": round(single_qps, 1),
-        "single_query_p50_ms": round(percentile(single, 50), 3),
-        "speedup_vs_single_query_launch": round(qps / single_qps, 2),
+        "single_query_qps": round(single_qps, 1) if single_qps else None,
+        "single_query_p50_ms": round(percentile(single, 50), 3) if single else None,
+        "speedup_vs_single_query_launch": round(qps / single_qps, 2) if single_qps else None,
+        "host_buffer_api_qps": round(host_qps, 1) if host_qps else None,
         "modmul_per_query": MODMUL_QUERY,
         "roofline": roofline,
         "roofline_hbm_front": roofline_hbm,

@@ -4,7 +4,8 @@
  * This is the drop-in boundary for the respond path of igor53627/inspire-rs
  * (Rust crate `inspire` 0.1.2).  Each entry point replaces one public Rust
  * function; the replaced function is cited as file:line in the reference.  A
- * Rust caller binds these with `extern "C"`.
+ * Rust caller binds these with `extern "C"` declarations of the same
+ * signatures and #[repr(C)] mirrors of the structs below.
  *
  * Conventions
  *  - Polynomials cross the boundary as the reference's `Poly.coeffs`
@@ -187,8 +188,8 @@ int inspire_respond_batch_device(inspire_ctx *ctx, const inspire_device_batch *b
 /* When enabled, every kernel launch of the respond path is bracketed by HIP
  * events on the context's stream.  inspire_ctx_profile reads the summed
  * duration (ms) and launch count of one kernel by name (rows_ntt,
- * expand_seeds, first_dim, cols_intt, fold, pack_b0, pack_step1, y_ntt,
- * pack_chain, db_load) and, with reset != 0, clears it. */
+ * expand_seeds, first_dim, cols_intt, fold, pack_front, pack_b0, pack_step1,
+ * y_ntt, pack_chain, db_load) and, with reset != 0, clears it. */
 int inspire_ctx_set_profiling(inspire_ctx *ctx, int enable);
 int inspire_ctx_profile(inspire_ctx *ctx, const char *kernel, double *total_ms, uint64_t *launches, int reset);
 
