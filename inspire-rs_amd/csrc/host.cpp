@@ -233,6 +233,9 @@ int inspire_ctx_create(const inspire_params* params, const uint8_t w_seed[32], i
         D.gadget_log = glog;
         D.gadget_mask = (uint32_t)(params->gadget_base - 1);
         D.q = ctx->q;
+        D.small_digits = 1;
+        for (uint32_t k = 0; k < ctx->K; k++)
+            if (params->gadget_base > ctx->hm[k].q) D.small_digits = 0;
         for (uint32_t k = 0; k < ctx->K; k++) {
             const uint32_t q = (uint32_t)ctx->hm[k].q;
             const uint32_t* base = ctx->tables.get<uint32_t>() + per * k;
@@ -245,6 +248,7 @@ int inspire_ctx_create(const inspire_params* params, const uint8_t w_seed[32], i
             D.m[k].itwp = base + 3 * ctx->N;
             D.delta[k] = (uint32_t)(ctx->delta % q);
             D.deltap[k] = shoup_pre(D.delta[k], q);
+            D.onep[k] = shoup_pre(1, q);
             D.mont[k] = (uint32_t)((1ull << 32) % q);
             D.montp[k] = shoup_pre(D.mont[k], q);
         }

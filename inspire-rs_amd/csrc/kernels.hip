@@ -88,8 +88,7 @@ __device__ __forceinline__ uint64_t crt_compose(const CtxDev& C, uint32_t a0, ui
 __device__ __forceinline__ uint32_t digit_of(const CtxDev& C, uint64_t x, int i, int k) {
     const uint32_t sh = i * C.gadget_log;
     const uint32_t v = sh < 64 ? (uint32_t)((x >> sh) & C.gadget_mask) : 0u;
-    const uint32_t q = C.m[k].q;
-    return v < q ? v : v % q;
+    return C.small_digits ? v : reduce32(v, C.m[k].q, C.onep[k]);
 }
 
 // Database preprocessing: h -> Delta*h (trivial_encrypt, rlwe/enc.rs:174-182)
@@ -371,10 +370,7 @@ k_encode_entries(const uint8_t* __restrict__ bytes, uint64_t num_entries_in_slic
         else if (bo < entry_size) v = p[bo];
     }
     uint32_t* o = out + sc * C.K * C.n;
-    for (uint32_t k = 0; k < C.K; k++) {
-        const uint32_t q = C.m[k].q;
-        o[(size_t)k * C.n + j] = v < q ? v : v % q;
-    }
+    for (uint32_t k = 0; k < C.K; k++) o[(size_t)k * C.n + j] = reduce32(v, C.m[k].q, C.onep[k]);
 }
 
 // ---------------------------------------------------------------------------

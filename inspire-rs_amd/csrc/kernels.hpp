@@ -20,6 +20,8 @@ struct CtxDev {
     uint32_t gadget_mask;         // gadget_base - 1
     uint32_t q0_inv_q1, q0_inv_q1p;  // CRT: q0^-1 mod q1 (Shoup pair, mod q1)
     uint32_t one_q1p;                // Shoup companion of 1 mod q1 (x mod q1 for x < 2^32)
+    uint32_t onep[kMaxK];            // Shoup companion of 1 mod q_k (reduce32)
+    uint32_t small_digits;           // gadget_base <= every q_k: digits need no reduction
     uint64_t q;                   // composite modulus
     uint32_t delta[kMaxK], deltap[kMaxK];  // Delta mod q_k (Shoup pairs)
     uint32_t mont[kMaxK], montp[kMaxK];    // 2^32 mod q_k (Montgomery-form conversion)

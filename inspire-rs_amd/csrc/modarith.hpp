@@ -38,6 +38,9 @@ IR_HD uint32_t shoup_mul(uint32_t x, uint32_t w, uint32_t wp, uint32_t q) {
 
 IR_HD uint32_t shoup_pre(uint32_t w, uint32_t q) { return (uint32_t)(((uint64_t)w << 32) / q); }
 
+// v mod q for any v < 2^32, with onep = shoup_pre(1, q).
+IR_HD uint32_t reduce32(uint32_t v, uint32_t q, uint32_t onep) { return csub(shoup_mul(v, 1u, onep, q), q); }
+
 IR_HD uint32_t mont_reduce(uint64_t t, uint32_t q, uint32_t qinv_neg) {
     uint32_t m = (uint32_t)t * qinv_neg;
     return (uint32_t)((t + (uint64_t)m * q) >> 32);

@@ -310,6 +310,8 @@ void pack_run(inspire_ctx* ctx, const Database& db, uint32_t Q, const uint32_t* 
     for (uint32_t k = 0; k < K; k++) {
         const uint64_t q = ctx->hm[k].q;
         (k ? A.ninv1 : A.ninv0) = (uint32_t)powmod64(ctx->N, q - 2, q);
+        A.ninvm[k] = (uint32_t)mulmod64(powmod64(ctx->N, q - 2, q), (1ull << 32) % q, q);
+        A.ninvmp[k] = shoup_pre(A.ninvm[k], (uint32_t)q);
     }
     const uint64_t q0 = ctx->hm[0].q;
     A.q0r64 = (uint32_t)pow2_64_mod(q0);

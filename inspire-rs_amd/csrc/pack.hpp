@@ -27,6 +27,7 @@ struct PackArgs {
     const uint64_t* bpart;       // [Q][nparts][gamma][K] partial sums of bhat (b_poly = n^-1 * sum)
     uint32_t nparts;
     uint32_t ninv0, ninv1;       // n^-1 mod q_k
+    uint32_t ninvm[2], ninvmp[2];  // n^-1 * 2^32 mod q_k (Shoup pairs)
     uint32_t* out;               // [Q][2][K][N] packed ciphertext (a_hat, b)
     uint32_t gamma;
     uint32_t residue_wise;       // 1: corrected automorphism on every residue
@@ -60,6 +61,12 @@ hipError_t launch_pack_front(const uint32_t* digits, const uint32_t* rows, const
                              bool rw, const CtxDev& C, hipStream_t st);
 hipError_t launch_pack_step1(const uint32_t* cols, const uint32_t* step1, uint32_t* r_pre, uint32_t Q,
                              uint32_t gamma, uint32_t kres, const CtxDev& C, hipStream_t st);
+// Two-modulus packing chain with every thread on both residues
+// (pack_chain.hip), for d <= 2048 and ell <= 3; launch_pack_chain_scaled
+// uses it whenever pack_chain_crt_supported holds.
+bool pack_chain_crt_supported(const CtxDev& C);
+hipError_t launch_pack_chain_crt(const PackArgs& A, uint32_t Q, const CtxDev& C, const ScaleDev& inv,
+                                 hipStream_t st);
 // inv: inverse-NTT scaling (plain inverse: n^-1 folded into the last stage).
 hipError_t launch_pack_chain_scaled(const PackArgs& A, uint32_t Q, const CtxDev& C, const ScaleDev& inv,
                                     hipStream_t st);

@@ -1,0 +1,468 @@
+// Packing chain for two CRT moduli: packing_offline step 2 and the online sum
+// (src/inspiring/inspiring2.rs:926-1041, :1185-1222), one query per
+// workgroup of 2 x N/8 threads; the first half works on residue 0, the second
+// on residue 1.
+//
+// Per step i = gamma-2 .. 0, in the NTT domain:
+//   T[i] = G^-1(INTT(R[i+1]));  R[i] = tau_{g^i}(r_pre[i]) + sum_j NTT(T[i][j]) * w_all[i][j]
+//   S   += sum_j NTT(T[i][j]) * y_all[i][j]          (online accumulator)
+// then a_hat = INTT(R[0]) and b = b_poly + INTT(S).
+//
+// Thread layout.  The radix-8 NTT passes leave every thread with eight fixed
+// positions.  The inverse transform's last pass and the forward transform's
+// first pass touch the same positions {t + k N/8} (the "coefficient
+// positions"), and the forward transform's last pass and the inverse
+// transform's first pass touch the same positions (the "slot positions").
+// So, per step, the CRT composition and the gadget digits of T[i] are formed
+// in registers between the two transforms (after one LDS exchange of the
+// coefficients between the halves), and the multiply-accumulate that produces
+// R[i] runs in registers on the slots the next inverse transform starts from.
+// R, the online accumulator S and the NTT of the client's y_body (rotated by
+// the automorphism table on the fly, staged in LDS) never leave the chip.
+//
+// Twiddles are kept in LDS in Montgomery form (w 2^32 mod q, one word each);
+// a butterfly's product is one Montgomery reduction.  Barriers wait for LDS
+// traffic only, so the global loads of the next multiply-accumulate's
+// operands (w_all, the r_pre gathers, the permutation) stay in flight across
+// the transforms.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+#include "ntt.hpp"
+#include "pack.hpp"
+
+namespace inspire {
+
+namespace {
+
+// Barrier over the workgroup for LDS data: this wave's LDS operations have
+// completed, then s_barrier.  Global loads issued earlier are not waited for.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+struct Mod {
+    uint32_t q, q2, qinv;
+};
+
+// a * b * 2^-32 mod q in [0, 2q), for a * b < q * 2^32.
+__device__ __forceinline__ uint32_t mmul(uint32_t a, uint32_t b, const Mod& M) {
+    return mont_reduce((uint64_t)a * b, M.q, M.qinv);
+}
+
+// Harvey CT butterfly with a Montgomery-form twiddle: X, Y < 4q -> X + wY, X - wY (< 4q).
+__device__ __forceinline__ void ct_m(uint32_t& X, uint32_t& Y, uint32_t wm, const Mod& M) {
+    const uint32_t x = csub(X, M.q2);
+    const uint32_t t = mmul(Y, wm, M);
+    X = x + t;
+    Y = x - t + M.q2;
+}
+
+// Harvey GS butterfly with a Montgomery-form twiddle: X, Y < 2q -> X + Y, w(X - Y) (< 2q).
+__device__ __forceinline__ void gs_m(uint32_t& X, uint32_t& Y, uint32_t wm, const Mod& M) {
+    const uint32_t s = csub(X + Y, M.q2);
+    const uint32_t d = X - Y + M.q2;
+    X = s;
+    Y = mmul(d, wm, M);
+}
+
+template <int LOGN>
+struct Lay {
+    static constexpr int N = 1 << LOGN, T = N / 8, LP = NttShape<LOGN>::LDS;
+    static constexpr int R8 = LOGN / 3, REM = LOGN % 3;
+    // first word of the two 4-element groups of a thread's slot positions
+    static __device__ __forceinline__ int group(int t, int g) {
+        if constexpr (REM != 0) return 4 * (t + g * T);
+        else return 8 * t + 4 * g;
+    }
+};
+
+template <int LOGN>
+__device__ __forceinline__ void load_slots(const uint32_t* p, int t, uint32_t (&v)[8]) {
+    using L = Lay<LOGN>;
+#pragma unroll
+    for (int g = 0; g < 2; g++) {
+        const uint4 a = *reinterpret_cast<const uint4*>(p + L::group(t, g));
+        v[4 * g + 0] = a.x;
+        v[4 * g + 1] = a.y;
+        v[4 * g + 2] = a.z;
+        v[4 * g + 3] = a.w;
+    }
+}
+
+// Inverse NTT of one residue, scaled by (c, cw) in the last stage: x at the
+// slot positions of thread t -> x at its coefficient positions (< 2q).  rb is
+// the residue's [LP] LDS buffer; no thread may still be reading it.
+template <int LOGN>
+__device__ __forceinline__ void intt1(uint32_t (&x)[8], int t, uint32_t* rb, const uint32_t* itw, const Mod& M,
+                                      uint32_t c, uint32_t cp, uint32_t cw, uint32_t cwp) {
+    using L = Lay<LOGN>;
+    constexpr int N = L::N;
+    if constexpr (L::REM != 0) {
+#pragma unroll
+        for (int g = 0; g < 2; g++) {
+            const int i = t + g * L::T;
+            uint32_t& x0 = x[4 * g + 0];
+            uint32_t& x1 = x[4 * g + 1];
+            uint32_t& x2 = x[4 * g + 2];
+            uint32_t& x3 = x[4 * g + 3];
+            gs_m(x0, x1, itw[N / 2 + 2 * i], M);
+            gs_m(x2, x3, itw[N / 2 + 2 * i + 1], M);
+            if constexpr (L::REM == 2) {
+                const uint32_t w = itw[N / 4 + i];
+                gs_m(x0, x2, w, M);
+                gs_m(x1, x3, w, M);
+            }
+            *reinterpret_cast<uint4*>(&rb[lds_ix(4 * i)]) = make_uint4(x0, x1, x2, x3);
+        }
+        lds_barrier();
+    }
+    int t0 = 1 << L::REM;
+#pragma unroll
+    for (int p = 0; p < L::R8; p++) {
+        const int i3 = t / t0, r = t - i3 * t0;
+        const int base = 8 * t0 * i3 + r;
+        if (!(L::REM == 0 && p == 0)) {
+#pragma unroll
+            for (int e = 0; e < 8; e++) x[e] = rb[lds_ix(base + e * t0)];
+        }
+        const int mA = N / (2 * t0), mB = N / (4 * t0), mC = N / (8 * t0);
+#pragma unroll
+        for (int h = 0; h < 4; h++) gs_m(x[2 * h], x[2 * h + 1], itw[mA + 4 * i3 + h], M);
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const uint32_t w = itw[mB + 2 * i3 + h];
+            gs_m(x[4 * h + 0], x[4 * h + 2], w, M);
+            gs_m(x[4 * h + 1], x[4 * h + 3], w, M);
+        }
+        if (p < L::R8 - 1) {
+            const uint32_t w = itw[mC + i3];
+#pragma unroll
+            for (int k = 0; k < 4; k++) gs_m(x[k], x[k + 4], w, M);
+#pragma unroll
+            for (int e = 0; e < 8; e++) rb[lds_ix(base + e * t0)] = x[e];
+            lds_barrier();
+        } else {
+            // last stage (m = 1) with the output scale: X' = c (X + Y), Y' = cw (X - Y)
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const uint32_t X = x[k], Y = x[k + 4];
+                x[k] = shoup_mul(X + Y, c, cp, M.q);
+                x[k + 4] = shoup_mul(X - Y + M.q2, cw, cwp, M.q);
+            }
+        }
+        t0 <<= 3;
+    }
+}
+
+template <int ELL>
+__device__ __forceinline__ void fwd_r8(uint32_t (&x)[ELL][8], int m0, int i, const uint32_t* tw, const Mod& M) {
+    {
+        const uint32_t w = tw[m0 + i];
+#pragma unroll
+        for (int j = 0; j < ELL; j++)
+#pragma unroll
+            for (int k = 0; k < 4; k++) ct_m(x[j][k], x[j][k + 4], w, M);
+    }
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const uint32_t w = tw[2 * m0 + 2 * i + h];
+#pragma unroll
+        for (int j = 0; j < ELL; j++) {
+            ct_m(x[j][4 * h + 0], x[j][4 * h + 2], w, M);
+            ct_m(x[j][4 * h + 1], x[j][4 * h + 3], w, M);
+        }
+    }
+#pragma unroll
+    for (int h = 0; h < 4; h++) {
+        const uint32_t w = tw[4 * m0 + 4 * i + h];
+#pragma unroll
+        for (int j = 0; j < ELL; j++) ct_m(x[j][2 * h], x[j][2 * h + 1], w, M);
+    }
+}
+
+// Forward NTT of ELL polys of one residue: x at the coefficient positions of
+// thread t -> x at its slot positions (< 4q).  db is [ELL][LP] LDS; no thread
+// may still be reading it.
+template <int LOGN, int ELL>
+__device__ __forceinline__ void ntt_digits(uint32_t (&x)[ELL][8], int t, uint32_t* db, const uint32_t* tw,
+                                           const Mod& M) {
+    using L = Lay<LOGN>;
+    constexpr int N = L::N, LP = L::LP;
+    int m0 = 1;
+#pragma unroll
+    for (int p = 0; p < L::R8; p++) {
+        const int t0 = N / (2 * m0), st = t0 >> 2;
+        const int i = t / st, r = t - i * st;
+        const int base = 2 * i * t0 + r;
+        if (p > 0) {
+#pragma unroll
+            for (int j = 0; j < ELL; j++)
+#pragma unroll
+                for (int e = 0; e < 8; e++) x[j][e] = db[j * LP + lds_ix(base + e * st)];
+        }
+        fwd_r8<ELL>(x, m0, i, tw, M);
+        if (!(L::REM == 0 && p == L::R8 - 1)) {
+#pragma unroll
+            for (int j = 0; j < ELL; j++)
+#pragma unroll
+                for (int e = 0; e < 8; e++) db[j * LP + lds_ix(base + e * st)] = x[j][e];
+            lds_barrier();
+        }
+        m0 <<= 3;
+    }
+    if constexpr (L::REM != 0) {
+#pragma unroll
+        for (int g = 0; g < 2; g++) {
+            const int i = t + g * L::T;
+#pragma unroll
+            for (int j = 0; j < ELL; j++) {
+                const uint4 v = *reinterpret_cast<const uint4*>(&db[j * LP + lds_ix(4 * i)]);
+                x[j][4 * g + 0] = v.x;
+                x[j][4 * g + 1] = v.y;
+                x[j][4 * g + 2] = v.z;
+                x[j][4 * g + 3] = v.w;
+            }
+            if constexpr (L::REM == 2) {
+                const uint32_t w = tw[N / 4 + i], w0 = tw[N / 2 + 2 * i], w1 = tw[N / 2 + 2 * i + 1];
+#pragma unroll
+                for (int j = 0; j < ELL; j++) {
+                    ct_m(x[j][4 * g + 0], x[j][4 * g + 2], w, M);
+                    ct_m(x[j][4 * g + 1], x[j][4 * g + 3], w, M);
+                    ct_m(x[j][4 * g + 0], x[j][4 * g + 1], w0, M);
+                    ct_m(x[j][4 * g + 2], x[j][4 * g + 3], w1, M);
+                }
+            } else {
+                const uint32_t w0 = tw[N / 2 + 2 * i], w1 = tw[N / 2 + 2 * i + 1];
+#pragma unroll
+                for (int j = 0; j < ELL; j++) {
+                    ct_m(x[j][4 * g + 0], x[j][4 * g + 1], w0, M);
+                    ct_m(x[j][4 * g + 2], x[j][4 * g + 3], w1, M);
+                }
+            }
+        }
+    }
+}
+
+}  // namespace
+
+template <int LOGN, int ELL>
+struct ChainCrtShape {
+    static constexpr int N = 1 << LOGN, LP = NttShape<LOGN>::LDS;
+    static constexpr size_t kBytes = (size_t)4 * N * 4      // Montgomery twiddles, both directions and residues
+                                     + (size_t)ELL * 2 * N * 4  // NTT of the client's y_body
+                                     + (size_t)2 * LP * 4       // inverse-transform exchange, per residue
+                                     + (size_t)2 * ELL * LP * 4;  // digit-transform exchange, per residue
+};
+
+template <int LOGN, int ELL>
+__global__ void __launch_bounds__(2 * NttShape<LOGN>::T, 1)
+k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
+    using L = Lay<LOGN>;
+    constexpr int N = L::N, T = L::T, LP = L::LP;
+    constexpr size_t KN = 2 * (size_t)N;
+    extern __shared__ __align__(16) uint32_t lds[];
+    uint32_t* twf = lds;                 // [2][N] psi^br(i) 2^32 mod q_k
+    uint32_t* twi = twf + 2 * N;         // [2][N] psi^-br(i) 2^32 mod q_k
+    uint32_t* yl = twi + 2 * N;          // [ELL][2][N]
+    uint32_t* rbuf = yl + ELL * 2 * N;   // [2][LP]
+    uint32_t* dbuf = rbuf + 2 * LP;      // [2][ELL][LP]
+
+    const int tid = threadIdx.x;
+    const int k = tid / T, t = tid - k * T;  // residue of this half, thread within it
+    const uint32_t qi = blockIdx.x, gamma = A.gamma;
+    const uint32_t* rpre = A.r_pre + (size_t)qi * gamma * KN;
+    const uint32_t* Y = A.y + (A.y_off ? A.y_off[qi] : (uint64_t)qi * ELL * KN);
+    const uint32_t ymode = A.y_mode ? A.y_mode[qi] : 0u;
+    const uint32_t ycount = A.y_count ? A.y_count[qi] : gamma - 1;
+    const Mod M{C.m[k].q, C.m[k].q2, C.m[k].qinv};
+    const uint32_t q0 = C.m[0].q, q1 = C.m[1].q;
+    uint32_t* rb = rbuf + k * LP;
+    uint32_t* db = dbuf + k * ELL * LP;
+    const uint32_t* tw = twf + k * N;
+    const uint32_t* itw = twi + k * N;
+
+    for (int e = tid; e < 2 * N; e += 2 * T) {
+        const int kk = e / N, i = e - kk * N;
+        const ModDev& D = C.m[kk];
+        twf[e] = csub(shoup_mul(D.tw[i], C.mont[kk], C.montp[kk], D.q), D.q);
+        twi[e] = csub(shoup_mul(D.itw[i], C.mont[kk], C.montp[kk], D.q), D.q);
+    }
+    if (ymode == 0) {
+        for (int e = tid; e < ELL * 2 * N / 4; e += 2 * T)
+            reinterpret_cast<uint4*>(yl)[e] = reinterpret_cast<const uint4*>(Y)[e];
+    }
+
+    // this residue's value of the reference's automorphism of r_pre[i] at a
+    // slot whose source slot is src (inspiring2.rs:279-296): residue 1 is
+    // rebuilt from residue 0's value in the reference's 2^64 Montgomery form,
+    // or permuted on its own in the corrected mode
+    auto r_init = [&](const uint32_t* Ri, uint32_t src) -> uint32_t {
+        const uint32_t v0 = Ri[src];
+        if (k == 0) return v0;
+        if (A.residue_wise) return Ri[N + src];
+        const uint32_t ref = csub(shoup_mul(v0, A.q0r64, A.q0r64p, q0), q0);
+        const uint32_t v1 = csub(shoup_mul(ref, 1u, A.one_q1p, q1), q1);
+        return csub(shoup_mul(v1, A.q1r64i, A.q1r64ip, q1), q1);
+    };
+
+    uint32_t R[8], S[8], pm[8], pmn[8];
+    load_slots<LOGN>(A.perm + (size_t)(gamma - 1) * N, t, pm);
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        R[e] = r_init(rpre + (size_t)(gamma - 1) * KN, pm[e]);
+        S[e] = 0;
+    }
+    if (gamma >= 2) load_slots<LOGN>(A.perm + (size_t)(gamma - 2) * N, t, pmn);
+    __syncthreads();
+
+    for (int i = (int)gamma - 2; i >= 0; i--) {
+        // The addresses below are recomputed from ts every step instead of
+        // being hoisted out of the loop, where they would occupy registers.
+        int ts = t;
+        asm volatile("" : "+v"(ts));
+#pragma unroll
+        for (int e = 0; e < 8; e++) pm[e] = pmn[e];
+        if (i > 0) load_slots<LOGN>(A.perm + (size_t)(i - 1) * N, ts, pmn);
+        // operands of this step's multiply-accumulate, loaded ahead
+        const uint32_t* Ri = rpre + (size_t)i * KN;
+        uint32_t rv[8], w[ELL][8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) rv[e] = Ri[pm[e]];
+        uint32_t rv1[8];
+        if (k == 1 && A.residue_wise) {
+#pragma unroll
+            for (int e = 0; e < 8; e++) rv1[e] = Ri[N + pm[e]];
+        }
+        const uint32_t* Wi = A.w_all + (size_t)i * ELL * KN + (size_t)k * N;
+#pragma unroll
+        for (int j = 0; j < ELL; j++) load_slots<LOGN>(Wi + (size_t)j * KN, ts, w[j]);
+
+        // T[i] = G^-1(INTT(R[i+1])): this residue's coefficients, then the other's via LDS
+        intt1<LOGN>(R, ts, rb, itw, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            R[e] = csub(R[e], M.q);
+            rb[lds_ix(ts + e * T)] = R[e];
+        }
+        lds_barrier();
+        const uint32_t* ro = rbuf + (1 - k) * LP;
+        uint32_t dg[ELL][8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const uint32_t other = ro[lds_ix(ts + e * T)];
+            const uint32_t c0 = k == 0 ? R[e] : other, c1 = k == 0 ? other : R[e];
+            const uint32_t a0m = csub(shoup_mul(c0, 1u, C.one_q1p, q1), q1);
+            const uint32_t u = csub(shoup_mul(c1 + q1 - a0m, C.q0_inv_q1, C.q0_inv_q1p, q1), q1);
+            const uint64_t x = (uint64_t)c0 + (uint64_t)q0 * u;
+#pragma unroll
+            for (int j = 0; j < ELL; j++) {
+                const uint32_t sh = j * C.gadget_log;
+                const uint32_t v = sh < 64 ? (uint32_t)((x >> sh) & C.gadget_mask) : 0u;
+                dg[j][e] = C.small_digits ? v : reduce32(v, M.q, C.onep[k]);
+            }
+        }
+        ntt_digits<LOGN, ELL>(dg, ts, db, tw, M);
+
+        const bool use_y = (uint32_t)i < ycount;
+        uint32_t yv[ELL][8];
+        if (use_y) {
+            if (ymode == 0) {
+#pragma unroll
+                for (int j = 0; j < ELL; j++)
+#pragma unroll
+                    for (int e = 0; e < 8; e++) yv[j][e] = yl[(j * 2 + k) * N + pm[e]];
+            } else {
+#pragma unroll
+                for (int j = 0; j < ELL; j++)
+                    load_slots<LOGN>(Y + (((size_t)i * ELL + j) * 2 + k) * N, ts, yv[j]);
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            uint64_t ar = 0, ab = 0;
+#pragma unroll
+            for (int j = 0; j < ELL; j++) {
+                const uint32_t x = csub(csub(dg[j][e], M.q2), M.q);
+                ar += (uint64_t)x * w[j][e];
+                if (use_y) ab += (uint64_t)x * yv[j][e];
+            }
+            uint32_t ri;
+            if (k == 0) ri = rv[e];
+            else if (A.residue_wise) ri = rv1[e];
+            else {
+                const uint32_t ref = csub(shoup_mul(rv[e], A.q0r64, A.q0r64p, q0), q0);
+                const uint32_t v1 = csub(shoup_mul(ref, 1u, A.one_q1p, q1), q1);
+                ri = csub(shoup_mul(v1, A.q1r64i, A.q1r64ip, q1), q1);
+            }
+            R[e] = csub(ri + csub(mont_reduce(ar, M.q, M.qinv), M.q), M.q);
+            if (use_y) S[e] = csub(S[e] + csub(mont_reduce(ab, M.q, M.qinv), M.q), M.q);
+        }
+    }
+
+    // a_hat = INTT(R[0]); b = b_poly + INTT(S)
+    uint32_t* out = A.out + (size_t)qi * 2 * KN + (size_t)k * N;
+    intt1<LOGN>(R, t, rb, itw, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
+#pragma unroll
+    for (int e = 0; e < 8; e++) out[t + e * T] = csub(R[e], M.q);
+    lds_barrier();  // every thread is past its last read of rb
+    intt1<LOGN>(S, t, rb, itw, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const uint32_t pos = t + e * T;
+        uint32_t v = csub(S[e], M.q);
+        if (pos < gamma) {
+            // b_poly coefficient: n^-1 * (sum of the column's bhat) mod q_k.
+            // The sum is below 2^40: (s 2^-32) (n^-1 2^32) = s n^-1.
+            uint64_t s = 0;
+            for (uint32_t pb = 0; pb < A.nparts; pb++)
+                s += A.bpart[(((size_t)qi * A.nparts + pb) * gamma + pos) * 2 + k];
+            const uint32_t a = csub(mont_reduce(s, M.q, M.qinv), M.q);
+            v = csub(v + csub(shoup_mul(a, A.ninvm[k], A.ninvmp[k], M.q), M.q), M.q);
+        }
+        out[KN + pos] = v;
+    }
+}
+
+template <int LOGN, int ELL>
+static hipError_t chain_crt_t(const PackArgs& A, uint32_t Q, const CtxDev& C, const ScaleDev& inv, hipStream_t st) {
+    constexpr size_t shmem = ChainCrtShape<LOGN, ELL>::kBytes;
+    auto kern = k_pack_chain_crt<LOGN, ELL>;
+    if (shmem > 64 * 1024) {
+        const hipError_t e =
+            hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(kern, dim3(Q), dim3(2 * NttShape<LOGN>::T), shmem, st, A, C, inv);
+    return hipGetLastError();
+}
+
+template <int LOGN>
+static hipError_t chain_crt_ell(const PackArgs& A, uint32_t Q, const CtxDev& C, const ScaleDev& inv,
+                                hipStream_t st) {
+    switch (C.ell) {
+        case 1: return chain_crt_t<LOGN, 1>(A, Q, C, inv, st);
+        case 2: return chain_crt_t<LOGN, 2>(A, Q, C, inv, st);
+        case 3: return chain_crt_t<LOGN, 3>(A, Q, C, inv, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+bool pack_chain_crt_supported(const CtxDev& C) {
+    if (C.K != 2 || C.ell < 1 || C.ell > 3 || C.logn < 8 || C.logn > 11) return false;
+    // every modulus must leave room for the lazy ranges: 4q < 2^32
+    if (C.m[0].q >= (1u << 30) || C.m[1].q >= (1u << 30)) return false;
+    return ChainCrtShape<11, 3>::kBytes <= 160 * 1024;
+}
+
+hipError_t launch_pack_chain_crt(const PackArgs& A, uint32_t Q, const CtxDev& C, const ScaleDev& inv,
+                                 hipStream_t st) {
+    if (!Q) return hipSuccess;
+    switch (C.logn) {
+        case 8: return chain_crt_ell<8>(A, Q, C, inv, st);
+        case 9: return chain_crt_ell<9>(A, Q, C, inv, st);
+        case 10: return chain_crt_ell<10>(A, Q, C, inv, st);
+        case 11: return chain_crt_ell<11>(A, Q, C, inv, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace inspire

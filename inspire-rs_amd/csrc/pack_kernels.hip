@@ -63,48 +63,59 @@ k_pack_b0(const uint32_t* __restrict__ cols, uint64_t* __restrict__ bpart, uint3
 // Column values never leave registers: ahat feeds the DFT, bhat feeds the
 // per-block b sums (bpart [Q][n/256][gamma][K] u64).
 // ---------------------------------------------------------------------------
-template <int LG, bool RW>
+template <int LG, bool RW, int ELL>
 __global__ void __launch_bounds__(256)
 k_pack_front(const uint32_t* __restrict__ digits, const uint32_t* __restrict__ rows, const int32_t* __restrict__ slot,
              uint32_t* __restrict__ r_pre, uint64_t* __restrict__ bpart, const uint32_t* __restrict__ twist,
              const uint2* __restrict__ dftw, CtxDev C) {
     constexpr int G = 1 << LG;
+    constexpr int CH = G < 16 ? G : 16;  // columns whose digits are loaded together
     __shared__ uint32_t red[2 * G][257];
     const int n = (int)C.n, nb = n / 256, tid = threadIdx.x;
     const uint32_t qi = blockIdx.x / nb, blk = blockIdx.x % nb;
     const int s = blk * 256 + tid;
     const int sl = slot[qi] < 0 ? 0 : slot[qi];  // a missing shard is reported by the host; read shard 0
     const size_t KN = (size_t)C.K * n;
-    const uint32_t* R = rows + (size_t)qi * C.ell * 2 * KN + s;
-    const uint32_t* Dq = digits + (size_t)sl * G * C.ell * KN + s;
+    const uint32_t* R = rows + (size_t)qi * ELL * 2 * KN + s;
+    const uint32_t* Dq = digits + (size_t)sl * G * ELL * KN + s;
     const uint32_t cls = ((2u * (__brev((uint32_t)s) >> (32 - C.logn)) + 1u) & (G - 1)) >> 1;
     for (int k = 0; k < (int)C.K; k++) {
         const uint32_t q = C.m[k].q, q2 = 2 * q, qinv = C.m[k].qinv;
         const bool want_a = k == 0 || RW;
-        uint32_t ca[kMaxEll], cb[kMaxEll];
-        for (uint32_t i = 0; i < C.ell; i++) {
+        uint32_t ca[ELL], cb[ELL];
+#pragma unroll
+        for (int i = 0; i < ELL; i++) {
             ca[i] = want_a ? R[((size_t)(2 * i) * C.K + k) * n] : 0u;
             cb[i] = R[((size_t)(2 * i + 1) * C.K + k) * n];
         }
         uint32_t v[G];
 #pragma unroll
-        for (int j = 0; j < G; j++) {
-            const uint32_t* D = Dq + ((size_t)j * C.ell * C.K + k) * n;
-            uint64_t aa = 0, bb = 0;
-            for (uint32_t i = 0; i < C.ell; i++) {
-                const uint64_t d = D[(size_t)i * KN];
-                aa += d * ca[i];
-                bb += d * cb[i];
-            }
-            red[k * G + j][tid] = csub(mont_reduce(bb, q, qinv), q);
-            if (want_a) {
-                const uint32_t a = csub(mont_reduce(aa, q, qinv), q);
-                // twist by gamma^-1 w_s^j (Montgomery form) and place at the bit-reversed index
-                const uint32_t tw = twist[((size_t)k * G + j) * n + s];
-                int p = 0;
+        for (int j0 = 0; j0 < G; j0 += CH) {
+            // every digit load of the chunk is issued before the first use
+            uint32_t dv[CH][ELL];
 #pragma unroll
-                for (int b = 0; b < LG; b++) p |= ((j >> b) & 1) << (LG - 1 - b);
-                v[p] = csub(mont_reduce((uint64_t)a * tw, q, qinv), q);
+            for (int jj = 0; jj < CH; jj++)
+#pragma unroll
+                for (int i = 0; i < ELL; i++) dv[jj][i] = Dq[((((size_t)(j0 + jj)) * ELL + i) * C.K + k) * n];
+#pragma unroll
+            for (int jj = 0; jj < CH; jj++) {
+                const int j = j0 + jj;
+                uint64_t aa = 0, bb = 0;
+#pragma unroll
+                for (int i = 0; i < ELL; i++) {
+                    aa += (uint64_t)dv[jj][i] * ca[i];
+                    bb += (uint64_t)dv[jj][i] * cb[i];
+                }
+                red[k * G + j][tid] = csub(mont_reduce(bb, q, qinv), q);
+                if (want_a) {
+                    const uint32_t a = csub(mont_reduce(aa, q, qinv), q);
+                    // twist by gamma^-1 w_s^j (Montgomery form) and place at the bit-reversed index
+                    const uint32_t tw = twist[((size_t)k * G + j) * n + s];
+                    int p = 0;
+#pragma unroll
+                    for (int b = 0; b < LG; b++) p |= ((j >> b) & 1) << (LG - 1 - b);
+                    v[p] = csub(mont_reduce((uint64_t)a * tw, q, qinv), q);
+                }
             }
         }
         if (want_a) {
@@ -283,9 +294,8 @@ k_pack_chain(PackArgs A, CtxDev C, ScaleDev inv) {
             for (uint32_t d = 0; d < ell; d++)
 #pragma unroll
                 for (int k = 0; k < KK; k++) {
-                    const uint32_t qk = C.m[k].q;
                     const uint32_t v = dig[m][d];
-                    buf(d, k)[lds_ix(j)] = v < qk ? v : v % qk;
+                    buf(d, k)[lds_ix(j)] = C.small_digits ? v : reduce32(v, C.m[k].q, C.onep[k]);
                 }
         }
         __syncthreads();
@@ -355,14 +365,31 @@ hipError_t launch_pack_b0(const uint32_t* cols, uint64_t* bpart, uint32_t Q, uin
     return hipGetLastError();
 }
 
+template <int LG, int ELL>
+static hipError_t front_t(const uint32_t* digits, const uint32_t* rows, const int32_t* slot, uint32_t* r_pre,
+                          uint64_t* bpart, const uint32_t* twist, const uint2* dftw, uint32_t Q, bool rw,
+                          const CtxDev& C, hipStream_t st) {
+    const dim3 grid(Q * (C.n / 256));
+    if (rw)
+        hipLaunchKernelGGL((k_pack_front<LG, true, ELL>), grid, dim3(256), 0, st, digits, rows, slot, r_pre, bpart,
+                           twist, dftw, C);
+    else
+        hipLaunchKernelGGL((k_pack_front<LG, false, ELL>), grid, dim3(256), 0, st, digits, rows, slot, r_pre, bpart,
+                           twist, dftw, C);
+    return hipGetLastError();
+}
+
 template <int LG>
 static hipError_t front_lg(const uint32_t* digits, const uint32_t* rows, const int32_t* slot, uint32_t* r_pre,
                            uint64_t* bpart, const uint32_t* twist, const uint2* dftw, uint32_t Q, bool rw,
                            const CtxDev& C, hipStream_t st) {
-    const dim3 grid(Q * (C.n / 256));
-    if (rw) hipLaunchKernelGGL((k_pack_front<LG, true>), grid, dim3(256), 0, st, digits, rows, slot, r_pre, bpart, twist, dftw, C);
-    else hipLaunchKernelGGL((k_pack_front<LG, false>), grid, dim3(256), 0, st, digits, rows, slot, r_pre, bpart, twist, dftw, C);
-    return hipGetLastError();
+    switch (C.ell) {
+        case 1: return front_t<LG, 1>(digits, rows, slot, r_pre, bpart, twist, dftw, Q, rw, C, st);
+        case 2: return front_t<LG, 2>(digits, rows, slot, r_pre, bpart, twist, dftw, Q, rw, C, st);
+        case 3: return front_t<LG, 3>(digits, rows, slot, r_pre, bpart, twist, dftw, Q, rw, C, st);
+        case 4: return front_t<LG, 4>(digits, rows, slot, r_pre, bpart, twist, dftw, Q, rw, C, st);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_pack_front(const uint32_t* digits, const uint32_t* rows, const int32_t* slot, uint32_t* r_pre,
@@ -409,6 +436,7 @@ static hipError_t chain_k(const PackArgs& A, uint32_t Q, const CtxDev& C, const 
 hipError_t launch_pack_chain_scaled(const PackArgs& A, uint32_t Q, const CtxDev& C, const ScaleDev& inv,
                                     hipStream_t st) {
     if (!Q) return hipSuccess;
+    if (pack_chain_crt_supported(C)) return launch_pack_chain_crt(A, Q, C, inv, st);
     switch (C.logn) {
         case 8: return chain_k<8>(A, Q, C, inv, st);
         case 9: return chain_k<9>(A, Q, C, inv, st);
