@@ -47,6 +47,7 @@ extern "C" {
 #define INSPIRE_ERR_OUT_OF_MEMORY 9
 #define INSPIRE_ERR_DEVICE 10
 #define INSPIRE_ERR_CAPACITY 11            /* an output buffer is too small */
+#define INSPIRE_ERR_PENDING 12             /* a serving-loop ticket is not finished yet */
 
 /* InspireVariant (src/params.rs:78-103) */
 #define INSPIRE_VARIANT_NO_PACKING 0
@@ -183,6 +184,70 @@ typedef struct {
 
 int inspire_respond_batch_device(inspire_ctx *ctx, const inspire_device_batch *batch);
 
+/* ---- serving loop ---- */
+
+/* Registers a client's InspiRING packing keys once: y_body of its
+ * ClientPackingKeys (src/inspiring/inspiring2.rs:603-628), uint64_t[ell][K][d].
+ * The keys stay resident in device memory (their NTT); requests to the
+ * serving loop name the client instead of carrying the keys.  Registering an
+ * id again replaces its keys. */
+int inspire_register_client_keys(inspire_ctx *ctx, uint32_t client_id, const uint64_t *y_body);
+
+typedef struct inspire_server inspire_server;
+
+typedef struct {
+    uint32_t lanes;        /* batches in flight on the device at once (one HIP stream each), 1..8 */
+    uint32_t max_batch;    /* queries per batch, 1..1024 */
+    uint64_t hbm_budget;   /* device bytes all in-flight queries may hold; 0: lanes * max_batch
+                              of the largest query footprint */
+    uint32_t max_wait_us;  /* a lane with fewer than max_batch queries waits at most this long for more
+                              while another lane is busy; 0: start at once */
+    uint32_t reserved;
+} inspire_server_config;
+
+/* One query to the serving loop (the respond_with_variant /
+ * respond_seeded_with_variant semantics of src/pir/respond.rs:129-183 with
+ * InspiRING packing), inputs in device-ready form, copied at submit:
+ *   seeded: rows = b polys of RGSW rows ell..2ell-1, uint32_t[ell][K][d];
+ *           seeds = their 32-byte seeds, uint8_t[ell][32]
+ *   full:   rows = RGSW rows ell..2ell-1, uint32_t[ell][2][K][d]
+ * (rows 0..ell-1 multiply the zero a-digits of the database and do not
+ * affect the response).  Packed variants use the registered keys of
+ * client_id.  arrival_ns: the query's arrival on CLOCK_MONOTONIC (0: now);
+ * latencies are measured from it. */
+typedef struct {
+    uint32_t db_id, shard_id, variant, seeded, client_id, reserved;
+    const uint32_t *rows;
+    const uint8_t *seeds;
+    uint64_t arrival_ns;
+} inspire_request;
+
+typedef struct {
+    uint64_t requests;          /* completed */
+    uint64_t batches;           /* launched */
+    uint64_t max_batch_seen;
+    uint64_t peak_inflight_bytes;
+    uint64_t peak_lanes_busy;   /* most batches on the device at once */
+} inspire_server_stats;
+
+/* Starts the loop: every lane has a worker thread that takes the queries
+ * waiting when it is free (up to max_batch, within the HBM budget), runs
+ * them as one batch (any databases and variants together), and completes
+ * them.  Queries arriving while other batches run are admitted by the next
+ * free lane, so batches overlap on the device. */
+int inspire_server_create(inspire_ctx *ctx, const inspire_server_config *cfg, inspire_server **out);
+/* Copies n requests in and returns their tickets. */
+int inspire_server_submit(inspire_server *s, uint32_t n, const inspire_request *reqs, uint64_t *tickets);
+/* Waits for a ticket (timeout_ms < 0: no limit) and releases it.  r receives
+ * the ServerResponse (as inspire_respond*), status the query's code, done_ns
+ * its completion time on CLOCK_MONOTONIC.  Returns INSPIRE_ERR_PENDING when
+ * the time ran out (the ticket stays valid). */
+int inspire_server_wait(inspire_server *s, uint64_t ticket, int32_t timeout_ms, inspire_response *r,
+                        int32_t *status, uint64_t *done_ns);
+int inspire_server_get_stats(inspire_server *s, inspire_server_stats *out);
+/* Finishes the queries already submitted, then stops the workers. */
+void inspire_server_destroy(inspire_server *s);
+
 /* ---- profiling ---- */
 
 /* When enabled, every kernel launch of the respond path is bracketed by HIP
@@ -191,6 +256,10 @@ int inspire_respond_batch_device(inspire_ctx *ctx, const inspire_device_batch *b
  * expand_seeds, first_dim, cols_intt, fold, pack_front, pack_b0, pack_step1,
  * y_ntt, pack_chain, db_load) and, with reset != 0, clears it. */
 int inspire_ctx_set_profiling(inspire_ctx *ctx, int enable);
+/* Restricts the bracketing to the named kernels (comma-separated names as
+ * above); NULL or "" brackets every kernel (the default).  Fewer events keep
+ * the profiled run closer to an unprofiled one. */
+int inspire_ctx_set_profiling_kernels(inspire_ctx *ctx, const char *kernels);
 int inspire_ctx_profile(inspire_ctx *ctx, const char *kernel, double *total_ms, uint64_t *launches, int reset);
 
 /* ---- math entry points (NttContext, external_product) ---- */

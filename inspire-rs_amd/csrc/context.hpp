@@ -7,6 +7,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <shared_mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -56,6 +57,7 @@ struct Database {
     uint32_t cols = 0;                                // polys per shard (gamma)
     std::unordered_map<uint32_t, uint32_t> slot_of;   // ShardData.id -> slot (first match)
     DevBuf digits;                                    // [shard][col][ell][K][N] u32, NTT domain
+    size_t digit_words_per_shard = 0;                 // cols * ell * K * N
 };
 
 // InspiRING tables for one number of packed columns gamma (PackParams +
@@ -79,11 +81,20 @@ extern const char* const kProfNames[P_COUNT];
 
 struct Profiler {
     bool on = false;
+    uint32_t mask = ~0u;  // kernels (bit = ProfId) bracketed while on
     struct Span { int id; hipEvent_t a, b; };
     std::vector<Span> spans;
     std::vector<hipEvent_t> pool;
     double total_ms[P_COUNT] = {};
     uint64_t launches[P_COUNT] = {};
+};
+
+// Client packing keys registered once and kept resident: the NTT of y_body
+// (Montgomery form), [slot][ell][K][N] u32.
+struct ClientPool {
+    DevBuf keys;
+    uint32_t capacity = 0, used = 0;
+    std::unordered_map<uint32_t, uint32_t> slot_of;  // client id -> slot
 };
 
 struct HostModulus {
@@ -105,6 +116,14 @@ struct inspire_ctx {
     std::recursive_mutex mu;
     std::map<uint32_t, inspire::Database> dbs;
     std::map<uint32_t, std::unique_ptr<inspire::PackTables>> packs;
+    inspire::ClientPool clients;
+    // Writers (database load/unload, client registration) hold it exclusively,
+    // serving lanes share it while a batch reads databases, tables and keys.
+    std::shared_mutex rw;
+    bool rw_held = false;  // this context's own thread holds rw exclusively
+    // A serving lane: its own stream and workspaces; databases, tables and
+    // client keys belong to the owner context.
+    inspire_ctx* owner = nullptr;
     // scratch reused across calls
     inspire::DevBuf ws0, ws1, ws2, ws3, ws4, ws5, ws6;
     std::vector<uint32_t> hbuf;
@@ -112,6 +131,9 @@ struct inspire_ctx {
 };
 
 namespace inspire {
+inline inspire_ctx* root(inspire_ctx* c) { return c->owner ? c->owner : c; }
+inline const inspire_ctx* root(const inspire_ctx* c) { return c->owner ? c->owner : c; }
+
 // Records HIP events around the launches in its scope when profiling is on.
 class ProfScope {
    public:

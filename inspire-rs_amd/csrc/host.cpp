@@ -16,6 +16,7 @@
 #include "context.hpp"
 #include "kernels.hpp"
 #include "modarith.hpp"
+#include "abi.hpp"
 #include "respond.hpp"
 
 using namespace inspire;
@@ -170,20 +171,6 @@ uint64_t pow2_64_mod(uint64_t q) { return (uint64_t)(((unsigned __int128)1 << 64
 // C ABI
 // ---------------------------------------------------------------------------
 
-#define INSPIRE_TRY(...)                                    \
-    try {                                                   \
-        __VA_ARGS__                                         \
-    } catch (const inspire::Error& e) {                     \
-        inspire::set_last_error(e.msg);                     \
-        return e.code;                                      \
-    } catch (const std::bad_alloc&) {                       \
-        inspire::set_last_error("host allocation failed");  \
-        return INSPIRE_ERR_OUT_OF_MEMORY;                   \
-    } catch (const std::exception& e) {                     \
-        inspire::set_last_error(e.what());                  \
-        return INSPIRE_ERR_DEVICE;                          \
-    }
-
 extern "C" {
 
 const char* inspire_last_error(void) { return inspire::g_last_error.c_str(); }
@@ -278,7 +265,7 @@ void inspire_ctx_destroy(inspire_ctx* ctx) {
 
 uint64_t inspire_ctx_device_bytes(const inspire_ctx* ctx) {
     if (!ctx) return 0;
-    uint64_t b = ctx->tables.bytes() + ctx->ws0.bytes() + ctx->ws1.bytes() + ctx->ws2.bytes() + ctx->ws3.bytes() +
+    uint64_t b = ctx->tables.bytes() + ctx->clients.keys.bytes() + ctx->ws0.bytes() + ctx->ws1.bytes() + ctx->ws2.bytes() + ctx->ws3.bytes() +
                  ctx->ws4.bytes() + ctx->ws5.bytes() + ctx->ws6.bytes();
     for (auto& kv : ctx->dbs) b += kv.second.digits.bytes();
     for (auto& kv : ctx->packs) b += kv.second->step1.bytes() + kv.second->perm.bytes() + kv.second->w_all.bytes();
@@ -300,6 +287,33 @@ int inspire_ctx_set_profiling(inspire_ctx* ctx, int enable) {
         std::lock_guard<std::recursive_mutex> g(ctx->mu);
         prof_collect(ctx);
         ctx->prof.on = enable != 0;
+        return INSPIRE_OK;
+    })
+}
+
+int inspire_ctx_set_profiling_kernels(inspire_ctx* ctx, const char* kernels) {
+    INSPIRE_TRY({
+        if (!ctx) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "null argument"};
+        std::lock_guard<std::recursive_mutex> g(ctx->mu);
+        prof_collect(ctx);
+        uint32_t mask = 0;
+        if (!kernels || !*kernels) {
+            mask = ~0u;
+        } else {
+            std::string all(kernels);
+            size_t pos = 0;
+            while (pos <= all.size()) {
+                const size_t end = std::min(all.find(',', pos), all.size());
+                const std::string name = all.substr(pos, end - pos);
+                int id = -1;
+                for (int i = 0; i < P_COUNT; i++)
+                    if (name == kProfNames[i]) id = i;
+                if (id < 0) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "unknown kernel name '" + name + "'"};
+                mask |= 1u << id;
+                pos = end + 1;
+            }
+        }
+        ctx->prof.mask = mask;
         return INSPIRE_OK;
     })
 }
@@ -374,6 +388,7 @@ int inspire_db_load(inspire_ctx* ctx, uint32_t db_id, uint32_t num_shards, const
         if (!ctx || (num_shards && !shard_ids) || (num_shards && polys_per_shard && !polys))
             throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "null argument"};
         std::lock_guard<std::recursive_mutex> g(ctx->mu);
+        std::unique_lock<std::shared_mutex> w(ctx->rw);
         db_load(ctx, db_id, num_shards, shard_ids, polys_per_shard, polys);
         return INSPIRE_OK;
     })
@@ -384,6 +399,7 @@ int inspire_db_load_entries(inspire_ctx* ctx, uint32_t db_id, const uint8_t* dat
     INSPIRE_TRY({
         if (!ctx || (len && !data)) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "null argument"};
         std::lock_guard<std::recursive_mutex> g(ctx->mu);
+        std::unique_lock<std::shared_mutex> w(ctx->rw);
         db_load_entries(ctx, db_id, data, len, entry_size);
         return INSPIRE_OK;
     })
@@ -393,6 +409,7 @@ int inspire_db_unload(inspire_ctx* ctx, uint32_t db_id) {
     INSPIRE_TRY({
         if (!ctx) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "null argument"};
         std::lock_guard<std::recursive_mutex> g(ctx->mu);
+        std::unique_lock<std::shared_mutex> w(ctx->rw);
         if (!ctx->dbs.erase(db_id)) throw Error{INSPIRE_ERR_UNKNOWN_DATABASE, "unknown database " + std::to_string(db_id)};
         return INSPIRE_OK;
     })

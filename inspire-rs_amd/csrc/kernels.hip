@@ -135,9 +135,8 @@ k_rlwe_digits(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, size_
 // q*out_stride + out_off + 2c (a) and + 1 (b), each K*N words.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
-k_first_dim(const uint32_t* __restrict__ digits, const uint32_t* __restrict__ rows,
-            const int32_t* __restrict__ slot, uint32_t* __restrict__ out, uint32_t Q, uint32_t cols,
-            uint32_t out_stride, uint32_t out_off, CtxDev C) {
+k_first_dim(const uint64_t* __restrict__ qdig, const uint32_t* __restrict__ rows, uint32_t* __restrict__ out,
+            uint32_t Q, uint32_t cols, uint32_t out_stride, uint32_t out_off, CtxDev C) {
     const uint32_t n4 = C.n / 4;
     const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t total = (size_t)Q * cols * C.K * n4;
@@ -147,10 +146,10 @@ k_first_dim(const uint32_t* __restrict__ digits, const uint32_t* __restrict__ ro
     const uint32_t k = (uint32_t)(r % C.K); r /= C.K;
     const uint32_t c = (uint32_t)(r % cols);
     const uint32_t qi = (uint32_t)(r / cols);
-    const int32_t sl = slot[qi];
-    if (sl < 0) return;
+    const uint32_t* dig = reinterpret_cast<const uint32_t*>(qdig[qi]);
+    if (!dig) return;  // a missing shard: reported by the host
     const size_t KN = (size_t)C.K * C.n;
-    const uint4* D = reinterpret_cast<const uint4*>(digits + ((size_t)sl * cols + c) * C.ell * KN + k * C.n) + s4;
+    const uint4* D = reinterpret_cast<const uint4*>(dig + (size_t)c * C.ell * KN + k * C.n) + s4;
     const uint4* R = reinterpret_cast<const uint4*>(rows + (size_t)qi * C.ell * 2 * KN + k * C.n) + s4;
     uint64_t aa[4] = {0, 0, 0, 0}, bb[4] = {0, 0, 0, 0};
     for (uint32_t i = 0; i < C.ell; i++) {
@@ -423,13 +422,12 @@ hipError_t launch_rlwe_digits(const uint32_t* in, uint32_t* out, size_t P, const
     return hipGetLastError();
 }
 
-hipError_t launch_first_dim(const uint32_t* digits, const uint32_t* rows, const int32_t* slot, uint32_t* out,
-                            uint32_t Q, uint32_t cols, uint32_t out_stride, uint32_t out_off, const CtxDev& C,
-                            hipStream_t st) {
+hipError_t launch_first_dim(const uint64_t* qdig, const uint32_t* rows, uint32_t* out, uint32_t Q, uint32_t cols,
+                            uint32_t out_stride, uint32_t out_off, const CtxDev& C, hipStream_t st) {
     const size_t total = (size_t)Q * cols * C.K * (C.n / 4);
     if (!total) return hipSuccess;
-    hipLaunchKernelGGL(k_first_dim, dim3(blocks_for(total, 256)), dim3(256), 0, st, digits, rows, slot, out, Q,
-                       cols, out_stride, out_off, C);
+    hipLaunchKernelGGL(k_first_dim, dim3(blocks_for(total, 256)), dim3(256), 0, st, qdig, rows, out, Q, cols,
+                       out_stride, out_off, C);
     return hipGetLastError();
 }
 

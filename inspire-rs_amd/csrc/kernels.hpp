@@ -44,9 +44,9 @@ hipError_t launch_encode_entries(const uint8_t* bytes, uint64_t num_entries, uin
                                  uint32_t shards, uint32_t* out, const CtxDev& C, hipStream_t st);
 hipError_t launch_db_digits(const uint32_t* in, uint32_t* out, size_t P, const CtxDev& C, hipStream_t st);
 hipError_t launch_rlwe_digits(const uint32_t* in, uint32_t* out, size_t P, const CtxDev& C, hipStream_t st);
-hipError_t launch_first_dim(const uint32_t* digits, const uint32_t* rows, const int32_t* slot, uint32_t* out,
-                            uint32_t Q, uint32_t cols, uint32_t out_stride, uint32_t out_off, const CtxDev& C,
-                            hipStream_t st);
+// qdig[q]: device address of query q's shard digits ([cols][ell][K][N]), 0 for a missing shard.
+hipError_t launch_first_dim(const uint64_t* qdig, const uint32_t* rows, uint32_t* out, uint32_t Q, uint32_t cols,
+                            uint32_t out_stride, uint32_t out_off, const CtxDev& C, hipStream_t st);
 hipError_t launch_ext_mac(const uint32_t* dig, const uint32_t* rgsw, uint32_t* out, size_t P, const CtxDev& C,
                           hipStream_t st);
 // seeds [count][8] u32 key words; seed i expands into out + i*out_stride ([K][N]).

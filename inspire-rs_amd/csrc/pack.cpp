@@ -130,8 +130,14 @@ GenInfo gen_info(uint32_t n, uint32_t gamma) {  // inspiring2.rs:114-128
 }  // namespace
 
 PackTables& pack_tables(inspire_ctx* ctx, uint32_t gamma) {
-    auto it = ctx->packs.find(gamma);
-    if (it != ctx->packs.end()) return *it->second;
+    auto& packs = root(ctx)->packs;
+    auto it = packs.find(gamma);
+    if (it != packs.end()) return *it->second;
+    if (ctx->owner) throw Error{INSPIRE_ERR_DEVICE, "packing tables missing for a serving lane"};
+    // Building changes the table map, which serving lanes read: writers hold
+    // ctx->rw exclusively.  Callers that already hold it say so.
+    std::unique_lock<std::shared_mutex> w(ctx->rw, std::defer_lock);
+    if (!ctx->rw_held) w.lock();
     const uint32_t n = ctx->N, K = ctx->K, ell = ctx->ell, logn = ctx->logn;
     const size_t KN = (size_t)K * n;
     std::unique_ptr<PackTables> T(new PackTables());
@@ -250,17 +256,17 @@ PackTables& pack_tables(inspire_ctx* ctx, uint32_t gamma) {
     hip_check(hipMemcpy(T->w_all.get(), w_all.data(), w_all.size() * 4, hipMemcpyHostToDevice), "w_all");
     T->gen_pows = G.gen_pows;
     auto& ref = *T;
-    ctx->packs[gamma] = std::move(T);
+    packs[gamma] = std::move(T);
     return ref;
 }
 
 // Device-only packing of a staged group: first dimension, b_poly, step 1 and
 // the chain.  y: NTT-domain (Montgomery) client keys, per query y_off/y_mode/
 // y_count (device arrays).  out: [Q][2][K][N].
-void pack_run(inspire_ctx* ctx, const Database& db, uint32_t Q, const uint32_t* rows, const int32_t* d_slot,
+void pack_run(inspire_ctx* ctx, uint32_t gamma, uint32_t Q, const uint32_t* rows, const uint64_t* d_qdig,
               const uint32_t* d_y, const uint64_t* d_yoff, const uint32_t* d_ymode, const uint32_t* d_ycount,
               uint32_t* d_out) {
-    const uint32_t gamma = db.cols, K = ctx->K;
+    const uint32_t K = ctx->K;
     const size_t KN = (size_t)K * ctx->N;
     PackTables& T = pack_tables(ctx, gamma);
     const bool rw = (ctx->params.flags & INSPIRE_FLAG_RESIDUE_WISE_AUTOMORPHISM) != 0;
@@ -272,8 +278,8 @@ void pack_run(inspire_ctx* ctx, const Database& db, uint32_t Q, const uint32_t* 
     uint64_t* d_bpart = reinterpret_cast<uint64_t*>(d_rpre + rpre_words + (rpre_words & 1));
     if (T.fast_front) {
         ProfScope ps(ctx, P_FRONT);
-        hip_check(launch_pack_front(db.digits.get<uint32_t>(), rows, d_slot, d_rpre, d_bpart, T.twist.get<uint32_t>(),
-                                    T.dftw.get<uint2>(), Q, gamma, rw || K == 1, ctx->dev, ctx->stream),
+        hip_check(launch_pack_front(d_qdig, rows, d_rpre, d_bpart, T.twist.get<uint32_t>(), T.dftw.get<uint2>(), Q,
+                                    gamma, rw || K == 1, ctx->dev, ctx->stream),
                   "pack front");
     } else {
         const size_t cols_words = (size_t)Q * 2 * gamma * KN;
@@ -281,8 +287,7 @@ void pack_run(inspire_ctx* ctx, const Database& db, uint32_t Q, const uint32_t* 
         uint32_t* d_cols = ctx->ws2.get<uint32_t>();
         {
             ProfScope ps(ctx, P_FIRST_DIM);
-            hip_check(launch_first_dim(db.digits.get<uint32_t>(), rows, d_slot, d_cols, Q, gamma, 2 * gamma, 0,
-                                       ctx->dev, ctx->stream),
+            hip_check(launch_first_dim(d_qdig, rows, d_cols, Q, gamma, 2 * gamma, 0, ctx->dev, ctx->stream),
                       "first dim");
         }
         {
@@ -338,9 +343,9 @@ static void upload_meta(inspire_ctx* ctx, uint32_t Q, const std::vector<uint64_t
     hip_check(hipMemcpyAsync(*d_ycount, y_count.data(), Q * 4, hipMemcpyHostToDevice, ctx->stream), "y count");
 }
 
-void pack_run_device_ybody(inspire_ctx* ctx, const Database& db, uint32_t Q, const uint32_t* rows,
-                           const int32_t* d_slot, const uint32_t* y_body, uint32_t* d_out) {
-    const uint32_t gamma = db.cols, ell = ctx->ell;
+void pack_run_device_ybody(inspire_ctx* ctx, uint32_t gamma, uint32_t Q, const uint32_t* rows,
+                           const uint64_t* d_qdig, const uint32_t* y_body, uint32_t* d_out) {
+    const uint32_t ell = ctx->ell;
     const size_t KN = (size_t)ctx->K * ctx->N;
     PackTables& T = pack_tables(ctx, gamma);
     for (uint32_t i = 0; i + 1 < gamma; i++)
@@ -357,13 +362,13 @@ void pack_run_device_ybody(inspire_ctx* ctx, const Database& db, uint32_t Q, con
                   "y ntt");
     }
     // y layout: query t at t*ell*K*N, rotated from y_body, gamma-1 rotations (the kernel's default)
-    pack_run(ctx, db, Q, rows, d_slot, d_y, nullptr, nullptr, nullptr, d_out);
+    pack_run(ctx, gamma, Q, rows, d_qdig, d_y, nullptr, nullptr, nullptr, d_out);
 }
 
-void pack_respond_host(inspire_ctx* ctx, const Database& db, const BatchItem* items, uint32_t Q, const uint32_t* rows,
-                       const int32_t* d_slot, const uint32_t* d_err, const inspire_query* qs, inspire_response* rs,
+void pack_respond_host(inspire_ctx* ctx, uint32_t gamma, const BatchItem* items, uint32_t Q, const uint32_t* rows,
+                       const uint64_t* d_qdig, const uint32_t* d_err, const inspire_query* qs, inspire_response* rs,
                        int32_t* status) {
-    const uint32_t gamma = db.cols, K = ctx->K, ell = ctx->ell;
+    const uint32_t K = ctx->K, ell = ctx->ell;
     const size_t KN = (size_t)K * ctx->N;
     PackTables& T = pack_tables(ctx, gamma);
     const uint32_t nrot = gamma - 1;
@@ -422,7 +427,7 @@ void pack_respond_host(inspire_ctx* ctx, const Database& db, const BatchItem* it
     const size_t out_words = (size_t)Q * 2 * KN;
     DevBuf& outb = ctx->ws6;
     outb.reserve(out_words * 4);
-    pack_run(ctx, db, Q, rows, d_slot, d_y, d_yoff, d_ymode, d_ycount, outb.get<uint32_t>());
+    pack_run(ctx, gamma, Q, rows, d_qdig, d_y, d_yoff, d_ymode, d_ycount, outb.get<uint32_t>());
     std::vector<uint32_t> hout(out_words);
     uint32_t herr = 0;
     hip_check(hipMemcpyAsync(hout.data(), outb.get(), out_words * 4, hipMemcpyDeviceToHost, ctx->stream), "out");

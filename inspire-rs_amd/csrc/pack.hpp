@@ -11,7 +11,7 @@ namespace inspire {
 // One runnable query of a respond batch (shared with respond.cpp).
 struct BatchItem {
     uint32_t idx;   // position in the caller's batch
-    int32_t slot;   // shard slot in its database
+    uint64_t dig;   // device address of the shard's digits
     bool packed;    // InsPIRe^1/^2
 };
 
@@ -42,23 +42,25 @@ PackTables& pack_tables(inspire_ctx* ctx, uint32_t gamma);
 // Device-only packing of a staged group (rows in NTT/Montgomery form):
 // out [Q][2][K][N].  y: NTT-domain Montgomery client keys with per-query
 // offsets, modes (0: y_body rotated on the fly, 1: explicit y_all) and counts.
-void pack_run(inspire_ctx* ctx, const Database& db, uint32_t Q, const uint32_t* rows, const int32_t* d_slot,
+// d_qdig: per-query device addresses of the shard digits (databases of gamma columns).
+void pack_run(inspire_ctx* ctx, uint32_t gamma, uint32_t Q, const uint32_t* rows, const uint64_t* d_qdig,
               const uint32_t* d_y, const uint64_t* d_yoff, const uint32_t* d_ymode, const uint32_t* d_ycount,
               uint32_t* d_out);
 // Device-resident y_body [Q][ell][K][N] (coefficient domain).
-void pack_run_device_ybody(inspire_ctx* ctx, const Database& db, uint32_t Q, const uint32_t* rows,
-                           const int32_t* d_slot, const uint32_t* y_body, uint32_t* d_out);
+void pack_run_device_ybody(inspire_ctx* ctx, uint32_t gamma, uint32_t Q, const uint32_t* rows,
+                           const uint64_t* d_qdig, const uint32_t* y_body, uint32_t* d_out);
 // Host-side keys of the queries `items` (y_body or y_all), responses copied back.
-void pack_respond_host(inspire_ctx* ctx, const Database& db, const BatchItem* items, uint32_t Q, const uint32_t* rows,
-                       const int32_t* d_slot, const uint32_t* d_err, const inspire_query* qs, inspire_response* rs,
+void pack_respond_host(inspire_ctx* ctx, uint32_t gamma, const BatchItem* items, uint32_t Q, const uint32_t* rows,
+                       const uint64_t* d_qdig, const uint32_t* d_err, const inspire_query* qs, inspire_response* rs,
                        int32_t* status);
 
 hipError_t launch_pack_b0(const uint32_t* cols, uint64_t* bpart, uint32_t Q, uint32_t gamma, const CtxDev& C,
                           hipStream_t st);
 // Fused first dimension + step 1 + b sums for gamma in {1, 2, ..., 64} with gamma^2 | 2n.
-hipError_t launch_pack_front(const uint32_t* digits, const uint32_t* rows, const int32_t* slot, uint32_t* r_pre,
-                             uint64_t* bpart, const uint32_t* twist, const uint2* dftw, uint32_t Q, uint32_t gamma,
-                             bool rw, const CtxDev& C, hipStream_t st);
+// qdig[q]: device address of query q's shard digits (a valid shard also for a missing one).
+hipError_t launch_pack_front(const uint64_t* qdig, const uint32_t* rows, uint32_t* r_pre, uint64_t* bpart,
+                             const uint32_t* twist, const uint2* dftw, uint32_t Q, uint32_t gamma, bool rw,
+                             const CtxDev& C, hipStream_t st);
 hipError_t launch_pack_step1(const uint32_t* cols, const uint32_t* step1, uint32_t* r_pre, uint32_t Q,
                              uint32_t gamma, uint32_t kres, const CtxDev& C, hipStream_t st);
 // Two-modulus packing chain with every thread on both residues

@@ -65,19 +65,19 @@ k_pack_b0(const uint32_t* __restrict__ cols, uint64_t* __restrict__ bpart, uint3
 // ---------------------------------------------------------------------------
 template <int LG, bool RW, int ELL>
 __global__ void __launch_bounds__(256)
-k_pack_front(const uint32_t* __restrict__ digits, const uint32_t* __restrict__ rows, const int32_t* __restrict__ slot,
-             uint32_t* __restrict__ r_pre, uint64_t* __restrict__ bpart, const uint32_t* __restrict__ twist,
-             const uint2* __restrict__ dftw, CtxDev C) {
+k_pack_front(const uint64_t* __restrict__ qdig, const uint32_t* __restrict__ rows, uint32_t* __restrict__ r_pre,
+             uint64_t* __restrict__ bpart, const uint32_t* __restrict__ twist, const uint2* __restrict__ dftw,
+             CtxDev C) {
     constexpr int G = 1 << LG;
     constexpr int CH = G < 16 ? G : 16;  // columns whose digits are loaded together
     __shared__ uint32_t red[2 * G][257];
     const int n = (int)C.n, nb = n / 256, tid = threadIdx.x;
     const uint32_t qi = blockIdx.x / nb, blk = blockIdx.x % nb;
     const int s = blk * 256 + tid;
-    const int sl = slot[qi] < 0 ? 0 : slot[qi];  // a missing shard is reported by the host; read shard 0
     const size_t KN = (size_t)C.K * n;
     const uint32_t* R = rows + (size_t)qi * ELL * 2 * KN + s;
-    const uint32_t* Dq = digits + (size_t)sl * G * ELL * KN + s;
+    // the host points a missing shard at a valid one and reports it
+    const uint32_t* Dq = reinterpret_cast<const uint32_t*>(qdig[qi]) + s;
     const uint32_t cls = ((2u * (__brev((uint32_t)s) >> (32 - C.logn)) + 1u) & (G - 1)) >> 1;
     for (int k = 0; k < (int)C.K; k++) {
         const uint32_t q = C.m[k].q, q2 = 2 * q, qinv = C.m[k].qinv;
@@ -366,44 +366,44 @@ hipError_t launch_pack_b0(const uint32_t* cols, uint64_t* bpart, uint32_t Q, uin
 }
 
 template <int LG, int ELL>
-static hipError_t front_t(const uint32_t* digits, const uint32_t* rows, const int32_t* slot, uint32_t* r_pre,
-                          uint64_t* bpart, const uint32_t* twist, const uint2* dftw, uint32_t Q, bool rw,
-                          const CtxDev& C, hipStream_t st) {
+static hipError_t front_t(const uint64_t* qdig, const uint32_t* rows, uint32_t* r_pre, uint64_t* bpart,
+                          const uint32_t* twist, const uint2* dftw, uint32_t Q, bool rw, const CtxDev& C,
+                          hipStream_t st) {
     const dim3 grid(Q * (C.n / 256));
     if (rw)
-        hipLaunchKernelGGL((k_pack_front<LG, true, ELL>), grid, dim3(256), 0, st, digits, rows, slot, r_pre, bpart,
-                           twist, dftw, C);
+        hipLaunchKernelGGL((k_pack_front<LG, true, ELL>), grid, dim3(256), 0, st, qdig, rows, r_pre, bpart, twist,
+                           dftw, C);
     else
-        hipLaunchKernelGGL((k_pack_front<LG, false, ELL>), grid, dim3(256), 0, st, digits, rows, slot, r_pre, bpart,
-                           twist, dftw, C);
+        hipLaunchKernelGGL((k_pack_front<LG, false, ELL>), grid, dim3(256), 0, st, qdig, rows, r_pre, bpart, twist,
+                           dftw, C);
     return hipGetLastError();
 }
 
 template <int LG>
-static hipError_t front_lg(const uint32_t* digits, const uint32_t* rows, const int32_t* slot, uint32_t* r_pre,
-                           uint64_t* bpart, const uint32_t* twist, const uint2* dftw, uint32_t Q, bool rw,
-                           const CtxDev& C, hipStream_t st) {
+static hipError_t front_lg(const uint64_t* qdig, const uint32_t* rows, uint32_t* r_pre, uint64_t* bpart,
+                           const uint32_t* twist, const uint2* dftw, uint32_t Q, bool rw, const CtxDev& C,
+                           hipStream_t st) {
     switch (C.ell) {
-        case 1: return front_t<LG, 1>(digits, rows, slot, r_pre, bpart, twist, dftw, Q, rw, C, st);
-        case 2: return front_t<LG, 2>(digits, rows, slot, r_pre, bpart, twist, dftw, Q, rw, C, st);
-        case 3: return front_t<LG, 3>(digits, rows, slot, r_pre, bpart, twist, dftw, Q, rw, C, st);
-        case 4: return front_t<LG, 4>(digits, rows, slot, r_pre, bpart, twist, dftw, Q, rw, C, st);
+        case 1: return front_t<LG, 1>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st);
+        case 2: return front_t<LG, 2>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st);
+        case 3: return front_t<LG, 3>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st);
+        case 4: return front_t<LG, 4>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st);
         default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_pack_front(const uint32_t* digits, const uint32_t* rows, const int32_t* slot, uint32_t* r_pre,
-                             uint64_t* bpart, const uint32_t* twist, const uint2* dftw, uint32_t Q, uint32_t gamma,
-                             bool rw, const CtxDev& C, hipStream_t st) {
+hipError_t launch_pack_front(const uint64_t* qdig, const uint32_t* rows, uint32_t* r_pre, uint64_t* bpart,
+                             const uint32_t* twist, const uint2* dftw, uint32_t Q, uint32_t gamma, bool rw,
+                             const CtxDev& C, hipStream_t st) {
     if (!Q) return hipSuccess;
     switch (gamma) {
-        case 1: return front_lg<0>(digits, rows, slot, r_pre, bpart, twist, dftw, Q, rw, C, st);
-        case 2: return front_lg<1>(digits, rows, slot, r_pre, bpart, twist, dftw, Q, rw, C, st);
-        case 4: return front_lg<2>(digits, rows, slot, r_pre, bpart, twist, dftw, Q, rw, C, st);
-        case 8: return front_lg<3>(digits, rows, slot, r_pre, bpart, twist, dftw, Q, rw, C, st);
-        case 16: return front_lg<4>(digits, rows, slot, r_pre, bpart, twist, dftw, Q, rw, C, st);
-        case 32: return front_lg<5>(digits, rows, slot, r_pre, bpart, twist, dftw, Q, rw, C, st);
-        case 64: return front_lg<6>(digits, rows, slot, r_pre, bpart, twist, dftw, Q, rw, C, st);
+        case 1: return front_lg<0>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st);
+        case 2: return front_lg<1>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st);
+        case 4: return front_lg<2>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st);
+        case 8: return front_lg<3>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st);
+        case 16: return front_lg<4>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st);
+        case 32: return front_lg<5>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st);
+        case 64: return front_lg<6>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st);
         default: return hipErrorInvalidValue;
     }
 }

@@ -28,7 +28,7 @@ const char* const kProfNames[P_COUNT] = {"rows_ntt", "expand_seeds", "first_dim"
                                          "pack_front"};
 
 ProfScope::ProfScope(inspire_ctx* ctx, int id) : ctx_(ctx), id_(id) {
-    if (!ctx_->prof.on) return;
+    if (!ctx_->prof.on || !((ctx_->prof.mask >> id) & 1u)) return;
     auto& P = ctx_->prof;
     if (P.pool.empty()) {
         hipEvent_t e;
@@ -102,16 +102,22 @@ static void db_digits(inspire_ctx* ctx, const uint32_t* h, size_t np, uint32_t* 
     hip_check(launch_ntt(dig, np * ctx->ell, ctx->dev, scale_none(), false, ctx->stream), "db ntt");
 }
 
-static void db_begin(Database& db, uint32_t num_shards, const uint32_t* shard_ids, uint32_t cols) {
+uint64_t shard_digits(const Database& db, uint32_t slot) {
+    return reinterpret_cast<uint64_t>(db.digits.get<uint32_t>() + (size_t)slot * db.digit_words_per_shard);
+}
+
+static void db_begin(const inspire_ctx* ctx, Database& db, uint32_t num_shards, const uint32_t* shard_ids,
+                     uint32_t cols) {
     db.num_shards = num_shards;
     db.cols = cols;
+    db.digit_words_per_shard = (size_t)cols * ctx->ell * ctx->K * ctx->N;
     for (uint32_t s = 0; s < num_shards; s++) db.slot_of.emplace(shard_ids ? shard_ids[s] : s, s);  // first id wins
 }
 
 void db_load(inspire_ctx* ctx, uint32_t db_id, uint32_t num_shards, const uint32_t* shard_ids,
              uint32_t polys_per_shard, const uint64_t* polys) {
     Database db;
-    db_begin(db, num_shards, shard_ids, polys_per_shard);
+    db_begin(ctx, db, num_shards, shard_ids, polys_per_shard);
     const size_t KN = (size_t)ctx->K * ctx->N;
     const size_t P = (size_t)num_shards * polys_per_shard;
     ctx->dbs.erase(db_id);  // release the old copy before allocating the new one
@@ -144,7 +150,7 @@ void db_load_entries(inspire_ctx* ctx, uint32_t db_id, const uint8_t* data, uint
     const uint32_t nshards = (uint32_t)nshards64;
     const size_t KN = (size_t)ctx->K * d;
     Database db;
-    db_begin(db, nshards, nullptr, cols);
+    db_begin(ctx, db, nshards, nullptr, cols);
     ctx->dbs.erase(db_id);
     db.digits.alloc((size_t)nshards * cols * ctx->ell * KN * 4);
     // slices of whole shards, bounded by a 256 MiB residue staging buffer
@@ -224,14 +230,12 @@ void stage_rows(inspire_ctx* ctx, uint32_t Q, uint32_t* rows_ws, const uint32_t*
 }
 
 // InsPIRe^0 on staged rows: out [Q][1 + cols][2][K][N] (combined, then columns).
-void run_respond0(inspire_ctx* ctx, const Database& db, uint32_t Q, const uint32_t* rows, const int32_t* d_slot,
+void run_respond0(inspire_ctx* ctx, uint32_t cols, uint32_t Q, const uint32_t* rows, const uint64_t* d_qdig,
                   uint32_t* out) {
-    const uint32_t cols = db.cols;
     {
         ProfScope ps(ctx, P_FIRST_DIM);
         // polys per query: 2 (combined) + 2 per column; column c at polys 2 + 2c, 3 + 2c
-        hip_check(launch_first_dim(db.digits.get<uint32_t>(), rows, d_slot, out, Q, cols, 2 * (1 + cols), 2, ctx->dev,
-                                   ctx->stream),
+        hip_check(launch_first_dim(d_qdig, rows, out, Q, cols, 2 * (1 + cols), 2, ctx->dev, ctx->stream),
                   "first dim");
     }
     {
@@ -263,7 +267,7 @@ void fail(int32_t* status, uint32_t i, int code, const std::string& msg) {
 // when the query must be computed; nullptr when it finished here (an error,
 // or an empty shard answered with zeros).
 static const Database* admit(inspire_ctx* ctx, const inspire_query& q, uint32_t v, inspire_response* r,
-                             int32_t* status, uint32_t i, bool direct_inspiring, int32_t* slot_out) {
+                             int32_t* status, uint32_t i, bool direct_inspiring, uint64_t* dig_out) {
     const size_t KN = (size_t)ctx->K * ctx->N;
     r->num_column_ciphertexts = 0;
     if (v > INSPIRE_VARIANT_TWO_PACKING) { fail(status, i, INSPIRE_ERR_INVALID_ARGUMENT, "unknown variant"); return nullptr; }
@@ -283,8 +287,9 @@ static const Database* admit(inspire_ctx* ctx, const inspire_query& q, uint32_t 
         fail(status, i, INSPIRE_ERR_PACKING_KEYS_MISSING, "InspiRING client packing keys missing from query");
         return nullptr;
     }
-    auto it = ctx->dbs.find(q.db_id);
-    if (it == ctx->dbs.end()) {
+    auto& dbs = root(ctx)->dbs;
+    auto it = dbs.find(q.db_id);
+    if (it == dbs.end()) {
         fail(status, i, INSPIRE_ERR_UNKNOWN_DATABASE, "unknown database " + std::to_string(q.db_id));
         return nullptr;
     }
@@ -314,28 +319,28 @@ static const Database* admit(inspire_ctx* ctx, const inspire_query& q, uint32_t 
         fail(status, i, INSPIRE_ERR_CAPACITY, "column_ciphertexts capacity too small");
         return nullptr;
     }
-    *slot_out = (int32_t)sit->second;
+    *dig_out = shard_digits(db, sit->second);
     return &db;
 }
 
 void respond_batch(inspire_ctx* ctx, uint32_t n, const inspire_query* qs, const uint32_t* variants,
                    inspire_response* rs, int32_t* status, bool direct_inspiring) {
     const size_t KN = (size_t)ctx->K * ctx->N, ell = ctx->ell;
-    std::map<uint32_t, std::vector<BatchItem>> groups;
+    std::map<uint32_t, std::vector<BatchItem>> groups;  // by column count: one launch spans databases
     for (uint32_t i = 0; i < n; i++) {
         status[i] = INSPIRE_OK;
         const inspire_query& q = qs[i];
         inspire_response& r = rs[i];
         if (!r.ciphertext) { fail(status, i, INSPIRE_ERR_INVALID_ARGUMENT, "response ciphertext buffer is null"); continue; }
         if (!q.rgsw || (q.seeded && !q.rgsw_seeds)) { fail(status, i, INSPIRE_ERR_INVALID_ARGUMENT, "query rgsw is null"); continue; }
-        int32_t slot = -1;
-        if (!admit(ctx, q, variants[i], &r, status, i, direct_inspiring, &slot)) continue;
-        groups[q.db_id].push_back(BatchItem{i, slot, variants[i] != INSPIRE_VARIANT_NO_PACKING});
+        uint64_t dig = 0;
+        const Database* db = admit(ctx, q, variants[i], &r, status, i, direct_inspiring, &dig);
+        if (!db) continue;
+        groups[db->cols].push_back(BatchItem{i, dig, variants[i] != INSPIRE_VARIANT_NO_PACKING});
     }
 
     for (auto& kv : groups) {
-        const Database& db = ctx->dbs.at(kv.first);
-        const uint32_t cols = db.cols;
+        const uint32_t cols = kv.first;
         for (int pass = 0; pass < 2; pass++) {  // 0: InsPIRe^0, 1: InspiRING
             std::vector<BatchItem> items;
             for (const BatchItem& it : kv.second)
@@ -344,18 +349,18 @@ void respond_batch(inspire_ctx* ctx, uint32_t n, const inspire_query* qs, const 
             const uint32_t Q = (uint32_t)items.size();
             // ---- stage rows ell..2ell-1 (the a-digits of the trivial encryption are 0)
             ctx->ws1.reserve((size_t)Q * ell * 2 * KN * 4);
-            ctx->ws4.reserve((size_t)Q * (ell * 8 + 1) * 4 + 64);
+            ctx->ws4.reserve((size_t)Q * (ell * 8 + 2) * 4 + 64);
             uint32_t* rows = ctx->ws1.get<uint32_t>();
-            int32_t* d_slot = ctx->ws4.get<int32_t>();
-            uint32_t* d_seeds = ctx->ws4.get<uint32_t>() + Q;
+            uint64_t* d_qdig = ctx->ws4.get<uint64_t>();
+            uint32_t* d_seeds = reinterpret_cast<uint32_t*>(d_qdig + Q);
             uint32_t* d_err = d_seeds + (size_t)Q * ell * 8;
             std::vector<uint32_t> hrows((size_t)Q * ell * 2 * KN, 0);
             std::vector<uint32_t> hseeds((size_t)Q * ell * 8, 0);
-            std::vector<int32_t> hslot(Q);
+            std::vector<uint64_t> hdig(Q);
             std::vector<std::pair<uint32_t, uint32_t>> runs;
             for (uint32_t t = 0; t < Q; t++) {
                 const inspire_query& q = qs[items[t].idx];
-                hslot[t] = items[t].slot;
+                hdig[t] = items[t].dig;
                 if (q.seeded) {
                     if (!runs.empty() && runs.back().second == t) runs.back().second = t + 1;
                     else runs.push_back({t, t + 1});
@@ -374,7 +379,7 @@ void respond_batch(inspire_ctx* ctx, uint32_t n, const inspire_query* qs, const 
                 }
             }
             hip_check(hipMemcpyAsync(rows, hrows.data(), hrows.size() * 4, hipMemcpyHostToDevice, ctx->stream), "rows");
-            hip_check(hipMemcpyAsync(d_slot, hslot.data(), Q * 4, hipMemcpyHostToDevice, ctx->stream), "slots");
+            hip_check(hipMemcpyAsync(d_qdig, hdig.data(), Q * 8, hipMemcpyHostToDevice, ctx->stream), "digits");
             hip_check(hipMemcpyAsync(d_seeds, hseeds.data(), hseeds.size() * 4, hipMemcpyHostToDevice, ctx->stream),
                       "seeds");
             hip_check(hipMemsetAsync(d_err, 0, 4, ctx->stream), "err");
@@ -383,7 +388,7 @@ void respond_batch(inspire_ctx* ctx, uint32_t n, const inspire_query* qs, const 
                 const size_t per = (size_t)(1 + cols) * 2 * KN;
                 ctx->ws2.reserve(Q * per * 4);
                 uint32_t* out = ctx->ws2.get<uint32_t>();
-                run_respond0(ctx, db, Q, rows, d_slot, out);
+                run_respond0(ctx, cols, Q, rows, d_qdig, out);
                 std::vector<uint32_t> hout(Q * per);
                 uint32_t herr = 0;
                 hip_check(hipMemcpyAsync(hout.data(), out, hout.size() * 4, hipMemcpyDeviceToHost, ctx->stream), "out");
@@ -400,7 +405,7 @@ void respond_batch(inspire_ctx* ctx, uint32_t n, const inspire_query* qs, const 
                         for (size_t w = 0; w < (size_t)cols * 2 * KN; w++) r.column_ciphertexts[w] = src[2 * KN + w];
                 }
             } else {
-                pack_respond_host(ctx, db, items.data(), Q, rows, d_slot, d_err, qs, rs, status);
+                pack_respond_host(ctx, cols, items.data(), Q, rows, d_qdig, d_err, qs, rs, status);
             }
         }
     }
@@ -421,21 +426,23 @@ void respond_batch_device(inspire_ctx* ctx, const inspire_device_batch& b) {
     const bool packed = b.variant != INSPIRE_VARIANT_NO_PACKING;
     if (packed && !b.y_body)
         throw Error{INSPIRE_ERR_PACKING_KEYS_MISSING, "InspiRING client packing keys missing from query"};
-    auto it = ctx->dbs.find(b.db_id);
-    if (it == ctx->dbs.end()) throw Error{INSPIRE_ERR_UNKNOWN_DATABASE, "unknown database " + std::to_string(b.db_id)};
+    auto& dbs = root(ctx)->dbs;
+    auto it = dbs.find(b.db_id);
+    if (it == dbs.end()) throw Error{INSPIRE_ERR_UNKNOWN_DATABASE, "unknown database " + std::to_string(b.db_id)};
     const Database& db = it->second;
     if (db.cols == 0) throw Error{INSPIRE_ERR_UNSUPPORTED, "device batches need a database with columns"};
     // slots (a query whose shard is missing gets slot -1 and status SHARD_NOT_FOUND)
-    ctx->ws4.reserve((size_t)Q * 4 + 64);
-    int32_t* d_slot = ctx->ws4.get<int32_t>();
-    uint32_t* d_err = reinterpret_cast<uint32_t*>(d_slot + Q);
-    std::vector<int32_t> hslot(Q);
+    ctx->ws4.reserve((size_t)Q * 8 + 64);
+    uint64_t* d_qdig = ctx->ws4.get<uint64_t>();
+    uint32_t* d_err = reinterpret_cast<uint32_t*>(d_qdig + Q);
+    std::vector<uint64_t> hdig(Q);
     for (uint32_t t = 0; t < Q; t++) {
         auto sit = db.slot_of.find(b.shard_ids[t]);
-        hslot[t] = sit == db.slot_of.end() ? -1 : (int32_t)sit->second;
-        b.status[t] = hslot[t] < 0 ? INSPIRE_ERR_SHARD_NOT_FOUND : INSPIRE_OK;
+        const bool found = sit != db.slot_of.end();
+        hdig[t] = shard_digits(db, found ? sit->second : 0);  // a missing shard reads shard 0, reported below
+        b.status[t] = found ? INSPIRE_OK : INSPIRE_ERR_SHARD_NOT_FOUND;
     }
-    hip_check(hipMemcpyAsync(d_slot, hslot.data(), Q * 4, hipMemcpyHostToDevice, ctx->stream), "slots");
+    hip_check(hipMemcpyAsync(d_qdig, hdig.data(), Q * 8, hipMemcpyHostToDevice, ctx->stream), "digits");
     hip_check(hipMemsetAsync(d_err, 0, 4, ctx->stream), "err");
     // rows: the caller's rows into the workspace layout
     ctx->ws1.reserve((size_t)Q * ell * 2 * KN * 4);
@@ -451,8 +458,8 @@ void respond_batch_device(inspire_ctx* ctx, const inspire_device_batch& b) {
                   "rows");
     }
     stage_rows(ctx, Q, rows, b.seeds, runs, d_err);
-    if (!packed) run_respond0(ctx, db, Q, rows, d_slot, b.out);
-    else pack_run_device_ybody(ctx, db, Q, rows, d_slot, b.y_body, b.out);
+    if (!packed) run_respond0(ctx, db.cols, Q, rows, d_qdig, b.out);
+    else pack_run_device_ybody(ctx, db.cols, Q, rows, d_qdig, b.y_body, b.out);
     uint32_t herr = 0;
     hip_check(hipMemcpyAsync(&herr, d_err, 4, hipMemcpyDeviceToHost, ctx->stream), "err");
     hip_check(hipStreamSynchronize(ctx->stream), "device batch");

@@ -1,6 +1,8 @@
 // Respond-path drivers shared by the C ABI entry points.
 #pragma once
 #include <cstdint>
+#include <utility>
+#include <vector>
 
 #include "context.hpp"
 
@@ -29,6 +31,19 @@ void external_product_batch(inspire_ctx* ctx, uint64_t count, const uint64_t* rl
 // dispatch, respond.rs:284-398) rather than respond_with_variant.
 void respond_batch(inspire_ctx* ctx, uint32_t n, const inspire_query* qs, const uint32_t* variants,
                    inspire_response* rs, int32_t* status, bool direct_inspiring);
+
+// RGSW rows ell..2ell-1 staged as [Q][ell][2][K][N] coefficient words: the
+// a-slots of the seeded runs [first, end) are expanded from seeds
+// ([Q][ell][8] words), then every row is transformed (NTT, Montgomery form).
+void stage_rows(inspire_ctx* ctx, uint32_t Q, uint32_t* rows_ws, const uint32_t* seeds,
+                const std::vector<std::pair<uint32_t, uint32_t>>& seeded_runs, uint32_t* d_err);
+
+// Device address of the digits of the shard in `slot`.
+uint64_t shard_digits(const Database& db, uint32_t slot);
+
+// InsPIRe^0 on staged NTT-domain rows: out [Q][1 + cols][2][K][N].
+void run_respond0(inspire_ctx* ctx, uint32_t cols, uint32_t Q, const uint32_t* rows, const uint64_t* d_qdig,
+                  uint32_t* out);
 
 // Same work for a batch whose inputs are resident in device memory.
 void respond_batch_device(inspire_ctx* ctx, const inspire_device_batch& b);

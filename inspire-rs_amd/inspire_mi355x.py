@@ -39,6 +39,7 @@ ERR_UNKNOWN_DATABASE = 8
 ERR_OUT_OF_MEMORY = 9
 ERR_DEVICE = 10
 ERR_CAPACITY = 11
+ERR_PENDING = 12
 
 NO_PACKING, ONE_PACKING, TWO_PACKING = 0, 1, 2  # InspireVariant (src/params.rs:78-103)
 PACKING_INSPIRING, PACKING_TREE = 0, 1  # PackingMode (src/pir/query.rs:27-40)
@@ -106,6 +107,40 @@ class _Response(C.Structure):
     ]
 
 
+class _ServerConfig(C.Structure):
+    _fields_ = [
+        ("lanes", C.c_uint32),
+        ("max_batch", C.c_uint32),
+        ("hbm_budget", C.c_uint64),
+        ("max_wait_us", C.c_uint32),
+        ("reserved", C.c_uint32),
+    ]
+
+
+class _Request(C.Structure):
+    _fields_ = [
+        ("db_id", C.c_uint32),
+        ("shard_id", C.c_uint32),
+        ("variant", C.c_uint32),
+        ("seeded", C.c_uint32),
+        ("client_id", C.c_uint32),
+        ("reserved", C.c_uint32),
+        ("rows", C.c_void_p),
+        ("seeds", C.c_void_p),
+        ("arrival_ns", C.c_uint64),
+    ]
+
+
+class _ServerStats(C.Structure):
+    _fields_ = [
+        ("requests", C.c_uint64),
+        ("batches", C.c_uint64),
+        ("max_batch_seen", C.c_uint64),
+        ("peak_inflight_bytes", C.c_uint64),
+        ("peak_lanes_busy", C.c_uint64),
+    ]
+
+
 _u64p = C.POINTER(C.c_uint64)
 
 # (name, restype, argtypes) of every entry point in include/inspire_mi355x.h
@@ -128,7 +163,18 @@ EXPORTS = [
         [C.c_void_p, C.c_uint32, C.POINTER(_Query), C.POINTER(C.c_uint32), C.POINTER(_Response), C.POINTER(C.c_int32)],
     ),
     ("inspire_respond_batch_device", C.c_int, [C.c_void_p, C.POINTER(_DeviceBatch)]),
+    ("inspire_register_client_keys", C.c_int, [C.c_void_p, C.c_uint32, _u64p]),
+    ("inspire_server_create", C.c_int, [C.c_void_p, C.POINTER(_ServerConfig), C.POINTER(C.c_void_p)]),
+    ("inspire_server_submit", C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(_Request), _u64p]),
+    (
+        "inspire_server_wait",
+        C.c_int,
+        [C.c_void_p, C.c_uint64, C.c_int32, C.POINTER(_Response), C.POINTER(C.c_int32), _u64p],
+    ),
+    ("inspire_server_get_stats", C.c_int, [C.c_void_p, C.POINTER(_ServerStats)]),
+    ("inspire_server_destroy", None, [C.c_void_p]),
     ("inspire_ctx_set_profiling", C.c_int, [C.c_void_p, C.c_int]),
+    ("inspire_ctx_set_profiling_kernels", C.c_int, [C.c_void_p, C.c_char_p]),
     (
         "inspire_ctx_profile",
         C.c_int,
@@ -295,6 +341,12 @@ class Context:
                                              entry_size))
         self._dbs[db_id] = (entry_size * 8 + 15) // 16 if (entry_size and len(data)) else 0
 
+    def register_client_keys(self, client_id: int, y_body: np.ndarray):
+        """Resident packing keys of a client: y_body of its ClientPackingKeys
+        (src/inspiring/inspiring2.rs:603-628), uint64[ell, K, d]."""
+        yb = np.ascontiguousarray(y_body, dtype=np.uint64)
+        _raise(lib().inspire_register_client_keys(self._h, client_id, _ptr64(yb)))
+
     def unload_database(self, db_id: int):
         _raise(lib().inspire_db_unload(self._h, db_id))
         self._dbs.pop(db_id, None)
@@ -383,6 +435,10 @@ class Context:
     def set_profiling(self, on: bool):
         _raise(lib().inspire_ctx_set_profiling(self._h, 1 if on else 0))
 
+    def set_profiling_kernels(self, names=None):
+        """Bracket only these kernels (an iterable of names); None: all."""
+        _raise(lib().inspire_ctx_set_profiling_kernels(self._h, ",".join(names).encode() if names else None))
+
     def profile(self, kernel: str, reset: bool = False):
         """(total milliseconds, launches) of one kernel since the last reset."""
         ms = C.c_double()
@@ -411,6 +467,88 @@ class Context:
         out = np.zeros((count, 2, self.K, self.d), dtype=np.uint64)
         _raise(lib().inspire_external_product(self._h, count, _ptr64(rl), _ptr64(rg), _ptr64(out)))
         return out
+
+
+@dataclass
+class Request:
+    """A query to the serving loop (inspire_request): rows ell..2ell-1 as
+    uint32 (seeded: b polys [ell, K, d] with seeds uint8[ell, 32]; full:
+    [ell, 2, K, d]).  Packed variants name a registered client."""
+
+    db_id: int
+    shard_id: int
+    variant: int
+    rows: np.ndarray
+    seeds: Optional[np.ndarray] = None
+    client_id: int = 0
+    arrival_ns: int = 0
+
+
+class Server:
+    """Continuous-batching serving loop over a Context (inspire_server_*)."""
+
+    def __init__(self, ctx: Context, lanes: int = 2, max_batch: int = 256, hbm_budget: int = 0,
+                 max_wait_us: int = 0):
+        self.ctx = ctx
+        cfg = _ServerConfig(lanes, max_batch, hbm_budget, max_wait_us, 0)
+        self._h = C.c_void_p()
+        _raise(lib().inspire_server_create(ctx._h, C.byref(cfg), C.byref(self._h)))
+
+    def submit(self, reqs: Sequence[Request]) -> np.ndarray:
+        n = len(reqs)
+        arr = (_Request * n)()
+        keep = []
+        for i, r in enumerate(reqs):
+            rows = np.ascontiguousarray(r.rows, dtype=np.uint32)
+            keep.append(rows)
+            seeds = None
+            if r.seeds is not None:
+                seeds = np.ascontiguousarray(r.seeds, dtype=np.uint8)
+                keep.append(seeds)
+            arr[i] = _Request(r.db_id, r.shard_id, r.variant, 1 if seeds is not None else 0, r.client_id, 0,
+                              rows.ctypes.data, seeds.ctypes.data if seeds is not None else None, r.arrival_ns)
+        tickets = np.zeros(n, dtype=np.uint64)
+        _raise(lib().inspire_server_submit(self._h, n, arr, _ptr64(tickets)))
+        return tickets
+
+    def wait(self, ticket: int, cols: int = 0, timeout_ms: int = -1):
+        """-> (status, ServerResponse or None, done_ns).  cols: column
+        capacity for InsPIRe^0 responses."""
+        K, d = self.ctx.K, self.ctx.d
+        ct = np.zeros((2, K, d), dtype=np.uint64)
+        cc = np.zeros((max(cols, 1), 2, K, d), dtype=np.uint64)
+        r = _Response(_ptr64(ct), _ptr64(cc) if cols else None, cols, 0)
+        st = C.c_int32()
+        done = C.c_uint64()
+        rc = lib().inspire_server_wait(self._h, int(ticket), timeout_ms, C.byref(r), C.byref(st), C.byref(done))
+        if rc == ERR_PENDING:
+            return ERR_PENDING, None, 0
+        _raise(rc)
+        if st.value != OK:
+            return st.value, None, done.value
+        return OK, ServerResponse(ct, cc[: r.num_column_ciphertexts]), done.value
+
+    def stats(self) -> dict:
+        s = _ServerStats()
+        _raise(lib().inspire_server_get_stats(self._h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in _ServerStats._fields_}
+
+    def close(self):
+        if self._h:
+            lib().inspire_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 class DeviceArray:
