@@ -148,6 +148,14 @@ def main():
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (a known-size kernel for "
                          "calibrating rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--serve", action="store_true",
+                    help="measure the serving loop instead: open-loop Poisson arrivals over 64 databases")
+    ap.add_argument("--serve-log2", type=int, default=28, help="total entries of the serving databases (log2)")
+    ap.add_argument("--serve-dbs", type=int, default=64)
+    ap.add_argument("--serve-rates", type=str, default="100,300,1000,3000,10000,30000,100000")
+    ap.add_argument("--serve-lanes", type=int, default=2)
+    ap.add_argument("--serve-max-batch", type=int, default=256)
+    ap.add_argument("--serve-pool", type=int, default=256, help="distinct queries (with CPU answers) drawn from")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the respond path shards by database across GPUs with no collective, "
@@ -158,6 +166,9 @@ def main():
 
     log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)
     w_seed = bytes(range(32))
+    if args.serve:
+        print(json.dumps(serve_main(args, M, log, w_seed)))
+        return
     rng = np.random.default_rng(2026)
     n_entries = 1 << args.db_log2
     nshards = n_entries // D
@@ -328,6 +339,216 @@ def main():
     dout.free()
     ctx.close()
     print(json.dumps(line))
+
+
+SERVE_METRIC = ("serving loop: queries/s + p50/p99 respond latency, InsPIRe^0/^1/^2 at 10/30/60 %, "
+                "open-loop Poisson arrivals, 2^28 x 32 B in 64 databases")
+
+
+def serve_sizes(rng, total_log2: int, ndb: int):
+    """Log-uniform sizes in [2^20, 2^26] entries, scaled to a total of
+    2^total_log2, in whole shards of D entries."""
+    s = 2.0 ** rng.uniform(20, 26, ndb)
+    target = (1 << total_log2) // D
+    shards = np.maximum(1, np.round(s * target / s.sum())).astype(np.int64)
+    shards[np.argmax(shards)] += target - shards.sum()
+    return shards * D
+
+
+def serve_main(args, M, log, w_seed: bytes):
+    """Open-loop Poisson arrivals into the serving loop (inspire_server_*)
+    over databases of uneven size: throughput and p50/p99 latency per offered
+    rate.  Queries are drawn from a pool of synthetic queries of the real
+    shape; the CPU leg computes the oracle's answer for each pool query (and
+    times it), and every served response is compared with that answer."""
+    import threading
+
+    rng = np.random.default_rng(2028)
+    sizes = serve_sizes(rng, args.serve_log2, args.serve_dbs)
+    total = int(sizes.sum())
+    log(f"serving databases: {len(sizes)} of {sizes.min()}..{sizes.max()} entries, total {total}")
+    raw = np.frombuffer(rng.bytes(total * 32), dtype=np.uint8)
+    offs = np.concatenate([[0], np.cumsum(sizes)]) * 32
+    ctx = M.Context(M.InspireParams(), w_seed)
+    t0 = time.perf_counter()
+    for i in range(len(sizes)):
+        ctx.load_entries(i, raw[offs[i]:offs[i + 1]], 32)
+    log(f"resident: {ctx.device_bytes / 2**30:.1f} GiB, load {time.perf_counter() - t0:.1f} s")
+
+    # clients with resident packing keys, and the query pool (synthetic, real shapes)
+    moduli = np.array([Q0, Q1], dtype=np.uint64)[:, None]
+    n_clients = 16
+    ybodies = [(rng.integers(0, 2**63, (ELL, K, D), dtype=np.uint64) % moduli) for _ in range(n_clients)]
+    for c, yb in enumerate(ybodies):
+        ctx.register_client_keys(c, yb)
+    P = args.serve_pool
+    pool = []
+    for i in range(P):
+        u = rng.random()
+        variant = 0 if u < 0.1 else (1 if u < 0.4 else 2)
+        db = int(rng.integers(0, len(sizes)))
+        shard = int(rng.integers(0, sizes[db] // D))
+        client = int(rng.integers(0, n_clients))
+        if variant == 1:  # full rows ell..2ell-1 (a, b)
+            rows = (rng.integers(0, 2**63, (ELL, 2, K, D), dtype=np.uint64) % moduli).astype(np.uint32)
+            seeds = None
+        else:  # seeded
+            rows = (rng.integers(0, 2**63, (ELL, K, D), dtype=np.uint64) % moduli).astype(np.uint32)
+            seeds = rng.integers(0, 256, (ELL, 32), dtype=np.uint8)
+        pool.append(dict(db=db, shard=shard, variant=variant, client=client, rows=rows, seeds=seeds))
+
+    # CPU leg: the oracle's respond / respond_inspiring for every pool query
+    cpu = serve_cpu_answers(pool, raw, offs, ybodies, w_seed, log)
+
+    structs, keep = [], []
+    for q in pool:
+        st, k = M.Server.request_struct(M.Request(q["db"], q["shard"], q["variant"], rows=q["rows"],
+                                                  seeds=q["seeds"], client_id=q["client"]), borrow=True)
+        structs.append(st)
+        keep.append(k)
+    results = []
+    rates = [float(x) for x in args.serve_rates.split(",") if x]
+    mismatches_total, checked_total = 0, 0
+    with M.Server(ctx, lanes=args.serve_lanes, max_batch=args.serve_max_batch) as srv:
+        for lam in rates:
+            n = int(min(50000, max(200, lam * 1.0)))
+            pick = rng.integers(0, P, n)
+            gaps = rng.exponential(1.0 / lam, n)
+            t_start = time.monotonic_ns() + 20_000_000
+            arrival = (t_start + np.cumsum(gaps) * 1e9).astype(np.int64)
+            tickets = np.zeros(n, dtype=np.uint64)
+            done = np.zeros(n, dtype=np.int64)
+            status = np.zeros(n, dtype=np.int32)
+            bad = [0]
+            submitted = [0]
+
+            def waiter():
+                for j in range(n):
+                    while submitted[0] <= j:
+                        time.sleep(50e-6)
+                    q = pool[pick[j]]
+                    st, r, dn = srv.wait(int(tickets[j]), cols=16 if q["variant"] == 0 else 0)
+                    status[j] = st
+                    done[j] = dn
+                    want = cpu[pick[j]]
+                    if st != M.OK or not np.array_equal(r.ciphertext, want[0]) or (
+                            q["variant"] == 0 and not np.array_equal(r.column_ciphertexts, want[1])):
+                        bad[0] += 1
+
+            th = threading.Thread(target=waiter)
+            th.start()
+            i = 0
+            while i < n:
+                now = time.monotonic_ns()
+                k = int(np.searchsorted(arrival, now, side="right"))
+                if k > i:
+                    arr = (M.RequestStruct * (k - i))()
+                    for j in range(i, k):
+                        s = structs[pick[j]]
+                        arr[j - i] = s
+                        arr[j - i].arrival_ns = int(arrival[j])
+                    tickets[i:k] = srv.submit_structs(arr, k - i)
+                    i = k
+                    submitted[0] = k
+                else:
+                    dt = (arrival[i] - now) / 1e9
+                    if dt > 300e-6:
+                        time.sleep(dt - 200e-6)
+            th.join()
+            lat = (done - arrival) / 1e6
+            span = (done.max() - arrival[0]) / 1e9
+            res = {
+                "offered_qps": lam,
+                "queries": n,
+                "achieved_qps": round(n / span, 1),
+                "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                "bit_exact": f"{n - bad[0]}/{n}",
+            }
+            mismatches_total += bad[0]
+            checked_total += n
+            results.append(res)
+            log(f"offered {lam:>8.0f} q/s: achieved {res['achieved_qps']:>9.0f} q/s, p50 {res['p50_ms']} ms, "
+                f"p99 {res['p99_ms']} ms, bit-exact {res['bit_exact']}")
+        stats = srv.stats()
+    ctx.close()
+    peak = max(results, key=lambda r: r["achieved_qps"])
+    return {
+        "metric": SERVE_METRIC,
+        "value": peak["achieved_qps"],
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {
+            "workload": "serving loop: open-loop Poisson arrivals, InsPIRe^0/^1/^2 at 10/30/60 %, "
+                        f"{len(sizes)} databases of log-uniform size totalling 2^{args.serve_log2} x 32 B, "
+                        f"{n_clients} clients with resident packing keys",
+            "db_entries_min": int(sizes.min()),
+            "db_entries_max": int(sizes.max()),
+            "db_entries_total": total,
+            "lanes": args.serve_lanes,
+            "max_batch": args.serve_max_batch,
+        },
+        "rates": results,
+        "bit_exact": f"{checked_total - mismatches_total}/{checked_total}",
+        "server": stats,
+        "cpu_baseline": cpu["baseline"],
+    }
+
+
+def serve_cpu_answers(pool, raw, offs, ybodies, w_seed: bytes, log):
+    """The oracle (C restatement of the reference respond) on this host's
+    cores for every pool query: its answers check the served responses, and
+    its time is the CPU baseline of the mix."""
+    sys.path.insert(0, ROOT)
+    import oracle as O
+
+    o = O.Oracle(O.baseline_params())
+    o.pack(GAMMA, w_seed)
+    threads = O.lib.or_num_threads()
+
+    def full_rows(q):
+        rg = np.zeros((2 * ELL, 2, K, D), dtype=np.uint64)
+        for r in range(ELL):
+            if q["variant"] == 1:
+                rg[ELL + r] = q["rows"][r]
+            else:
+                rg[ELL + r, 0] = o.poly_from_seed(q["seeds"][r].tobytes())
+                rg[ELL + r, 1] = q["rows"][r]
+        return rg
+
+    def shard_polys(q):
+        base = offs[q["db"]] + q["shard"] * D * 32
+        return o.encode_shard(raw[base:base + D * 32], 32, 0)
+
+    answers = {}
+    packed = [i for i, q in enumerate(pool) if q["variant"] != 0]
+    zero = [i for i, q in enumerate(pool) if q["variant"] == 0]
+    ins = [(shard_polys(pool[i]), full_rows(pool[i]), ybodies[pool[i]["client"]]) for i in packed]
+    t0 = time.perf_counter()
+    rc, outs = o.respond_inspiring_many(w_seed, [x[0] for x in ins], [x[1] for x in ins], [x[2] for x in ins])
+    t_cpu = time.perf_counter() - t0
+    for j, i in enumerate(packed):
+        assert rc[j] == 0
+        answers[i] = (outs[j], None)
+    for i in zero:
+        sp, rg = shard_polys(pool[i]), full_rows(pool[i])
+        t0 = time.perf_counter()
+        ct, cols = o.respond(sp, rg)
+        t_cpu += time.perf_counter() - t0
+        answers[i] = (ct, cols)
+    log(f"cpu answers: {len(pool)} queries ({len(zero)} InsPIRe^0) in {t_cpu:.1f} s")
+    answers["baseline"] = {
+        "value": round(len(pool) / t_cpu, 1),
+        "unit": "queries/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"the {len(pool)} queries of the serving pool (every served query is one of them): packed ones "
+                  f"{threads} at a time, InsPIRe^0 one at a time; {t_cpu:.1f} s",
+    }
+    return answers
 
 
 if __name__ == "__main__":

@@ -212,15 +212,21 @@ typedef struct {
  *           seeds = their 32-byte seeds, uint8_t[ell][32]
  *   full:   rows = RGSW rows ell..2ell-1, uint32_t[ell][2][K][d]
  * (rows 0..ell-1 multiply the zero a-digits of the database and do not
- * affect the response).  Packed variants use the registered keys of
- * client_id.  arrival_ns: the query's arrival on CLOCK_MONOTONIC (0: now);
- * latencies are measured from it. */
+ * affect the response), unless flags has INSPIRE_REQUEST_BORROW.  Packed
+ * variants use the registered keys of client_id.  arrival_ns: the query's
+ * arrival on CLOCK_MONOTONIC (0: now); latencies are measured from it. */
 typedef struct {
-    uint32_t db_id, shard_id, variant, seeded, client_id, reserved;
+    uint32_t db_id, shard_id, variant, seeded, client_id;
+    uint32_t flags;        /* INSPIRE_REQUEST_* */
     const uint32_t *rows;
     const uint8_t *seeds;
     uint64_t arrival_ns;
 } inspire_request;
+
+/* inspire_request.flags: the caller keeps rows and seeds unchanged until the
+ * ticket's wait returns; the loop reads them in place instead of copying at
+ * submit. */
+#define INSPIRE_REQUEST_BORROW 0x1u
 
 typedef struct {
     uint64_t requests;          /* completed */

@@ -85,6 +85,10 @@ struct SReq {
     uint32_t db_id = 0, shard_id = 0, variant = 0, seeded = 0, client_id = 0;
     std::vector<uint32_t> rows;   // seeded: [ell][K][N] b polys; full: [ell][2][K][N]
     uint32_t seeds[kMaxEll * 8] = {};
+    const uint32_t* borrowed = nullptr;  // the caller's rows when borrowed
+    const uint8_t* borrowed_seeds = nullptr;
+    const uint32_t* rows_ptr() const { return borrowed ? borrowed : rows.data(); }
+    const void* seeds_ptr() const { return borrowed_seeds ? (const void*)borrowed_seeds : (const void*)seeds; }
     uint64_t arrival = 0, done_ns = 0;
     int32_t status = INSPIRE_OK;
     std::string err;
@@ -232,10 +236,10 @@ void run_batch(inspire_server* s, Lane& L, std::vector<std::shared_ptr<SReq>>& b
             reinterpret_cast<uint64_t*>(pin + g.o_dig)[t] = r->dig;
             reinterpret_cast<uint64_t*>(pin + g.o_yoff)[t] = r->yoff;
             if (t < g.S) {
-                std::memcpy(pin + g.o_seeds + t * ell * 32, r->seeds, ell * 32);
-                std::memcpy(pin + g.o_b + t * ell * KN * 4, r->rows.data(), ell * KN * 4);
+                std::memcpy(pin + g.o_seeds + t * ell * 32, r->seeds_ptr(), ell * 32);
+                std::memcpy(pin + g.o_b + t * ell * KN * 4, r->rows_ptr(), ell * KN * 4);
             } else {
-                std::memcpy(pin + g.o_full + (t - g.S) * ell * 2 * KN * 4, r->rows.data(), ell * 2 * KN * 4);
+                std::memcpy(pin + g.o_full + (t - g.S) * ell * 2 * KN * 4, r->rows_ptr(), ell * 2 * KN * 4);
             }
         }
         *reinterpret_cast<uint32_t*>(pin + g.o_err) = 0;
@@ -523,9 +527,14 @@ int inspire_server_submit(inspire_server* s, uint32_t n, const inspire_request* 
                     resolve(s, *r);
                 }
                 if (r->status == INSPIRE_OK) {
-                    const size_t words = (r->seeded ? 1 : 2) * ell * KN;
-                    r->rows.assign(q.rows, q.rows + words);
-                    if (r->seeded) std::memcpy(r->seeds, q.seeds, ell * 32);
+                    if (q.flags & INSPIRE_REQUEST_BORROW) {
+                        r->borrowed = q.rows;
+                        r->borrowed_seeds = r->seeded ? q.seeds : nullptr;
+                    } else {
+                        const size_t words = (r->seeded ? 1 : 2) * ell * KN;
+                        r->rows.assign(q.rows, q.rows + words);
+                        if (r->seeded) std::memcpy(r->seeds, q.seeds, ell * 32);
+                    }
                 }
                 made[i] = std::move(r);
             }

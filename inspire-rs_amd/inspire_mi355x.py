@@ -40,6 +40,7 @@ ERR_OUT_OF_MEMORY = 9
 ERR_DEVICE = 10
 ERR_CAPACITY = 11
 ERR_PENDING = 12
+REQUEST_BORROW = 0x1
 
 NO_PACKING, ONE_PACKING, TWO_PACKING = 0, 1, 2  # InspireVariant (src/params.rs:78-103)
 PACKING_INSPIRING, PACKING_TREE = 0, 1  # PackingMode (src/pir/query.rs:27-40)
@@ -124,7 +125,7 @@ class _Request(C.Structure):
         ("variant", C.c_uint32),
         ("seeded", C.c_uint32),
         ("client_id", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("flags", C.c_uint32),
         ("rows", C.c_void_p),
         ("seeds", C.c_void_p),
         ("arrival_ns", C.c_uint64),
@@ -141,6 +142,7 @@ class _ServerStats(C.Structure):
     ]
 
 
+RequestStruct = _Request  # inspire_request, for callers that prepare many at once
 _u64p = C.POINTER(C.c_uint64)
 
 # (name, restype, argtypes) of every entry point in include/inspire_mi355x.h
@@ -507,6 +509,23 @@ class Server:
                 keep.append(seeds)
             arr[i] = _Request(r.db_id, r.shard_id, r.variant, 1 if seeds is not None else 0, r.client_id, 0,
                               rows.ctypes.data, seeds.ctypes.data if seeds is not None else None, r.arrival_ns)
+        tickets = np.zeros(n, dtype=np.uint64)
+        _raise(lib().inspire_server_submit(self._h, n, arr, _ptr64(tickets)))
+        return tickets
+
+    @staticmethod
+    def request_struct(r: Request, borrow: bool = False):
+        """-> (inspire_request struct, arrays it points to).  With borrow, the
+        arrays must stay alive and unchanged until the ticket is waited."""
+        rows = np.ascontiguousarray(r.rows, dtype=np.uint32)
+        seeds = np.ascontiguousarray(r.seeds, dtype=np.uint8) if r.seeds is not None else None
+        st = _Request(r.db_id, r.shard_id, r.variant, 1 if seeds is not None else 0, r.client_id,
+                      REQUEST_BORROW if borrow else 0, rows.ctypes.data,
+                      seeds.ctypes.data if seeds is not None else None, r.arrival_ns)
+        return st, (rows, seeds)
+
+    def submit_structs(self, arr, n: int) -> np.ndarray:
+        """Submits n prepared inspire_request structs (a ctypes array)."""
         tickets = np.zeros(n, dtype=np.uint64)
         _raise(lib().inspire_server_submit(self._h, n, arr, _ptr64(tickets)))
         return tickets
