@@ -51,6 +51,23 @@ class DevBuf {
     size_t n_ = 0;
 };
 
+// Owning pinned host allocation (hipHostMalloc), grown on demand.
+class PinBuf {
+   public:
+    PinBuf() = default;
+    ~PinBuf() { release(); }
+    PinBuf(const PinBuf&) = delete;
+    PinBuf& operator=(const PinBuf&) = delete;
+    void reserve(size_t bytes);
+    void release();
+    template <typename T = uint8_t>
+    T* get() const { return reinterpret_cast<T*>(p_); }
+
+   private:
+    void* p_ = nullptr;
+    size_t n_ = 0;
+};
+
 // One resident EncodedDatabase.
 struct Database {
     uint32_t num_shards = 0;
@@ -126,6 +143,7 @@ struct inspire_ctx {
     inspire_ctx* owner = nullptr;
     // scratch reused across calls
     inspire::DevBuf ws0, ws1, ws2, ws3, ws4, ws5, ws6;
+    inspire::PinBuf pin;  // small per-batch metadata
     std::vector<uint32_t> hbuf;
     inspire::Profiler prof;
 };

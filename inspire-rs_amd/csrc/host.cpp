@@ -45,6 +45,23 @@ void DevBuf::release() {
     n_ = 0;
 }
 
+void PinBuf::reserve(size_t bytes) {
+    if (bytes <= n_) return;
+    release();
+    if (hipHostMalloc(&p_, bytes, hipHostMallocDefault) != hipSuccess) {
+        p_ = nullptr;
+        (void)hipGetLastError();
+        throw Error{INSPIRE_ERR_OUT_OF_MEMORY, "pinned host allocation of " + std::to_string(bytes) + " bytes failed"};
+    }
+    n_ = bytes;
+}
+
+void PinBuf::release() {
+    if (p_) (void)hipHostFree(p_);
+    p_ = nullptr;
+    n_ = 0;
+}
+
 void hip_check(hipError_t e, const char* what) {
     if (e != hipSuccess) throw Error{INSPIRE_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e)};
 }

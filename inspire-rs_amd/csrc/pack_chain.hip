@@ -43,6 +43,12 @@ struct Mod {
     uint32_t q, q2, qinv;
 };
 
+// Padded LDS index of element base + e*st, from the padded index of base.
+// For the radix-8 passes (st a power of two, base = an aligned block of 8*st
+// elements plus r < st) the padding of the eight elements differs from that
+// of base by a constant: the addresses are one register plus immediates.
+__device__ __forceinline__ int lds_step(int base_ix, int e, int st) { return base_ix + e * st + 4 * ((e * st) >> 5); }
+
 // a * b * 2^-32 mod q in [0, 2q), for a * b < q * 2^32.
 __device__ __forceinline__ uint32_t mmul(uint32_t a, uint32_t b, const Mod& M) {
     return mont_reduce((uint64_t)a * b, M.q, M.qinv);
@@ -119,10 +125,10 @@ __device__ __forceinline__ void intt1(uint32_t (&x)[8], int t, uint32_t* rb, con
 #pragma unroll
     for (int p = 0; p < L::R8; p++) {
         const int i3 = t / t0, r = t - i3 * t0;
-        const int base = 8 * t0 * i3 + r;
+        const int bix = lds_ix(8 * t0 * i3 + r);
         if (!(L::REM == 0 && p == 0)) {
 #pragma unroll
-            for (int e = 0; e < 8; e++) x[e] = rb[lds_ix(base + e * t0)];
+            for (int e = 0; e < 8; e++) x[e] = rb[lds_step(bix, e, t0)];
         }
         const int mA = N / (2 * t0), mB = N / (4 * t0), mC = N / (8 * t0);
 #pragma unroll
@@ -138,7 +144,7 @@ __device__ __forceinline__ void intt1(uint32_t (&x)[8], int t, uint32_t* rb, con
 #pragma unroll
             for (int k = 0; k < 4; k++) gs_m(x[k], x[k + 4], w, M);
 #pragma unroll
-            for (int e = 0; e < 8; e++) rb[lds_ix(base + e * t0)] = x[e];
+            for (int e = 0; e < 8; e++) rb[lds_step(bix, e, t0)] = x[e];
             lds_barrier();
         } else {
             // last stage (m = 1) with the output scale: X' = c (X + Y), Y' = cw (X - Y)
@@ -192,19 +198,19 @@ __device__ __forceinline__ void ntt_digits(uint32_t (&x)[ELL][8], int t, uint32_
     for (int p = 0; p < L::R8; p++) {
         const int t0 = N / (2 * m0), st = t0 >> 2;
         const int i = t / st, r = t - i * st;
-        const int base = 2 * i * t0 + r;
+        const int bix = lds_ix(2 * i * t0 + r);
         if (p > 0) {
 #pragma unroll
             for (int j = 0; j < ELL; j++)
 #pragma unroll
-                for (int e = 0; e < 8; e++) x[j][e] = db[j * LP + lds_ix(base + e * st)];
+                for (int e = 0; e < 8; e++) x[j][e] = db[j * LP + lds_step(bix, e, st)];
         }
         fwd_r8<ELL>(x, m0, i, tw, M);
         if (!(L::REM == 0 && p == L::R8 - 1)) {
 #pragma unroll
             for (int j = 0; j < ELL; j++)
 #pragma unroll
-                for (int e = 0; e < 8; e++) db[j * LP + lds_ix(base + e * st)] = x[j][e];
+                for (int e = 0; e < 8; e++) db[j * LP + lds_step(bix, e, st)] = x[j][e];
             lds_barrier();
         }
         m0 <<= 3;
@@ -253,7 +259,9 @@ struct ChainCrtShape {
                                      + (size_t)2 * ELL * LP * 4;  // digit-transform exchange, per residue
 };
 
-template <int LOGN, int ELL>
+// LAZY: ELL * 4q * q < q * 2^32, so the multiply-accumulate takes the
+// transform outputs (< 4q) without reducing them first.
+template <int LOGN, int ELL, bool LAZY>
 __global__ void __launch_bounds__(2 * NttShape<LOGN>::T, 1)
 k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
     using L = Lay<LOGN>;
@@ -338,17 +346,18 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
 
         // T[i] = G^-1(INTT(R[i+1])): this residue's coefficients, then the other's via LDS
         intt1<LOGN>(R, ts, rb, itw, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
+        const int cix = lds_ix(ts);
 #pragma unroll
         for (int e = 0; e < 8; e++) {
             R[e] = csub(R[e], M.q);
-            rb[lds_ix(ts + e * T)] = R[e];
+            rb[lds_step(cix, e, T)] = R[e];
         }
         lds_barrier();
         const uint32_t* ro = rbuf + (1 - k) * LP;
         uint32_t dg[ELL][8];
 #pragma unroll
         for (int e = 0; e < 8; e++) {
-            const uint32_t other = ro[lds_ix(ts + e * T)];
+            const uint32_t other = ro[lds_step(cix, e, T)];
             const uint32_t c0 = k == 0 ? R[e] : other, c1 = k == 0 ? other : R[e];
             const uint32_t a0m = csub(shoup_mul(c0, 1u, C.one_q1p, q1), q1);
             const uint32_t u = csub(shoup_mul(c1 + q1 - a0m, C.q0_inv_q1, C.q0_inv_q1p, q1), q1);
@@ -381,7 +390,7 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
             uint64_t ar = 0, ab = 0;
 #pragma unroll
             for (int j = 0; j < ELL; j++) {
-                const uint32_t x = csub(csub(dg[j][e], M.q2), M.q);
+                const uint32_t x = LAZY ? dg[j][e] : csub(csub(dg[j][e], M.q2), M.q);
                 ar += (uint64_t)x * w[j][e];
                 if (use_y) ab += (uint64_t)x * yv[j][e];
             }
@@ -422,17 +431,34 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
     }
 }
 
-template <int LOGN, int ELL>
-static hipError_t chain_crt_t(const PackArgs& A, uint32_t Q, const CtxDev& C, const ScaleDev& inv, hipStream_t st) {
+// Raises the kernel's dynamic LDS limit once per instantiation (a function-
+// local static is initialised exactly once, also under concurrent callers).
+template <int LOGN, int ELL, bool LAZY>
+static hipError_t chain_crt_lds_limit() {
+    static const hipError_t e =
+        ChainCrtShape<LOGN, ELL>::kBytes > 64 * 1024
+            ? hipFuncSetAttribute((const void*)k_pack_chain_crt<LOGN, ELL, LAZY>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)ChainCrtShape<LOGN, ELL>::kBytes)
+            : hipSuccess;
+    return e;
+}
+
+template <int LOGN, int ELL, bool LAZY>
+static hipError_t chain_crt_launch(const PackArgs& A, uint32_t Q, const CtxDev& C, const ScaleDev& inv,
+                                   hipStream_t st) {
+    const hipError_t e = chain_crt_lds_limit<LOGN, ELL, LAZY>();
+    if (e != hipSuccess) return e;
+    auto kern = k_pack_chain_crt<LOGN, ELL, LAZY>;
     constexpr size_t shmem = ChainCrtShape<LOGN, ELL>::kBytes;
-    auto kern = k_pack_chain_crt<LOGN, ELL>;
-    if (shmem > 64 * 1024) {
-        const hipError_t e =
-            hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
-        if (e != hipSuccess) return e;
-    }
     hipLaunchKernelGGL(kern, dim3(Q), dim3(2 * NttShape<LOGN>::T), shmem, st, A, C, inv);
     return hipGetLastError();
+}
+
+template <int LOGN, int ELL>
+static hipError_t chain_crt_t(const PackArgs& A, uint32_t Q, const CtxDev& C, const ScaleDev& inv, hipStream_t st) {
+    const uint64_t qmax = C.m[0].q > C.m[1].q ? C.m[0].q : C.m[1].q;
+    if ((uint64_t)ELL * 4 * qmax < (1ull << 32)) return chain_crt_launch<LOGN, ELL, true>(A, Q, C, inv, st);
+    return chain_crt_launch<LOGN, ELL, false>(A, Q, C, inv, st);
 }
 
 template <int LOGN>

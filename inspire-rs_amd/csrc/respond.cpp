@@ -229,6 +229,36 @@ void stage_rows(inspire_ctx* ctx, uint32_t Q, uint32_t* rows_ws, const uint32_t*
               "rows ntt");
 }
 
+// Rows from the caller's buffers straight into the workspace: queries [0, S)
+// are seeded (a expanded from seeds [S][ell][8], b read from bsrc
+// [S][ell][K][N]), queries [S, Q) bring full rows fsrc [Q-S][ell][2][K][N].
+// Every row is transformed (NTT, Montgomery form) from its source into
+// rows_ws [Q][ell][2][K][N]; no copy of the inputs is made.
+void stage_rows_from(inspire_ctx* ctx, uint32_t Q, uint32_t S, uint32_t* rows_ws, const uint32_t* seeds,
+                     const uint32_t* bsrc, const uint32_t* fsrc, uint32_t* d_err) {
+    const size_t KN = (size_t)ctx->K * ctx->N, ell = ctx->ell;
+    if (S) {
+        ProfScope ps(ctx, P_EXPAND);
+        hip_check(launch_expand_seeds(seeds, rows_ws, (size_t)S * ell, 2 * KN, d_err, ctx->dev, ctx->stream),
+                  "expand seeds");
+    }
+    uint64_t mont[kMaxK];
+    for (uint32_t k = 0; k < ctx->K; k++) mont[k] = mont_of(ctx, k);
+    const ScaleDev sc = scale_fwd(ctx, mont);
+    ProfScope ps(ctx, P_ROWS_NTT);
+    if (S) {
+        hip_check(launch_ntt_strided(rows_ws, 2 * KN, rows_ws, 2 * KN, (size_t)S * ell, ctx->dev, sc, false,
+                                     ctx->stream),
+                  "rows ntt a");
+        hip_check(launch_ntt_strided(bsrc, KN, rows_ws + KN, 2 * KN, (size_t)S * ell, ctx->dev, sc, false, ctx->stream),
+                  "rows ntt b");
+    }
+    if (Q > S)
+        hip_check(launch_ntt_strided(fsrc, KN, rows_ws + (size_t)S * ell * 2 * KN, KN, (size_t)(Q - S) * ell * 2,
+                                     ctx->dev, sc, false, ctx->stream),
+                  "rows ntt full");
+}
+
 // InsPIRe^0 on staged rows: out [Q][1 + cols][2][K][N] (combined, then columns).
 void run_respond0(inspire_ctx* ctx, uint32_t cols, uint32_t Q, const uint32_t* rows, const uint64_t* d_qdig,
                   uint32_t* out) {
@@ -432,39 +462,33 @@ void respond_batch_device(inspire_ctx* ctx, const inspire_device_batch& b) {
     const Database& db = it->second;
     if (db.cols == 0) throw Error{INSPIRE_ERR_UNSUPPORTED, "device batches need a database with columns"};
     // slots (a query whose shard is missing gets slot -1 and status SHARD_NOT_FOUND)
-    ctx->ws4.reserve((size_t)Q * 8 + 64);
+    // per-query digit addresses and the error word, staged in pinned memory
+    const size_t meta = (size_t)Q * 8 + 8;
+    ctx->ws4.reserve(meta + 64);
+    ctx->pin.reserve(meta + 8);
     uint64_t* d_qdig = ctx->ws4.get<uint64_t>();
     uint32_t* d_err = reinterpret_cast<uint32_t*>(d_qdig + Q);
-    std::vector<uint64_t> hdig(Q);
+    uint64_t* hdig = ctx->pin.get<uint64_t>();
     for (uint32_t t = 0; t < Q; t++) {
         auto sit = db.slot_of.find(b.shard_ids[t]);
         const bool found = sit != db.slot_of.end();
         hdig[t] = shard_digits(db, found ? sit->second : 0);  // a missing shard reads shard 0, reported below
         b.status[t] = found ? INSPIRE_OK : INSPIRE_ERR_SHARD_NOT_FOUND;
     }
-    hip_check(hipMemcpyAsync(d_qdig, hdig.data(), Q * 8, hipMemcpyHostToDevice, ctx->stream), "digits");
-    hip_check(hipMemsetAsync(d_err, 0, 4, ctx->stream), "err");
-    // rows: the caller's rows into the workspace layout
+    hdig[Q] = 0;  // the error word
+    hip_check(hipMemcpyAsync(d_qdig, hdig, meta, hipMemcpyHostToDevice, ctx->stream), "digits");
+    // rows: transformed from the caller's buffers into the workspace layout
     ctx->ws1.reserve((size_t)Q * ell * 2 * KN * 4);
     uint32_t* rows = ctx->ws1.get<uint32_t>();
-    std::vector<std::pair<uint32_t, uint32_t>> runs;
-    if (b.seeded) {
-        hip_check(hipMemcpy2DAsync(rows + KN, 2 * KN * 4, b.rows, KN * 4, KN * 4, (size_t)Q * ell,
-                                   hipMemcpyDeviceToDevice, ctx->stream),
-                  "rows b");
-        runs.push_back({0, Q});
-    } else {
-        hip_check(hipMemcpyAsync(rows, b.rows, (size_t)Q * ell * 2 * KN * 4, hipMemcpyDeviceToDevice, ctx->stream),
-                  "rows");
-    }
-    stage_rows(ctx, Q, rows, b.seeds, runs, d_err);
+    if (b.seeded) stage_rows_from(ctx, Q, Q, rows, b.seeds, b.rows, nullptr, d_err);
+    else stage_rows_from(ctx, Q, 0, rows, nullptr, nullptr, b.rows, d_err);
     if (!packed) run_respond0(ctx, db.cols, Q, rows, d_qdig, b.out);
     else pack_run_device_ybody(ctx, db.cols, Q, rows, d_qdig, b.y_body, b.out);
-    uint32_t herr = 0;
-    hip_check(hipMemcpyAsync(&herr, d_err, 4, hipMemcpyDeviceToHost, ctx->stream), "err");
+    uint32_t* herr = reinterpret_cast<uint32_t*>(ctx->pin.get<uint64_t>() + Q + 1);
+    hip_check(hipMemcpyAsync(herr, d_err, 4, hipMemcpyDeviceToHost, ctx->stream), "err");
     hip_check(hipStreamSynchronize(ctx->stream), "device batch");
     prof_collect(ctx);
-    if (herr) throw Error{INSPIRE_ERR_DEVICE, "seed expansion exhausted its draw budget"};
+    if (*herr) throw Error{INSPIRE_ERR_DEVICE, "seed expansion exhausted its draw budget"};
 }
 
 }  // namespace inspire

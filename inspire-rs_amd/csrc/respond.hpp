@@ -38,6 +38,11 @@ void respond_batch(inspire_ctx* ctx, uint32_t n, const inspire_query* qs, const 
 void stage_rows(inspire_ctx* ctx, uint32_t Q, uint32_t* rows_ws, const uint32_t* seeds,
                 const std::vector<std::pair<uint32_t, uint32_t>>& seeded_runs, uint32_t* d_err);
 
+// The same from the caller's buffers: queries [0, S) seeded (b in bsrc
+// [S][ell][K][N]), [S, Q) full rows fsrc [Q-S][ell][2][K][N].
+void stage_rows_from(inspire_ctx* ctx, uint32_t Q, uint32_t S, uint32_t* rows_ws, const uint32_t* seeds,
+                     const uint32_t* bsrc, const uint32_t* fsrc, uint32_t* d_err);
+
 // Device address of the digits of the shard in `slot`.
 uint64_t shard_digits(const Database& db, uint32_t slot);
 

@@ -51,33 +51,6 @@ uint64_t now_ns() {
         .count();
 }
 
-// Pinned host buffer (hipHostMalloc), grown on demand.
-class PinBuf {
-   public:
-    ~PinBuf() { release(); }
-    void reserve(size_t bytes) {
-        if (bytes <= n_) return;
-        release();
-        if (hipHostMalloc(&p_, bytes, hipHostMallocDefault) != hipSuccess) {
-            p_ = nullptr;
-            (void)hipGetLastError();
-            throw Error{INSPIRE_ERR_OUT_OF_MEMORY, "pinned host allocation of " + std::to_string(bytes) + " bytes failed"};
-        }
-        n_ = bytes;
-    }
-    template <typename T = uint8_t>
-    T* get() const { return reinterpret_cast<T*>(p_); }
-
-   private:
-    void release() {
-        if (p_) (void)hipHostFree(p_);
-        p_ = nullptr;
-        n_ = 0;
-    }
-    void* p_ = nullptr;
-    size_t n_ = 0;
-};
-
 }  // namespace
 
 struct SReq {
@@ -255,18 +228,10 @@ void run_batch(inspire_server* s, Lane& L, std::vector<std::shared_ptr<SReq>>& b
         Group& g = kv.second;
         const uint32_t Q = (uint32_t)g.reqs.size();
         uint32_t* grows = rows + g.o_rows;
-        if (g.S)
-            hip_check(hipMemcpy2DAsync(grows + KN, 2 * KN * 4, inbox + g.o_b, KN * 4, KN * 4, (size_t)g.S * ell,
-                                       hipMemcpyDeviceToDevice, st),
-                      "serve rows b");
-        if (Q > g.S)
-            hip_check(hipMemcpyAsync(grows + (size_t)g.S * ell * 2 * KN, inbox + g.o_full,
-                                     (size_t)(Q - g.S) * ell * 2 * KN * 4, hipMemcpyDeviceToDevice, st),
-                      "serve rows full");
-        std::vector<std::pair<uint32_t, uint32_t>> runs;
-        if (g.S) runs.push_back({0, g.S});
         uint32_t* d_err = reinterpret_cast<uint32_t*>(inbox + g.o_err);
-        stage_rows(ctx, Q, grows, reinterpret_cast<const uint32_t*>(inbox + g.o_seeds), runs, d_err);
+        stage_rows_from(ctx, Q, g.S, grows, reinterpret_cast<const uint32_t*>(inbox + g.o_seeds),
+                        reinterpret_cast<const uint32_t*>(inbox + g.o_b),
+                        reinterpret_cast<const uint32_t*>(inbox + g.o_full), d_err);
         const uint64_t* d_qdig = reinterpret_cast<const uint64_t*>(inbox + g.o_dig);
         if (g.packed)
             pack_run(ctx, g.cols, Q, grows, d_qdig, own->clients.keys.get<uint32_t>(),
