@@ -346,6 +346,93 @@ k_expand_seeds(const uint32_t* __restrict__ seeds, uint32_t* __restrict__ out, s
 }
 
 // ---------------------------------------------------------------------------
+// Seeded RGSW rows in one pass (d = 2048, 256 threads = the expansion's draw
+// lanes = the NTT's N/8): row i's a is expanded from its seed into LDS as
+// above, b is read from bsrc + i * KN, both are transformed and scaled to
+// Montgomery form, and written to rows + i * 2KN (a) and + KN (b).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_rows_seeded(const uint32_t* __restrict__ seeds, const uint32_t* __restrict__ bsrc, uint32_t* __restrict__ rows,
+              uint32_t* __restrict__ err, CtxDev C, ScaleDev sc) {
+    using S = NttShape<11>;
+    constexpr int N = S::N, LP = S::LDS;
+    __shared__ __align__(16) uint32_t poly[2 * kMaxK][LP];  // a residues, then b residues
+    __shared__ uint32_t sh[4];
+    __shared__ uint32_t s_last;
+    const int tid = threadIdx.x;
+    const size_t KN = (size_t)C.K * N;
+    uint32_t key[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) key[i] = seeds[(size_t)blockIdx.x * 8 + i];
+    const uint32_t* b = bsrc + (size_t)blockIdx.x * KN;
+    for (uint32_t k = 0; k < C.K; k++)
+#pragma unroll
+        for (int j = 0; j < 8; j++) poly[kMaxK + k][lds_ix(tid + j * S::T)] = b[(size_t)k * N + tid + j * S::T];
+    uint32_t k = 0, filled = 0;
+    uint64_t pos = 0;
+    for (int chunk = 0; chunk < kExpandMaxChunks && k < C.K; chunk++) {
+        const uint64_t c0 = (uint64_t)chunk * 2048;
+        uint32_t w[16];
+        chacha20_block(key, (uint64_t)chunk * 256 + tid, w);
+        uint64_t v[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) v[j] = (uint64_t)w[2 * j] | ((uint64_t)w[2 * j + 1] << 32);
+        while (k < C.K) {
+            const uint64_t q = C.m[k].q;
+            const uint64_t zone = (q << __clzll(q)) - 1;
+            uint32_t flags = 0, cnt = 0;
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const uint64_t idx = c0 + 8 * tid + j;
+                const bool ok = idx >= pos && v[j] * q <= zone;
+                flags |= (uint32_t)ok << j;
+                cnt += ok;
+            }
+            uint32_t excl;
+            const uint32_t total = block_scan256(cnt, excl, sh);
+            const uint32_t need = N - filled;
+            uint32_t rank = excl;
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                if (flags >> j & 1) {
+                    if (rank < need) poly[k][lds_ix(filled + rank)] = (uint32_t)__umul64hi(v[j], q);
+                    if (rank == need - 1) s_last = 8 * tid + j;
+                    rank++;
+                }
+            }
+            __syncthreads();
+            if (total >= need) {
+                pos = c0 + s_last + 1;
+                filled = 0;
+                k++;
+                __syncthreads();
+                continue;
+            }
+            filled += total;
+            pos = c0 + 2048;
+            break;
+        }
+    }
+    if (k < C.K) {  // draw budget exhausted (never in practice): report, write nothing
+        if (tid == 0) atomicOr(err, 1u);
+        return;
+    }
+    __syncthreads();
+    uint32_t* out = rows + (size_t)blockIdx.x * 2 * KN;
+    for (uint32_t kk = 0; kk < C.K; kk++) {
+        const ModDev& M = C.m[kk];
+        for (int ab = 0; ab < 2; ab++) {
+            uint32_t* s = poly[ab * kMaxK + kk];
+            ntt_fwd_lds<11>(s, tid, M, BlockSync{});
+            uint32_t* o = out + (size_t)ab * KN + (size_t)kk * N;
+#pragma unroll
+            for (int j = 0; j < 8; j++)
+                o[tid + j * S::T] = csub(shoup_mul(s[lds_ix(tid + j * S::T)], sc.c[kk], sc.cp[kk], M.q), M.q);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // encode_database (encode_db.rs:106-164) on the device: entry e of the byte
 // stream lands in shard e / N, local index e % N; column c is the LE u16 of
 // bytes (2c, 2c+1), a lone last byte, or 0 (extract_column_value,
@@ -443,6 +530,14 @@ hipError_t launch_sum_cols(uint32_t* resp, uint32_t Q, uint32_t cols, const CtxD
     const size_t total = (size_t)Q * 2 * C.K * C.n;
     if (!total) return hipSuccess;
     hipLaunchKernelGGL(k_sum_cols, dim3(blocks_for(total, 256)), dim3(256), 0, st, resp, Q, cols, C);
+    return hipGetLastError();
+}
+
+hipError_t launch_rows_seeded(const uint32_t* seeds, const uint32_t* bsrc, uint32_t* rows, size_t count,
+                              uint32_t* err, const CtxDev& C, const ScaleDev& sc, hipStream_t st) {
+    if (!count) return hipSuccess;
+    if (C.logn != 11 || count > 0x7fffffffu) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_rows_seeded, dim3((unsigned)count), dim3(256), 0, st, seeds, bsrc, rows, err, C, sc);
     return hipGetLastError();
 }
 

@@ -53,5 +53,10 @@ hipError_t launch_ext_mac(const uint32_t* dig, const uint32_t* rgsw, uint32_t* o
 hipError_t launch_expand_seeds(const uint32_t* seeds, uint32_t* out, size_t count, size_t out_stride, uint32_t* err,
                                const CtxDev& C, hipStream_t st);
 hipError_t launch_sum_cols(uint32_t* resp, uint32_t Q, uint32_t cols, const CtxDev& C, hipStream_t st);
+// d = 2048: seeded rows expanded and transformed in one pass.  Row i: seed
+// seeds + 8i, b at bsrc + i*K*N; out rows + i*2*K*N (a, then b), Montgomery
+// form via sc.  Returns hipErrorInvalidValue for other degrees.
+hipError_t launch_rows_seeded(const uint32_t* seeds, const uint32_t* bsrc, uint32_t* rows, size_t count,
+                              uint32_t* err, const CtxDev& C, const ScaleDev& sc, hipStream_t st);
 
 }  // namespace inspire

@@ -265,7 +265,7 @@ PackTables& pack_tables(inspire_ctx* ctx, uint32_t gamma) {
 // y_count (device arrays).  out: [Q][2][K][N].
 void pack_run(inspire_ctx* ctx, uint32_t gamma, uint32_t Q, const uint32_t* rows, const uint64_t* d_qdig,
               const uint32_t* d_y, const uint64_t* d_yoff, const uint32_t* d_ymode, const uint32_t* d_ycount,
-              uint32_t* d_out) {
+              uint32_t* d_out, bool y_coeff) {
     const uint32_t K = ctx->K;
     const size_t KN = (size_t)K * ctx->N;
     PackTables& T = pack_tables(ctx, gamma);
@@ -307,6 +307,7 @@ void pack_run(inspire_ctx* ctx, uint32_t gamma, uint32_t Q, const uint32_t* rows
     A.y_off = d_yoff;
     A.y_mode = d_ymode;
     A.y_count = d_ycount;
+    A.y_coeff = y_coeff ? 1u : 0u;
     A.bpart = d_bpart;
     A.nparts = nparts;
     A.out = d_out;
@@ -351,6 +352,11 @@ void pack_run_device_ybody(inspire_ctx* ctx, uint32_t gamma, uint32_t Q, const u
     for (uint32_t i = 0; i + 1 < gamma; i++)
         if (!(T.gen_pows[i] & 1))
             throw Error{INSPIRE_ERR_UNSUPPORTED, "device batches need an odd InspiRING generator (gamma | 2d)"};
+    // y layout: query t at t*ell*K*N, rotated from y_body, gamma-1 rotations (the kernel's default)
+    if (pack_chain_crt_supported(ctx->dev)) {  // the chain kernel transforms y_body itself
+        pack_run(ctx, gamma, Q, rows, d_qdig, y_body, nullptr, nullptr, nullptr, d_out, true);
+        return;
+    }
     ctx->ws0.reserve((size_t)Q * ell * KN * 4);
     uint32_t* d_y = ctx->ws0.get<uint32_t>();
     uint64_t mont[kMaxK];
@@ -361,8 +367,7 @@ void pack_run_device_ybody(inspire_ctx* ctx, uint32_t gamma, uint32_t Q, const u
                                      ctx->stream),
                   "y ntt");
     }
-    // y layout: query t at t*ell*K*N, rotated from y_body, gamma-1 rotations (the kernel's default)
-    pack_run(ctx, gamma, Q, rows, d_qdig, d_y, nullptr, nullptr, nullptr, d_out);
+    pack_run(ctx, gamma, Q, rows, d_qdig, d_y, nullptr, nullptr, nullptr, d_out, false);
 }
 
 void pack_respond_host(inspire_ctx* ctx, uint32_t gamma, const BatchItem* items, uint32_t Q, const uint32_t* rows,
@@ -427,7 +432,7 @@ void pack_respond_host(inspire_ctx* ctx, uint32_t gamma, const BatchItem* items,
     const size_t out_words = (size_t)Q * 2 * KN;
     DevBuf& outb = ctx->ws6;
     outb.reserve(out_words * 4);
-    pack_run(ctx, gamma, Q, rows, d_qdig, d_y, d_yoff, d_ymode, d_ycount, outb.get<uint32_t>());
+    pack_run(ctx, gamma, Q, rows, d_qdig, d_y, d_yoff, d_ymode, d_ycount, outb.get<uint32_t>(), false);
     std::vector<uint32_t> hout(out_words);
     uint32_t herr = 0;
     hip_check(hipMemcpyAsync(hout.data(), outb.get(), out_words * 4, hipMemcpyDeviceToHost, ctx->stream), "out");

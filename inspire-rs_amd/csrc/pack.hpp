@@ -24,6 +24,8 @@ struct PackArgs {
     const uint64_t* y_off;       // [Q] word offset of each query's y block
     const uint32_t* y_mode;      // [Q] 0: rotate y_body by perm, 1: explicit y_all
     const uint32_t* y_count;     // [Q] rotations available (y_all.len(), or gamma-1 for y_body)
+    uint32_t y_coeff;            // 1: y holds y_body in the coefficient domain (plain); the chain
+                                 //    kernel transforms it (two-modulus kernel only)
     const uint64_t* bpart;       // [Q][nparts][gamma][K] partial sums of bhat (b_poly = n^-1 * sum)
     uint32_t nparts;
     uint32_t ninv0, ninv1;       // n^-1 mod q_k
@@ -43,9 +45,10 @@ PackTables& pack_tables(inspire_ctx* ctx, uint32_t gamma);
 // out [Q][2][K][N].  y: NTT-domain Montgomery client keys with per-query
 // offsets, modes (0: y_body rotated on the fly, 1: explicit y_all) and counts.
 // d_qdig: per-query device addresses of the shard digits (databases of gamma columns).
+// y_coeff: d_y holds coefficient-domain y_body (only when pack_chain_crt_supported).
 void pack_run(inspire_ctx* ctx, uint32_t gamma, uint32_t Q, const uint32_t* rows, const uint64_t* d_qdig,
               const uint32_t* d_y, const uint64_t* d_yoff, const uint32_t* d_ymode, const uint32_t* d_ycount,
-              uint32_t* d_out);
+              uint32_t* d_out, bool y_coeff);
 // Device-resident y_body [Q][ell][K][N] (coefficient domain).
 void pack_run_device_ybody(inspire_ctx* ctx, uint32_t gamma, uint32_t Q, const uint32_t* rows,
                            const uint64_t* d_qdig, const uint32_t* y_body, uint32_t* d_out);

@@ -294,7 +294,27 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
         twf[e] = csub(shoup_mul(D.tw[i], C.mont[kk], C.montp[kk], D.q), D.q);
         twi[e] = csub(shoup_mul(D.itw[i], C.mont[kk], C.montp[kk], D.q), D.q);
     }
-    if (ymode == 0) {
+    if (ymode == 0 && A.y_coeff) {
+        // NTT of this query's y_body (both halves, one residue each), into
+        // LDS at the slot positions in Montgomery form
+        __syncthreads();
+        uint32_t yv[ELL][8];
+#pragma unroll
+        for (int j = 0; j < ELL; j++)
+#pragma unroll
+            for (int e = 0; e < 8; e++) yv[j][e] = Y[(j * 2 + k) * N + t + e * T];
+        ntt_digits<LOGN, ELL>(yv, t, db, tw, M);
+#pragma unroll
+        for (int j = 0; j < ELL; j++)
+#pragma unroll
+            for (int g = 0; g < 2; g++) {
+                uint32_t v[4];
+#pragma unroll
+                for (int c = 0; c < 4; c++)
+                    v[c] = csub(shoup_mul(yv[j][4 * g + c], C.mont[k], C.montp[k], M.q), M.q);
+                *reinterpret_cast<uint4*>(&yl[(j * 2 + k) * N + L::group(t, g)]) = make_uint4(v[0], v[1], v[2], v[3]);
+            }
+    } else if (ymode == 0) {
         for (int e = tid; e < ELL * 2 * N / 4; e += 2 * T)
             reinterpret_cast<uint4*>(yl)[e] = reinterpret_cast<const uint4*>(Y)[e];
     }

@@ -237,26 +237,32 @@ void stage_rows(inspire_ctx* ctx, uint32_t Q, uint32_t* rows_ws, const uint32_t*
 void stage_rows_from(inspire_ctx* ctx, uint32_t Q, uint32_t S, uint32_t* rows_ws, const uint32_t* seeds,
                      const uint32_t* bsrc, const uint32_t* fsrc, uint32_t* d_err) {
     const size_t KN = (size_t)ctx->K * ctx->N, ell = ctx->ell;
-    if (S) {
-        ProfScope ps(ctx, P_EXPAND);
-        hip_check(launch_expand_seeds(seeds, rows_ws, (size_t)S * ell, 2 * KN, d_err, ctx->dev, ctx->stream),
-                  "expand seeds");
-    }
     uint64_t mont[kMaxK];
     for (uint32_t k = 0; k < ctx->K; k++) mont[k] = mont_of(ctx, k);
     const ScaleDev sc = scale_fwd(ctx, mont);
-    ProfScope ps(ctx, P_ROWS_NTT);
-    if (S) {
+    if (S && ctx->logn == 11) {  // expansion and both transforms in one pass
+        ProfScope ps(ctx, P_EXPAND);
+        hip_check(launch_rows_seeded(seeds, bsrc, rows_ws, (size_t)S * ell, d_err, ctx->dev, sc, ctx->stream),
+                  "seeded rows");
+    } else if (S) {
+        {
+            ProfScope ps(ctx, P_EXPAND);
+            hip_check(launch_expand_seeds(seeds, rows_ws, (size_t)S * ell, 2 * KN, d_err, ctx->dev, ctx->stream),
+                      "expand seeds");
+        }
+        ProfScope ps(ctx, P_ROWS_NTT);
         hip_check(launch_ntt_strided(rows_ws, 2 * KN, rows_ws, 2 * KN, (size_t)S * ell, ctx->dev, sc, false,
                                      ctx->stream),
                   "rows ntt a");
         hip_check(launch_ntt_strided(bsrc, KN, rows_ws + KN, 2 * KN, (size_t)S * ell, ctx->dev, sc, false, ctx->stream),
                   "rows ntt b");
     }
-    if (Q > S)
+    if (Q > S) {
+        ProfScope ps(ctx, P_ROWS_NTT);
         hip_check(launch_ntt_strided(fsrc, KN, rows_ws + (size_t)S * ell * 2 * KN, KN, (size_t)(Q - S) * ell * 2,
                                      ctx->dev, sc, false, ctx->stream),
                   "rows ntt full");
+    }
 }
 
 // InsPIRe^0 on staged rows: out [Q][1 + cols][2][K][N] (combined, then columns).

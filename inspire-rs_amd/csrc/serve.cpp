@@ -235,7 +235,7 @@ void run_batch(inspire_server* s, Lane& L, std::vector<std::shared_ptr<SReq>>& b
         const uint64_t* d_qdig = reinterpret_cast<const uint64_t*>(inbox + g.o_dig);
         if (g.packed)
             pack_run(ctx, g.cols, Q, grows, d_qdig, own->clients.keys.get<uint32_t>(),
-                     reinterpret_cast<const uint64_t*>(inbox + g.o_yoff), nullptr, nullptr, out + g.o_out);
+                     reinterpret_cast<const uint64_t*>(inbox + g.o_yoff), nullptr, nullptr, out + g.o_out, false);
         else
             run_respond0(ctx, g.cols, Q, grows, d_qdig, out + g.o_out);
     }
