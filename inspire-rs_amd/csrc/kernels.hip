@@ -368,15 +368,24 @@ k_rows_seeded(const uint32_t* __restrict__ seeds, const uint32_t* __restrict__ b
     for (uint32_t k = 0; k < C.K; k++)
 #pragma unroll
         for (int j = 0; j < 8; j++) poly[kMaxK + k][lds_ix(tid + j * S::T)] = b[(size_t)k * N + tid + j * S::T];
+    // Draws are generated in chunks of up to 256 ChaCha blocks (8 draws per
+    // thread).  For the last residue the chunk is sized to the draws still
+    // needed plus a margin (a short chunk just continues in the next one), so
+    // the rejected draws of the last residue do not cost a whole chunk.
     uint32_t k = 0, filled = 0;
-    uint64_t pos = 0;
+    uint64_t pos = 0, c0 = 0;  // first unconsumed draw; first draw of the chunk
     for (int chunk = 0; chunk < kExpandMaxChunks && k < C.K; chunk++) {
-        const uint64_t c0 = (uint64_t)chunk * 2048;
+        uint32_t active = 256;
+        if (k == C.K - 1) {
+            const uint32_t need_now = N - filled;
+            active = min(256u, (need_now + need_now / 8 + 64 + 7) / 8);
+        }
+        const bool on = (uint32_t)tid < active;
         uint32_t w[16];
-        chacha20_block(key, (uint64_t)chunk * 256 + tid, w);
+        if (on) chacha20_block(key, c0 / 8 + tid, w);
         uint64_t v[8];
 #pragma unroll
-        for (int j = 0; j < 8; j++) v[j] = (uint64_t)w[2 * j] | ((uint64_t)w[2 * j + 1] << 32);
+        for (int j = 0; j < 8; j++) v[j] = on ? ((uint64_t)w[2 * j] | ((uint64_t)w[2 * j + 1] << 32)) : 0;
         while (k < C.K) {
             const uint64_t q = C.m[k].q;
             const uint64_t zone = (q << __clzll(q)) - 1;
@@ -384,7 +393,7 @@ k_rows_seeded(const uint32_t* __restrict__ seeds, const uint32_t* __restrict__ b
 #pragma unroll
             for (int j = 0; j < 8; j++) {
                 const uint64_t idx = c0 + 8 * tid + j;
-                const bool ok = idx >= pos && v[j] * q <= zone;
+                const bool ok = on && idx >= pos && v[j] * q <= zone;
                 flags |= (uint32_t)ok << j;
                 cnt += ok;
             }
@@ -409,9 +418,10 @@ k_rows_seeded(const uint32_t* __restrict__ seeds, const uint32_t* __restrict__ b
                 continue;
             }
             filled += total;
-            pos = c0 + 2048;
+            pos = c0 + 8 * (uint64_t)active;
             break;
         }
+        c0 += 8 * (uint64_t)active;
     }
     if (k < C.K) {  // draw budget exhausted (never in practice): report, write nothing
         if (tid == 0) atomicOr(err, 1u);
