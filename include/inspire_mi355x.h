@@ -130,6 +130,13 @@ int inspire_db_load(inspire_ctx *ctx, uint32_t db_id, uint32_t num_shards, const
  * by inspire_db_load; shards hold d entries (setup.rs:149-153). */
 int inspire_db_load_entries(inspire_ctx *ctx, uint32_t db_id, const uint8_t *data, uint64_t len,
                             uint32_t entry_size);
+/* load_shard_binary (src/pir/mmap.rs:55-85) of every shard-*.bin file in
+ * dir, the files save_shards_binary writes (mmap.rs:19-52): the shards are
+ * ingested into device memory once, where the reference's respond_mmap*
+ * (src/pir/respond.rs:492-640) read one from disk per query.  Every file must
+ * match the context's ring dimension and moduli and hold the same number of
+ * polynomials. */
+int inspire_db_load_shard_files(inspire_ctx *ctx, uint32_t db_id, const char *dir);
 int inspire_db_unload(inspire_ctx *ctx, uint32_t db_id);
 
 /* ---- respond (src/pir/respond.rs) ---- */

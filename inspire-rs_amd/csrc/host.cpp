@@ -422,6 +422,16 @@ int inspire_db_load_entries(inspire_ctx* ctx, uint32_t db_id, const uint8_t* dat
     })
 }
 
+int inspire_db_load_shard_files(inspire_ctx* ctx, uint32_t db_id, const char* dir) {
+    INSPIRE_TRY({
+        if (!ctx || !dir) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "null argument"};
+        std::lock_guard<std::recursive_mutex> g(ctx->mu);
+        std::unique_lock<std::shared_mutex> w(ctx->rw);
+        db_load_shard_files(ctx, db_id, dir);
+        return INSPIRE_OK;
+    })
+}
+
 int inspire_db_unload(inspire_ctx* ctx, uint32_t db_id) {
     INSPIRE_TRY({
         if (!ctx) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "null argument"};

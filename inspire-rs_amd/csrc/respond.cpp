@@ -12,6 +12,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <filesystem>
+#include <fstream>
 #include <string>
 #include <vector>
 
@@ -131,6 +133,100 @@ void db_load(inspire_ctx* ctx, uint32_t db_id, uint32_t num_shards, const uint32
             db_digits(ctx, ctx->ws0.get<uint32_t>(), np, db.digits.get<uint32_t>() + p0 * ctx->ell * KN);
         }
         hip_check(hipStreamSynchronize(ctx->stream), "db load");
+    }
+    prof_collect(ctx);
+    ctx->dbs[db_id] = std::move(db);
+}
+
+// load_shard_binary (src/pir/mmap.rs:55-85) of every shard-*.bin file in a
+// directory (the layout save_shards_binary writes, mmap.rs:19-52): LE u32 id,
+// u32 polynomial count, then (count > 0) u32 ring_dim, u32 crt_count and u64
+// moduli, then the polynomials' residue-major u64 coefficients.  The shards
+// are ingested into device memory once instead of being read per query
+// (respond_mmap*, respond.rs:492-640).
+void db_load_shard_files(inspire_ctx* ctx, uint32_t db_id, const char* dir) {
+    namespace fs = std::filesystem;
+    struct File {
+        std::string path;
+        uint32_t id = 0, polys = 0;
+        size_t data_off = 0;
+    };
+    std::vector<File> files;
+    std::error_code ec;
+    fs::directory_iterator it(dir, ec);
+    if (ec) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, std::string("cannot read shard directory ") + dir};
+    for (const auto& e : it) {
+        const std::string name = e.path().filename().string();
+        if (name.rfind("shard-", 0) == 0 && name.size() > 4 && name.compare(name.size() - 4, 4, ".bin") == 0)
+            files.push_back(File{e.path().string()});
+    }
+    const size_t KN = (size_t)ctx->K * ctx->N;
+    auto bad = [](const std::string& f, const std::string& why) {
+        throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "shard file " + f + ": " + why};
+    };
+    uint32_t cols = 0;
+    bool first = true;
+    for (File& f : files) {
+        std::ifstream in(f.path, std::ios::binary);
+        uint32_t h[4] = {0, 0, 0, 0};
+        if (!in.read(reinterpret_cast<char*>(h), 8)) bad(f.path, "truncated header");
+        f.id = h[0];
+        f.polys = h[1];
+        if (!in.read(reinterpret_cast<char*>(h + 2), 8)) bad(f.path, "truncated header");
+        f.data_off = 16;
+        if (f.polys) {
+            if (h[2] != ctx->N) bad(f.path, "ring dimension " + std::to_string(h[2]) + " differs from the context's");
+            if (h[3] != ctx->K) bad(f.path, "CRT modulus count " + std::to_string(h[3]) + " differs from the context's");
+            for (uint32_t k = 0; k < ctx->K; k++) {
+                uint64_t m = 0;
+                if (!in.read(reinterpret_cast<char*>(&m), 8)) bad(f.path, "truncated header");
+                if (m != ctx->hm[k].q) bad(f.path, "CRT moduli differ from the context's");
+            }
+            f.data_off += 8ull * ctx->K;
+            const uint64_t want = f.data_off + (uint64_t)f.polys * KN * 8;
+            if (fs::file_size(f.path, ec) < want || ec) bad(f.path, "truncated polynomial data");
+        }
+        if (first) cols = f.polys;
+        else if (f.polys != cols)
+            throw Error{INSPIRE_ERR_UNSUPPORTED, "shards with different polynomial counts (resident shards share one layout)"};
+        first = false;
+    }
+    std::stable_sort(files.begin(), files.end(), [](const File& a, const File& b) { return a.id < b.id; });
+    std::vector<uint32_t> ids(files.size());
+    for (size_t i = 0; i < files.size(); i++) ids[i] = files[i].id;
+    Database db;
+    db_begin(ctx, db, (uint32_t)files.size(), ids.data(), cols);
+    ctx->dbs.erase(db_id);
+    const size_t P = files.size() * (size_t)cols;
+    if (P) {
+        db.digits.alloc(P * ctx->ell * KN * 4);
+        // slices of whole files, up to 64 MiB of u32 residues
+        const size_t per_file = (size_t)cols * KN;
+        const size_t files_per_slice = std::max<size_t>(1, (size_t)(16u << 20) / per_file);
+        ctx->ws0.reserve(std::min(files.size(), files_per_slice) * per_file * 4);
+        ctx->pin.reserve(std::min(files.size(), files_per_slice) * per_file * 4);
+        std::vector<uint64_t> raw(per_file);
+        for (size_t f0 = 0; f0 < files.size(); f0 += files_per_slice) {
+            const size_t nf = std::min(files_per_slice, files.size() - f0);
+            uint32_t* stage = ctx->pin.get<uint32_t>();
+            for (size_t j = 0; j < nf; j++) {
+                const File& f = files[f0 + j];
+                std::ifstream in(f.path, std::ios::binary);
+                in.seekg((std::streamoff)f.data_off);
+                if (!in.read(reinterpret_cast<char*>(raw.data()), (std::streamsize)(per_file * 8)))
+                    bad(f.path, "truncated polynomial data");
+                uint32_t* dst = stage + j * per_file;
+                for (size_t w = 0; w < per_file; w++) {
+                    const uint64_t q = ctx->hm[(w / ctx->N) % ctx->K].q;
+                    if (raw[w] >= q) bad(f.path, "a residue is not reduced");
+                    dst[w] = (uint32_t)raw[w];
+                }
+            }
+            hip_check(hipMemcpyAsync(ctx->ws0.get(), stage, nf * per_file * 4, hipMemcpyHostToDevice, ctx->stream),
+                      "shard upload");
+            db_digits(ctx, ctx->ws0.get<uint32_t>(), nf * cols, db.digits.get<uint32_t>() + f0 * cols * ctx->ell * KN);
+            hip_check(hipStreamSynchronize(ctx->stream), "shard slice");
+        }
     }
     prof_collect(ctx);
     ctx->dbs[db_id] = std::move(db);

@@ -22,6 +22,7 @@ void download_u64(inspire_ctx* ctx, const uint32_t* src, size_t words, uint64_t*
 void db_load(inspire_ctx* ctx, uint32_t db_id, uint32_t num_shards, const uint32_t* shard_ids,
              uint32_t polys_per_shard, const uint64_t* polys);
 void db_load_entries(inspire_ctx* ctx, uint32_t db_id, const uint8_t* data, uint64_t len, uint32_t entry_size);
+void db_load_shard_files(inspire_ctx* ctx, uint32_t db_id, const char* dir);
 
 void external_product_batch(inspire_ctx* ctx, uint64_t count, const uint64_t* rlwe, const uint64_t* rgsw,
                             uint64_t* out);

@@ -153,6 +153,7 @@ EXPORTS = [
     ("inspire_ctx_device_bytes", C.c_uint64, [C.c_void_p]),
     ("inspire_db_load", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32), C.c_uint32, _u64p]),
     ("inspire_db_load_entries", C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint8), C.c_uint64, C.c_uint32]),
+    ("inspire_db_load_shard_files", C.c_int, [C.c_void_p, C.c_uint32, C.c_char_p]),
     ("inspire_db_unload", C.c_int, [C.c_void_p, C.c_uint32]),
     ("inspire_respond", C.c_int, [C.c_void_p, C.POINTER(_Query), C.POINTER(_Response)]),
     ("inspire_respond_seeded", C.c_int, [C.c_void_p, C.POINTER(_Query), C.POINTER(_Response)]),
@@ -342,6 +343,18 @@ class Context:
         _raise(lib().inspire_db_load_entries(self._h, db_id, buf.ctypes.data_as(C.POINTER(C.c_uint8)), buf.size,
                                              entry_size))
         self._dbs[db_id] = (entry_size * 8 + 15) // 16 if (entry_size and len(data)) else 0
+
+    def load_shard_files(self, db_id: int, directory: str):
+        """Every shard-*.bin file of a directory in the layout of
+        save_shards_binary (src/pir/mmap.rs:19-52), ingested into HBM."""
+        _raise(lib().inspire_db_load_shard_files(self._h, db_id, os.fsencode(directory)))
+        cols = 0
+        for name in os.listdir(directory):  # every file holds the same count (checked by the library)
+            if name.startswith("shard-") and name.endswith(".bin"):
+                with open(os.path.join(directory, name), "rb") as f:
+                    cols = int(np.frombuffer(f.read(8), dtype="<u4")[1])
+                break
+        self._dbs[db_id] = cols
 
     def register_client_keys(self, client_id: int, y_body: np.ndarray):
         """Resident packing keys of a client: y_body of its ClientPackingKeys

@@ -64,3 +64,24 @@ def uniform_polys(o: O.Oracle, count: int, seed: int, lead=()) -> np.ndarray:
 
 def decrypt_columns(o: O.Oracle, sk, response_cols, ncols):
     return [int(o.decrypt(response_cols[c], sk)[0]) for c in range(ncols)]
+
+
+def save_shards_binary(directory, ids, shard_polys, moduli, d: int):
+    """Writes shard files in the layout of save_shards_binary
+    (src/pir/mmap.rs:19-52): per shard `shard-{id:04}.bin` holding LE u32 id,
+    u32 polynomial count, then (count > 0) u32 ring_dim, u32 crt_count, u64
+    moduli, then every polynomial's residue-major u64 coefficients."""
+    import os
+
+    os.makedirs(directory, exist_ok=True)
+    for sid, polys in zip(ids, shard_polys):
+        polys = np.ascontiguousarray(polys, dtype=np.uint64)
+        count = polys.shape[0] if polys.size else 0
+        with open(os.path.join(directory, f"shard-{sid:04d}.bin"), "wb") as f:
+            f.write(np.array([sid, count], dtype="<u4").tobytes())
+            if count:
+                f.write(np.array([d, len(moduli)], dtype="<u4").tobytes())
+                f.write(np.array(moduli, dtype="<u8").tobytes())
+                f.write(polys.astype("<u8").tobytes())
+            else:
+                f.write(np.array([0, 0], dtype="<u4").tobytes())
