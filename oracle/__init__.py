@@ -104,6 +104,24 @@ def _load():
              C.POINTER(_u64p), C.POINTER(_u64p), C.POINTER(C.c_int)],
         ),
         "or_num_threads": (C.c_int, []),
+        "or_sample_extract_coeff0": (None, [P, _u64p, _u64p, _u64p]),
+        "or_homomorphic_automorph": (None, [P, C.c_void_p, _u64p, C.c_uint32, _u64p, C.c_uint32, _u64p]),
+        "or_pack_lwes": (
+            C.c_uint32,
+            [P, C.c_void_p, C.c_uint32, _u64p, _u64p, _u64p, C.c_uint32, C.c_int, C.c_int, _u64p],
+        ),
+        "or_respond_one_packing": (
+            C.c_uint32,
+            [P, C.c_void_p, _u64p, C.c_uint32, _u64p, _u64p, C.c_uint32, C.c_int, C.c_int, _u64p],
+        ),
+        "or_respond_one_packing_many": (
+            None,
+            [P, C.c_void_p, C.c_uint32, C.POINTER(_u64p), C.c_uint32, C.POINTER(_u64p), _u64p, C.c_uint32,
+             C.POINTER(_u64p)],
+        ),
+        "or_automorphism_ks_matrix": (
+            None, [P, C.c_void_p, _u64p, C.c_uint32, C.POINTER(OrChacha), C.c_double, _u64p]),
+        "or_galois_keys": (None, [P, C.c_void_p, _u64p, C.POINTER(OrChacha), C.c_double, _u64p]),
         "or_chacha20_block": (None, [_u32p, C.c_uint64, C.c_uint64, _u32p]),
         "or_chacha_seed": (None, [C.POINTER(OrChacha), _u8p]),
         "or_chacha_next_u32": (C.c_uint32, [C.POINTER(OrChacha)]),
@@ -346,6 +364,83 @@ class Oracle:
         lib.or_respond_inspiring_many(C.byref(self.cp), self.ntt, pk, n, arr(keep[:n]), ncols, arr(keep[n:2 * n]),
                                       arr(keep[2 * n:]), arr(outs), rc)
         return list(rc), outs
+
+    # -- tree packing (inspiring/automorph_pack.rs)
+    @property
+    def log_d(self):
+        return self.params.d.bit_length() - 1
+
+    def sample_extract(self, ct):
+        """sample_extract_coeff0 (rlwe/enc.rs:195-208) -> (a [d] composite, b composite)."""
+        a = np.zeros(self.params.d, dtype=np.uint64)
+        b = np.zeros(1, dtype=np.uint64)
+        lib.or_sample_extract_coeff0(C.byref(self.cp), _p64(np.ascontiguousarray(ct, dtype=np.uint64)), _p64(a),
+                                     _p64(b))
+        return a, int(b[0])
+
+    def homomorphic_automorph(self, ct, t, ks):
+        ks = np.ascontiguousarray(ks, dtype=np.uint64)
+        out = self.zeros(2)
+        lib.or_homomorphic_automorph(C.byref(self.cp), self.ntt, _p64(np.ascontiguousarray(ct, dtype=np.uint64)), t,
+                                     _p64(ks), ks.shape[0], _p64(out))
+        return out
+
+    def _tree_out(self, words, bdim):
+        K, d = self.params.K, self.params.d
+        return words[:K * d].reshape(K, d).copy(), words[K * d:K * d + K * bdim].reshape(K, bdim).copy()
+
+    def pack_lwes(self, lwe_a, lwe_b, galois_keys, literal=False, residue_wise=None):
+        """pack_lwes (automorph_pack.rs:418-487) -> (a [K][d], b [K][bdim])."""
+        rw = self.residue_wise if residue_wise is None else residue_wise
+        K, d = self.params.K, self.params.d
+        la = np.ascontiguousarray(lwe_a, dtype=np.uint64).reshape(-1, d)
+        lb = np.ascontiguousarray(lwe_b, dtype=np.uint64).reshape(-1)
+        gk = np.ascontiguousarray(galois_keys, dtype=np.uint64)
+        words = np.zeros(3 * K * d, dtype=np.uint64)
+        bdim = lib.or_pack_lwes(C.byref(self.cp), self.ntt, la.shape[0], _p64(la), _p64(lb), _p64(gk), gk.shape[0],
+                                int(literal), int(rw), _p64(words))
+        return self._tree_out(words, bdim)
+
+    def respond_one_packing(self, shard_polys, rgsw, galois_keys, literal=False, residue_wise=None):
+        """respond_one_packing (respond.rs:213-268) -> (a [K][d], b [K][bdim]);
+        an empty shard gives the zero ciphertext (bdim = d)."""
+        rw = self.residue_wise if residue_wise is None else residue_wise
+        K, d = self.params.K, self.params.d
+        shard_polys = np.ascontiguousarray(shard_polys, dtype=np.uint64)
+        ncols = shard_polys.shape[0] if shard_polys.size else 0
+        gk = np.ascontiguousarray(galois_keys, dtype=np.uint64)
+        words = np.zeros(3 * K * d, dtype=np.uint64)
+        bdim = lib.or_respond_one_packing(C.byref(self.cp), self.ntt, _p64(shard_polys) if ncols else None, ncols,
+                                          _p64(np.ascontiguousarray(rgsw, dtype=np.uint64)), _p64(gk), gk.shape[0],
+                                          int(literal), int(rw), _p64(words))
+        return self._tree_out(words, bdim or d)
+
+    def respond_one_packing_many(self, shards, rgsws, galois_keys):
+        """Many respond_one_packing calls (reference b layout), one per OpenMP
+        thread at a time -> list of flat ciphertext words (a then b)."""
+        n = len(shards)
+        K, d = self.params.K, self.params.d
+        ncols = shards[0].shape[0]
+        keep = [np.ascontiguousarray(x, dtype=np.uint64) for x in list(shards) + list(rgsws)]
+        gk = np.ascontiguousarray(galois_keys, dtype=np.uint64)
+        outs = [np.zeros(3 * K * d, dtype=np.uint64) for _ in range(n)]
+        arr = lambda xs: (_u64p * n)(*[_p64(x) for x in xs])
+        lib.or_respond_one_packing_many(C.byref(self.cp), self.ntt, n, arr(keep[:n]), ncols, arr(keep[n:]), _p64(gk),
+                                        gk.shape[0], arr(outs))
+        bdim = 2 * d if K == 2 else d
+        return [o[:K * d + K * bdim] for o in outs]
+
+    def automorphism_ks_matrix(self, sk, t, r, sigma=6.4):
+        out = self.zeros(self.params.gadget_len, 2)
+        lib.or_automorphism_ks_matrix(C.byref(self.cp), self.ntt, _p64(np.ascontiguousarray(sk)), t, C.byref(r), sigma,
+                                      _p64(out))
+        return out
+
+    def galois_keys(self, sk, r, sigma=6.4):
+        """ServerCrs.galois_keys (setup.rs:167-173): [log_d][ell][2][K][d]."""
+        out = self.zeros(self.log_d, self.params.gadget_len, 2)
+        lib.or_galois_keys(C.byref(self.cp), self.ntt, _p64(np.ascontiguousarray(sk)), C.byref(r), sigma, _p64(out))
+        return out
 
     # -- automorphisms
     def automorph_table(self, t):

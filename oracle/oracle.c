@@ -945,6 +945,209 @@ int or_num_threads(void) {
 }
 
 /* ------------------------------------------------------------------ */
+/* Tree packing: inspiring/automorph_pack.rs, rlwe/galois.rs            */
+/* ------------------------------------------------------------------ */
+
+/* RlweCiphertext::add (rlwe/enc.rs:114-119) */
+static void ct_add(const or_params *pr, const uint64_t *x, const uint64_t *y, uint64_t *out) {
+    poly_add(pr, x, y, out);
+    poly_add(pr, x + psz(pr), y + psz(pr), out + psz(pr));
+}
+
+static int ct_is_zero(const or_params *pr, const uint64_t *ct) {
+    for (size_t x = 0; x < 2 * psz(pr); x++)
+        if (ct[x]) return 0;
+    return 1;
+}
+
+/* sample_extract_coeff0 (rlwe/enc.rs:195-208): a'_0 = a_0, a'_i = a_(d-i),
+ * b = b_0, all as composite values. */
+void or_sample_extract_coeff0(const or_params *pr, const uint64_t *ct, uint64_t *a_out, uint64_t *b_out) {
+    const uint32_t d = pr->d;
+    const uint64_t inv = crt_inv(pr);
+    a_out[0] = poly_coeff_inv(pr, ct, 0, inv);
+    for (uint32_t i = 1; i < d; i++) a_out[i] = poly_coeff_inv(pr, ct, d - i, inv);
+    *b_out = poly_coeff_inv(pr, ct + psz(pr), 0, inv);
+}
+
+/* homomorphic_automorph (automorph_pack.rs:44-78): tau_t on both parts
+ * (galois.rs:64-69), gadget digits of tau_t(a), then
+ * a' = sum_i D_i * ks[i].a and b' = tau_t(b) + sum_i D_i * ks[i].b, each
+ * product through mul_ntt (poly.rs:505-532). */
+void or_homomorphic_automorph(const or_params *pr, const or_ntt *c, const uint64_t *ct, uint32_t t,
+                              const uint64_t *ks, uint32_t ks_rows, uint64_t *out) {
+    const size_t P = psz(pr);
+    const uint32_t ell = pr->gadget_len;
+    uint64_t *ta = palloc(pr, 2), *dig = palloc(pr, ell), *tm = palloc(pr, 1);
+    or_apply_automorphism(pr, ct, t, ta);
+    or_apply_automorphism(pr, ct + P, t, ta + P);
+    or_gadget_decompose(pr, ta, dig);
+    uint64_t *ra = out, *rb = out + P;
+    memset(ra, 0, P * 8);
+    memcpy(rb, ta + P, P * 8);
+    for (uint32_t i = 0; i < ell && i < ks_rows; i++) {
+        const uint64_t *row = ks + (size_t)i * 2 * P;
+        or_poly_mul_ntt(pr, c, dig + (size_t)i * P, row, tm);
+        poly_add(pr, ra, tm, ra);
+        or_poly_mul_ntt(pr, c, dig + (size_t)i * P, row + P, tm);
+        poly_add(pr, rb, tm, rb);
+    }
+    free(ta); free(dig); free(tm);
+}
+
+typedef struct {
+    const or_params *pr;
+    const or_ntt *c;
+    const uint64_t *leaves;    /* [d][2][K][d] prepped RLWEs */
+    const uint8_t *leaf_nz;    /* leaf is nonzero; NULL in the literal recursion */
+    const uint64_t *keys;      /* galois keys [nkeys][ell][2][K][d] */
+    uint32_t nkeys, log_n;
+    const uint64_t *y, *neg_y; /* [log_d][K][d] YConstants (automorph_pack.rs:86-148) */
+} tree_ctx;
+
+/* Returns 1 when the node's ciphertext is exactly zero and was left unset
+ * (only when leaf_nz is given: a subtree of zero leaves packs to zero, since
+ * every step is linear with exact zero inputs). */
+static int pack_inner(const tree_ctx *T, uint32_t ell, uint32_t start, uint64_t *out) { /* :171-256 */
+    const or_params *pr = T->pr;
+    const size_t P = psz(pr), CT = 2 * P;
+    if (ell == 0) {
+        if (T->leaf_nz && !T->leaf_nz[start]) return 1;
+        memcpy(out, T->leaves + (size_t)start * CT, CT * 8);
+        return 0;
+    }
+    uint32_t step = 1u << (T->log_n - ell);
+    uint64_t *ev = palloc(pr, 2), *od = palloc(pr, 2);
+    int ez = pack_inner(T, ell - 1, start, ev);
+    int oz = pack_inner(T, ell - 1, start + step, od);
+    if (ez && oz) { free(ev); free(od); return 1; }
+    if (ez) memset(ev, 0, CT * 8);
+    if (oz) memset(od, 0, CT * 8);
+    const uint64_t *y = T->y + (size_t)(ell - 1) * P, *ny = T->neg_y + (size_t)(ell - 1) * P;
+    uint64_t *s0 = palloc(pr, 2), *s1 = palloc(pr, 2), *tm = palloc(pr, 2);
+    /* ct_sum_0 = even + y*odd, ct_sum_1 = even + (-y)*odd (RlweCiphertext::poly_mul, rlwe/enc.rs:147-152) */
+    or_poly_mul_ntt(pr, T->c, od, y, tm);
+    or_poly_mul_ntt(pr, T->c, od + P, y, tm + P);
+    ct_add(pr, ev, tm, s0);
+    or_poly_mul_ntt(pr, T->c, od, ny, tm);
+    or_poly_mul_ntt(pr, T->c, od + P, ny, tm + P);
+    ct_add(pr, ev, tm, s1);
+    uint32_t t = (1u << ell) + 1, ks_idx = T->nkeys - ell;
+    const size_t KS = (size_t)pr->gadget_len * CT;
+    or_homomorphic_automorph(pr, T->c, s1, t, T->keys + (size_t)ks_idx * KS, pr->gadget_len, tm);
+    ct_add(pr, s0, tm, out);
+    free(ev); free(od); free(s0); free(s1); free(tm);
+    return 0;
+}
+
+/* pack_lwes (automorph_pack.rs:418-487) of n LWEs: lwe_a [n][d] and lwe_b[n]
+ * composite values as sample_extract_coeff0 gives them.  ct_out receives a
+ * [K][d] followed by b [K][bdim]; the return value is bdim.  With two moduli
+ * the reference adds b_val*d to the first d of the 2d raw residue words of b
+ * and rebuilds b from all 2d words as composite coefficients
+ * (Poly::from_coeffs_moduli, poly.rs:127-159), so bdim = 2d; residue_wise = 1
+ * adds b_val*d to each residue instead (bdim = d, a corrected mode). */
+uint32_t or_pack_lwes(const or_params *pr, const or_ntt *c, uint32_t n, const uint64_t *lwe_a,
+                      const uint64_t *lwe_b, const uint64_t *galois_keys, uint32_t nkeys, int literal,
+                      int residue_wise, uint64_t *ct_out) {
+    const uint32_t d = pr->d, K = pr->K;
+    const size_t P = psz(pr), CT = 2 * P;
+    const uint64_t q = pr->q;
+    if (n == 0) { memset(ct_out, 0, CT * 8); return d; }               /* :426-431 */
+    uint64_t *cf = (uint64_t *)calloc(2 * (size_t)d, sizeof(uint64_t));
+    uint32_t m = n < d ? n : d;
+    uint64_t *leaves = palloc(pr, 2 * (size_t)d);                        /* padded to d (:448-454) */
+    uint8_t *nz = (uint8_t *)calloc(d, 1);
+    for (uint32_t j = 0; j < m; j++) {                                   /* prep_pack_lwes (:383-405) */
+        const uint64_t *a = lwe_a + (size_t)j * d;
+        cf[0] = a[0];                                                    /* invert_sample_extract (:349-363) */
+        for (uint32_t i = 1; i < d; i++) cf[i] = a[d - i];
+        or_poly_from_coeffs(pr, cf, leaves + (size_t)j * CT);
+        nz[j] = !ct_is_zero(pr, leaves + (size_t)j * CT);
+    }
+    if (n == 1) {                                                        /* :436-443 */
+        memcpy(ct_out, leaves, P * 8);
+        memset(cf, 0, (size_t)d * 8);
+        cf[0] = lwe_b[0];
+        or_poly_from_coeffs(pr, cf, ct_out + P);
+        free(cf); free(leaves); free(nz);
+        return d;
+    }
+    uint32_t log_d = 0;
+    while ((1u << log_d) < d) log_d++;
+    uint64_t *y = palloc(pr, log_d), *ny = palloc(pr, log_d);
+    for (uint32_t l = 0; l < log_d; l++) {                               /* YConstants::generate (:104-131) */
+        uint32_t st = d >> (l + 1);
+        memset(cf, 0, (size_t)d * 8);
+        cf[st] = 1;
+        or_poly_from_coeffs(pr, cf, y + (size_t)l * P);
+        cf[st] = q - 1;
+        or_poly_from_coeffs(pr, cf, ny + (size_t)l * P);
+    }
+    tree_ctx T = {pr, c, leaves, literal ? NULL : nz, galois_keys, nkeys, log_d, y, ny};
+    uint64_t *packed = palloc(pr, 2);
+    if (pack_inner(&T, log_d, 0, packed)) memset(packed, 0, CT * 8);
+    memcpy(ct_out, packed, P * 8);
+    uint32_t bdim = d;
+    if (residue_wise || K == 1) {
+        for (uint32_t k = 0; k < K; k++) {
+            uint64_t qk = pr->moduli[k];
+            for (uint32_t z = 0; z < d; z++) {
+                uint64_t sc = z < m ? (uint64_t)(((u128)lwe_b[z] * d) % q) : 0;
+                ct_out[P + (size_t)k * d + z] = (uint64_t)(((u128)packed[P + (size_t)k * d + z] + sc % qk) % qk);
+            }
+        }
+    } else {                                                             /* :476-484 */
+        bdim = 2 * d;
+        uint64_t *raw = packed + P;                                      /* b.coeffs(): [res0 | res1] */
+        for (uint32_t z = 0; z < d; z++) {
+            uint64_t sc = z < m ? (uint64_t)(((u128)lwe_b[z] * d) % q) : 0; /* ModQ::mul */
+            raw[z] = (uint64_t)(((u128)raw[z] + sc) % q);                    /* ModQ::add */
+        }
+        for (uint32_t k = 0; k < K; k++)
+            for (uint32_t i = 0; i < 2 * d; i++) ct_out[P + (size_t)k * 2 * d + i] = raw[i] % pr->moduli[k];
+    }
+    free(cf); free(leaves); free(nz); free(y); free(ny); free(packed);
+    return bdim;
+}
+
+/* respond_one_packing (respond.rs:213-268): columns by external product, an
+ * LWE from coefficient 0 of each, pack_lwes with the CRS galois keys.
+ * Returns bdim (see or_pack_lwes), or 0 for the empty shard (ct_out = the
+ * zero ciphertext; the reference also returns column_ciphertexts = [0]). */
+uint32_t or_respond_one_packing(const or_params *pr, const or_ntt *c, const uint64_t *shard_polys,
+                                uint32_t ncols, const uint64_t *rgsw, const uint64_t *galois_keys,
+                                uint32_t nkeys, int literal, int residue_wise, uint64_t *ct_out) {
+    const size_t P = psz(pr);
+    const uint32_t d = pr->d;
+    if (ncols == 0) { memset(ct_out, 0, 2 * P * 8); return 0; }          /* :230-236 */
+    uint64_t *cols = palloc(pr, 2 * (size_t)ncols);
+#pragma omp parallel for schedule(dynamic, 1) if (!literal)
+    for (uint32_t col = 0; col < ncols; col++) {                         /* :241-249 */
+        uint64_t *rl = palloc(pr, 2);
+        trivial_encrypt(pr, shard_polys + (size_t)col * P, rl);
+        or_external_product(pr, c, rl, rgsw, cols + (size_t)col * 2 * P, literal);
+        free(rl);
+    }
+    uint64_t *la = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)ncols * d);
+    uint64_t *lb = (uint64_t *)malloc(sizeof(uint64_t) * ncols);
+    for (uint32_t col = 0; col < ncols; col++)                            /* :253-256 */
+        or_sample_extract_coeff0(pr, cols + (size_t)col * 2 * P, la + (size_t)col * d, lb + col);
+    uint32_t bdim = or_pack_lwes(pr, c, ncols, la, lb, galois_keys, nkeys, literal, residue_wise, ct_out);
+    free(cols); free(la); free(lb);
+    return bdim;
+}
+
+/* n respond_one_packing calls, one query per thread. */
+void or_respond_one_packing_many(const or_params *pr, const or_ntt *c, uint32_t n,
+                                 const uint64_t *const *shards, uint32_t ncols, const uint64_t *const *rgsw,
+                                 const uint64_t *galois_keys, uint32_t nkeys, uint64_t *const *ct_out) {
+#pragma omp parallel for schedule(dynamic, 1)
+    for (uint32_t i = 0; i < n; i++)
+        or_respond_one_packing(pr, c, shards[i], ncols, rgsw[i], galois_keys, nkeys, 0, 0, ct_out[i]);
+}
+
+/* ------------------------------------------------------------------ */
 /* Client side: fixtures and decrypt checks only                        */
 /* ------------------------------------------------------------------ */
 
@@ -1113,4 +1316,39 @@ void or_encode_shard(const or_params *pr, const uint8_t *db, uint64_t num_entrie
         or_poly_from_coeffs(pr, cf, polys + (size_t)pi * psz(pr));
     }
     free(cf);
+}
+
+/* generate_automorphism_ks_matrix (ks/setup.rs:218-247) through
+ * generate_ks_matrix (ks/setup.rs:118-166): from tau_t(s) to s, rows
+ * (a_i, -a_i s + e_i + tau_t(s) z^i), out [ell][2][K][d].  The draws come from
+ * the ChaCha stream r (the reference samples a with thread_rng). */
+void or_automorphism_ks_matrix(const or_params *pr, const or_ntt *c, const uint64_t *sk, uint32_t t,
+                               or_chacha *r, double sigma, uint64_t *out) {
+    const size_t P = psz(pr);
+    uint64_t pw[16];
+    gadget_powers(pr, pw);
+    uint64_t *ts = palloc(pr, 1), *e = palloc(pr, 1), *as = palloc(pr, 1), *sm = palloc(pr, 1);
+    or_apply_automorphism(pr, sk, t, ts);
+    for (uint32_t i = 0; i < pr->gadget_len; i++) {
+        uint64_t *ra = out + (size_t)i * 2 * P, *rb = ra + P;
+        poly_random_rng(pr, r, ra);
+        or_sample_gaussian_poly(pr, r, sigma, e);
+        or_poly_mul_ntt(pr, c, ra, sk, as);
+        poly_neg(pr, as, as);
+        poly_add(pr, as, e, rb);
+        poly_scalar_mul(pr, ts, pw[i], sm);
+        poly_add(pr, rb, sm, rb);
+    }
+    free(ts); free(e); free(as); free(sm);
+}
+
+/* ServerCrs.galois_keys (setup.rs:167-173): for i < log_d, the matrix of
+ * t = d/2^i + 1; out [log_d][ell][2][K][d]. */
+void or_galois_keys(const or_params *pr, const or_ntt *c, const uint64_t *sk, or_chacha *r, double sigma,
+                    uint64_t *out) {
+    uint32_t log_d = 0;
+    while ((1u << log_d) < pr->d) log_d++;
+    const size_t KS = (size_t)pr->gadget_len * 2 * psz(pr);
+    for (uint32_t i = 0; i < log_d; i++)
+        or_automorphism_ks_matrix(pr, c, sk, (pr->d >> i) + 1, r, sigma, out + (size_t)i * KS);
 }

@@ -118,6 +118,28 @@ void or_respond_inspiring_many(const or_params *pr, const or_ntt *c, const or_pa
                                const uint64_t *const *y_body, uint64_t *const *ct_out, int *rc);
 int or_num_threads(void);
 
+/* ---- tree packing: inspiring/automorph_pack.rs ---- */
+/* sample_extract_coeff0 (rlwe/enc.rs:195-208): composite a [d], b */
+void or_sample_extract_coeff0(const or_params *pr, const uint64_t *ct, uint64_t *a_out, uint64_t *b_out);
+/* homomorphic_automorph (:44-78); ks = KeySwitchingMatrix rows [ks_rows][2][K][d] */
+void or_homomorphic_automorph(const or_params *pr, const or_ntt *c, const uint64_t *ct, uint32_t t,
+                              const uint64_t *ks, uint32_t ks_rows, uint64_t *out);
+/* pack_lwes (:418-487).  galois_keys [nkeys][ell][2][K][d] (ServerCrs.galois_keys).
+ * literal = 1 runs the recursion over all d leaves as written; 0 skips subtrees
+ * of zero leaves (identical output).  ct_out: a [K][d] then b [K][bdim]; returns
+ * bdim (2d for the reference's two-modulus b, :476-484; d otherwise and with
+ * residue_wise = 1, the corrected mode). */
+uint32_t or_pack_lwes(const or_params *pr, const or_ntt *c, uint32_t n, const uint64_t *lwe_a,
+                      const uint64_t *lwe_b, const uint64_t *galois_keys, uint32_t nkeys, int literal,
+                      int residue_wise, uint64_t *ct_out);
+/* respond_one_packing (respond.rs:213-268); returns bdim, 0 for an empty shard */
+uint32_t or_respond_one_packing(const or_params *pr, const or_ntt *c, const uint64_t *shard_polys,
+                                uint32_t ncols, const uint64_t *rgsw, const uint64_t *galois_keys,
+                                uint32_t nkeys, int literal, int residue_wise, uint64_t *ct_out);
+void or_respond_one_packing_many(const or_params *pr, const or_ntt *c, uint32_t n,
+                                 const uint64_t *const *shards, uint32_t ncols, const uint64_t *const *rgsw,
+                                 const uint64_t *galois_keys, uint32_t nkeys, uint64_t *const *ct_out);
+
 /* ---- ChaCha20Rng (rand_chacha 0.3) + rand 0.8 gen_range ---- */
 typedef struct {
     uint32_t key[8];
@@ -146,6 +168,12 @@ void or_inverse_monomial(const or_params *pr, uint32_t k, uint64_t *out);   /* e
 void or_client_packing_keys(const or_params *pr, const or_ntt *c, const or_pack *pk,
                             const uint64_t *sk, or_chacha *r, double sigma,
                             uint64_t *y_body, uint64_t *y_all);             /* inspiring2.rs:640-707 */
+/* generate_automorphism_ks_matrix (ks/setup.rs:218-247): [ell][2][K][d] */
+void or_automorphism_ks_matrix(const or_params *pr, const or_ntt *c, const uint64_t *sk, uint32_t t,
+                               or_chacha *r, double sigma, uint64_t *out);
+/* ServerCrs.galois_keys (setup.rs:167-173): [log_d][ell][2][K][d] */
+void or_galois_keys(const or_params *pr, const or_ntt *c, const uint64_t *sk, or_chacha *r, double sigma,
+                    uint64_t *out);
 void or_decrypt(const or_params *pr, const or_ntt *c, const uint64_t *ct, const uint64_t *sk,
                 uint64_t *msg_out);                                         /* rlwe/enc.rs:90-108 */
 /* encode_database (encode_db.rs:106-164) for shard `shard_idx`: writes ncols polys. */

@@ -85,3 +85,25 @@ def save_shards_binary(directory, ids, shard_polys, moduli, d: int):
                 f.write(polys.astype("<u8").tobytes())
             else:
                 f.write(np.array([0, 0], dtype="<u4").tobytes())
+
+
+def rlwe_encrypt(o: O.Oracle, sk, msg_poly, r) -> np.ndarray:
+    """RlweCiphertext::encrypt (rlwe/enc.rs:32-51): (a, -a*s + e + Delta*m),
+    a uniform and e Gaussian from the ChaCha stream r."""
+    a = o.zeros()
+    for k, m in enumerate(o.params.moduli):
+        a[k] = [O.lib.or_gen_range(r, m) for _ in range(o.params.d)]
+    e = o.gaussian_poly(r)
+    a_s = o.mul_ntt(a, sk)
+    b = np.zeros_like(a)
+    for k, m in enumerate(o.params.moduli):
+        m = int(m)
+        dm = o.params.delta % m
+        b[k] = [((m - int(x)) % m + int(y) + dm * int(z)) % m for x, y, z in zip(a_s[k], e[k], msg_poly[k])]
+    return np.stack([a, b])
+
+
+def coeff0_message(o: O.Oracle, value: int) -> np.ndarray:
+    mp = o.zeros()
+    mp[:, 0] = value
+    return mp
