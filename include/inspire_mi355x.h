@@ -98,12 +98,20 @@ typedef struct {
     const uint64_t *y_all;   /* uint64_t[y_all_len][ell][K][d], or NULL */
 } inspire_query;
 
-/* ServerResponse (src/pir/respond.rs:30-37). */
+/* ServerResponse (src/pir/respond.rs:30-37).  The ciphertext is a.coeffs
+ * followed by b.coeffs: a = uint64_t[K][d], b = uint64_t[K][ciphertext_b_dim].
+ * b has d coefficients except in a tree-packed response (respond_one_packing)
+ * with two moduli and at least two columns, where the reference rebuilds b
+ * from its 2d residue words as 2d composite coefficients
+ * (src/inspiring/automorph_pack.rs:476-484, Poly::from_coeffs_moduli): then
+ * ciphertext_b_dim = 2d and the ciphertext holds 3*K*d words. */
 typedef struct {
-    uint64_t *ciphertext;          /* out: uint64_t[2][K][d] */
+    uint64_t *ciphertext;          /* out: a then b, see above */
     uint64_t *column_ciphertexts;  /* out: uint64_t[column_capacity][2][K][d], may be NULL */
     uint32_t column_capacity;
     uint32_t num_column_ciphertexts; /* out: the reference's column_ciphertexts.len() */
+    uint32_t ciphertext_capacity;  /* in: words the ciphertext buffer holds; 0 means 2*K*d */
+    uint32_t ciphertext_b_dim;     /* out: coefficients per residue of b (d or 2d) */
 } inspire_response;
 
 /* ---- context ---- */
@@ -112,6 +120,12 @@ typedef struct {
  * reference's roots (ntt.rs:116-164) and the InspiRING tables derived from the
  * CRS seed w_seed (ServerCrs.inspiring_w_seed, src/pir/setup.rs:197-201). */
 int inspire_ctx_create(const inspire_params *params, const uint8_t w_seed[32], inspire_ctx **out);
+/* ServerCrs.galois_keys (src/pir/setup.rs:167-173), the key-switching
+ * matrices tree packing uses: count = log2(d) matrices, matrix i for the
+ * automorphism t = d/2^i + 1 (ks/setup.rs:218-247), each gadget_len rows
+ * (a, b): keys = uint64_t[count][gadget_len][2][K][d], canonical residues.
+ * They stay resident in device memory; setting them again replaces them. */
+int inspire_ctx_set_galois_keys(inspire_ctx *ctx, const uint64_t *keys, uint32_t count);
 void inspire_ctx_destroy(inspire_ctx *ctx);
 const char *inspire_last_error(void);
 /* Device memory held by the context, in bytes. */
@@ -153,6 +167,14 @@ int inspire_respond_seeded_inspiring(inspire_ctx *ctx, const inspire_query *q, i
  * (respond.rs:161-183); q->seeded selects between them. */
 int inspire_respond_with_variant(inspire_ctx *ctx, const inspire_query *q, uint32_t variant,
                                  inspire_response *r);
+/* respond_one_packing (respond.rs:213-268): InsPIRe^1 with tree packing of
+ * the column LWEs (pack_lwes, src/inspiring/automorph_pack.rs:418-487) under
+ * the CRS galois keys (inspire_ctx_set_galois_keys).  The response's b has 2d
+ * coefficients per residue with two moduli and at least two columns (see
+ * inspire_response); q->packing_mode is not read. */
+int inspire_respond_one_packing(inspire_ctx *ctx, const inspire_query *q, inspire_response *r);
+/* respond_seeded_packed (respond.rs:428-435): the same for a SeededClientQuery. */
+int inspire_respond_seeded_packed(inspire_ctx *ctx, const inspire_query *q, inspire_response *r);
 /* Many queries (any variants, databases and shards) in one batched launch
  * sequence.  status[i] receives each query's code; the return value is
  * INSPIRE_OK when the batch ran (individual queries may still have failed). */
@@ -171,8 +193,11 @@ int inspire_respond_batch(inspire_ctx *ctx, uint32_t n, const inspire_query *qs,
  *          (rows 0..ell-1 multiply the zero a-digits of the database's
  *          trivial encryption and do not affect the response)
  *   seeds  seeded: [count][ell][8] (32-byte seeds of rows ell..2ell-1)
- *   y_body packed variants: [count][ell][K][d]
- *   out    packed variants: [count][2][K][d];
+ *   y_body packed variants with InspiRING packing: [count][ell][K][d]
+ *          (not read with tree packing)
+ *   out    InspiRING packing: [count][2][K][d];
+ *          tree packing: [count][K*d + K*b_dim] (a, then b; b_dim as in
+ *          inspire_response);
  *          InsPIRe^0: [count][1 + cols][2][K][d] (ciphertext, then columns)
  * shard_ids and status are host arrays.  status[i] is INSPIRE_OK or
  * INSPIRE_ERR_SHARD_NOT_FOUND.  The call returns when the batch has finished. */
@@ -187,6 +212,8 @@ typedef struct {
     const uint32_t *y_body;
     uint32_t *out;
     int32_t *status;
+    uint32_t packing_mode;   /* INSPIRE_PACKING_* for the packed variants */
+    uint32_t reserved;
 } inspire_device_batch;
 
 int inspire_respond_batch_device(inspire_ctx *ctx, const inspire_device_batch *batch);

@@ -92,7 +92,8 @@ struct PackTables {
 
 // HIP-event timing of each kernel category on the context's stream.
 enum ProfId {
-    P_ROWS_NTT, P_EXPAND, P_FIRST_DIM, P_COLS_INTT, P_FOLD, P_B0, P_STEP1, P_Y_NTT, P_CHAIN, P_DB, P_FRONT, P_COUNT
+    P_ROWS_NTT, P_EXPAND, P_FIRST_DIM, P_COLS_INTT, P_FOLD, P_B0, P_STEP1, P_Y_NTT, P_CHAIN, P_DB, P_FRONT,
+    P_TREE_LEAVES, P_TREE_COMBINE, P_COUNT
 };
 extern const char* const kProfNames[P_COUNT];
 
@@ -134,6 +135,10 @@ struct inspire_ctx {
     std::map<uint32_t, inspire::Database> dbs;
     std::map<uint32_t, std::unique_ptr<inspire::PackTables>> packs;
     inspire::ClientPool clients;
+    // ServerCrs.galois_keys (tree packing): [log N][ell][2][K][N] NTT domain,
+    // Montgomery form, and NTT(X^(N >> l)) per level, [log N][K][N]
+    inspire::DevBuf galois, yhat;
+    uint32_t galois_count = 0;
     // Writers (database load/unload, client registration) hold it exclusively,
     // serving lanes share it while a batch reads databases, tables and keys.
     std::shared_mutex rw;

@@ -18,6 +18,7 @@
 #include "modarith.hpp"
 #include "abi.hpp"
 #include "respond.hpp"
+#include "tree.hpp"
 
 using namespace inspire;
 
@@ -283,7 +284,7 @@ void inspire_ctx_destroy(inspire_ctx* ctx) {
 uint64_t inspire_ctx_device_bytes(const inspire_ctx* ctx) {
     if (!ctx) return 0;
     uint64_t b = ctx->tables.bytes() + ctx->clients.keys.bytes() + ctx->ws0.bytes() + ctx->ws1.bytes() + ctx->ws2.bytes() + ctx->ws3.bytes() +
-                 ctx->ws4.bytes() + ctx->ws5.bytes() + ctx->ws6.bytes();
+                 ctx->ws4.bytes() + ctx->ws5.bytes() + ctx->ws6.bytes() + ctx->galois.bytes() + ctx->yhat.bytes();
     for (auto& kv : ctx->dbs) b += kv.second.digits.bytes();
     for (auto& kv : ctx->packs) b += kv.second->step1.bytes() + kv.second->perm.bytes() + kv.second->w_all.bytes();
     return b;
@@ -443,7 +444,7 @@ int inspire_db_unload(inspire_ctx* ctx, uint32_t db_id) {
 }
 
 static int respond_one(inspire_ctx* ctx, const inspire_query* q, uint32_t variant, inspire_response* r,
-                       uint32_t seeded) {
+                       uint32_t seeded, Route route) {
     INSPIRE_TRY({
         if (!ctx || !q || !r) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "null argument"};
         if (q->seeded != seeded)
@@ -452,25 +453,43 @@ static int respond_one(inspire_ctx* ctx, const inspire_query* q, uint32_t varian
                                : "this entry point takes a ClientQuery (seeded = 0)"};
         std::lock_guard<std::recursive_mutex> g(ctx->mu);
         int32_t st = 0;
-        respond_batch(ctx, 1, q, &variant, r, &st, /*direct_inspiring=*/variant != INSPIRE_VARIANT_NO_PACKING);
+        respond_batch(ctx, 1, q, &variant, r, &st, route);
         return st;
     })
 }
 
 int inspire_respond(inspire_ctx* ctx, const inspire_query* q, inspire_response* r) {
-    return respond_one(ctx, q, INSPIRE_VARIANT_NO_PACKING, r, 0);
+    return respond_one(ctx, q, INSPIRE_VARIANT_NO_PACKING, r, 0, Route::ByVariant);
 }
 
 int inspire_respond_seeded(inspire_ctx* ctx, const inspire_query* q, inspire_response* r) {
-    return respond_one(ctx, q, INSPIRE_VARIANT_NO_PACKING, r, 1);
+    return respond_one(ctx, q, INSPIRE_VARIANT_NO_PACKING, r, 1, Route::ByVariant);
 }
 
 int inspire_respond_inspiring(inspire_ctx* ctx, const inspire_query* q, inspire_response* r) {
-    return respond_one(ctx, q, INSPIRE_VARIANT_ONE_PACKING, r, 0);
+    return respond_one(ctx, q, INSPIRE_VARIANT_ONE_PACKING, r, 0, Route::Inspiring);
 }
 
 int inspire_respond_seeded_inspiring(inspire_ctx* ctx, const inspire_query* q, inspire_response* r) {
-    return respond_one(ctx, q, INSPIRE_VARIANT_TWO_PACKING, r, 1);
+    return respond_one(ctx, q, INSPIRE_VARIANT_TWO_PACKING, r, 1, Route::Inspiring);
+}
+
+int inspire_respond_one_packing(inspire_ctx* ctx, const inspire_query* q, inspire_response* r) {
+    return respond_one(ctx, q, INSPIRE_VARIANT_ONE_PACKING, r, 0, Route::Tree);
+}
+
+int inspire_respond_seeded_packed(inspire_ctx* ctx, const inspire_query* q, inspire_response* r) {
+    return respond_one(ctx, q, INSPIRE_VARIANT_ONE_PACKING, r, 1, Route::Tree);
+}
+
+int inspire_ctx_set_galois_keys(inspire_ctx* ctx, const uint64_t* keys, uint32_t count) {
+    INSPIRE_TRY({
+        if (!ctx || (count && !keys)) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "null argument"};
+        std::lock_guard<std::recursive_mutex> g(ctx->mu);
+        std::unique_lock<std::shared_mutex> w(ctx->rw);
+        set_galois_keys(ctx, keys, count);
+        return INSPIRE_OK;
+    })
 }
 
 int inspire_respond_with_variant(inspire_ctx* ctx, const inspire_query* q, uint32_t variant, inspire_response* r) {
@@ -478,7 +497,7 @@ int inspire_respond_with_variant(inspire_ctx* ctx, const inspire_query* q, uint3
         if (!ctx || !q || !r) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "null argument"};
         std::lock_guard<std::recursive_mutex> g(ctx->mu);
         int32_t st = 0;
-        respond_batch(ctx, 1, q, &variant, r, &st, false);
+        respond_batch(ctx, 1, q, &variant, r, &st, Route::ByVariant);
         return st;
     })
 }
@@ -489,7 +508,7 @@ int inspire_respond_batch(inspire_ctx* ctx, uint32_t n, const inspire_query* qs,
         if (!ctx || (n && (!qs || !variants || !rs || !status)))
             throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "null argument"};
         std::lock_guard<std::recursive_mutex> g(ctx->mu);
-        respond_batch(ctx, n, qs, variants, rs, status, false);
+        respond_batch(ctx, n, qs, variants, rs, status, Route::ByVariant);
         return INSPIRE_OK;
     })
 }

@@ -8,11 +8,18 @@
 
 namespace inspire {
 
+// How a packed variant is answered.
+enum BatchKind : uint8_t {
+    KIND_PLAIN = 0,      // InsPIRe^0 (respond / respond_seeded)
+    KIND_INSPIRING = 1,  // InspiRING packing (respond_inspiring, respond.rs:284-386)
+    KIND_TREE = 2,       // tree packing (respond_one_packing, respond.rs:213-268)
+};
+
 // One runnable query of a respond batch (shared with respond.cpp).
 struct BatchItem {
     uint32_t idx;   // position in the caller's batch
     uint64_t dig;   // device address of the shard's digits
-    bool packed;    // InsPIRe^1/^2
+    uint8_t kind;   // BatchKind
 };
 
 // Kernel-side description of a packing batch.

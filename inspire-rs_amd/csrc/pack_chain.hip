@@ -30,225 +30,9 @@
 #include "kernels.hpp"
 #include "ntt.hpp"
 #include "pack.hpp"
+#include "reg_ntt.hpp"
 
 namespace inspire {
-
-namespace {
-
-// Barrier over the workgroup for LDS data: this wave's LDS operations have
-// completed, then s_barrier.  Global loads issued earlier are not waited for.
-__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
-
-struct Mod {
-    uint32_t q, q2, qinv;
-};
-
-// Padded LDS index of element base + e*st, from the padded index of base.
-// For the radix-8 passes (st a power of two, base = an aligned block of 8*st
-// elements plus r < st) the padding of the eight elements differs from that
-// of base by a constant: the addresses are one register plus immediates.
-__device__ __forceinline__ int lds_step(int base_ix, int e, int st) { return base_ix + e * st + 4 * ((e * st) >> 5); }
-
-// a * b * 2^-32 mod q in [0, 2q), for a * b < q * 2^32.
-__device__ __forceinline__ uint32_t mmul(uint32_t a, uint32_t b, const Mod& M) {
-    return mont_reduce((uint64_t)a * b, M.q, M.qinv);
-}
-
-// Harvey CT butterfly with a Montgomery-form twiddle: X, Y < 4q -> X + wY, X - wY (< 4q).
-__device__ __forceinline__ void ct_m(uint32_t& X, uint32_t& Y, uint32_t wm, const Mod& M) {
-    const uint32_t x = csub(X, M.q2);
-    const uint32_t t = mmul(Y, wm, M);
-    X = x + t;
-    Y = x - t + M.q2;
-}
-
-// Harvey GS butterfly with a Montgomery-form twiddle: X, Y < 2q -> X + Y, w(X - Y) (< 2q).
-__device__ __forceinline__ void gs_m(uint32_t& X, uint32_t& Y, uint32_t wm, const Mod& M) {
-    const uint32_t s = csub(X + Y, M.q2);
-    const uint32_t d = X - Y + M.q2;
-    X = s;
-    Y = mmul(d, wm, M);
-}
-
-template <int LOGN>
-struct Lay {
-    static constexpr int N = 1 << LOGN, T = N / 8, LP = NttShape<LOGN>::LDS;
-    static constexpr int R8 = LOGN / 3, REM = LOGN % 3;
-    // first word of the two 4-element groups of a thread's slot positions
-    static __device__ __forceinline__ int group(int t, int g) {
-        if constexpr (REM != 0) return 4 * (t + g * T);
-        else return 8 * t + 4 * g;
-    }
-};
-
-template <int LOGN>
-__device__ __forceinline__ void load_slots(const uint32_t* p, int t, uint32_t (&v)[8]) {
-    using L = Lay<LOGN>;
-#pragma unroll
-    for (int g = 0; g < 2; g++) {
-        const uint4 a = *reinterpret_cast<const uint4*>(p + L::group(t, g));
-        v[4 * g + 0] = a.x;
-        v[4 * g + 1] = a.y;
-        v[4 * g + 2] = a.z;
-        v[4 * g + 3] = a.w;
-    }
-}
-
-// Inverse NTT of one residue, scaled by (c, cw) in the last stage: x at the
-// slot positions of thread t -> x at its coefficient positions (< 2q).  rb is
-// the residue's [LP] LDS buffer; no thread may still be reading it.
-template <int LOGN>
-__device__ __forceinline__ void intt1(uint32_t (&x)[8], int t, uint32_t* rb, const uint32_t* itw, const Mod& M,
-                                      uint32_t c, uint32_t cp, uint32_t cw, uint32_t cwp) {
-    using L = Lay<LOGN>;
-    constexpr int N = L::N;
-    if constexpr (L::REM != 0) {
-#pragma unroll
-        for (int g = 0; g < 2; g++) {
-            const int i = t + g * L::T;
-            uint32_t& x0 = x[4 * g + 0];
-            uint32_t& x1 = x[4 * g + 1];
-            uint32_t& x2 = x[4 * g + 2];
-            uint32_t& x3 = x[4 * g + 3];
-            gs_m(x0, x1, itw[N / 2 + 2 * i], M);
-            gs_m(x2, x3, itw[N / 2 + 2 * i + 1], M);
-            if constexpr (L::REM == 2) {
-                const uint32_t w = itw[N / 4 + i];
-                gs_m(x0, x2, w, M);
-                gs_m(x1, x3, w, M);
-            }
-            *reinterpret_cast<uint4*>(&rb[lds_ix(4 * i)]) = make_uint4(x0, x1, x2, x3);
-        }
-        lds_barrier();
-    }
-    int t0 = 1 << L::REM;
-#pragma unroll
-    for (int p = 0; p < L::R8; p++) {
-        const int i3 = t / t0, r = t - i3 * t0;
-        const int bix = lds_ix(8 * t0 * i3 + r);
-        if (!(L::REM == 0 && p == 0)) {
-#pragma unroll
-            for (int e = 0; e < 8; e++) x[e] = rb[lds_step(bix, e, t0)];
-        }
-        const int mA = N / (2 * t0), mB = N / (4 * t0), mC = N / (8 * t0);
-#pragma unroll
-        for (int h = 0; h < 4; h++) gs_m(x[2 * h], x[2 * h + 1], itw[mA + 4 * i3 + h], M);
-#pragma unroll
-        for (int h = 0; h < 2; h++) {
-            const uint32_t w = itw[mB + 2 * i3 + h];
-            gs_m(x[4 * h + 0], x[4 * h + 2], w, M);
-            gs_m(x[4 * h + 1], x[4 * h + 3], w, M);
-        }
-        if (p < L::R8 - 1) {
-            const uint32_t w = itw[mC + i3];
-#pragma unroll
-            for (int k = 0; k < 4; k++) gs_m(x[k], x[k + 4], w, M);
-#pragma unroll
-            for (int e = 0; e < 8; e++) rb[lds_step(bix, e, t0)] = x[e];
-            lds_barrier();
-        } else {
-            // last stage (m = 1) with the output scale: X' = c (X + Y), Y' = cw (X - Y)
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const uint32_t X = x[k], Y = x[k + 4];
-                x[k] = shoup_mul(X + Y, c, cp, M.q);
-                x[k + 4] = shoup_mul(X - Y + M.q2, cw, cwp, M.q);
-            }
-        }
-        t0 <<= 3;
-    }
-}
-
-template <int ELL>
-__device__ __forceinline__ void fwd_r8(uint32_t (&x)[ELL][8], int m0, int i, const uint32_t* tw, const Mod& M) {
-    {
-        const uint32_t w = tw[m0 + i];
-#pragma unroll
-        for (int j = 0; j < ELL; j++)
-#pragma unroll
-            for (int k = 0; k < 4; k++) ct_m(x[j][k], x[j][k + 4], w, M);
-    }
-#pragma unroll
-    for (int h = 0; h < 2; h++) {
-        const uint32_t w = tw[2 * m0 + 2 * i + h];
-#pragma unroll
-        for (int j = 0; j < ELL; j++) {
-            ct_m(x[j][4 * h + 0], x[j][4 * h + 2], w, M);
-            ct_m(x[j][4 * h + 1], x[j][4 * h + 3], w, M);
-        }
-    }
-#pragma unroll
-    for (int h = 0; h < 4; h++) {
-        const uint32_t w = tw[4 * m0 + 4 * i + h];
-#pragma unroll
-        for (int j = 0; j < ELL; j++) ct_m(x[j][2 * h], x[j][2 * h + 1], w, M);
-    }
-}
-
-// Forward NTT of ELL polys of one residue: x at the coefficient positions of
-// thread t -> x at its slot positions (< 4q).  db is [ELL][LP] LDS; no thread
-// may still be reading it.
-template <int LOGN, int ELL>
-__device__ __forceinline__ void ntt_digits(uint32_t (&x)[ELL][8], int t, uint32_t* db, const uint32_t* tw,
-                                           const Mod& M) {
-    using L = Lay<LOGN>;
-    constexpr int N = L::N, LP = L::LP;
-    int m0 = 1;
-#pragma unroll
-    for (int p = 0; p < L::R8; p++) {
-        const int t0 = N / (2 * m0), st = t0 >> 2;
-        const int i = t / st, r = t - i * st;
-        const int bix = lds_ix(2 * i * t0 + r);
-        if (p > 0) {
-#pragma unroll
-            for (int j = 0; j < ELL; j++)
-#pragma unroll
-                for (int e = 0; e < 8; e++) x[j][e] = db[j * LP + lds_step(bix, e, st)];
-        }
-        fwd_r8<ELL>(x, m0, i, tw, M);
-        if (!(L::REM == 0 && p == L::R8 - 1)) {
-#pragma unroll
-            for (int j = 0; j < ELL; j++)
-#pragma unroll
-                for (int e = 0; e < 8; e++) db[j * LP + lds_step(bix, e, st)] = x[j][e];
-            lds_barrier();
-        }
-        m0 <<= 3;
-    }
-    if constexpr (L::REM != 0) {
-#pragma unroll
-        for (int g = 0; g < 2; g++) {
-            const int i = t + g * L::T;
-#pragma unroll
-            for (int j = 0; j < ELL; j++) {
-                const uint4 v = *reinterpret_cast<const uint4*>(&db[j * LP + lds_ix(4 * i)]);
-                x[j][4 * g + 0] = v.x;
-                x[j][4 * g + 1] = v.y;
-                x[j][4 * g + 2] = v.z;
-                x[j][4 * g + 3] = v.w;
-            }
-            if constexpr (L::REM == 2) {
-                const uint32_t w = tw[N / 4 + i], w0 = tw[N / 2 + 2 * i], w1 = tw[N / 2 + 2 * i + 1];
-#pragma unroll
-                for (int j = 0; j < ELL; j++) {
-                    ct_m(x[j][4 * g + 0], x[j][4 * g + 2], w, M);
-                    ct_m(x[j][4 * g + 1], x[j][4 * g + 3], w, M);
-                    ct_m(x[j][4 * g + 0], x[j][4 * g + 1], w0, M);
-                    ct_m(x[j][4 * g + 2], x[j][4 * g + 3], w1, M);
-                }
-            } else {
-                const uint32_t w0 = tw[N / 2 + 2 * i], w1 = tw[N / 2 + 2 * i + 1];
-#pragma unroll
-                for (int j = 0; j < ELL; j++) {
-                    ct_m(x[j][4 * g + 0], x[j][4 * g + 1], w0, M);
-                    ct_m(x[j][4 * g + 2], x[j][4 * g + 3], w1, M);
-                }
-            }
-        }
-    }
-}
-
-}  // namespace
 
 template <int LOGN, int ELL>
 struct ChainCrtShape {

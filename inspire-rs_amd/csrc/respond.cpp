@@ -22,12 +22,13 @@
 #include "modarith.hpp"
 #include "pack.hpp"
 #include "respond.hpp"
+#include "tree.hpp"
 
 namespace inspire {
 
 const char* const kProfNames[P_COUNT] = {"rows_ntt", "expand_seeds", "first_dim", "cols_intt", "fold",
                                          "pack_b0",  "pack_step1",   "y_ntt",     "pack_chain", "db_load",
-                                         "pack_front"};
+                                         "pack_front", "tree_leaves", "tree_combine"};
 
 ProfScope::ProfScope(inspire_ctx* ctx, int id) : ctx_(ctx), id_(id) {
     if (!ctx_->prof.on || !((ctx_->prof.mask >> id) & 1u)) return;
@@ -399,22 +400,23 @@ void fail(int32_t* status, uint32_t i, int code, const std::string& msg) {
 // when the query must be computed; nullptr when it finished here (an error,
 // or an empty shard answered with zeros).
 static const Database* admit(inspire_ctx* ctx, const inspire_query& q, uint32_t v, inspire_response* r,
-                             int32_t* status, uint32_t i, bool direct_inspiring, uint64_t* dig_out) {
+                             int32_t* status, uint32_t i, Route route, uint64_t* dig_out, uint8_t* kind) {
     const size_t KN = (size_t)ctx->K * ctx->N;
     r->num_column_ciphertexts = 0;
+    r->ciphertext_b_dim = ctx->N;
     if (v > INSPIRE_VARIANT_TWO_PACKING) { fail(status, i, INSPIRE_ERR_INVALID_ARGUMENT, "unknown variant"); return nullptr; }
-    const bool packed = v != INSPIRE_VARIANT_NO_PACKING;
-    if (packed && !direct_inspiring) {  // respond_with_variant (respond.rs:135-149)
+    *kind = v == INSPIRE_VARIANT_NO_PACKING ? KIND_PLAIN : KIND_INSPIRING;
+    if (*kind != KIND_PLAIN && route == Route::Tree) *kind = KIND_TREE;
+    if (*kind != KIND_PLAIN && route == Route::ByVariant) {  // respond_with_variant (respond.rs:135-149)
         if (q.packing_mode == INSPIRE_PACKING_TREE) {
-            fail(status, i, INSPIRE_ERR_UNSUPPORTED, "tree packing (packing_mode=tree) is not on the GPU path");
-            return nullptr;
-        }
-        if (!q.has_packing_keys) {
+            *kind = KIND_TREE;
+        } else if (!q.has_packing_keys) {
             fail(status, i, INSPIRE_ERR_PACKING_KEYS_MISSING_TREE,
                  "InspiRING packing keys missing (set packing_mode=tree to use tree packing)");
             return nullptr;
         }
     }
+    const bool packed = *kind == KIND_INSPIRING;  // InspiRING's key checks (tree packing reads none)
     if (packed && !q.has_packing_keys) {  // respond.rs:296-299
         fail(status, i, INSPIRE_ERR_PACKING_KEYS_MISSING, "InspiRING client packing keys missing from query");
         return nullptr;
@@ -447,16 +449,30 @@ static const Database* admit(inspire_ctx* ctx, const inspire_query& q, uint32_t 
         fail(status, i, INSPIRE_ERR_PACKING_KEYS_EMPTY, "InspiRING packing keys invalid: y_all and y_body are both empty");
         return nullptr;
     }
-    if (!packed && r->column_ciphertexts && r->column_capacity < db.cols) {
+    if (*kind == KIND_PLAIN && r->column_ciphertexts && r->column_capacity < db.cols) {
         fail(status, i, INSPIRE_ERR_CAPACITY, "column_ciphertexts capacity too small");
         return nullptr;
+    }
+    if (*kind == KIND_TREE) {
+        if (!root(ctx)->galois_count) {  // the reference indexes ServerCrs.galois_keys (automorph_pack.rs:237-250)
+            fail(status, i, INSPIRE_ERR_INVALID_ARGUMENT,
+                 "tree packing needs the CRS galois keys (inspire_ctx_set_galois_keys)");
+            return nullptr;
+        }
+        const size_t cap = r->ciphertext_capacity ? r->ciphertext_capacity : 2 * KN;
+        if (cap < tree_out_words(ctx, db.cols)) {
+            fail(status, i, INSPIRE_ERR_CAPACITY,
+                 "ciphertext capacity too small: a tree-packed response holds " +
+                     std::to_string(tree_out_words(ctx, db.cols)) + " words");
+            return nullptr;
+        }
     }
     *dig_out = shard_digits(db, sit->second);
     return &db;
 }
 
 void respond_batch(inspire_ctx* ctx, uint32_t n, const inspire_query* qs, const uint32_t* variants,
-                   inspire_response* rs, int32_t* status, bool direct_inspiring) {
+                   inspire_response* rs, int32_t* status, Route route) {
     const size_t KN = (size_t)ctx->K * ctx->N, ell = ctx->ell;
     std::map<uint32_t, std::vector<BatchItem>> groups;  // by column count: one launch spans databases
     for (uint32_t i = 0; i < n; i++) {
@@ -466,17 +482,18 @@ void respond_batch(inspire_ctx* ctx, uint32_t n, const inspire_query* qs, const 
         if (!r.ciphertext) { fail(status, i, INSPIRE_ERR_INVALID_ARGUMENT, "response ciphertext buffer is null"); continue; }
         if (!q.rgsw || (q.seeded && !q.rgsw_seeds)) { fail(status, i, INSPIRE_ERR_INVALID_ARGUMENT, "query rgsw is null"); continue; }
         uint64_t dig = 0;
-        const Database* db = admit(ctx, q, variants[i], &r, status, i, direct_inspiring, &dig);
+        uint8_t kind = KIND_PLAIN;
+        const Database* db = admit(ctx, q, variants[i], &r, status, i, route, &dig, &kind);
         if (!db) continue;
-        groups[db->cols].push_back(BatchItem{i, dig, variants[i] != INSPIRE_VARIANT_NO_PACKING});
+        groups[db->cols].push_back(BatchItem{i, dig, kind});
     }
 
     for (auto& kv : groups) {
         const uint32_t cols = kv.first;
-        for (int pass = 0; pass < 2; pass++) {  // 0: InsPIRe^0, 1: InspiRING
+        for (uint8_t pass = 0; pass < 3; pass++) {  // KIND_PLAIN, KIND_INSPIRING, KIND_TREE
             std::vector<BatchItem> items;
             for (const BatchItem& it : kv.second)
-                if (it.packed == (pass == 1)) items.push_back(it);
+                if (it.kind == pass) items.push_back(it);
             if (items.empty()) continue;
             const uint32_t Q = (uint32_t)items.size();
             // ---- stage rows ell..2ell-1 (the a-digits of the trivial encryption are 0)
@@ -536,8 +553,27 @@ void respond_batch(inspire_ctx* ctx, uint32_t n, const inspire_query* qs, const 
                     if (r.column_ciphertexts)
                         for (size_t w = 0; w < (size_t)cols * 2 * KN; w++) r.column_ciphertexts[w] = src[2 * KN + w];
                 }
-            } else {
+            } else if (pass == KIND_INSPIRING) {
                 pack_respond_host(ctx, cols, items.data(), Q, rows, d_qdig, d_err, qs, rs, status);
+            } else {
+                const size_t ow = tree_out_words(ctx, cols);
+                ctx->ws2.reserve(Q * ow * 4);
+                uint32_t* out = ctx->ws2.get<uint32_t>();
+                tree_run(ctx, cols, Q, rows, d_qdig, out);
+                std::vector<uint32_t> hout(Q * ow);
+                uint32_t herr = 0;
+                hip_check(hipMemcpyAsync(hout.data(), out, hout.size() * 4, hipMemcpyDeviceToHost, ctx->stream), "out");
+                hip_check(hipMemcpyAsync(&herr, d_err, 4, hipMemcpyDeviceToHost, ctx->stream), "err");
+                hip_check(hipStreamSynchronize(ctx->stream), "respond");
+                for (uint32_t t = 0; t < Q; t++) {
+                    const uint32_t i = items[t].idx;
+                    if (herr) { fail(status, i, INSPIRE_ERR_DEVICE, "seed expansion exhausted its draw budget"); continue; }
+                    inspire_response& r = rs[i];
+                    const uint32_t* src = hout.data() + t * ow;
+                    for (size_t w = 0; w < ow; w++) r.ciphertext[w] = src[w];
+                    r.ciphertext_b_dim = tree_b_dim(ctx, cols);
+                    r.num_column_ciphertexts = 0;  // respond.rs:262-267
+                }
             }
         }
     }
@@ -555,9 +591,14 @@ void respond_batch_device(inspire_ctx* ctx, const inspire_device_batch& b) {
     if (!b.shard_ids || !b.rows || !b.out || !b.status || (b.seeded && !b.seeds))
         throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "null argument in device batch"};
     if (b.variant > INSPIRE_VARIANT_TWO_PACKING) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "unknown variant"};
-    const bool packed = b.variant != INSPIRE_VARIANT_NO_PACKING;
+    if (b.variant != INSPIRE_VARIANT_NO_PACKING && b.packing_mode > INSPIRE_PACKING_TREE)
+        throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "unknown packing mode"};
+    const bool tree = b.variant != INSPIRE_VARIANT_NO_PACKING && b.packing_mode == INSPIRE_PACKING_TREE;
+    const bool packed = b.variant != INSPIRE_VARIANT_NO_PACKING && !tree;  // InspiRING
     if (packed && !b.y_body)
         throw Error{INSPIRE_ERR_PACKING_KEYS_MISSING, "InspiRING client packing keys missing from query"};
+    if (tree && !root(ctx)->galois_count)
+        throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "tree packing needs the CRS galois keys (inspire_ctx_set_galois_keys)"};
     auto& dbs = root(ctx)->dbs;
     auto it = dbs.find(b.db_id);
     if (it == dbs.end()) throw Error{INSPIRE_ERR_UNKNOWN_DATABASE, "unknown database " + std::to_string(b.db_id)};
@@ -584,7 +625,8 @@ void respond_batch_device(inspire_ctx* ctx, const inspire_device_batch& b) {
     uint32_t* rows = ctx->ws1.get<uint32_t>();
     if (b.seeded) stage_rows_from(ctx, Q, Q, rows, b.seeds, b.rows, nullptr, d_err);
     else stage_rows_from(ctx, Q, 0, rows, nullptr, nullptr, b.rows, d_err);
-    if (!packed) run_respond0(ctx, db.cols, Q, rows, d_qdig, b.out);
+    if (tree) tree_run(ctx, db.cols, Q, rows, d_qdig, b.out);
+    else if (!packed) run_respond0(ctx, db.cols, Q, rows, d_qdig, b.out);
     else pack_run_device_ybody(ctx, db.cols, Q, rows, d_qdig, b.y_body, b.out);
     uint32_t* herr = reinterpret_cast<uint32_t*>(ctx->pin.get<uint64_t>() + Q + 1);
     hip_check(hipMemcpyAsync(herr, d_err, 4, hipMemcpyDeviceToHost, ctx->stream), "err");

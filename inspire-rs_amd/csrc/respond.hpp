@@ -27,11 +27,16 @@ void db_load_shard_files(inspire_ctx* ctx, uint32_t db_id, const char* dir);
 void external_product_batch(inspire_ctx* ctx, uint64_t count, const uint64_t* rlwe, const uint64_t* rgsw,
                             uint64_t* out);
 
-// Runs n queries.  variants[i] selects InsPIRe^0/1/2.  direct_inspiring: the
-// caller is respond_inspiring / respond_seeded_inspiring (no packing_mode
-// dispatch, respond.rs:284-398) rather than respond_with_variant.
+// How the packed variants of a call are answered.
+enum class Route {
+    ByVariant,  // respond_with_variant: the query's packing_mode decides (respond.rs:129-183)
+    Inspiring,  // respond_inspiring / respond_seeded_inspiring (respond.rs:284-398)
+    Tree,       // respond_one_packing / respond_seeded_packed (respond.rs:213-268, 428-435)
+};
+
+// Runs n queries.  variants[i] selects InsPIRe^0/1/2.
 void respond_batch(inspire_ctx* ctx, uint32_t n, const inspire_query* qs, const uint32_t* variants,
-                   inspire_response* rs, int32_t* status, bool direct_inspiring);
+                   inspire_response* rs, int32_t* status, Route route);
 
 // RGSW rows ell..2ell-1 staged as [Q][ell][2][K][N] coefficient words: the
 // a-slots of the seeded runs [first, end) are expanded from seeds

@@ -550,6 +550,7 @@ int inspire_server_wait(inspire_server* s, uint64_t ticket, int32_t timeout_ms, 
         if (r && q->status == INSPIRE_OK) {
             const size_t KN = (size_t)s->ctx->K * s->ctx->N;
             if (!r->ciphertext) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "response ciphertext buffer is null"};
+            r->ciphertext_b_dim = s->ctx->N;
             if (q->packed()) {
                 for (size_t w = 0; w < 2 * KN; w++) r->ciphertext[w] = q->out[w];
                 r->num_column_ciphertexts = 0;  // respond.rs:382-385

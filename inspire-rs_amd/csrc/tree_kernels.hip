@@ -1,0 +1,454 @@
+// Tree packing of the column LWEs (respond_one_packing, src/pir/respond.rs:
+// 213-268; pack_lwes / pack_lwes_inner / homomorphic_automorph,
+// src/inspiring/automorph_pack.rs:44-78, 171-256, 418-487) on gfx950.
+//
+// The reference pads the m column LWEs to d leaves and recurses over all of
+// them.  A subtree whose leaves are all padding is exactly zero (every step
+// is linear in its inputs), so only the nodes that hold a column take work:
+// at level l (1..log d) the nodes are s < min(m, 2^(log d - l)), node s
+// combines node s of level l-1 (even) with node s + 2^(log d - l) (odd, when
+// that index is < m).  Skipping the zero nodes leaves every output word as
+// the full recursion computes it.
+//
+// Node state: a in the coefficient domain (the gadget digits of the
+// automorphism need it), b in the NTT domain (b only ever enters linearly:
+// the automorphism is a slot gather and y = X^(d/2^l) a pointwise product),
+// both canonical u32 residues, [2][K][N] per node.  One workgroup of K x N/8
+// threads works on one node, the k-th group of N/8 threads on residue k.
+//
+// One level (pack_lwes_inner with t = 2^l + 1 and the galois key of index
+// log d - l, whose rows are kept as NTT-domain Montgomery words):
+//   s0 = even + y odd,  s1 = even - y odd          (y odd: negacyclic shift)
+//   D_i = NTT(G^-1(tau_t(s1.a)))                    (gadget digits, i < ell)
+//   a   = s0.a + INTT(sum_i D_i * key[i].a)
+//   b   = s0.b + tau_t(s1.b) + sum_i D_i * key[i].b (NTT domain)
+// which is s0 + homomorphic_automorph(s1, t) term for term.
+//
+// k_tree_leaves computes each column's ciphertext from the shard digits and
+// the query's RGSW rows (the first dimension), takes the LWE of its
+// coefficient 0 (sample_extract_coeff0: a stays the column's a, b is
+// coefficient 0 of the column's b, n^-1 times the sum of its NTT slots) and
+// runs the levels at which no node has an odd child.  k_tree_combine runs one
+// later level; at the root it finishes pack_lwes: b leaves the NTT domain and
+// b_val * d is added at coefficient z (automorph_pack.rs:473-484).
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+#include "reg_ntt.hpp"
+#include "tree.hpp"
+
+namespace inspire {
+
+namespace {
+
+template <int LOGN, int K, int ELL>
+struct TreeShape {
+    static constexpr int N = 1 << LOGN, T = N / 8, LP = NttShape<LOGN>::LDS;
+    static constexpr size_t kTw = (size_t)2 * K * N;      // Montgomery twiddles, both directions
+    static constexpr size_t kA = (size_t)K * LP;          // s1.a (automorphism source), then INTT exchange
+    static constexpr size_t kB = (size_t)K * LP;          // s1.b (automorphism source)
+    static constexpr size_t kD = (size_t)K * ELL * LP;    // digit-transform exchange
+    static constexpr int W = T < 64 ? T : 64;              // lanes of one reduction group
+    static constexpr size_t kRed = 2 * K * (T / W);       // u64 partial sums (two words each)
+    static constexpr size_t kBytes = 4 * (kTw + kA + kB + kD + kRed);
+};
+
+template <int LOGN, int K, int ELL>
+struct TreeLds {
+    using S = TreeShape<LOGN, K, ELL>;
+    uint32_t *twf, *twi, *abuf, *bbuf, *dbuf;
+    uint64_t* red;
+    __device__ explicit TreeLds(uint32_t* lds) {
+        twf = lds;
+        twi = twf + K * S::N;
+        abuf = twi + K * S::N;
+        bbuf = abuf + S::kA;
+        dbuf = bbuf + S::kB;
+        red = reinterpret_cast<uint64_t*>(dbuf + S::kD);
+    }
+};
+
+// Twiddles of every residue into LDS, Montgomery form.
+template <int LOGN, int K, int ELL>
+__device__ __forceinline__ void stage_twiddles(const TreeLds<LOGN, K, ELL>& L, const CtxDev& C) {
+    constexpr int N = 1 << LOGN;
+    for (int e = threadIdx.x; e < K * N; e += blockDim.x) {
+        const int kk = e / N, i = e - kk * N;
+        const ModDev& D = C.m[kk];
+        L.twf[e] = csub(shoup_mul(D.tw[i], C.mont[kk], C.montp[kk], D.q), D.q);
+        L.twi[e] = csub(shoup_mul(D.itw[i], C.mont[kk], C.montp[kk], D.q), D.q);
+    }
+}
+
+__device__ __forceinline__ uint32_t addq(uint32_t x, uint32_t y, uint32_t q) { return csub(x + y, q); }
+__device__ __forceinline__ uint32_t subq(uint32_t x, uint32_t y, uint32_t q) { return csub(x + q - y, q); }
+__device__ __forceinline__ uint32_t negq(uint32_t x, uint32_t q) { return x ? q - x : 0u; }
+
+// Inverse of an odd t modulo 2^32 (Newton: each step doubles the correct bits).
+__device__ __forceinline__ uint32_t inv_odd(uint32_t t) {
+    uint32_t y = t;  // correct to 3 bits
+#pragma unroll
+    for (int i = 0; i < 4; i++) y *= 2u - t * y;
+    return y;
+}
+
+// Slot p of NTT(tau_t(x)) is slot br((t (2 br(p) + 1) mod 2N - 1) / 2) of
+// NTT(x): slot p holds the evaluation at psi^(2 br(p) + 1).
+template <int LOGN>
+__device__ __forceinline__ uint32_t aut_slot(uint32_t p, uint32_t t) {
+    constexpr uint32_t N = 1u << LOGN;
+    const uint32_t e = (t * (2u * (__brev(p) >> (32 - LOGN)) + 1u)) & (2u * N - 1u);
+    return __brev((e - 1u) >> 1) >> (32 - LOGN);
+}
+
+// One level of pack_lwes_inner on the node held by this workgroup (see the
+// file comment).  a0/a1: s0.a/s1.a at this thread's coefficient positions,
+// b0/b1: s0.b/s1.b at its slot positions (canonical).  On return a0, b0 hold
+// the new node (canonical).  key: [ell][2][K][N] NTT-domain Montgomery rows.
+template <int LOGN, int K, int ELL, bool LAZY>
+__device__ __forceinline__ void tree_level(uint32_t (&a0)[8], const uint32_t (&a1)[8], uint32_t (&b0)[8],
+                                           const uint32_t (&b1)[8], uint32_t level, const uint32_t* key,
+                                           const TreeLds<LOGN, K, ELL>& L, const CtxDev& C, const ScaleDev& inv,
+                                           int k, int t) {
+    using Ly = Lay<LOGN>;
+    constexpr int N = Ly::N, T = Ly::T, LP = Ly::LP;
+    constexpr size_t KN = (size_t)K * N;
+    const Mod M{C.m[k].q, C.m[k].q2, C.m[k].qinv};
+    const uint32_t q = M.q;
+    const uint32_t taut = (1u << level) + 1u;
+    const uint32_t tinv = inv_odd(taut) & (2u * N - 1u);
+
+    // the key rows this residue multiplies by, loaded ahead of the transforms
+    uint32_t ka[ELL][8], kb[ELL][8];
+#pragma unroll
+    for (int j = 0; j < ELL; j++) {
+        load_slots<LOGN>(key + ((size_t)j * 2 + 0) * KN + (size_t)k * N, t, ka[j]);
+        load_slots<LOGN>(key + ((size_t)j * 2 + 1) * KN + (size_t)k * N, t, kb[j]);
+    }
+
+    lds_barrier();  // every thread is past its reads of the previous level
+    const int cix = lds_ix(t);
+    uint32_t* ab = L.abuf + k * LP;
+    uint32_t* bb = L.bbuf + k * LP;
+#pragma unroll
+    for (int e = 0; e < 8; e++) ab[lds_step(cix, e, T)] = a1[e];
+#pragma unroll
+    for (int g = 0; g < 2; g++)
+        *reinterpret_cast<uint4*>(&bb[lds_ix(Ly::group(t, g))]) =
+            make_uint4(b1[4 * g + 0], b1[4 * g + 1], b1[4 * g + 2], b1[4 * g + 3]);
+    lds_barrier();
+
+    // gadget digits of tau_t(s1.a) at the coefficient positions:
+    // tau_t(x)_j = x_i for t i = j mod 2N, and -x_i for t i = j + N mod 2N
+    uint32_t dg[ELL][8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const uint32_t j = (uint32_t)(t + e * T);
+        const uint32_t src = (tinv * j) & (2u * N - 1u);
+        const bool neg = src >= (uint32_t)N;
+        const int sx = lds_ix((int)(src & (N - 1)));
+        uint32_t c[K];
+#pragma unroll
+        for (int kk = 0; kk < K; kk++) {
+            const uint32_t v = L.abuf[kk * LP + sx];
+            c[kk] = neg ? negq(v, C.m[kk].q) : v;
+        }
+        uint64_t x = c[0];
+        if constexpr (K == 2) {
+            const uint32_t q1 = C.m[1].q;
+            const uint32_t a0m = csub(shoup_mul(c[0], 1u, C.one_q1p, q1), q1);
+            const uint32_t u = csub(shoup_mul(c[1] + q1 - a0m, C.q0_inv_q1, C.q0_inv_q1p, q1), q1);
+            x = (uint64_t)c[0] + (uint64_t)C.m[0].q * u;
+        }
+#pragma unroll
+        for (int j2 = 0; j2 < ELL; j2++) {
+            const uint32_t sh = j2 * C.gadget_log;
+            const uint32_t v = sh < 64 ? (uint32_t)((x >> sh) & C.gadget_mask) : 0u;
+            dg[j2][e] = C.small_digits ? v : reduce32(v, q, C.onep[k]);
+        }
+    }
+    // tau_t(s1.b) at the slot positions
+    uint32_t bt[8];
+#pragma unroll
+    for (int g = 0; g < 2; g++)
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+            bt[4 * g + c] = bb[lds_ix((int)aut_slot<LOGN>((uint32_t)(Ly::group(t, g) + c), taut))];
+
+    ntt_digits<LOGN, ELL>(dg, t, L.dbuf + k * ELL * LP, L.twf + k * N, M);
+
+    uint32_t A[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        uint64_t sa = 0, sb = 0;
+#pragma unroll
+        for (int j = 0; j < ELL; j++) {
+            const uint32_t x = LAZY ? dg[j][e] : csub(csub(dg[j][e], M.q2), q);
+            sa += (uint64_t)x * ka[j][e];
+            sb += (uint64_t)x * kb[j][e];
+        }
+        A[e] = mont_reduce(sa, q, M.qinv);  // < 2q, the inverse transform's input range
+        b0[e] = addq(addq(b0[e], bt[e], q), csub(mont_reduce(sb, q, M.qinv), q), q);
+    }
+    // the digit transform's barriers ordered every read of abuf before this
+    intt1<LOGN>(A, t, ab, L.twi + k * N, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
+#pragma unroll
+    for (int e = 0; e < 8; e++) a0[e] = addq(a0[e], csub(A[e], q), q);
+}
+
+template <int LOGN, int K, int ELL, bool LAZY>
+__global__ void __launch_bounds__(K * NttShape<LOGN>::T, 1)
+k_tree_leaves(TreeArgs A, CtxDev C, ScaleDev inv) {
+    using Ly = Lay<LOGN>;
+    constexpr int N = Ly::N, T = Ly::T, LP = Ly::LP;
+    constexpr size_t KN = (size_t)K * N;
+    extern __shared__ __align__(16) uint32_t lds[];
+    const TreeLds<LOGN, K, ELL> L(lds);
+    const int tid = threadIdx.x, k = tid / T, t = tid - k * T;
+    const uint32_t qi = blockIdx.x / A.m, s = blockIdx.x - qi * A.m;
+    const Mod M{C.m[k].q, C.m[k].q2, C.m[k].qinv};
+    const uint32_t q = M.q;
+
+    // first dimension of column s (external_product of (0, Delta h) with RGSW
+    // rows ell..2ell-1; the a-digits are 0): NTT domain, slot positions
+    const uint32_t* dig = reinterpret_cast<const uint32_t*>(A.qdig[qi]) + (size_t)s * ELL * KN + (size_t)k * N;
+    const uint32_t* rw = A.rows + (size_t)qi * ELL * 2 * KN + (size_t)k * N;
+    uint32_t dv[ELL][8], ra[ELL][8], rb[ELL][8];
+#pragma unroll
+    for (int j = 0; j < ELL; j++) {
+        load_slots<LOGN>(dig + (size_t)j * KN, t, dv[j]);
+        load_slots<LOGN>(rw + (size_t)j * 2 * KN, t, ra[j]);
+        load_slots<LOGN>(rw + (size_t)j * 2 * KN + KN, t, rb[j]);
+    }
+    stage_twiddles(L, C);
+    uint32_t ah[8];
+    uint64_t bsum = 0;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        uint64_t sa = 0, sb = 0;
+#pragma unroll
+        for (int j = 0; j < ELL; j++) {
+            sa += (uint64_t)dv[j][e] * ra[j][e];
+            sb += (uint64_t)dv[j][e] * rb[j][e];
+        }
+        ah[e] = mont_reduce(sa, q, M.qinv);
+        bsum += csub(mont_reduce(sb, q, M.qinv), q);
+    }
+    // LWE b = coefficient 0 of the column's b = n^-1 * sum of its slots
+    constexpr int W = TreeShape<LOGN, K, ELL>::W;  // a group never spans two residues
+#pragma unroll
+    for (int o = W / 2; o >= 1; o >>= 1) bsum += __shfl_xor(bsum, o, W);
+    if (tid % W == 0) L.red[tid / W] = bsum;
+    __syncthreads();  // also orders the twiddle stores before their first use
+    uint64_t tot = 0;
+#pragma unroll
+    for (int w = 0; w < T / W; w++) tot += L.red[k * (T / W) + w];
+    // (tot 2^-32) (n^-1 2^32) = tot n^-1; tot < N q < q 2^32
+    const uint32_t lwe_b =
+        csub(shoup_mul(csub(mont_reduce(tot, q, M.qinv), q), A.ninvm[k], A.ninvmp[k], q), q);
+
+    // the leaf: (a, 0) with a the column's a (invert_sample_extract undoes
+    // sample_extract_coeff0, automorph_pack.rs:349-405)
+    uint32_t a[8], b[8];
+    intt1<LOGN>(ah, t, L.abuf + k * LP, L.twi + k * N, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        a[e] = csub(ah[e], q);
+        b[e] = 0;
+    }
+
+    if (A.m == 1) {  // a single LWE is not tree-packed (automorph_pack.rs:436-443)
+        uint32_t* o = A.out + (size_t)qi * A.out_words;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int j = t + e * T;
+            o[(size_t)k * N + j] = a[e];
+            o[KN + (size_t)k * N + j] = j == 0 ? lwe_b : 0u;
+        }
+        return;
+    }
+    if (t == 0) A.lweb[((size_t)qi * A.m + s) * K + k] = lwe_b;
+
+    for (uint32_t lv = A.level_lo; lv <= A.level_hi; lv++)
+        tree_level<LOGN, K, ELL, LAZY>(a, a, b, b, lv, A.ks + (size_t)(A.logn - lv) * ELL * 2 * KN, L, C, inv, k,
+                                       t);
+
+    uint32_t* nd = A.node + ((size_t)qi * A.m + s) * 2 * KN + (size_t)k * N;
+#pragma unroll
+    for (int e = 0; e < 8; e++) nd[t + e * T] = a[e];
+#pragma unroll
+    for (int g = 0; g < 2; g++)
+        *reinterpret_cast<uint4*>(nd + KN + Ly::group(t, g)) = make_uint4(b[4 * g], b[4 * g + 1], b[4 * g + 2], b[4 * g + 3]);
+}
+
+// One level after the leaves: node s of level `level` from nodes s and
+// s + 2^(log d - level) of the level below (in place: outputs s <
+// 2^(log d - level) never hold an odd input).  At the root the packed
+// ciphertext is finished into out.
+template <int LOGN, int K, int ELL, bool LAZY>
+__global__ void __launch_bounds__(K * NttShape<LOGN>::T, 1)
+k_tree_combine(TreeArgs A, CtxDev C, ScaleDev inv) {
+    using Ly = Lay<LOGN>;
+    constexpr int N = Ly::N, T = Ly::T, LP = Ly::LP;
+    constexpr size_t KN = (size_t)K * N;
+    extern __shared__ __align__(16) uint32_t lds[];
+    const TreeLds<LOGN, K, ELL> L(lds);
+    const int tid = threadIdx.x, k = tid / T, t = tid - k * T;
+    const uint32_t level = A.level_lo;
+    const uint32_t half = 1u << (LOGN - level);  // nodes at this level: s < min(m, half)
+    const uint32_t nn = A.m < half ? A.m : half;
+    const uint32_t qi = blockIdx.x / nn, s = blockIdx.x - qi * nn;
+    const bool odd = s + half < A.m;
+    const uint32_t q = C.m[k].q;
+    const uint32_t* ev = A.node + ((size_t)qi * A.m + s) * 2 * KN + (size_t)k * N;
+    const uint32_t* od = A.node + ((size_t)qi * A.m + s + half) * 2 * KN + (size_t)k * N;
+
+    uint32_t a0[8], a1[8], b0[8], b1[8];
+    const uint32_t ysh = (uint32_t)N >> level;  // y = X^(N / 2^level)
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const uint32_t j = (uint32_t)(t + e * T);
+        const uint32_t x = ev[j];
+        uint32_t v = 0;
+        if (odd) v = j >= ysh ? od[j - ysh] : negq(od[j + N - ysh], q);  // (X^ysh odd.a)_j
+        a0[e] = addq(x, v, q);
+        a1[e] = subq(x, v, q);
+    }
+    {
+        uint32_t eb[8], ob[8], yh[8];
+        load_slots<LOGN>(ev + KN, t, eb);
+        if (odd) {
+            load_slots<LOGN>(od + KN, t, ob);
+            load_slots<LOGN>(A.yhat + ((size_t)(level - 1) * K + k) * N, t, yh);
+        }
+        const Mod M{q, C.m[k].q2, C.m[k].qinv};
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const uint32_t v = odd ? csub(mmul(ob[e], yh[e], M), q) : 0u;
+            b0[e] = addq(eb[e], v, q);
+            b1[e] = subq(eb[e], v, q);
+        }
+    }
+    stage_twiddles(L, C);  // the level's first barrier orders these stores
+    tree_level<LOGN, K, ELL, LAZY>(a0, a1, b0, b1, level, A.ks + (size_t)(A.logn - level) * ELL * 2 * KN, L, C,
+                                   inv, k, t);
+
+    if (level < (uint32_t)LOGN) {
+        // each thread rewrites exactly the words of node s it read
+        uint32_t* nd = A.node + ((size_t)qi * A.m + s) * 2 * KN + (size_t)k * N;
+#pragma unroll
+        for (int e = 0; e < 8; e++) nd[t + e * T] = a0[e];
+#pragma unroll
+        for (int g = 0; g < 2; g++)
+            *reinterpret_cast<uint4*>(nd + KN + Ly::group(t, g)) =
+                make_uint4(b0[4 * g], b0[4 * g + 1], b0[4 * g + 2], b0[4 * g + 3]);
+        return;
+    }
+
+    // root: b to the coefficient domain, then b_val * d at coefficient z < m
+    const Mod M{q, C.m[k].q2, C.m[k].qinv};
+    lds_barrier();  // every thread is past the level's last reads of abuf
+    intt1<LOGN>(b0, t, L.abuf + k * LP, L.twi + k * N, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
+    uint32_t* o = A.out + (size_t)qi * A.out_words;
+    uint32_t* ob = o + KN;
+    const uint32_t* lw = A.lweb + (size_t)qi * A.m * K;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const uint32_t j = (uint32_t)(t + e * T);
+        o[(size_t)k * N + j] = a0[e];
+        const uint32_t bv = csub(b0[e], q);
+        if (!A.ref_layout) {
+            uint32_t v = bv;
+            if (j < A.m) v = addq(v, csub(shoup_mul(lw[(size_t)j * K + k], A.dmod[k], A.dmodp[k], q), q), q);
+            ob[(size_t)k * N + j] = v;
+        } else if (k == 0) {
+            // b.coeffs() = [res0 | res1] read as 2N composite coefficients
+            // (Poly::from_coeffs_moduli, poly.rs:127-159): word z < N is
+            // (b0_z + b_val_z d) mod q, cut into both residues
+#pragma unroll
+            for (int kk = 0; kk < K; kk++) {
+                const uint32_t qk = C.m[kk].q;
+                uint32_t v = kk == 0 ? bv : reduce32(bv, qk, C.onep[kk]);
+                if (j < A.m) v = addq(v, csub(shoup_mul(lw[(size_t)j * K + kk], A.dmod[kk], A.dmodp[kk], qk), qk), qk);
+                ob[(size_t)kk * 2 * N + j] = v;
+            }
+        } else {
+            // word N + j is b1_j, below q1, cut into both residues
+#pragma unroll
+            for (int kk = 0; kk < K; kk++)
+                ob[(size_t)kk * 2 * N + N + j] = kk == 1 ? bv : reduce32(bv, C.m[kk].q, C.onep[kk]);
+        }
+    }
+}
+
+template <int LOGN, int K, int ELL, bool LAZY>
+static hipError_t tree_launch(const TreeArgs& A, bool leaves, uint32_t blocks, const CtxDev& C, const ScaleDev& inv,
+                              hipStream_t st) {
+    constexpr size_t shmem = TreeShape<LOGN, K, ELL>::kBytes;
+    static_assert(shmem <= 160 * 1024, "tree kernel LDS");
+    auto kern = leaves ? k_tree_leaves<LOGN, K, ELL, LAZY> : k_tree_combine<LOGN, K, ELL, LAZY>;
+    // raise the dynamic LDS limit once per instantiation
+    static const hipError_t e0 =
+        shmem > 64 * 1024 ? hipFuncSetAttribute((const void*)k_tree_leaves<LOGN, K, ELL, LAZY>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem)
+                          : hipSuccess;
+    static const hipError_t e1 =
+        shmem > 64 * 1024 ? hipFuncSetAttribute((const void*)k_tree_combine<LOGN, K, ELL, LAZY>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem)
+                          : hipSuccess;
+    if (e0 != hipSuccess) return e0;
+    if (e1 != hipSuccess) return e1;
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(K * NttShape<LOGN>::T), shmem, st, A, C, inv);
+    return hipGetLastError();
+}
+
+template <int LOGN, int K, int ELL>
+static hipError_t tree_lazy(const TreeArgs& A, bool leaves, uint32_t blocks, const CtxDev& C, const ScaleDev& inv,
+                            hipStream_t st) {
+    uint64_t qmax = 0;
+    for (int kk = 0; kk < K; kk++) qmax = C.m[kk].q > qmax ? C.m[kk].q : qmax;
+    if ((uint64_t)ELL * 4 * qmax < (1ull << 32)) return tree_launch<LOGN, K, ELL, true>(A, leaves, blocks, C, inv, st);
+    return tree_launch<LOGN, K, ELL, false>(A, leaves, blocks, C, inv, st);
+}
+
+template <int LOGN, int K>
+static hipError_t tree_ell(const TreeArgs& A, bool leaves, uint32_t blocks, const CtxDev& C, const ScaleDev& inv,
+                           hipStream_t st) {
+    switch (C.ell) {
+        case 1: return tree_lazy<LOGN, K, 1>(A, leaves, blocks, C, inv, st);
+        case 2: return tree_lazy<LOGN, K, 2>(A, leaves, blocks, C, inv, st);
+        case 3: return tree_lazy<LOGN, K, 3>(A, leaves, blocks, C, inv, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <int LOGN>
+static hipError_t tree_k(const TreeArgs& A, bool leaves, uint32_t blocks, const CtxDev& C, const ScaleDev& inv,
+                         hipStream_t st) {
+    if (C.K == 1) return tree_ell<LOGN, 1>(A, leaves, blocks, C, inv, st);
+    if (C.K == 2) return tree_ell<LOGN, 2>(A, leaves, blocks, C, inv, st);
+    return hipErrorInvalidValue;
+}
+
+}  // namespace
+
+bool tree_supported(const CtxDev& C) {
+    if (C.K < 1 || C.K > 2 || C.ell < 1 || C.ell > 3 || C.logn < 8 || C.logn > 11) return false;
+    for (uint32_t k = 0; k < C.K; k++)
+        if (C.m[k].q >= (1u << 30)) return false;  // lazy ranges: 4q < 2^32
+    return true;
+}
+
+hipError_t launch_tree(const TreeArgs& A, bool leaves, uint32_t blocks, const CtxDev& C, const ScaleDev& inv,
+                       hipStream_t st) {
+    if (!blocks) return hipSuccess;
+    switch (C.logn) {
+        case 8: return tree_k<8>(A, leaves, blocks, C, inv, st);
+        case 9: return tree_k<9>(A, leaves, blocks, C, inv, st);
+        case 10: return tree_k<10>(A, leaves, blocks, C, inv, st);
+        case 11: return tree_k<11>(A, leaves, blocks, C, inv, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace inspire

@@ -8,6 +8,8 @@ argument meaning and errors follow the Rust crate `inspire`:
     respond_inspiring(ctx, db_id, query)       src/pir/respond.rs:284-386
     respond_seeded_inspiring(ctx, db_id, query) src/pir/respond.rs:391-398
     respond_with_variant(ctx, db_id, query, v) src/pir/respond.rs:129-183
+    respond_one_packing(ctx, db_id, query)     src/pir/respond.rs:213-268
+    respond_seeded_packed(ctx, db_id, query)   src/pir/respond.rs:428-435
 
 Errors raise PirError carrying the reference's message text
 (src/pir/error.rs).  There is no CPU fallback: importing this module without
@@ -96,6 +98,8 @@ class _DeviceBatch(C.Structure):
         ("y_body", C.c_void_p),
         ("out", C.c_void_p),
         ("status", C.POINTER(C.c_int32)),
+        ("packing_mode", C.c_uint32),
+        ("reserved", C.c_uint32),
     ]
 
 
@@ -105,6 +109,8 @@ class _Response(C.Structure):
         ("column_ciphertexts", C.POINTER(C.c_uint64)),
         ("column_capacity", C.c_uint32),
         ("num_column_ciphertexts", C.c_uint32),
+        ("ciphertext_capacity", C.c_uint32),
+        ("ciphertext_b_dim", C.c_uint32),
     ]
 
 
@@ -160,6 +166,9 @@ EXPORTS = [
     ("inspire_respond_inspiring", C.c_int, [C.c_void_p, C.POINTER(_Query), C.POINTER(_Response)]),
     ("inspire_respond_seeded_inspiring", C.c_int, [C.c_void_p, C.POINTER(_Query), C.POINTER(_Response)]),
     ("inspire_respond_with_variant", C.c_int, [C.c_void_p, C.POINTER(_Query), C.c_uint32, C.POINTER(_Response)]),
+    ("inspire_respond_one_packing", C.c_int, [C.c_void_p, C.POINTER(_Query), C.POINTER(_Response)]),
+    ("inspire_respond_seeded_packed", C.c_int, [C.c_void_p, C.POINTER(_Query), C.POINTER(_Response)]),
+    ("inspire_ctx_set_galois_keys", C.c_int, [C.c_void_p, _u64p, C.c_uint32]),
     (
         "inspire_respond_batch",
         C.c_int,
@@ -281,10 +290,31 @@ class ClientQuery:
 
 @dataclass
 class ServerResponse:
-    """ServerResponse (src/pir/respond.rs:30-37)."""
+    """ServerResponse (src/pir/respond.rs:30-37).
 
-    ciphertext: np.ndarray                       # uint64[2, K, d]
+    ciphertext is uint64[2, K, d], except for a tree-packed response whose b
+    has 2d coefficients per residue (two moduli, at least two columns,
+    src/inspiring/automorph_pack.rs:476-484): then it is the flat words of
+    a.coeffs followed by b.coeffs.  a and b give the two parts either way."""
+
+    ciphertext: np.ndarray
     column_ciphertexts: np.ndarray = field(default=None)  # uint64[n, 2, K, d]
+    b_dim: int = 0
+
+    @property
+    def a(self) -> np.ndarray:
+        K, d = self.ciphertext.shape[1:] if self.ciphertext.ndim == 3 else self._kd()
+        return self.ciphertext.reshape(-1)[:K * d].reshape(K, d)
+
+    @property
+    def b(self) -> np.ndarray:
+        K, d = self.ciphertext.shape[1:] if self.ciphertext.ndim == 3 else self._kd()
+        return self.ciphertext.reshape(-1)[K * d:].reshape(K, -1)
+
+    def _kd(self):
+        words = self.ciphertext.size  # K d + K b_dim with b_dim = 2d
+        d = self.b_dim // 2
+        return words // (3 * d), d
 
 
 class Context:
@@ -356,6 +386,17 @@ class Context:
                 break
         self._dbs[db_id] = cols
 
+    def set_galois_keys(self, galois_keys: np.ndarray):
+        """ServerCrs.galois_keys (src/pir/setup.rs:167-173): log2(d)
+        key-switching matrices, uint64[log2 d, ell, 2, K, d]."""
+        gk = np.ascontiguousarray(galois_keys, dtype=np.uint64)
+        _raise(lib().inspire_ctx_set_galois_keys(self._h, _ptr64(gk), gk.shape[0] if gk.ndim else 0))
+
+    def tree_b_dim(self, cols: int) -> int:
+        """b coefficients per residue of a tree-packed response."""
+        rw = self.params.flags & FLAG_RESIDUE_WISE_AUTOMORPHISM
+        return 2 * self.d if (self.K == 2 and not rw and cols >= 2) else self.d
+
     def register_client_keys(self, client_id: int, y_body: np.ndarray):
         """Resident packing keys of a client: y_body of its ClientPackingKeys
         (src/inspiring/inspiring2.rs:603-628), uint64[ell, K, d]."""
@@ -393,21 +434,28 @@ class Context:
         return cq
 
     def _response(self, cols: int, keep: list):
-        ct = np.zeros((2, self.K, self.d), dtype=np.uint64)
+        ct = np.zeros(3 * self.K * self.d, dtype=np.uint64)  # room for a tree-packed b of 2d coefficients
         cc = np.zeros((max(cols, 1), 2, self.K, self.d), dtype=np.uint64)
         keep += [ct, cc]
         r = _Response()
         r.ciphertext = _ptr64(ct)
         r.column_ciphertexts = _ptr64(cc)
         r.column_capacity = cc.shape[0]
+        r.ciphertext_capacity = ct.size
         return r, ct, cc
+
+    def _result(self, r: _Response, ct: np.ndarray, cc: np.ndarray) -> ServerResponse:
+        bd = r.ciphertext_b_dim or self.d
+        words = ct[: self.K * self.d + self.K * bd]
+        ciphertext = words.reshape(2, self.K, self.d) if bd == self.d else words
+        return ServerResponse(ciphertext, cc[: r.num_column_ciphertexts], bd)
 
     def _call(self, fn, db_id: int, q: ClientQuery, *extra) -> ServerResponse:
         keep: list = []
         cq = self._query(db_id, q, keep)
         r, ct, cc = self._response(self._dbs.get(db_id, 0), keep)
         _raise(fn(self._h, C.byref(cq), *extra, C.byref(r)))
-        return ServerResponse(ct, cc[: r.num_column_ciphertexts])
+        return self._result(r, ct, cc)
 
     def respond_batch(self, db_ids: Sequence[int], queries: Sequence[ClientQuery], variants: Sequence[int]):
         """Many queries in one batched launch sequence -> list of ServerResponse or PirError."""
@@ -430,12 +478,13 @@ class Context:
                 res.append(PirError(st[i], f"query {i} failed with status {st[i]}"))
             else:
                 ct, cc = outs[i]
-                res.append(ServerResponse(ct, cc[: ra[i].num_column_ciphertexts]))
+                res.append(self._result(ra[i], ct, cc))
         return res
 
     # -- device-resident batches and profiling
     def respond_batch_device(self, db_id: int, variant: int, shard_ids, rows_ptr: int, out_ptr: int,
-                             seeds_ptr: int = 0, y_body_ptr: int = 0, seeded: bool = True):
+                             seeds_ptr: int = 0, y_body_ptr: int = 0, seeded: bool = True,
+                             packing_mode: int = PACKING_INSPIRING):
         """Runs one batch whose inputs live in device memory (pointers from e.g.
         torch tensors; layouts in include/inspire_mi355x.h).  Returns the
         per-query status array."""
@@ -443,7 +492,7 @@ class Context:
         st = np.zeros(ids.size, dtype=np.int32)
         b = _DeviceBatch(ids.size, db_id, variant, 1 if seeded else 0, ids.ctypes.data_as(C.POINTER(C.c_uint32)),
                          rows_ptr, seeds_ptr or None, y_body_ptr or None, out_ptr,
-                         st.ctypes.data_as(C.POINTER(C.c_int32)))
+                         st.ctypes.data_as(C.POINTER(C.c_int32)), packing_mode, 0)
         _raise(lib().inspire_respond_batch_device(self._h, C.byref(b)))
         return st
 
@@ -549,7 +598,7 @@ class Server:
         K, d = self.ctx.K, self.ctx.d
         ct = np.zeros((2, K, d), dtype=np.uint64)
         cc = np.zeros((max(cols, 1), 2, K, d), dtype=np.uint64)
-        r = _Response(_ptr64(ct), _ptr64(cc) if cols else None, cols, 0)
+        r = _Response(_ptr64(ct), _ptr64(cc) if cols else None, cols, 0, ct.size, 0)
         st = C.c_int32()
         done = C.c_uint64()
         rc = lib().inspire_server_wait(self._h, int(ticket), timeout_ms, C.byref(r), C.byref(st), C.byref(done))
@@ -664,3 +713,14 @@ def respond_with_variant(ctx: Context, db_id: int, query: ClientQuery, variant: 
 def respond_seeded_with_variant(ctx: Context, db_id: int, query: ClientQuery, variant: int) -> ServerResponse:
     assert query.seeded
     return ctx._call(lib().inspire_respond_with_variant, db_id, query, variant)
+
+
+def respond_one_packing(ctx: Context, db_id: int, query: ClientQuery) -> ServerResponse:
+    """respond_one_packing (src/pir/respond.rs:213-268): tree packing with the
+    CRS galois keys (Context.set_galois_keys)."""
+    return ctx._call(lib().inspire_respond_one_packing, db_id, query)
+
+
+def respond_seeded_packed(ctx: Context, db_id: int, query: ClientQuery) -> ServerResponse:
+    """respond_seeded_packed (src/pir/respond.rs:428-435)."""
+    return ctx._call(lib().inspire_respond_seeded_packed, db_id, query)

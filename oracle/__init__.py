@@ -427,7 +427,7 @@ class Oracle:
         arr = lambda xs: (_u64p * n)(*[_p64(x) for x in xs])
         lib.or_respond_one_packing_many(C.byref(self.cp), self.ntt, n, arr(keep[:n]), ncols, arr(keep[n:]), _p64(gk),
                                         gk.shape[0], arr(outs))
-        bdim = 2 * d if K == 2 else d
+        bdim = 2 * d if (K == 2 and not self.residue_wise and ncols >= 2) else d
         return [o[:K * d + K * bdim] for o in outs]
 
     def automorphism_ks_matrix(self, sk, t, r, sigma=6.4):

@@ -40,8 +40,15 @@ def inputs():
     yb, ya = cl.packing_keys(16, W_SEED)
     cs = Client(single, 9)
     db_s = random_db(256, 32, seed=25, values_below_256=True)
+    # ServerCrs.galois_keys under the client's key, from a stream of their own
+    gk = crt.galois_keys(cl.sk, crt.rng(bytes(range(40, 72))))
     return dict(crt=crt, single=single, cl=cl, db=db, db_small=db_small, rows=rows, seeds=seeds, b=b,
-                yb=yb, ya=ya, cs=cs, db_s=db_s, rows_s=cs.query_rows(42), keys_s=cs.packing_keys(16, W_SEED))
+                yb=yb, ya=ya, cs=cs, db_s=db_s, rows_s=cs.query_rows(42), keys_s=cs.packing_keys(16, W_SEED), gk=gk)
+
+
+def tree_words(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """A tree-packed ciphertext as the flat words a.coeffs ++ b.coeffs."""
+    return np.concatenate([np.ascontiguousarray(a).reshape(-1), np.ascontiguousarray(b).reshape(-1)])
 
 
 def build_cases(inp=None):
@@ -76,6 +83,9 @@ def build_cases(inp=None):
     s_shards = encode_db(single, inp["db_s"], 32)
     _, ct = single.respond_inspiring(W_SEED, s_shards[0], inp["rows_s"], y_body=inp["keys_s"][0])
     cases.append(("single_inspiring", ct))
+    cases.append(("tree_full", tree_words(*crt.respond_one_packing(shards[0], inp["rows"], inp["gk"]))))
+    cases.append(("tree_seeded", tree_words(*crt.respond_one_packing(shards[0], rows_x, inp["gk"]))))
+    cases.append(("tree_residue_wise", tree_words(*rw.respond_one_packing(small[0], inp["rows"], inp["gk"]))))
     return cases
 
 

@@ -107,3 +107,14 @@ def coeff0_message(o: O.Oracle, value: int) -> np.ndarray:
     mp = o.zeros()
     mp[:, 0] = value
     return mp
+
+
+def phase_error(o: O.Oracle, sk, a, b, want) -> float:
+    """max over coefficients of |a*s + b - Delta*want| (centred mod q) / Delta."""
+    q, delta = o.params.q, o.params.delta
+    ph = o.mul_ntt(a, sk)
+    for k, m in enumerate(o.params.moduli):
+        ph[k] = (ph[k] + b[k]) % m
+    comp = [int(x) for x in o.coeffs(ph)]
+    errs = [(c - delta * int(w)) % q for c, w in zip(comp, want)]
+    return max(min(e, q - e) for e in errs) / delta

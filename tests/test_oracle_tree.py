@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from helpers import Client, coeff0_message, encode_db, random_db, rlwe_encrypt
+from helpers import Client, coeff0_message, encode_db, phase_error, random_db, rlwe_encrypt
 
 
 @pytest.fixture(scope="module")
@@ -41,17 +41,6 @@ def lwes(o, sk, r, msgs):
 
 def dec(o, sk, a, b):
     return o.decrypt(np.stack([a, b]), sk)
-
-
-def phase_error(o, sk, a, b, want):
-    """max over coefficients of |a*s + b - Delta*want| (centred mod q) / Delta."""
-    q, delta = o.params.q, o.params.delta
-    ph = o.mul_ntt(a, sk)
-    for k, m in enumerate(o.params.moduli):
-        ph[k] = (ph[k] + b[k]) % m
-    comp = [int(x) for x in o.coeffs(ph)]
-    errs = [(c - delta * int(w)) % q for c, w in zip(comp, want)]
-    return max(min(e, q - e) for e in errs) / delta
 
 
 def test_galois_keys_are_the_crs_set(single):
