@@ -59,6 +59,27 @@ HBM_PEAK_GBS = 8000.0
 MODMUL_PEAK_T = 157.3 / 2 / 2 / 3
 
 
+def tree_modmul(m: int = GAMMA):
+    """Algorithmic modular multiplications per query of tree packing
+    (respond_one_packing) over m columns: (leaves kernel, combine kernels).
+    A tree node at level l costs the digit NTTs (ell K), the key MAC
+    (2 ell K d) and the INTT of the new a (K); nodes with an odd child also
+    multiply odd.b by y in the NTT domain (K d).  The leaves kernel adds the
+    first-dimension MAC (2 ell K d per column) and each column's a INTT (K);
+    the root adds b's INTT (K)."""
+    L = D.bit_length() - 1
+    node = (ELL * K + K) * NTT + 2 * ELL * K * D
+    lm = (m - 1).bit_length()
+    leaves = m * (2 * ELL * K * D + K * NTT) + m * (L - lm) * node
+    combine = 0
+    for lv in range(L - lm + 1, L + 1):
+        half = 1 << (L - lv)
+        nodes = min(m, half)
+        odd = max(0, min(m - half, half))
+        combine += nodes * node + odd * K * D
+    return leaves, combine + K * NTT
+
+
 def synthetic_queries(rng: np.random.Generator, n: int, nshards: int):
     moduli = np.array([Q0, Q1], dtype=np.uint64)[:, None]
     b = (rng.integers(0, 2**63, (n, ELL, K, D), dtype=np.uint64) % moduli).astype(np.uint32)
@@ -130,7 +151,8 @@ def cpu_baseline(db_bytes: np.ndarray, qset, gpu_out: np.ndarray, budget_s: floa
         "unit": "queries/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{done} InsPIRe^2 queries of the timed workload, {t_total:.1f} s of CPU time; "
+        "sample": f"{done} InsPIRe^2 queries of the timed workload, {t_total:.2f} s wall on {threads} threads "
+                  f"({t_total * threads:.1f} core-s); "
                   f"PackParams/OfflinePackingKeys hoisted; bit-exact vs GPU: {done - mismatches}/{done}",
         "bit_exact_sampled": f"{done - mismatches}/{done}",
     }
@@ -146,6 +168,8 @@ def main():
     ap.add_argument("--single-steps", type=int, default=32, help="steps of the one-query-per-launch run")
     ap.add_argument("--host-steps", type=int, default=3, help="steps of the host-buffer (PCIe-inclusive) run")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--tree-steps", type=int, default=10,
+                    help="steps of the tree-packing (respond_seeded_packed) batch leg; 0 skips it")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (a known-size kernel for "
@@ -330,6 +354,8 @@ def main():
         "kernel_ms_per_step": {k_: round(ms / n, 4) for k_, (ms, n) in breakdown.items() if n},
         "cpu_baseline": None,
     }
+    if args.tree_steps:
+        line["tree_packing"] = tree_leg(args, M, ctx, db, dev, qsets, log)
     if not args.no_cpu:
         shards, b, seeds, y = qsets[(args.steps - 1) % n_sets]
         step(args.steps - 1)  # the batch of this query set, responses in pout
@@ -341,6 +367,115 @@ def main():
     dout.free()
     ctx.close()
     print(json.dumps(line))
+
+
+def tree_leg(args, M, ctx, db, dev, qsets, log):
+    """Tree packing (respond_seeded_packed, src/pir/respond.rs:428-435) of the
+    same seeded query batches over the same resident database, with
+    synthetic CRS galois keys: queries/s, per-kernel times, the leaves
+    kernel's modmul roofline, and a sample checked bit for bit against the
+    oracle (which is also the CPU baseline of this leg)."""
+    B = args.batch
+    rng = np.random.default_rng(2027)
+    moduli = np.array([Q0, Q1], dtype=np.uint64)[:, None]
+    L = D.bit_length() - 1
+    gk = rng.integers(0, 2**63, (L, ELL, 2, K, D), dtype=np.uint64) % moduli
+    ctx.set_galois_keys(gk)
+    words = K * D + K * 2 * D
+    tout = M.DeviceArray(B * words * 4)
+
+    def tstep(i):
+        shards, db_, ds_, _ = dev[i % len(dev)]
+        st = ctx.respond_batch_device(0, M.TWO_PACKING, shards, db_.ptr, tout.ptr, seeds_ptr=ds_.ptr,
+                                      packing_mode=M.PACKING_TREE)
+        if (st != 0).any():
+            raise RuntimeError(f"status {st[st != 0][:4]}")
+
+    for i in range(args.warmup):
+        tstep(i)
+    names = ("tree_leaves", "tree_combine", "expand_seeds")
+    ctx.set_profiling_kernels(["tree_leaves", "tree_combine"])
+    ctx.set_profiling(True)
+    for nm in names:
+        ctx.profile(nm, reset=True)
+    lat = []
+    t_start = time.perf_counter()
+    for i in range(args.tree_steps):
+        t0 = time.perf_counter()
+        tstep(i)
+        lat.append((time.perf_counter() - t0) * 1e3)
+    elapsed = time.perf_counter() - t_start
+    prof = {nm: ctx.profile(nm, reset=True) for nm in names}
+    ctx.set_profiling(False)
+    ctx.set_profiling_kernels(None)
+    qps = B * args.tree_steps / elapsed
+    mm_leaves, mm_combine = tree_modmul()
+    lv_ms, lv_n = prof["tree_leaves"]
+    cb_ms, cb_n = prof["tree_combine"]
+    lv_avg = lv_ms / max(lv_n, 1) / 1e3
+    achieved = mm_leaves * B / lv_avg / 1e12
+    log(f"tree packing: {qps:.0f} queries/s, {elapsed / args.tree_steps * 1e3:.2f} ms/step, "
+        f"leaves {lv_avg * 1e3:.3f} ms ({achieved:.2f} Tmodmul/s), combine {cb_ms / max(lv_n, 1):.3f} ms/step")
+    out = {
+        "workload": f"respond_seeded_packed (tree packing, 16 LWEs, 11 levels): {B} seeded queries per step, "
+                    "same 2^26 x 32 B database, synthetic CRS galois keys",
+        "value": round(qps, 1),
+        "unit": "queries/s",
+        "steps": args.tree_steps,
+        "ms_per_step": round(elapsed / args.tree_steps * 1e3, 3),
+        "p50_ms": round(percentile(lat, 50), 3),
+        "p99_ms": round(percentile(lat, 99), 3),
+        "modmul_per_query": mm_leaves + mm_combine + 2 * ELL * K * NTT,
+        "kernel_ms_per_step": {"tree_leaves": round(lv_avg * 1e3, 4),
+                               "tree_combine": round(cb_ms / max(lv_n, 1), 4)},
+        "roofline": {
+            "bound": "valu",
+            "kernel": "k_tree_leaves",
+            "achieved": round(achieved, 3),
+            "peak": round(MODMUL_PEAK_T, 2),
+            "unit": "Tmodmul/s",
+            "frac": round(achieved / MODMUL_PEAK_T, 4),
+            "traffic": measured_traffic("k_tree_leaves", B),
+            "algorithmic_per_launch": f"{mm_leaves} modmul/query x {B} queries",
+            "avg_launch_ms": round(lv_avg * 1e3, 4),
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, ROOT)
+        import oracle as O
+
+        o = O.Oracle(O.baseline_params())
+        i_last = args.tree_steps - 1
+        tstep(i_last)
+        gout = tout.download(np.zeros((B, words), dtype=np.uint32))
+        shards, b, seeds, _ = qsets[i_last % len(qsets)]
+        n = min(B, O.lib.or_num_threads())
+        polys, rgsws = [], []
+        for i in range(n):
+            s = int(shards[i])
+            polys.append(o.encode_shard(db[s * D * 32:(s + 1) * D * 32], 32, 0))
+            rg = np.zeros((2 * ELL, 2, K, D), dtype=np.uint64)
+            for r in range(ELL):
+                rg[ELL + r, 0] = o.poly_from_seed(seeds[i, r].astype("<u4").tobytes())
+                rg[ELL + r, 1] = b[i, r]
+            rgsws.append(rg)
+        t0 = time.perf_counter()
+        outs = o.respond_one_packing_many(polys, rgsws, gk)
+        t_cpu = time.perf_counter() - t0
+        ok = sum(np.array_equal(outs[i].astype(np.uint32), gout[i]) for i in range(n))
+        out["cpu_baseline"] = {
+            "value": n / t_cpu,
+            "unit": "queries/s",
+            "cores": O.lib.or_num_threads(),
+            "kind": "port",
+            "sample": f"{n} tree-packed queries of the timed workload, one per thread, {t_cpu:.1f} s wall; "
+                      f"bit-exact vs GPU: {ok}/{n}",
+            "bit_exact_sampled": f"{ok}/{n}",
+        }
+        log(f"tree cpu baseline: {n / t_cpu:.1f} queries/s, bit-exact {ok}/{n}")
+    tout.free()
+    return out
 
 
 SERVE_METRIC = ("serving loop: queries/s + p50/p99 respond latency, InsPIRe^0/^1/^2 at 10/30/60 %, "
