@@ -288,6 +288,48 @@ int inspire_server_get_stats(inspire_server *s, inspire_server_stats *out);
 /* Finishes the queries already submitted, then stops the workers. */
 void inspire_server_destroy(inspire_server *s);
 
+/* ---- wire formats ---- */
+
+/* The byte formats the reference's server and clients exchange, for the
+ * parameter set `params` (no device is touched):
+ *  - bincode 1.3 (bincode::serialize; ServerResponse::to_binary /
+ *    from_binary, src/pir/respond.rs:43-51): little-endian, u64 lengths,
+ *    fields in declaration order, skip_serializing_if fields left out;
+ *  - serde_json, the HTTP bodies of src/bin/server.rs:120-236.
+ * Polynomials are the reference's Poly (src/math/poly.rs:63-77): coeffs
+ * (residue-major words), moduli, q, dim, crt_q0_inv_mod_q1, is_ntt.
+ * The encoders write *len bytes; with out == NULL or a capacity below the
+ * size they return INSPIRE_ERR_CAPACITY with *len set, so a first call with
+ * out == NULL sizes the buffer. */
+int inspire_response_to_binary(const inspire_params *params, const inspire_response *r, uint8_t *out,
+                               uint64_t capacity, uint64_t *len);
+/* serde_json::to_vec of the ServerResponse (the "response" member of the
+ * server's QueryResponse, src/bin/server.rs:88-92). */
+int inspire_response_to_json(const inspire_params *params, const inspire_response *r, char *out,
+                             uint64_t capacity, uint64_t *len);
+/* ServerResponse::from_binary into caller buffers (ciphertext_capacity and
+ * column_capacity as for a respond call; ciphertext_b_dim is set). */
+int inspire_response_from_binary(const inspire_params *params, const uint8_t *in, uint64_t len,
+                                 inspire_response *r);
+
+/* A decoded query: owns the polynomials the inspire_query filled by the
+ * decoders points into.  Free it after the respond call. */
+typedef struct inspire_wire_query inspire_wire_query;
+/* A ClientQuery or SeededClientQuery (src/pir/query.rs:119-150) as the
+ * server's /query and /query_seeded handlers receive it (serde_json; a
+ * seeded query is recognised by its rows' seeds).  Fills q (db_id = 0) with
+ * pointers into *out.  The polynomials must carry the parameters' moduli and
+ * ring dimension, and the gadget their base and length; full-packing keys
+ * (z_body) are refused with INSPIRE_ERR_UNSUPPORTED. */
+int inspire_query_from_json(const inspire_params *params, const char *json, uint64_t len,
+                            inspire_wire_query **out, inspire_query *q);
+/* The same from bincode::serialize of a ClientQuery (seeded = 0) or a
+ * SeededClientQuery (seeded = 1).  inspiring_packing_keys is absent when
+ * None and ClientPackingKeys.z_body when empty (skip_serializing_if). */
+int inspire_query_from_binary(const inspire_params *params, const uint8_t *in, uint64_t len, uint32_t seeded,
+                              inspire_wire_query **out, inspire_query *q);
+void inspire_wire_query_free(inspire_wire_query *w);
+
 /* ---- profiling ---- */
 
 /* When enabled, every kernel launch of the respond path is bracketed by HIP

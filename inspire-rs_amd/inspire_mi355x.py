@@ -192,6 +192,16 @@ EXPORTS = [
         C.c_int,
         [C.c_void_p, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.c_int],
     ),
+    ("inspire_response_to_binary", C.c_int,
+     [C.POINTER(_Params), C.POINTER(_Response), C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
+    ("inspire_response_to_json", C.c_int,
+     [C.POINTER(_Params), C.POINTER(_Response), C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
+    ("inspire_response_from_binary", C.c_int, [C.POINTER(_Params), C.c_void_p, C.c_uint64, C.POINTER(_Response)]),
+    ("inspire_query_from_json", C.c_int,
+     [C.POINTER(_Params), C.c_char_p, C.c_uint64, C.POINTER(C.c_void_p), C.POINTER(_Query)]),
+    ("inspire_query_from_binary", C.c_int,
+     [C.POINTER(_Params), C.c_char_p, C.c_uint64, C.c_uint32, C.POINTER(C.c_void_p), C.POINTER(_Query)]),
+    ("inspire_wire_query_free", None, [C.c_void_p]),
     ("inspire_ntt_forward", C.c_int, [C.c_void_p, _u64p, C.c_uint64]),
     ("inspire_ntt_inverse", C.c_int, [C.c_void_p, _u64p, C.c_uint64]),
     ("inspire_external_product", C.c_int, [C.c_void_p, C.c_uint64, _u64p, _u64p, _u64p]),
@@ -724,3 +734,96 @@ def respond_one_packing(ctx: Context, db_id: int, query: ClientQuery) -> ServerR
 def respond_seeded_packed(ctx: Context, db_id: int, query: ClientQuery) -> ServerResponse:
     """respond_seeded_packed (src/pir/respond.rs:428-435)."""
     return ctx._call(lib().inspire_respond_seeded_packed, db_id, query)
+
+
+# -- wire formats (bincode / serde_json of the reference's types)
+
+def _response_struct(params: InspireParams, resp: ServerResponse, keep: list) -> _Response:
+    K, d = params.crt_count, params.ring_dim
+    ct = np.ascontiguousarray(resp.ciphertext, dtype=np.uint64).reshape(-1)
+    cols = resp.column_ciphertexts if resp.column_ciphertexts is not None else np.zeros((0, 2, K, d), np.uint64)
+    cc = np.ascontiguousarray(cols, dtype=np.uint64)
+    keep += [ct, cc]
+    r = _Response()
+    r.ciphertext = _ptr64(ct)
+    r.column_ciphertexts = _ptr64(cc) if cc.size else None
+    r.column_capacity = cc.shape[0]
+    r.num_column_ciphertexts = cc.shape[0]
+    r.ciphertext_capacity = ct.size
+    r.ciphertext_b_dim = resp.b_dim or (ct.size // K - d)
+    return r
+
+
+def _encode(fn, params: InspireParams, resp: ServerResponse) -> bytes:
+    keep: list = []
+    r = _response_struct(params, resp, keep)
+    p = params._c()
+    n = C.c_uint64()
+    rc = fn(C.byref(p), C.byref(r), None, 0, C.byref(n))
+    if rc != ERR_CAPACITY:
+        _raise(rc)
+    buf = C.create_string_buffer(n.value)
+    _raise(fn(C.byref(p), C.byref(r), buf, n.value, C.byref(n)))
+    return buf.raw[: n.value]
+
+
+def response_to_binary(params: InspireParams, resp: ServerResponse) -> bytes:
+    """ServerResponse::to_binary (src/pir/respond.rs:43-46): bincode 1.3."""
+    return _encode(lib().inspire_response_to_binary, params, resp)
+
+
+def response_to_json(params: InspireParams, resp: ServerResponse) -> bytes:
+    """serde_json::to_vec(&ServerResponse) (the body of src/bin/server.rs:88-92)."""
+    return _encode(lib().inspire_response_to_json, params, resp)
+
+
+def response_from_binary(params: InspireParams, data: bytes, max_columns: int = 64) -> ServerResponse:
+    """ServerResponse::from_binary (src/pir/respond.rs:48-51)."""
+    K, d = params.crt_count, params.ring_dim
+    ct = np.zeros(3 * K * d, dtype=np.uint64)
+    cc = np.zeros((max(max_columns, 1), 2, K, d), dtype=np.uint64)
+    r = _Response(_ptr64(ct), _ptr64(cc), cc.shape[0], 0, ct.size, 0)
+    p = params._c()
+    buf = C.create_string_buffer(bytes(data), len(data))
+    _raise(lib().inspire_response_from_binary(C.byref(p), buf, len(data), C.byref(r)))
+    bd = r.ciphertext_b_dim
+    words = ct[: K * d + K * bd]
+    return ServerResponse(words.reshape(2, K, d) if bd == d else words, cc[: r.num_column_ciphertexts], bd)
+
+
+def _decoded(params: InspireParams, handle: C.c_void_p, q: _Query) -> ClientQuery:
+    K, d, ell = params.crt_count, params.ring_dim, params.gadget_len
+    try:
+        if q.seeded:
+            rgsw = np.ctypeslib.as_array(q.rgsw, shape=(2 * ell * K * d,)).reshape(2 * ell, K, d).copy()
+            seeds = np.ctypeslib.as_array(q.rgsw_seeds, shape=(2 * ell * 32,)).reshape(2 * ell, 32).copy()
+        else:
+            rgsw = np.ctypeslib.as_array(q.rgsw, shape=(4 * ell * K * d,)).reshape(2 * ell, 2, K, d).copy()
+            seeds = None
+        y_body = None
+        if q.y_body:
+            y_body = np.ctypeslib.as_array(q.y_body, shape=(ell * K * d,)).reshape(ell, K, d).copy()
+        return ClientQuery(shard_id=q.shard_id, rgsw=rgsw, seeds=seeds, packing_mode=q.packing_mode,
+                           has_packing_keys=bool(q.has_packing_keys), y_body=y_body)
+    finally:
+        lib().inspire_wire_query_free(handle)
+
+
+def query_from_json(params: InspireParams, data: bytes) -> ClientQuery:
+    """A ClientQuery / SeededClientQuery as the server's handlers receive it
+    (serde_json, src/bin/server.rs:120-236)."""
+    p = params._c()
+    h = C.c_void_p()
+    q = _Query()
+    _raise(lib().inspire_query_from_json(C.byref(p), bytes(data), len(data), C.byref(h), C.byref(q)))
+    return _decoded(params, h, q)
+
+
+def query_from_binary(params: InspireParams, data: bytes, seeded: bool) -> ClientQuery:
+    """bincode::deserialize-side of a ClientQuery (seeded=False) or SeededClientQuery."""
+    p = params._c()
+    h = C.c_void_p()
+    q = _Query()
+    _raise(lib().inspire_query_from_binary(C.byref(p), bytes(data), len(data), 1 if seeded else 0, C.byref(h),
+                                           C.byref(q)))
+    return _decoded(params, h, q)
