@@ -126,6 +126,15 @@ int inspire_ctx_create(const inspire_params *params, const uint8_t w_seed[32], i
  * (a, b): keys = uint64_t[count][gadget_len][2][K][d], canonical residues.
  * They stay resident in device memory; setting them again replaces them. */
 int inspire_ctx_set_galois_keys(inspire_ctx *ctx, const uint64_t *keys, uint32_t count);
+/* A lane of ctx: a second handle on the same resident state (databases,
+ * CRS tables, galois keys, registered client keys) with its own HIP stream
+ * and workspaces.  Calls on different lanes, from different threads, run
+ * concurrently on the device, so one lane's host work and transfers overlap
+ * another's kernels (the reference's server answers concurrent requests from
+ * tokio handlers over one shared state, src/bin/server.rs:53-57,120-236).
+ * Loads, unloads and key registration go through the owner.  Destroy lanes
+ * (inspire_ctx_destroy) before their owner. */
+int inspire_ctx_create_lane(inspire_ctx *ctx, inspire_ctx **out);
 void inspire_ctx_destroy(inspire_ctx *ctx);
 const char *inspire_last_error(void);
 /* Device memory held by the context, in bytes. */

@@ -73,6 +73,13 @@ struct Database {
     uint32_t num_shards = 0;
     uint32_t cols = 0;                                // polys per shard (gamma)
     std::unordered_map<uint32_t, uint32_t> slot_of;   // ShardData.id -> slot (first match)
+    bool dense_ids = false;                           // ids are 0..num_shards-1 in slot order
+    // slot of a shard id, or -1 (the reference's find by id, respond.rs:78-82)
+    int64_t slot(uint32_t id) const {
+        if (dense_ids) return id < num_shards ? (int64_t)id : -1;
+        auto it = slot_of.find(id);
+        return it == slot_of.end() ? -1 : (int64_t)it->second;
+    }
     DevBuf digits;                                    // [shard][col][ell][K][N] u32, NTT domain
     size_t digit_words_per_shard = 0;                 // cols * ell * K * N
 };
@@ -142,7 +149,7 @@ struct inspire_ctx {
     // Writers (database load/unload, client registration) hold it exclusively,
     // serving lanes share it while a batch reads databases, tables and keys.
     std::shared_mutex rw;
-    bool rw_held = false;  // this context's own thread holds rw exclusively
+    std::mutex packs_mu;  // the packing-table map (built on first use, never erased)
     // A serving lane: its own stream and workspaces; databases, tables and
     // client keys belong to the owner context.
     inspire_ctx* owner = nullptr;
@@ -170,4 +177,9 @@ class ProfScope {
 };
 // Folds finished spans into the totals (synchronises the stream).
 void prof_collect(inspire_ctx* ctx);
+
+// A lane of `owner`: its own stream and workspaces over the owner's resident
+// state (databases, tables, keys).
+inspire_ctx* make_lane_ctx(inspire_ctx* owner);
+void destroy_lane_ctx(inspire_ctx* lane);
 }  // namespace inspire

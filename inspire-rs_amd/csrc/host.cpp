@@ -7,6 +7,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <memory>
+#include <mutex>
+#include <shared_mutex>
 #include <cstdio>
 #include <cstring>
 #include <stdexcept>
@@ -139,6 +142,43 @@ static void build_modulus(HostModulus& m, uint64_t q, uint32_t n, uint32_t logn)
         m.itwp[i] = shoup_pre(m.itw[i], (uint32_t)q);
     }
 }
+
+inspire_ctx* make_lane_ctx(inspire_ctx* owner) {
+    std::unique_ptr<inspire_ctx> c(new inspire_ctx());
+    c->params = owner->params;
+    c->N = owner->N;
+    c->K = owner->K;
+    c->logn = owner->logn;
+    c->ell = owner->ell;
+    c->q = owner->q;
+    c->delta = owner->delta;
+    std::memcpy(c->w_seed, owner->w_seed, 32);
+    for (uint32_t k = 0; k < owner->K; k++) c->hm[k] = owner->hm[k];
+    c->dev = owner->dev;  // device tables belong to the owner
+    c->owner = owner;
+    hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "lane stream");
+    return c.release();
+}
+
+void destroy_lane_ctx(inspire_ctx* c) { inspire_ctx_destroy(c); }
+
+namespace {
+
+// A lane reads its owner's databases, tables and keys: it holds the owner's
+// reader lock for the whole call (loads and registrations hold it exclusively).
+struct LaneRead {
+    std::shared_lock<std::shared_mutex> lk;
+    explicit LaneRead(inspire_ctx* c) {
+        if (c->owner) lk = std::shared_lock<std::shared_mutex>(c->owner->rw);
+    }
+};
+
+void owner_only(const inspire_ctx* ctx, const char* what) {
+    if (ctx->owner)
+        throw Error{INSPIRE_ERR_INVALID_ARGUMENT, std::string(what) + " goes through the lane's owner context"};
+}
+
+}  // namespace
 
 }  // namespace inspire
 
@@ -286,14 +326,26 @@ uint64_t inspire_ctx_device_bytes(const inspire_ctx* ctx) {
     uint64_t b = ctx->tables.bytes() + ctx->clients.keys.bytes() + ctx->ws0.bytes() + ctx->ws1.bytes() + ctx->ws2.bytes() + ctx->ws3.bytes() +
                  ctx->ws4.bytes() + ctx->ws5.bytes() + ctx->ws6.bytes() + ctx->galois.bytes() + ctx->yhat.bytes();
     for (auto& kv : ctx->dbs) b += kv.second.digits.bytes();
+    std::lock_guard<std::mutex> lk(const_cast<inspire_ctx*>(ctx)->packs_mu);
     for (auto& kv : ctx->packs) b += kv.second->step1.bytes() + kv.second->perm.bytes() + kv.second->w_all.bytes();
     return b;
+}
+
+int inspire_ctx_create_lane(inspire_ctx* ctx, inspire_ctx** out) {
+    INSPIRE_TRY({
+        if (!ctx || !out) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "null argument"};
+        *out = nullptr;
+        if (ctx->owner) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "a lane cannot own lanes"};
+        *out = make_lane_ctx(ctx);
+        return INSPIRE_OK;
+    })
 }
 
 int inspire_respond_batch_device(inspire_ctx* ctx, const inspire_device_batch* batch) {
     INSPIRE_TRY({
         if (!ctx || !batch) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "null argument"};
         std::lock_guard<std::recursive_mutex> g(ctx->mu);
+        LaneRead lr(ctx);
         respond_batch_device(ctx, *batch);
         return INSPIRE_OK;
     })
@@ -405,6 +457,7 @@ int inspire_db_load(inspire_ctx* ctx, uint32_t db_id, uint32_t num_shards, const
     INSPIRE_TRY({
         if (!ctx || (num_shards && !shard_ids) || (num_shards && polys_per_shard && !polys))
             throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "null argument"};
+        owner_only(ctx, "loading");
         std::lock_guard<std::recursive_mutex> g(ctx->mu);
         std::unique_lock<std::shared_mutex> w(ctx->rw);
         db_load(ctx, db_id, num_shards, shard_ids, polys_per_shard, polys);
@@ -416,6 +469,7 @@ int inspire_db_load_entries(inspire_ctx* ctx, uint32_t db_id, const uint8_t* dat
                             uint32_t entry_size) {
     INSPIRE_TRY({
         if (!ctx || (len && !data)) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "null argument"};
+        owner_only(ctx, "loading");
         std::lock_guard<std::recursive_mutex> g(ctx->mu);
         std::unique_lock<std::shared_mutex> w(ctx->rw);
         db_load_entries(ctx, db_id, data, len, entry_size);
@@ -426,6 +480,7 @@ int inspire_db_load_entries(inspire_ctx* ctx, uint32_t db_id, const uint8_t* dat
 int inspire_db_load_shard_files(inspire_ctx* ctx, uint32_t db_id, const char* dir) {
     INSPIRE_TRY({
         if (!ctx || !dir) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "null argument"};
+        owner_only(ctx, "loading");
         std::lock_guard<std::recursive_mutex> g(ctx->mu);
         std::unique_lock<std::shared_mutex> w(ctx->rw);
         db_load_shard_files(ctx, db_id, dir);
@@ -436,6 +491,7 @@ int inspire_db_load_shard_files(inspire_ctx* ctx, uint32_t db_id, const char* di
 int inspire_db_unload(inspire_ctx* ctx, uint32_t db_id) {
     INSPIRE_TRY({
         if (!ctx) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "null argument"};
+        owner_only(ctx, "unloading");
         std::lock_guard<std::recursive_mutex> g(ctx->mu);
         std::unique_lock<std::shared_mutex> w(ctx->rw);
         if (!ctx->dbs.erase(db_id)) throw Error{INSPIRE_ERR_UNKNOWN_DATABASE, "unknown database " + std::to_string(db_id)};
@@ -452,6 +508,7 @@ static int respond_one(inspire_ctx* ctx, const inspire_query* q, uint32_t varian
                         seeded ? "this entry point takes a SeededClientQuery (seeded = 1)"
                                : "this entry point takes a ClientQuery (seeded = 0)"};
         std::lock_guard<std::recursive_mutex> g(ctx->mu);
+        LaneRead lr(ctx);
         int32_t st = 0;
         respond_batch(ctx, 1, q, &variant, r, &st, route);
         return st;
@@ -485,6 +542,7 @@ int inspire_respond_seeded_packed(inspire_ctx* ctx, const inspire_query* q, insp
 int inspire_ctx_set_galois_keys(inspire_ctx* ctx, const uint64_t* keys, uint32_t count) {
     INSPIRE_TRY({
         if (!ctx || (count && !keys)) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "null argument"};
+        owner_only(ctx, "loading");
         std::lock_guard<std::recursive_mutex> g(ctx->mu);
         std::unique_lock<std::shared_mutex> w(ctx->rw);
         set_galois_keys(ctx, keys, count);
@@ -496,6 +554,7 @@ int inspire_respond_with_variant(inspire_ctx* ctx, const inspire_query* q, uint3
     INSPIRE_TRY({
         if (!ctx || !q || !r) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "null argument"};
         std::lock_guard<std::recursive_mutex> g(ctx->mu);
+        LaneRead lr(ctx);
         int32_t st = 0;
         respond_batch(ctx, 1, q, &variant, r, &st, Route::ByVariant);
         return st;
@@ -508,6 +567,7 @@ int inspire_respond_batch(inspire_ctx* ctx, uint32_t n, const inspire_query* qs,
         if (!ctx || (n && (!qs || !variants || !rs || !status)))
             throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "null argument"};
         std::lock_guard<std::recursive_mutex> g(ctx->mu);
+        LaneRead lr(ctx);
         respond_batch(ctx, n, qs, variants, rs, status, Route::ByVariant);
         return INSPIRE_OK;
     })

@@ -130,14 +130,13 @@ GenInfo gen_info(uint32_t n, uint32_t gamma) {  // inspiring2.rs:114-128
 }  // namespace
 
 PackTables& pack_tables(inspire_ctx* ctx, uint32_t gamma) {
+    // The map is shared by the owner and its lanes: lookups and the first
+    // build of a gamma's tables happen under its own lock.  Entries are never
+    // erased, so references stay valid while other gammas are added.
+    std::lock_guard<std::mutex> lk(root(ctx)->packs_mu);
     auto& packs = root(ctx)->packs;
     auto it = packs.find(gamma);
     if (it != packs.end()) return *it->second;
-    if (ctx->owner) throw Error{INSPIRE_ERR_DEVICE, "packing tables missing for a serving lane"};
-    // Building changes the table map, which serving lanes read: writers hold
-    // ctx->rw exclusively.  Callers that already hold it say so.
-    std::unique_lock<std::shared_mutex> w(ctx->rw, std::defer_lock);
-    if (!ctx->rw_held) w.lock();
     const uint32_t n = ctx->N, K = ctx->K, ell = ctx->ell, logn = ctx->logn;
     const size_t KN = (size_t)K * n;
     std::unique_ptr<PackTables> T(new PackTables());

@@ -114,7 +114,13 @@ static void db_begin(const inspire_ctx* ctx, Database& db, uint32_t num_shards, 
     db.num_shards = num_shards;
     db.cols = cols;
     db.digit_words_per_shard = (size_t)cols * ctx->ell * ctx->K * ctx->N;
-    for (uint32_t s = 0; s < num_shards; s++) db.slot_of.emplace(shard_ids ? shard_ids[s] : s, s);  // first id wins
+    db.dense_ids = true;
+    for (uint32_t s = 0; s < num_shards; s++) {
+        const uint32_t id = shard_ids ? shard_ids[s] : s;
+        db.dense_ids = db.dense_ids && id == s;
+    }
+    if (!db.dense_ids)
+        for (uint32_t s = 0; s < num_shards; s++) db.slot_of.emplace(shard_ids[s], s);  // first id wins
 }
 
 void db_load(inspire_ctx* ctx, uint32_t db_id, uint32_t num_shards, const uint32_t* shard_ids,
@@ -428,8 +434,8 @@ static const Database* admit(inspire_ctx* ctx, const inspire_query& q, uint32_t 
         return nullptr;
     }
     const Database& db = it->second;
-    auto sit = db.slot_of.find(q.shard_id);
-    if (sit == db.slot_of.end()) {  // respond.rs:78-82
+    const int64_t slot = db.slot(q.shard_id);
+    if (slot < 0) {  // respond.rs:78-82
         fail(status, i, INSPIRE_ERR_SHARD_NOT_FOUND, "Shard " + std::to_string(q.shard_id) + " not found");
         return nullptr;
     }
@@ -467,7 +473,7 @@ static const Database* admit(inspire_ctx* ctx, const inspire_query& q, uint32_t 
             return nullptr;
         }
     }
-    *dig_out = shard_digits(db, sit->second);
+    *dig_out = shard_digits(db, (uint32_t)slot);
     return &db;
 }
 
@@ -613,9 +619,9 @@ void respond_batch_device(inspire_ctx* ctx, const inspire_device_batch& b) {
     uint32_t* d_err = reinterpret_cast<uint32_t*>(d_qdig + Q);
     uint64_t* hdig = ctx->pin.get<uint64_t>();
     for (uint32_t t = 0; t < Q; t++) {
-        auto sit = db.slot_of.find(b.shard_ids[t]);
-        const bool found = sit != db.slot_of.end();
-        hdig[t] = shard_digits(db, found ? sit->second : 0);  // a missing shard reads shard 0, reported below
+        const int64_t sl = db.slot(b.shard_ids[t]);
+        const bool found = sl >= 0;
+        hdig[t] = shard_digits(db, found ? (uint32_t)sl : 0);  // a missing shard reads shard 0, reported below
         b.status[t] = found ? INSPIRE_OK : INSPIRE_ERR_SHARD_NOT_FOUND;
     }
     hdig[Q] = 0;  // the error word

@@ -111,32 +111,6 @@ size_t request_footprint(const inspire_ctx* ctx, uint32_t cols, bool packed) {
     return in + rows + (size_t)(1 + cols) * 2 * KN * 4;
 }
 
-inspire_ctx* make_lane_ctx(inspire_ctx* owner) {
-    std::unique_ptr<inspire_ctx> c(new inspire_ctx());
-    c->params = owner->params;
-    c->N = owner->N;
-    c->K = owner->K;
-    c->logn = owner->logn;
-    c->ell = owner->ell;
-    c->q = owner->q;
-    c->delta = owner->delta;
-    std::memcpy(c->w_seed, owner->w_seed, 32);
-    for (uint32_t k = 0; k < owner->K; k++) c->hm[k] = owner->hm[k];
-    c->dev = owner->dev;  // device tables belong to the owner
-    c->owner = owner;
-    hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "lane stream");
-    return c.release();
-}
-
-void destroy_lane_ctx(inspire_ctx* c) {
-    if (!c) return;
-    if (c->stream) {
-        (void)hipStreamSynchronize(c->stream);
-        (void)hipStreamDestroy(c->stream);
-    }
-    delete c;
-}
-
 void resolve(inspire_server* s, SReq& r);
 
 // One group of a batch: same variant class and column count.
@@ -352,8 +326,8 @@ void resolve(inspire_server* s, SReq& r) {
         }
         r.yoff = (uint64_t)c->second * ctx->ell * ctx->K * ctx->N;
     }
-    auto sit = db.slot_of.find(r.shard_id);
-    if (sit == db.slot_of.end()) {  // respond.rs:78-82
+    const int64_t slot = db.slot(r.shard_id);
+    if (slot < 0) {  // respond.rs:78-82
         r.status = INSPIRE_ERR_SHARD_NOT_FOUND;
         r.err = "Shard " + std::to_string(r.shard_id) + " not found";
         return;
@@ -363,13 +337,18 @@ void resolve(inspire_server* s, SReq& r) {
         r.err = "the serving loop needs a database with columns";
         return;
     }
-    if (r.packed() && !ctx->packs.count(db.cols)) {
+    bool have_tables = true;
+    if (r.packed()) {
+        std::lock_guard<std::mutex> lk(ctx->packs_mu);
+        have_tables = ctx->packs.count(db.cols) != 0;
+    }
+    if (!have_tables) {
         r.status = INSPIRE_ERR_UNSUPPORTED;
         r.err = "no packing tables for this database (loaded after the server started)";
         return;
     }
     r.cols = db.cols;
-    r.dig = shard_digits(db, sit->second);
+    r.dig = shard_digits(db, (uint32_t)slot);
     r.footprint = request_footprint(ctx, db.cols, r.packed());
     if (r.footprint > s->budget) {  // could never be admitted
         r.status = INSPIRE_ERR_CAPACITY;
@@ -388,6 +367,7 @@ extern "C" {
 int inspire_register_client_keys(inspire_ctx* ctx, uint32_t client_id, const uint64_t* y_body) {
     INSPIRE_TRY({
         if (!ctx || !y_body) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "null argument"};
+        if (ctx->owner) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "registering keys goes through the lane's owner context"};
         std::lock_guard<std::recursive_mutex> g(ctx->mu);
         std::unique_lock<std::shared_mutex> w(ctx->rw);
         const size_t KN = (size_t)ctx->K * ctx->N, per = ctx->ell * KN;
@@ -436,7 +416,6 @@ int inspire_server_create(inspire_ctx* ctx, const inspire_server_config* cfg, in
         {
             std::lock_guard<std::recursive_mutex> g(ctx->mu);
             std::unique_lock<std::shared_mutex> w(ctx->rw);
-            ctx->rw_held = true;
             for (auto& kv : ctx->dbs) {
                 const uint32_t cols = kv.second.cols;
                 if (!cols) continue;
@@ -447,7 +426,6 @@ int inspire_server_create(inspire_ctx* ctx, const inspire_server_config* cfg, in
                     // a column count InspiRING cannot pack: packed queries to it fail at submit
                 }
             }
-            ctx->rw_held = false;
         }
         s->budget = cfg->hbm_budget ? std::max<uint64_t>(cfg->hbm_budget, max_fp)
                                     : (uint64_t)cfg->lanes * cfg->max_batch * max_fp;

@@ -155,6 +155,7 @@ _u64p = C.POINTER(C.c_uint64)
 EXPORTS = [
     ("inspire_ctx_create", C.c_int, [C.POINTER(_Params), C.POINTER(C.c_uint8), C.POINTER(C.c_void_p)]),
     ("inspire_ctx_destroy", None, [C.c_void_p]),
+    ("inspire_ctx_create_lane", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     ("inspire_last_error", C.c_char_p, []),
     ("inspire_ctx_device_bytes", C.c_uint64, [C.c_void_p]),
     ("inspire_db_load", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32), C.c_uint32, _u64p]),
@@ -340,6 +341,16 @@ class Context:
         self.d = params.ring_dim
         self.ell = params.gadget_len
         self._dbs = {}
+
+    def lane(self) -> "Context":
+        """inspire_ctx_create_lane: a handle with its own stream and
+        workspaces over this context's resident state."""
+        ln = Context.__new__(Context)
+        ln.params, ln.K, ln.d, ln.ell, ln._dbs = self.params, self.K, self.d, self.ell, self._dbs
+        ln._h = C.c_void_p()
+        _raise(lib().inspire_ctx_create_lane(self._h, C.byref(ln._h)))
+        ln._owner = self  # the owner outlives its lanes
+        return ln
 
     def close(self):
         if self._h:
