@@ -20,8 +20,9 @@
 // R, the online accumulator S and the NTT of the client's y_body (rotated by
 // the automorphism table on the fly, staged in LDS) never leave the chip.
 //
-// Twiddles are kept in LDS in Montgomery form (w 2^32 mod q, one word each);
-// a butterfly's product is one Montgomery reduction.  Barriers wait for LDS
+// Twiddles are kept in LDS in Montgomery form (w 2^32 mod q, one word each),
+// the forward table only (the inverse transforms read it mirrored,
+// reg_ntt.hpp); a butterfly's product is one Montgomery reduction.  Barriers wait for LDS
 // traffic only, so the global loads of the next multiply-accumulate's
 // operands (w_all, the r_pre gathers, the permutation) stay in flight across
 // the transforms.
@@ -37,10 +38,10 @@ namespace inspire {
 template <int LOGN, int ELL>
 struct ChainCrtShape {
     static constexpr int N = 1 << LOGN, LP = NttShape<LOGN>::LDS;
-    static constexpr size_t kBytes = (size_t)4 * N * 4      // Montgomery twiddles, both directions and residues
+    static constexpr size_t kBytes = (size_t)2 * N * 4      // forward Montgomery twiddles of both residues
                                      + (size_t)ELL * 2 * N * 4  // NTT of the client's y_body
-                                     + (size_t)2 * LP * 4       // inverse-transform exchange, per residue
-                                     + (size_t)2 * ELL * LP * 4;  // digit-transform exchange, per residue
+                                     + (size_t)2 * ELL * LP * 4;  // transform exchange, per residue (the
+                                                                  // inverse uses the first digit buffer)
 };
 
 // LAZY: ELL * 4q * q < q * 2^32, so the multiply-accumulate takes the
@@ -53,10 +54,8 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
     constexpr size_t KN = 2 * (size_t)N;
     extern __shared__ __align__(16) uint32_t lds[];
     uint32_t* twf = lds;                 // [2][N] psi^br(i) 2^32 mod q_k
-    uint32_t* twi = twf + 2 * N;         // [2][N] psi^-br(i) 2^32 mod q_k
-    uint32_t* yl = twi + 2 * N;          // [ELL][2][N]
-    uint32_t* rbuf = yl + ELL * 2 * N;   // [2][LP]
-    uint32_t* dbuf = rbuf + 2 * LP;      // [2][ELL][LP]
+    uint32_t* yl = twf + 2 * N;          // [ELL][2][N]
+    uint32_t* dbuf = yl + ELL * 2 * N;   // [2][ELL][LP]
 
     const int tid = threadIdx.x;
     const int k = tid / T, t = tid - k * T;  // residue of this half, thread within it
@@ -67,16 +66,14 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
     const uint32_t ycount = A.y_count ? A.y_count[qi] : gamma - 1;
     const Mod M{C.m[k].q, C.m[k].q2, C.m[k].qinv};
     const uint32_t q0 = C.m[0].q, q1 = C.m[1].q;
-    uint32_t* rb = rbuf + k * LP;
     uint32_t* db = dbuf + k * ELL * LP;
+    uint32_t* rb = db;  // the inverse transform's exchange: this residue's first digit buffer
     const uint32_t* tw = twf + k * N;
-    const uint32_t* itw = twi + k * N;
 
     for (int e = tid; e < 2 * N; e += 2 * T) {
         const int kk = e / N, i = e - kk * N;
         const ModDev& D = C.m[kk];
         twf[e] = csub(shoup_mul(D.tw[i], C.mont[kk], C.montp[kk], D.q), D.q);
-        twi[e] = csub(shoup_mul(D.itw[i], C.mont[kk], C.montp[kk], D.q), D.q);
     }
     if (ymode == 0 && A.y_coeff) {
         // NTT of this query's y_body (both halves, one residue each), into
@@ -149,7 +146,7 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
         for (int j = 0; j < ELL; j++) load_slots<LOGN>(Wi + (size_t)j * KN, ts, w[j]);
 
         // T[i] = G^-1(INTT(R[i+1])): this residue's coefficients, then the other's via LDS
-        intt1<LOGN>(R, ts, rb, itw, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
+        intt1<LOGN>(R, ts, rb, tw, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
         const int cix = lds_ix(ts);
 #pragma unroll
         for (int e = 0; e < 8; e++) {
@@ -157,7 +154,7 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
             rb[lds_step(cix, e, T)] = R[e];
         }
         lds_barrier();
-        const uint32_t* ro = rbuf + (1 - k) * LP;
+        const uint32_t* ro = dbuf + (1 - k) * ELL * LP;  // the other residue's exchange buffer
         uint32_t dg[ELL][8];
 #pragma unroll
         for (int e = 0; e < 8; e++) {
@@ -173,6 +170,7 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
                 dg[j][e] = C.small_digits ? v : reduce32(v, M.q, C.onep[k]);
             }
         }
+        lds_barrier();  // the other half is past its reads of this half's exchange buffer
         ntt_digits<LOGN, ELL>(dg, ts, db, tw, M);
 
         const bool use_y = (uint32_t)i < ycount;
@@ -213,11 +211,11 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
 
     // a_hat = INTT(R[0]); b = b_poly + INTT(S)
     uint32_t* out = A.out + (size_t)qi * 2 * KN + (size_t)k * N;
-    intt1<LOGN>(R, t, rb, itw, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
+    intt1<LOGN>(R, t, rb, tw, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
 #pragma unroll
     for (int e = 0; e < 8; e++) out[t + e * T] = csub(R[e], M.q);
     lds_barrier();  // every thread is past its last read of rb
-    intt1<LOGN>(S, t, rb, itw, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
+    intt1<LOGN>(S, t, rb, tw, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
 #pragma unroll
     for (int e = 0; e < 8; e++) {
         const uint32_t pos = t + e * T;

@@ -11,9 +11,15 @@
 // therefore runs in registers between an inverse and a forward transform, and
 // NTT-domain work (multiply-accumulate) between a forward and an inverse one.
 //
-// Twiddles are Montgomery-form words (w 2^32 mod q), normally staged in LDS;
-// a butterfly's product is one Montgomery reduction.  Barriers wait for LDS
-// traffic only, so global loads issued earlier stay in flight across them.
+// Twiddles are Montgomery-form words (w 2^32 mod q) of the forward table,
+// normally staged in LDS; a butterfly's product is one Montgomery reduction
+// (two 64-bit multiply-adds and one 32-bit multiply).  The inverse transform
+// needs no table of its own: psi^-br(m) = -psi^br(3 2^s - 1 - m) for m in
+// stage s (2^s <= m < 2^(s+1)), because psi^N = -1 and the stage's
+// bit-reversed exponents mirror each other about N; the Gentleman-Sande
+// butterfly takes the negation by subtracting the other way round.  Barriers
+// wait for LDS traffic only, so global loads issued earlier stay in flight
+// across them.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -47,13 +53,18 @@ __device__ __forceinline__ void ct_m(uint32_t& X, uint32_t& Y, uint32_t wm, cons
     Y = x - t + M.q2;
 }
 
-// Harvey GS butterfly with a Montgomery-form twiddle: X, Y < 2q -> X + Y, w(X - Y) (< 2q).
+// Harvey GS butterfly with the mirrored forward twiddle wm standing for -wm:
+// X, Y < 2q -> X + Y, (-w)(X - Y) = w(Y - X) (< 2q).
 __device__ __forceinline__ void gs_m(uint32_t& X, uint32_t& Y, uint32_t wm, const Mod& M) {
     const uint32_t s = csub(X + Y, M.q2);
-    const uint32_t d = X - Y + M.q2;
+    const uint32_t d = Y - X + M.q2;
     X = s;
     Y = mmul(d, wm, M);
 }
+
+// Twiddle of the inverse transform's entry m = mS + o of the stage with base
+// mS (a power of two): the forward twiddle at the mirrored index 2 mS - 1 - o.
+__device__ __forceinline__ uint32_t inv_tw(const uint32_t* tw, int mS, int o) { return tw[2 * mS - 1 - o]; }
 
 template <int LOGN>
 struct Lay {
@@ -83,7 +94,7 @@ __device__ __forceinline__ void load_slots(const uint32_t* p, int t, uint32_t (&
 // slot positions of thread t -> x at its coefficient positions (< 2q).  rb is
 // the residue's [LP] LDS buffer; no thread may still be reading it.
 template <int LOGN>
-__device__ __forceinline__ void intt1(uint32_t (&x)[8], int t, uint32_t* rb, const uint32_t* itw, const Mod& M,
+__device__ __forceinline__ void intt1(uint32_t (&x)[8], int t, uint32_t* rb, const uint32_t* tw, const Mod& M,
                                       uint32_t c, uint32_t cp, uint32_t cw, uint32_t cwp) {
     using L = Lay<LOGN>;
     constexpr int N = L::N;
@@ -95,10 +106,10 @@ __device__ __forceinline__ void intt1(uint32_t (&x)[8], int t, uint32_t* rb, con
             uint32_t& x1 = x[4 * g + 1];
             uint32_t& x2 = x[4 * g + 2];
             uint32_t& x3 = x[4 * g + 3];
-            gs_m(x0, x1, itw[N / 2 + 2 * i], M);
-            gs_m(x2, x3, itw[N / 2 + 2 * i + 1], M);
+            gs_m(x0, x1, inv_tw(tw, N / 2, 2 * i), M);
+            gs_m(x2, x3, inv_tw(tw, N / 2, 2 * i + 1), M);
             if constexpr (L::REM == 2) {
-                const uint32_t w = itw[N / 4 + i];
+                const uint32_t w = inv_tw(tw, N / 4, i);
                 gs_m(x0, x2, w, M);
                 gs_m(x1, x3, w, M);
             }
@@ -117,15 +128,15 @@ __device__ __forceinline__ void intt1(uint32_t (&x)[8], int t, uint32_t* rb, con
         }
         const int mA = N / (2 * t0), mB = N / (4 * t0), mC = N / (8 * t0);
 #pragma unroll
-        for (int h = 0; h < 4; h++) gs_m(x[2 * h], x[2 * h + 1], itw[mA + 4 * i3 + h], M);
+        for (int h = 0; h < 4; h++) gs_m(x[2 * h], x[2 * h + 1], inv_tw(tw, mA, 4 * i3 + h), M);
 #pragma unroll
         for (int h = 0; h < 2; h++) {
-            const uint32_t w = itw[mB + 2 * i3 + h];
+            const uint32_t w = inv_tw(tw, mB, 2 * i3 + h);
             gs_m(x[4 * h + 0], x[4 * h + 2], w, M);
             gs_m(x[4 * h + 1], x[4 * h + 3], w, M);
         }
         if (p < L::R8 - 1) {
-            const uint32_t w = itw[mC + i3];
+            const uint32_t w = inv_tw(tw, mC, i3);
 #pragma unroll
             for (int k = 0; k < 4; k++) gs_m(x[k], x[k + 4], w, M);
 #pragma unroll

@@ -44,7 +44,7 @@ namespace {
 template <int LOGN, int K, int ELL>
 struct TreeShape {
     static constexpr int N = 1 << LOGN, T = N / 8, LP = NttShape<LOGN>::LDS;
-    static constexpr size_t kTw = (size_t)2 * K * N;      // Montgomery twiddles, both directions
+    static constexpr size_t kTw = (size_t)K * N;          // forward Montgomery twiddles (the inverse reads them mirrored)
     static constexpr size_t kA = (size_t)K * LP;          // s1.a (automorphism source), then INTT exchange
     static constexpr size_t kB = (size_t)K * LP;          // s1.b (automorphism source)
     static constexpr size_t kD = (size_t)K * ELL * LP;    // digit-transform exchange
@@ -56,19 +56,18 @@ struct TreeShape {
 template <int LOGN, int K, int ELL>
 struct TreeLds {
     using S = TreeShape<LOGN, K, ELL>;
-    uint32_t *twf, *twi, *abuf, *bbuf, *dbuf;
+    uint32_t *twf, *abuf, *bbuf, *dbuf;
     uint64_t* red;
     __device__ explicit TreeLds(uint32_t* lds) {
         twf = lds;
-        twi = twf + K * S::N;
-        abuf = twi + K * S::N;
+        abuf = twf + K * S::N;
         bbuf = abuf + S::kA;
         dbuf = bbuf + S::kB;
         red = reinterpret_cast<uint64_t*>(dbuf + S::kD);
     }
 };
 
-// Twiddles of every residue into LDS, Montgomery form.
+// Forward twiddles of every residue into LDS, Montgomery form.
 template <int LOGN, int K, int ELL>
 __device__ __forceinline__ void stage_twiddles(const TreeLds<LOGN, K, ELL>& L, const CtxDev& C) {
     constexpr int N = 1 << LOGN;
@@ -76,7 +75,6 @@ __device__ __forceinline__ void stage_twiddles(const TreeLds<LOGN, K, ELL>& L, c
         const int kk = e / N, i = e - kk * N;
         const ModDev& D = C.m[kk];
         L.twf[e] = csub(shoup_mul(D.tw[i], C.mont[kk], C.montp[kk], D.q), D.q);
-        L.twi[e] = csub(shoup_mul(D.itw[i], C.mont[kk], C.montp[kk], D.q), D.q);
     }
 }
 
@@ -191,7 +189,7 @@ __device__ __forceinline__ void tree_level(uint32_t (&a0)[8], const uint32_t (&a
         b0[e] = addq(addq(b0[e], bt[e], q), csub(mont_reduce(sb, q, M.qinv), q), q);
     }
     // the digit transform's barriers ordered every read of abuf before this
-    intt1<LOGN>(A, t, ab, L.twi + k * N, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
+    intt1<LOGN>(A, t, ab, L.twf + k * N, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
 #pragma unroll
     for (int e = 0; e < 8; e++) a0[e] = addq(a0[e], csub(A[e], q), q);
 }
@@ -250,7 +248,7 @@ k_tree_leaves(TreeArgs A, CtxDev C, ScaleDev inv) {
     // the leaf: (a, 0) with a the column's a (invert_sample_extract undoes
     // sample_extract_coeff0, automorph_pack.rs:349-405)
     uint32_t a[8], b[8];
-    intt1<LOGN>(ah, t, L.abuf + k * LP, L.twi + k * N, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
+    intt1<LOGN>(ah, t, L.abuf + k * LP, L.twf + k * N, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
 #pragma unroll
     for (int e = 0; e < 8; e++) {
         a[e] = csub(ah[e], q);
@@ -348,7 +346,7 @@ k_tree_combine(TreeArgs A, CtxDev C, ScaleDev inv) {
     // root: b to the coefficient domain, then b_val * d at coefficient z < m
     const Mod M{q, C.m[k].q2, C.m[k].qinv};
     lds_barrier();  // every thread is past the level's last reads of abuf
-    intt1<LOGN>(b0, t, L.abuf + k * LP, L.twi + k * N, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
+    intt1<LOGN>(b0, t, L.abuf + k * LP, L.twf + k * N, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
     uint32_t* o = A.out + (size_t)qi * A.out_words;
     uint32_t* ob = o + KN;
     const uint32_t* lw = A.lweb + (size_t)qi * A.m * K;
