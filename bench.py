@@ -37,10 +37,14 @@ D, K, ELL, GAMMA = 2048, 2, 3, 16
 # Algorithmic modular multiplications per query in each kernel (one NTT of one
 # residue = (d/2) log2 d = 11264; one pointwise product of one residue = d).
 # The packing-chain kernel (k_pack_chain_crt) runs packing_offline step 2, the
-# online sum, the a_hat and b inverse transforms, and the NTT of y_body.
+# online sum, the a_hat and b inverse transforms, and the NTT of y_body.  Per
+# step it transforms R (K INTTs) and ell-1 gadget digits (K NTTs each); the
+# last digit's transform is recovered with 2 products per slot and residue
+# (NTT(D_last) = z^-(ell-1) (R - sum_j z^j NTT(D_j)), exact because the digits
+# sum to INTT(R) and the NTT is linear).
 NTT = (D // 2) * 11
-MODMUL_CHAIN = ((GAMMA - 1) * (2 + 2 * ELL) * NTT + (GAMMA - 1) * 2 * ELL * K * D + 4 * NTT
-                + ELL * K * NTT)
+MODMUL_CHAIN = ((GAMMA - 1) * (2 + 2 * (ELL - 1)) * NTT + (GAMMA - 1) * 2 * K * D
+                + (GAMMA - 1) * 2 * ELL * K * D + 4 * NTT + ELL * K * NTT)
 MODMUL_QUERY = (
     2 * ELL * K * NTT            # RGSW rows ell..2ell-1, a and b
     + GAMMA * 2 * ELL * K * D    # first dimension MAC

@@ -317,6 +317,14 @@ void pack_run(inspire_ctx* ctx, uint32_t gamma, uint32_t Q, const uint32_t* rows
         (k ? A.ninv1 : A.ninv0) = (uint32_t)powmod64(ctx->N, q - 2, q);
         A.ninvm[k] = (uint32_t)mulmod64(powmod64(ctx->N, q - 2, q), (1ull << 32) % q, q);
         A.ninvmp[k] = shoup_pre(A.ninvm[k], (uint32_t)q);
+        // gadget powers for the chain's recovered last digit, Montgomery form
+        const uint64_t z = (1ull << ctx->dev.gadget_log) % q, r32 = (1ull << 32) % q;
+        uint64_t zj = 1;
+        for (uint32_t j = 0; j < ctx->ell && j < 4; j++) {
+            A.zpowm[k][j] = (uint32_t)mulmod64(zj, r32, q);
+            if (j + 1 < ctx->ell) zj = mulmod64(zj, z, q);
+        }
+        A.zinvm[k] = (uint32_t)mulmod64(powmod64(zj, q - 2, q), r32, q);  // zj = z^(ell-1)
     }
     const uint64_t q0 = ctx->hm[0].q;
     A.q0r64 = (uint32_t)pow2_64_mod(q0);

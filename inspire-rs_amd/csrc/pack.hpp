@@ -43,6 +43,8 @@ struct PackArgs {
     uint32_t q0r64, q0r64p;      // 2^64 mod q0 (Shoup pair): plain -> reference Montgomery value
     uint32_t q1r64i, q1r64ip;    // 2^-64 mod q1 (Shoup pair): reference value -> plain
     uint32_t one_q1p;            // Shoup companion of 1 mod q1
+    uint32_t zpowm[2][4];        // z^j 2^32 mod q_k (gadget base z), j < ell
+    uint32_t zinvm[2];           // z^-(ell-1) 2^32 mod q_k
 };
 
 // Builds (once per gamma) the resident PackParams/OfflinePackingKeys tables.

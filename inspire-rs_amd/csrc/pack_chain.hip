@@ -35,27 +35,40 @@
 
 namespace inspire {
 
-template <int LOGN, int ELL>
+// REC: the last gadget digit's transform is recovered instead of computed.
+// The digits D_j of T = INTT(R) satisfy T = sum_j z^j D_j exactly when
+// z^ell >= q (the last digit is never truncated), and the NTT is linear mod
+// q_k, so NTT(D_last) = z^-(ell-1) (R - sum_{j<ell-1} z^j NTT(D_j)) mod q_k:
+// two modular products per slot replace a transform, and one LDS digit
+// buffer is not needed.
+template <int ELL, bool REC>
+struct ChainDigits {
+    static constexpr int ND = REC ? ELL - 1 : ELL;  // digits transformed
+    static constexpr int DB = ND > 0 ? ND : 1;      // LDS exchange buffers per residue
+};
+
+template <int LOGN, int ELL, bool REC>
 struct ChainCrtShape {
     static constexpr int N = 1 << LOGN, LP = NttShape<LOGN>::LDS;
     static constexpr size_t kBytes = (size_t)2 * N * 4      // forward Montgomery twiddles of both residues
                                      + (size_t)ELL * 2 * N * 4  // NTT of the client's y_body
-                                     + (size_t)2 * ELL * LP * 4;  // transform exchange, per residue (the
-                                                                  // inverse uses the first digit buffer)
+                                     + (size_t)2 * ChainDigits<ELL, REC>::DB * LP * 4;  // transform exchange,
+                                     // per residue (the inverse uses the first digit buffer)
 };
 
 // LAZY: ELL * 4q * q < q * 2^32, so the multiply-accumulate takes the
 // transform outputs (< 4q) without reducing them first.
-template <int LOGN, int ELL, bool LAZY>
+template <int LOGN, int ELL, bool LAZY, bool REC>
 __global__ void __launch_bounds__(2 * NttShape<LOGN>::T, 1)
 k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
     using L = Lay<LOGN>;
     constexpr int N = L::N, T = L::T, LP = L::LP;
+    constexpr int ND = ChainDigits<ELL, REC>::ND, DB = ChainDigits<ELL, REC>::DB;
     constexpr size_t KN = 2 * (size_t)N;
     extern __shared__ __align__(16) uint32_t lds[];
     uint32_t* twf = lds;                 // [2][N] psi^br(i) 2^32 mod q_k
     uint32_t* yl = twf + 2 * N;          // [ELL][2][N]
-    uint32_t* dbuf = yl + ELL * 2 * N;   // [2][ELL][LP]
+    uint32_t* dbuf = yl + ELL * 2 * N;   // [2][DB][LP]
 
     const int tid = threadIdx.x;
     const int k = tid / T, t = tid - k * T;  // residue of this half, thread within it
@@ -66,7 +79,7 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
     const uint32_t ycount = A.y_count ? A.y_count[qi] : gamma - 1;
     const Mod M{C.m[k].q, C.m[k].q2, C.m[k].qinv};
     const uint32_t q0 = C.m[0].q, q1 = C.m[1].q;
-    uint32_t* db = dbuf + k * ELL * LP;
+    uint32_t* db = dbuf + k * DB * LP;
     uint32_t* rb = db;  // the inverse transform's exchange: this residue's first digit buffer
     const uint32_t* tw = twf + k * N;
 
@@ -79,22 +92,30 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
         // NTT of this query's y_body (both halves, one residue each), into
         // LDS at the slot positions in Montgomery form
         __syncthreads();
-        uint32_t yv[ELL][8];
+        // DB polys at a time through the DB exchange buffers
 #pragma unroll
-        for (int j = 0; j < ELL; j++)
+        for (int j0 = 0; j0 < ELL; j0 += DB) {
+            uint32_t yv[DB][8];
 #pragma unroll
-            for (int e = 0; e < 8; e++) yv[j][e] = Y[(j * 2 + k) * N + t + e * T];
-        ntt_digits<LOGN, ELL>(yv, t, db, tw, M);
+            for (int j = 0; j < DB; j++)
 #pragma unroll
-        for (int j = 0; j < ELL; j++)
+                for (int e = 0; e < 8; e++) yv[j][e] = j0 + j < ELL ? Y[((j0 + j) * 2 + k) * N + t + e * T] : 0u;
+            if (j0 > 0) lds_barrier();  // every thread is past the previous transform's reads
+            ntt_digits<LOGN, DB>(yv, t, db, tw, M);
 #pragma unroll
-            for (int g = 0; g < 2; g++) {
-                uint32_t v[4];
+            for (int j = 0; j < DB; j++) {
+                if (j0 + j >= ELL) break;
 #pragma unroll
-                for (int c = 0; c < 4; c++)
-                    v[c] = csub(shoup_mul(yv[j][4 * g + c], C.mont[k], C.montp[k], M.q), M.q);
-                *reinterpret_cast<uint4*>(&yl[(j * 2 + k) * N + L::group(t, g)]) = make_uint4(v[0], v[1], v[2], v[3]);
+                for (int g = 0; g < 2; g++) {
+                    uint32_t v[4];
+#pragma unroll
+                    for (int c = 0; c < 4; c++)
+                        v[c] = csub(shoup_mul(yv[j][4 * g + c], C.mont[k], C.montp[k], M.q), M.q);
+                    *reinterpret_cast<uint4*>(&yl[((j0 + j) * 2 + k) * N + L::group(t, g)]) =
+                        make_uint4(v[0], v[1], v[2], v[3]);
+                }
             }
+        }
     } else if (ymode == 0) {
         for (int e = tid; e < ELL * 2 * N / 4; e += 2 * T)
             reinterpret_cast<uint4*>(yl)[e] = reinterpret_cast<const uint4*>(Y)[e];
@@ -146,6 +167,11 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
         for (int j = 0; j < ELL; j++) load_slots<LOGN>(Wi + (size_t)j * KN, ts, w[j]);
 
         // T[i] = G^-1(INTT(R[i+1])): this residue's coefficients, then the other's via LDS
+        uint32_t Rs[8];  // R[i+1] itself, for the recovered digit
+        if constexpr (REC) {
+#pragma unroll
+            for (int e = 0; e < 8; e++) Rs[e] = R[e];
+        }
         intt1<LOGN>(R, ts, rb, tw, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
         const int cix = lds_ix(ts);
 #pragma unroll
@@ -154,7 +180,7 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
             rb[lds_step(cix, e, T)] = R[e];
         }
         lds_barrier();
-        const uint32_t* ro = dbuf + (1 - k) * ELL * LP;  // the other residue's exchange buffer
+        const uint32_t* ro = dbuf + (1 - k) * DB * LP;  // the other residue's exchange buffer
         uint32_t dg[ELL][8];
 #pragma unroll
         for (int e = 0; e < 8; e++) {
@@ -164,14 +190,26 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
             const uint32_t u = csub(shoup_mul(c1 + q1 - a0m, C.q0_inv_q1, C.q0_inv_q1p, q1), q1);
             const uint64_t x = (uint64_t)c0 + (uint64_t)q0 * u;
 #pragma unroll
-            for (int j = 0; j < ELL; j++) {
+            for (int j = 0; j < ND; j++) {
                 const uint32_t sh = j * C.gadget_log;
                 const uint32_t v = sh < 64 ? (uint32_t)((x >> sh) & C.gadget_mask) : 0u;
                 dg[j][e] = C.small_digits ? v : reduce32(v, M.q, C.onep[k]);
             }
         }
         lds_barrier();  // the other half is past its reads of this half's exchange buffer
-        ntt_digits<LOGN, ELL>(dg, ts, db, tw, M);
+        if constexpr (ND > 0) ntt_digits<LOGN, ND>(reinterpret_cast<uint32_t(&)[ND][8]>(dg), ts, db, tw, M);
+        if constexpr (REC) {
+            // NTT(D_last) = z^-(ell-1) (R - sum_{j < ell-1} z^j NTT(D_j)): transform outputs are
+            // below 4q, the Montgomery products below 2q, so the difference stays in (0, 7q)
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                uint32_t d = Rs[e];
+                if constexpr (ND >= 1) d += 4 * M.q - dg[0][e];
+#pragma unroll
+                for (int j = 1; j < ND; j++) d += M.q2 - mmul(dg[j][e], A.zpowm[k][j], M);
+                dg[ELL - 1][e] = ND > 0 ? mmul(d, A.zinvm[k], M) : d;
+            }
+        }
 
         const bool use_y = (uint32_t)i < ycount;
         uint32_t yv[ELL][8];
@@ -235,23 +273,23 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
 
 // Raises the kernel's dynamic LDS limit once per instantiation (a function-
 // local static is initialised exactly once, also under concurrent callers).
-template <int LOGN, int ELL, bool LAZY>
+template <int LOGN, int ELL, bool LAZY, bool REC>
 static hipError_t chain_crt_lds_limit() {
+    constexpr size_t shmem = ChainCrtShape<LOGN, ELL, REC>::kBytes;
     static const hipError_t e =
-        ChainCrtShape<LOGN, ELL>::kBytes > 64 * 1024
-            ? hipFuncSetAttribute((const void*)k_pack_chain_crt<LOGN, ELL, LAZY>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)ChainCrtShape<LOGN, ELL>::kBytes)
-            : hipSuccess;
+        shmem > 64 * 1024 ? hipFuncSetAttribute((const void*)k_pack_chain_crt<LOGN, ELL, LAZY, REC>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem)
+                          : hipSuccess;
     return e;
 }
 
-template <int LOGN, int ELL, bool LAZY>
+template <int LOGN, int ELL, bool LAZY, bool REC>
 static hipError_t chain_crt_launch(const PackArgs& A, uint32_t Q, const CtxDev& C, const ScaleDev& inv,
                                    hipStream_t st) {
-    const hipError_t e = chain_crt_lds_limit<LOGN, ELL, LAZY>();
+    const hipError_t e = chain_crt_lds_limit<LOGN, ELL, LAZY, REC>();
     if (e != hipSuccess) return e;
-    auto kern = k_pack_chain_crt<LOGN, ELL, LAZY>;
-    constexpr size_t shmem = ChainCrtShape<LOGN, ELL>::kBytes;
+    auto kern = k_pack_chain_crt<LOGN, ELL, LAZY, REC>;
+    constexpr size_t shmem = ChainCrtShape<LOGN, ELL, REC>::kBytes;
     hipLaunchKernelGGL(kern, dim3(Q), dim3(2 * NttShape<LOGN>::T), shmem, st, A, C, inv);
     return hipGetLastError();
 }
@@ -259,8 +297,14 @@ static hipError_t chain_crt_launch(const PackArgs& A, uint32_t Q, const CtxDev& 
 template <int LOGN, int ELL>
 static hipError_t chain_crt_t(const PackArgs& A, uint32_t Q, const CtxDev& C, const ScaleDev& inv, hipStream_t st) {
     const uint64_t qmax = C.m[0].q > C.m[1].q ? C.m[0].q : C.m[1].q;
-    if ((uint64_t)ELL * 4 * qmax < (1ull << 32)) return chain_crt_launch<LOGN, ELL, true>(A, Q, C, inv, st);
-    return chain_crt_launch<LOGN, ELL, false>(A, Q, C, inv, st);
+    // the recovered digit needs the composite value to fit ell digits: q <= z^ell
+    const bool rec = (uint64_t)C.gadget_log * ELL >= 64 || C.q <= (1ull << (C.gadget_log * ELL));
+    if ((uint64_t)ELL * 4 * qmax < (1ull << 32)) {
+        if (rec) return chain_crt_launch<LOGN, ELL, true, true>(A, Q, C, inv, st);
+        return chain_crt_launch<LOGN, ELL, true, false>(A, Q, C, inv, st);
+    }
+    if (rec) return chain_crt_launch<LOGN, ELL, false, true>(A, Q, C, inv, st);
+    return chain_crt_launch<LOGN, ELL, false, false>(A, Q, C, inv, st);
 }
 
 template <int LOGN>
@@ -278,7 +322,7 @@ bool pack_chain_crt_supported(const CtxDev& C) {
     if (C.K != 2 || C.ell < 1 || C.ell > 3 || C.logn < 8 || C.logn > 11) return false;
     // every modulus must leave room for the lazy ranges: 4q < 2^32
     if (C.m[0].q >= (1u << 30) || C.m[1].q >= (1u << 30)) return false;
-    return ChainCrtShape<11, 3>::kBytes <= 160 * 1024;
+    return ChainCrtShape<11, 3, false>::kBytes <= 160 * 1024;
 }
 
 hipError_t launch_pack_chain_crt(const PackArgs& A, uint32_t Q, const CtxDev& C, const ScaleDev& inv,
