@@ -67,13 +67,15 @@ MODMUL_PEAK_T = 157.3 / 2 / 2 / 3
 def tree_modmul(m: int = GAMMA):
     """Algorithmic modular multiplications per query of tree packing
     (respond_one_packing) over m columns: (leaves kernel, combine kernels).
-    A tree node at level l costs the digit NTTs (ell K), the key MAC
-    (2 ell K d) and the INTT of the new a (K); nodes with an odd child also
-    multiply odd.b by y in the NTT domain (K d).  The leaves kernel adds the
-    first-dimension MAC (2 ell K d per column) and each column's a INTT (K);
-    the root adds b's INTT (K)."""
+    A tree node at level l costs the digit NTTs ((ell - 1) K: the last digit's
+    transform is recovered from tau(s1.a^) with ell - 1 products per slot,
+    exact since q <= z^ell), the key MAC (2 ell K d) and the INTT of the new a
+    (K); nodes with an odd child also multiply odd.a^ and odd.b by y in the
+    NTT domain (2 K d).  The leaves kernel adds the first-dimension MAC
+    (2 ell K d per column) and each column's a INTT (K); the root adds b's
+    INTT (K)."""
     L = D.bit_length() - 1
-    node = (ELL * K + K) * NTT + 2 * ELL * K * D
+    node = ((ELL - 1) * K + K) * NTT + 2 * ELL * K * D + (ELL - 1) * K * D
     lm = (m - 1).bit_length()
     leaves = m * (2 * ELL * K * D + K * NTT) + m * (L - lm) * node
     combine = 0
@@ -81,7 +83,7 @@ def tree_modmul(m: int = GAMMA):
         half = 1 << (L - lv)
         nodes = min(m, half)
         odd = max(0, min(m - half, half))
-        combine += nodes * node + odd * K * D
+        combine += nodes * node + odd * 2 * K * D
     return leaves, combine + K * NTT
 
 

@@ -102,8 +102,16 @@ void tree_run(inspire_ctx* ctx, uint32_t cols, uint32_t Q, const uint32_t* rows,
         A.ninvmp[k] = shoup_pre(A.ninvm[k], (uint32_t)q);
         A.dmod[k] = (uint32_t)(N % q);
         A.dmodp[k] = shoup_pre(A.dmod[k], (uint32_t)q);
+        // gadget powers for the recovered last digit, Montgomery form
+        const uint64_t z = (1ull << ctx->dev.gadget_log) % q, r32 = (1ull << 32) % q;
+        uint64_t zj = 1;
+        for (uint32_t j = 0; j < ctx->ell && j < 4; j++) {
+            A.zpowm[k][j] = (uint32_t)mulmod64(zj, r32, q);
+            if (j + 1 < ctx->ell) zj = mulmod64(zj, z, q);
+        }
+        A.zinvm[k] = (uint32_t)mulmod64(powmod64(zj, q - 2, q), r32, q);  // zj = z^(ell-1)
     }
-    const size_t node_words = (size_t)Q * m * 2 * KN, lweb_words = (size_t)Q * m * K;
+    const size_t node_words = (size_t)Q * m * 3 * KN, lweb_words = (size_t)Q * m * K;
     ctx->ws3.reserve((node_words + lweb_words) * 4);
     A.node = ctx->ws3.get<uint32_t>();
     A.lweb = A.node + node_words;

@@ -16,7 +16,7 @@ struct TreeArgs {
     const uint32_t* rows;   // [Q][ell][2][K][N] RGSW rows ell..2ell-1, NTT domain, Montgomery form
     const uint32_t* ks;     // [log N][ell][2][K][N] galois keys (ServerCrs.galois_keys), NTT, Montgomery
     const uint32_t* yhat;   // [log N][K][N] NTT(X^(N >> l)) of level l at l - 1, Montgomery form
-    uint32_t* node;         // [Q][m][2][K][N] tree nodes: a coefficient domain, b NTT domain
+    uint32_t* node;         // [Q][m][3][K][N] tree nodes: a (coefficient domain), a^ and b (NTT domain)
     uint32_t* lweb;         // [Q][m][K] the LWE b of each column (coefficient 0 of its b)
     uint32_t* out;          // [Q][out_words] a [K][N], then b [K][b_dim]
     uint32_t m;             // LWEs packed (columns, at most N)
@@ -26,6 +26,8 @@ struct TreeArgs {
     uint32_t out_words;
     uint32_t ninvm[2], ninvmp[2];  // n^-1 2^32 mod q_k (Shoup pairs)
     uint32_t dmod[2], dmodp[2];    // N mod q_k (Shoup pairs)
+    uint32_t zpowm[2][4];          // z^j 2^32 mod q_k (gadget base z), j < ell
+    uint32_t zinvm[2];             // z^-(ell-1) 2^32 mod q_k
 };
 
 bool tree_supported(const CtxDev& C);

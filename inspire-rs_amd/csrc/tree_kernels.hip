@@ -11,18 +11,23 @@
 // the full recursion computes it.
 //
 // Node state: a in the coefficient domain (the gadget digits of the
-// automorphism need it), b in the NTT domain (b only ever enters linearly:
-// the automorphism is a slot gather and y = X^(d/2^l) a pointwise product),
-// both canonical u32 residues, [2][K][N] per node.  One workgroup of K x N/8
-// threads works on one node, the k-th group of N/8 threads on residue k.
+// automorphism need it) and in the NTT domain (a^), b in the NTT domain (b
+// only ever enters linearly: the automorphism is a slot gather and y =
+// X^(d/2^l) a pointwise product), all canonical u32 residues, [3][K][N] per
+// node.  One workgroup of K x N/8 threads works on one node, the k-th group
+// of N/8 threads on residue k.
 //
 // One level (pack_lwes_inner with t = 2^l + 1 and the galois key of index
 // log d - l, whose rows are kept as NTT-domain Montgomery words):
 //   s0 = even + y odd,  s1 = even - y odd          (y odd: negacyclic shift)
 //   D_i = NTT(G^-1(tau_t(s1.a)))                    (gadget digits, i < ell)
-//   a   = s0.a + INTT(sum_i D_i * key[i].a)
+//   a^  = s0.a^ + sum_i D_i * key[i].a,  a = s0.a + INTT(sum_i D_i * key[i].a)
 //   b   = s0.b + tau_t(s1.b) + sum_i D_i * key[i].b (NTT domain)
-// which is s0 + homomorphic_automorph(s1, t) term for term.
+// which is s0 + homomorphic_automorph(s1, t) term for term.  When the digits
+// cover the composite value (q <= z^ell), sum_i z^i D_i = tau_t(s1.a)^ exactly
+// and the last digit's transform is recovered from the others instead of
+// computed: D_last = z^-(ell-1) (tau_t(s1.a^) - sum_{i<ell-1} z^i D_i), with
+// tau_t(s1.a^) a slot gather.
 //
 // k_tree_leaves computes each column's ciphertext from the shard digits and
 // the query's RGSW rows (the first dimension), takes the LWE of its
@@ -41,35 +46,43 @@ namespace inspire {
 
 namespace {
 
-template <int LOGN, int K, int ELL>
+template <int ELL, bool REC>
+struct TreeDigits {
+    static constexpr int ND = REC ? ELL - 1 : ELL;  // digits transformed
+    static constexpr int DB = ND > 0 ? ND : 1;      // LDS exchange buffers per residue
+};
+
+template <int LOGN, int K, int ELL, bool REC>
 struct TreeShape {
     static constexpr int N = 1 << LOGN, T = N / 8, LP = NttShape<LOGN>::LDS;
     static constexpr size_t kTw = (size_t)K * N;          // forward Montgomery twiddles (the inverse reads them mirrored)
     static constexpr size_t kA = (size_t)K * LP;          // s1.a (automorphism source), then INTT exchange
     static constexpr size_t kB = (size_t)K * LP;          // s1.b (automorphism source)
-    static constexpr size_t kD = (size_t)K * ELL * LP;    // digit-transform exchange
+    static constexpr size_t kH = REC ? (size_t)K * LP : 0;  // s1.a^ (automorphism source of the recovered digit)
+    static constexpr size_t kD = (size_t)K * TreeDigits<ELL, REC>::DB * LP;  // digit-transform exchange
     static constexpr int W = T < 64 ? T : 64;              // lanes of one reduction group
     static constexpr size_t kRed = 2 * K * (T / W);       // u64 partial sums (two words each)
-    static constexpr size_t kBytes = 4 * (kTw + kA + kB + kD + kRed);
+    static constexpr size_t kBytes = 4 * (kTw + kA + kB + kH + kD + kRed);
 };
 
-template <int LOGN, int K, int ELL>
+template <int LOGN, int K, int ELL, bool REC>
 struct TreeLds {
-    using S = TreeShape<LOGN, K, ELL>;
-    uint32_t *twf, *abuf, *bbuf, *dbuf;
+    using S = TreeShape<LOGN, K, ELL, REC>;
+    uint32_t *twf, *abuf, *bbuf, *hbuf, *dbuf;
     uint64_t* red;
     __device__ explicit TreeLds(uint32_t* lds) {
         twf = lds;
         abuf = twf + K * S::N;
         bbuf = abuf + S::kA;
-        dbuf = bbuf + S::kB;
+        hbuf = bbuf + S::kB;
+        dbuf = hbuf + S::kH;
         red = reinterpret_cast<uint64_t*>(dbuf + S::kD);
     }
 };
 
 // Forward twiddles of every residue into LDS, Montgomery form.
-template <int LOGN, int K, int ELL>
-__device__ __forceinline__ void stage_twiddles(const TreeLds<LOGN, K, ELL>& L, const CtxDev& C) {
+template <int LOGN, int K, int ELL, bool REC>
+__device__ __forceinline__ void stage_twiddles(const TreeLds<LOGN, K, ELL, REC>& L, const CtxDev& C) {
     constexpr int N = 1 << LOGN;
     for (int e = threadIdx.x; e < K * N; e += blockDim.x) {
         const int kk = e / N, i = e - kk * N;
@@ -101,15 +114,17 @@ __device__ __forceinline__ uint32_t aut_slot(uint32_t p, uint32_t t) {
 
 // One level of pack_lwes_inner on the node held by this workgroup (see the
 // file comment).  a0/a1: s0.a/s1.a at this thread's coefficient positions,
-// b0/b1: s0.b/s1.b at its slot positions (canonical).  On return a0, b0 hold
-// the new node (canonical).  key: [ell][2][K][N] NTT-domain Montgomery rows.
-template <int LOGN, int K, int ELL, bool LAZY>
-__device__ __forceinline__ void tree_level(uint32_t (&a0)[8], const uint32_t (&a1)[8], uint32_t (&b0)[8],
-                                           const uint32_t (&b1)[8], uint32_t level, const uint32_t* key,
-                                           const TreeLds<LOGN, K, ELL>& L, const CtxDev& C, const ScaleDev& inv,
-                                           int k, int t) {
+// h0/h1: s0.a^/s1.a^ and b0/b1: s0.b/s1.b at its slot positions (canonical).
+// On return a0, h0, b0 hold the new node (canonical).  key: [ell][2][K][N]
+// NTT-domain Montgomery rows.
+template <int LOGN, int K, int ELL, bool LAZY, bool REC>
+__device__ __forceinline__ void tree_level(uint32_t (&a0)[8], const uint32_t (&a1)[8], uint32_t (&h0)[8],
+                                           const uint32_t (&h1)[8], uint32_t (&b0)[8], const uint32_t (&b1)[8],
+                                           uint32_t level, const uint32_t* key, const TreeLds<LOGN, K, ELL, REC>& L,
+                                           const TreeArgs& TA, const CtxDev& C, const ScaleDev& inv, int k, int t) {
     using Ly = Lay<LOGN>;
     constexpr int N = Ly::N, T = Ly::T, LP = Ly::LP;
+    constexpr int ND = TreeDigits<ELL, REC>::ND, DB = TreeDigits<ELL, REC>::DB;
     constexpr size_t KN = (size_t)K * N;
     const Mod M{C.m[k].q, C.m[k].q2, C.m[k].qinv};
     const uint32_t q = M.q;
@@ -128,12 +143,17 @@ __device__ __forceinline__ void tree_level(uint32_t (&a0)[8], const uint32_t (&a
     const int cix = lds_ix(t);
     uint32_t* ab = L.abuf + k * LP;
     uint32_t* bb = L.bbuf + k * LP;
+    uint32_t* hb = L.hbuf + k * LP;
 #pragma unroll
     for (int e = 0; e < 8; e++) ab[lds_step(cix, e, T)] = a1[e];
 #pragma unroll
-    for (int g = 0; g < 2; g++)
+    for (int g = 0; g < 2; g++) {
         *reinterpret_cast<uint4*>(&bb[lds_ix(Ly::group(t, g))]) =
             make_uint4(b1[4 * g + 0], b1[4 * g + 1], b1[4 * g + 2], b1[4 * g + 3]);
+        if constexpr (REC)
+            *reinterpret_cast<uint4*>(&hb[lds_ix(Ly::group(t, g))]) =
+                make_uint4(h1[4 * g + 0], h1[4 * g + 1], h1[4 * g + 2], h1[4 * g + 3]);
+    }
     lds_barrier();
 
     // gadget digits of tau_t(s1.a) at the coefficient positions:
@@ -159,21 +179,38 @@ __device__ __forceinline__ void tree_level(uint32_t (&a0)[8], const uint32_t (&a
             x = (uint64_t)c[0] + (uint64_t)C.m[0].q * u;
         }
 #pragma unroll
-        for (int j2 = 0; j2 < ELL; j2++) {
+        for (int j2 = 0; j2 < ND; j2++) {
             const uint32_t sh = j2 * C.gadget_log;
             const uint32_t v = sh < 64 ? (uint32_t)((x >> sh) & C.gadget_mask) : 0u;
             dg[j2][e] = C.small_digits ? v : reduce32(v, q, C.onep[k]);
         }
     }
-    // tau_t(s1.b) at the slot positions
-    uint32_t bt[8];
+    // tau_t(s1.b), and tau_t(s1.a^) for the recovered digit, at the slot positions
+    uint32_t bt[8], ht[8];
 #pragma unroll
     for (int g = 0; g < 2; g++)
 #pragma unroll
-        for (int c = 0; c < 4; c++)
-            bt[4 * g + c] = bb[lds_ix((int)aut_slot<LOGN>((uint32_t)(Ly::group(t, g) + c), taut))];
+        for (int c = 0; c < 4; c++) {
+            const int sx = lds_ix((int)aut_slot<LOGN>((uint32_t)(Ly::group(t, g) + c), taut));
+            bt[4 * g + c] = bb[sx];
+            if constexpr (REC) ht[4 * g + c] = hb[sx];
+        }
 
-    ntt_digits<LOGN, ELL>(dg, t, L.dbuf + k * ELL * LP, L.twf + k * N, M);
+    if constexpr (ND > 0)
+        ntt_digits<LOGN, ND>(reinterpret_cast<uint32_t(&)[ND][8]>(dg), t, L.dbuf + k * DB * LP, L.twf + k * N, M);
+    else
+        lds_barrier();  // no transform: every thread is past its reads of abuf before the INTT reuses it
+    if constexpr (REC) {
+        // transform outputs < 4q and Montgomery products < 2q keep the difference in (0, 7q)
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            uint32_t d = ht[e];
+            if constexpr (ND >= 1) d += 4 * M.q - dg[0][e];
+#pragma unroll
+            for (int j = 1; j < ND; j++) d += M.q2 - mmul(dg[j][e], TA.zpowm[k][j], M);
+            dg[ELL - 1][e] = ND > 0 ? mmul(d, TA.zinvm[k], M) : d;
+        }
+    }
 
     uint32_t A[8];
 #pragma unroll
@@ -186,6 +223,7 @@ __device__ __forceinline__ void tree_level(uint32_t (&a0)[8], const uint32_t (&a
             sb += (uint64_t)x * kb[j][e];
         }
         A[e] = mont_reduce(sa, q, M.qinv);  // < 2q, the inverse transform's input range
+        h0[e] = addq(h0[e], csub(A[e], q), q);
         b0[e] = addq(addq(b0[e], bt[e], q), csub(mont_reduce(sb, q, M.qinv), q), q);
     }
     // the digit transform's barriers ordered every read of abuf before this
@@ -194,14 +232,14 @@ __device__ __forceinline__ void tree_level(uint32_t (&a0)[8], const uint32_t (&a
     for (int e = 0; e < 8; e++) a0[e] = addq(a0[e], csub(A[e], q), q);
 }
 
-template <int LOGN, int K, int ELL, bool LAZY>
+template <int LOGN, int K, int ELL, bool LAZY, bool REC>
 __global__ void __launch_bounds__(K * NttShape<LOGN>::T, 1)
 k_tree_leaves(TreeArgs A, CtxDev C, ScaleDev inv) {
     using Ly = Lay<LOGN>;
     constexpr int N = Ly::N, T = Ly::T, LP = Ly::LP;
     constexpr size_t KN = (size_t)K * N;
     extern __shared__ __align__(16) uint32_t lds[];
-    const TreeLds<LOGN, K, ELL> L(lds);
+    const TreeLds<LOGN, K, ELL, REC> L(lds);
     const int tid = threadIdx.x, k = tid / T, t = tid - k * T;
     const uint32_t qi = blockIdx.x / A.m, s = blockIdx.x - qi * A.m;
     const Mod M{C.m[k].q, C.m[k].q2, C.m[k].qinv};
@@ -233,7 +271,7 @@ k_tree_leaves(TreeArgs A, CtxDev C, ScaleDev inv) {
         bsum += csub(mont_reduce(sb, q, M.qinv), q);
     }
     // LWE b = coefficient 0 of the column's b = n^-1 * sum of its slots
-    constexpr int W = TreeShape<LOGN, K, ELL>::W;  // a group never spans two residues
+    constexpr int W = TreeShape<LOGN, K, ELL, REC>::W;  // a group never spans two residues
 #pragma unroll
     for (int o = W / 2; o >= 1; o >>= 1) bsum += __shfl_xor(bsum, o, W);
     if (tid % W == 0) L.red[tid / W] = bsum;
@@ -247,13 +285,16 @@ k_tree_leaves(TreeArgs A, CtxDev C, ScaleDev inv) {
 
     // the leaf: (a, 0) with a the column's a (invert_sample_extract undoes
     // sample_extract_coeff0, automorph_pack.rs:349-405)
-    uint32_t a[8], b[8];
-    intt1<LOGN>(ah, t, L.abuf + k * LP, L.twf + k * N, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
+    uint32_t a[8], h[8], b[8];
 #pragma unroll
     for (int e = 0; e < 8; e++) {
-        a[e] = csub(ah[e], q);
+        h[e] = csub(ah[e], q);
+        a[e] = h[e];
         b[e] = 0;
     }
+    intt1<LOGN>(a, t, L.abuf + k * LP, L.twf + k * N, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
+#pragma unroll
+    for (int e = 0; e < 8; e++) a[e] = csub(a[e], q);
 
     if (A.m == 1) {  // a single LWE is not tree-packed (automorph_pack.rs:436-443)
         uint32_t* o = A.out + (size_t)qi * A.out_words;
@@ -268,29 +309,32 @@ k_tree_leaves(TreeArgs A, CtxDev C, ScaleDev inv) {
     if (t == 0) A.lweb[((size_t)qi * A.m + s) * K + k] = lwe_b;
 
     for (uint32_t lv = A.level_lo; lv <= A.level_hi; lv++)
-        tree_level<LOGN, K, ELL, LAZY>(a, a, b, b, lv, A.ks + (size_t)(A.logn - lv) * ELL * 2 * KN, L, C, inv, k,
-                                       t);
+        tree_level<LOGN, K, ELL, LAZY, REC>(a, a, h, h, b, b, lv, A.ks + (size_t)(A.logn - lv) * ELL * 2 * KN, L, A,
+                                            C, inv, k, t);
 
-    uint32_t* nd = A.node + ((size_t)qi * A.m + s) * 2 * KN + (size_t)k * N;
+    uint32_t* nd = A.node + ((size_t)qi * A.m + s) * 3 * KN + (size_t)k * N;
 #pragma unroll
     for (int e = 0; e < 8; e++) nd[t + e * T] = a[e];
 #pragma unroll
-    for (int g = 0; g < 2; g++)
-        *reinterpret_cast<uint4*>(nd + KN + Ly::group(t, g)) = make_uint4(b[4 * g], b[4 * g + 1], b[4 * g + 2], b[4 * g + 3]);
+    for (int g = 0; g < 2; g++) {
+        *reinterpret_cast<uint4*>(nd + KN + Ly::group(t, g)) = make_uint4(h[4 * g], h[4 * g + 1], h[4 * g + 2], h[4 * g + 3]);
+        *reinterpret_cast<uint4*>(nd + 2 * KN + Ly::group(t, g)) =
+            make_uint4(b[4 * g], b[4 * g + 1], b[4 * g + 2], b[4 * g + 3]);
+    }
 }
 
 // One level after the leaves: node s of level `level` from nodes s and
 // s + 2^(log d - level) of the level below (in place: outputs s <
 // 2^(log d - level) never hold an odd input).  At the root the packed
 // ciphertext is finished into out.
-template <int LOGN, int K, int ELL, bool LAZY>
+template <int LOGN, int K, int ELL, bool LAZY, bool REC>
 __global__ void __launch_bounds__(K * NttShape<LOGN>::T, 1)
 k_tree_combine(TreeArgs A, CtxDev C, ScaleDev inv) {
     using Ly = Lay<LOGN>;
     constexpr int N = Ly::N, T = Ly::T, LP = Ly::LP;
     constexpr size_t KN = (size_t)K * N;
     extern __shared__ __align__(16) uint32_t lds[];
-    const TreeLds<LOGN, K, ELL> L(lds);
+    const TreeLds<LOGN, K, ELL, REC> L(lds);
     const int tid = threadIdx.x, k = tid / T, t = tid - k * T;
     const uint32_t level = A.level_lo;
     const uint32_t half = 1u << (LOGN - level);  // nodes at this level: s < min(m, half)
@@ -298,10 +342,10 @@ k_tree_combine(TreeArgs A, CtxDev C, ScaleDev inv) {
     const uint32_t qi = blockIdx.x / nn, s = blockIdx.x - qi * nn;
     const bool odd = s + half < A.m;
     const uint32_t q = C.m[k].q;
-    const uint32_t* ev = A.node + ((size_t)qi * A.m + s) * 2 * KN + (size_t)k * N;
-    const uint32_t* od = A.node + ((size_t)qi * A.m + s + half) * 2 * KN + (size_t)k * N;
+    const uint32_t* ev = A.node + ((size_t)qi * A.m + s) * 3 * KN + (size_t)k * N;
+    const uint32_t* od = A.node + ((size_t)qi * A.m + s + half) * 3 * KN + (size_t)k * N;
 
-    uint32_t a0[8], a1[8], b0[8], b1[8];
+    uint32_t a0[8], a1[8], h0[8], h1[8], b0[8], b1[8];
     const uint32_t ysh = (uint32_t)N >> level;  // y = X^(N / 2^level)
 #pragma unroll
     for (int e = 0; e < 8; e++) {
@@ -313,33 +357,42 @@ k_tree_combine(TreeArgs A, CtxDev C, ScaleDev inv) {
         a1[e] = subq(x, v, q);
     }
     {
-        uint32_t eb[8], ob[8], yh[8];
-        load_slots<LOGN>(ev + KN, t, eb);
+        // a^ and b: s0 = even + y^ odd, s1 = even - y^ odd, slot by slot
+        uint32_t eh[8], oh[8], eb[8], ob[8], yh[8];
+        load_slots<LOGN>(ev + KN, t, eh);
+        load_slots<LOGN>(ev + 2 * KN, t, eb);
         if (odd) {
-            load_slots<LOGN>(od + KN, t, ob);
+            load_slots<LOGN>(od + KN, t, oh);
+            load_slots<LOGN>(od + 2 * KN, t, ob);
             load_slots<LOGN>(A.yhat + ((size_t)(level - 1) * K + k) * N, t, yh);
         }
         const Mod M{q, C.m[k].q2, C.m[k].qinv};
 #pragma unroll
         for (int e = 0; e < 8; e++) {
+            const uint32_t vh = odd ? csub(mmul(oh[e], yh[e], M), q) : 0u;
+            h0[e] = addq(eh[e], vh, q);
+            h1[e] = subq(eh[e], vh, q);
             const uint32_t v = odd ? csub(mmul(ob[e], yh[e], M), q) : 0u;
             b0[e] = addq(eb[e], v, q);
             b1[e] = subq(eb[e], v, q);
         }
     }
     stage_twiddles(L, C);  // the level's first barrier orders these stores
-    tree_level<LOGN, K, ELL, LAZY>(a0, a1, b0, b1, level, A.ks + (size_t)(A.logn - level) * ELL * 2 * KN, L, C,
-                                   inv, k, t);
+    tree_level<LOGN, K, ELL, LAZY, REC>(a0, a1, h0, h1, b0, b1, level, A.ks + (size_t)(A.logn - level) * ELL * 2 * KN,
+                                        L, A, C, inv, k, t);
 
     if (level < (uint32_t)LOGN) {
         // each thread rewrites exactly the words of node s it read
-        uint32_t* nd = A.node + ((size_t)qi * A.m + s) * 2 * KN + (size_t)k * N;
+        uint32_t* nd = A.node + ((size_t)qi * A.m + s) * 3 * KN + (size_t)k * N;
 #pragma unroll
         for (int e = 0; e < 8; e++) nd[t + e * T] = a0[e];
 #pragma unroll
-        for (int g = 0; g < 2; g++)
+        for (int g = 0; g < 2; g++) {
             *reinterpret_cast<uint4*>(nd + KN + Ly::group(t, g)) =
+                make_uint4(h0[4 * g], h0[4 * g + 1], h0[4 * g + 2], h0[4 * g + 3]);
+            *reinterpret_cast<uint4*>(nd + 2 * KN + Ly::group(t, g)) =
                 make_uint4(b0[4 * g], b0[4 * g + 1], b0[4 * g + 2], b0[4 * g + 3]);
+        }
         return;
     }
 
@@ -379,19 +432,19 @@ k_tree_combine(TreeArgs A, CtxDev C, ScaleDev inv) {
     }
 }
 
-template <int LOGN, int K, int ELL, bool LAZY>
+template <int LOGN, int K, int ELL, bool LAZY, bool REC>
 static hipError_t tree_launch(const TreeArgs& A, bool leaves, uint32_t blocks, const CtxDev& C, const ScaleDev& inv,
                               hipStream_t st) {
-    constexpr size_t shmem = TreeShape<LOGN, K, ELL>::kBytes;
+    constexpr size_t shmem = TreeShape<LOGN, K, ELL, REC>::kBytes;
     static_assert(shmem <= 160 * 1024, "tree kernel LDS");
-    auto kern = leaves ? k_tree_leaves<LOGN, K, ELL, LAZY> : k_tree_combine<LOGN, K, ELL, LAZY>;
+    auto kern = leaves ? k_tree_leaves<LOGN, K, ELL, LAZY, REC> : k_tree_combine<LOGN, K, ELL, LAZY, REC>;
     // raise the dynamic LDS limit once per instantiation
     static const hipError_t e0 =
-        shmem > 64 * 1024 ? hipFuncSetAttribute((const void*)k_tree_leaves<LOGN, K, ELL, LAZY>,
+        shmem > 64 * 1024 ? hipFuncSetAttribute((const void*)k_tree_leaves<LOGN, K, ELL, LAZY, REC>,
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem)
                           : hipSuccess;
     static const hipError_t e1 =
-        shmem > 64 * 1024 ? hipFuncSetAttribute((const void*)k_tree_combine<LOGN, K, ELL, LAZY>,
+        shmem > 64 * 1024 ? hipFuncSetAttribute((const void*)k_tree_combine<LOGN, K, ELL, LAZY, REC>,
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem)
                           : hipSuccess;
     if (e0 != hipSuccess) return e0;
@@ -405,8 +458,14 @@ static hipError_t tree_lazy(const TreeArgs& A, bool leaves, uint32_t blocks, con
                             hipStream_t st) {
     uint64_t qmax = 0;
     for (int kk = 0; kk < K; kk++) qmax = C.m[kk].q > qmax ? C.m[kk].q : qmax;
-    if ((uint64_t)ELL * 4 * qmax < (1ull << 32)) return tree_launch<LOGN, K, ELL, true>(A, leaves, blocks, C, inv, st);
-    return tree_launch<LOGN, K, ELL, false>(A, leaves, blocks, C, inv, st);
+    // the recovered digit needs the composite value to fit ell digits: q <= z^ell
+    const bool rec = (uint64_t)C.gadget_log * ELL >= 64 || C.q <= (1ull << (C.gadget_log * ELL));
+    if ((uint64_t)ELL * 4 * qmax < (1ull << 32)) {
+        if (rec) return tree_launch<LOGN, K, ELL, true, true>(A, leaves, blocks, C, inv, st);
+        return tree_launch<LOGN, K, ELL, true, false>(A, leaves, blocks, C, inv, st);
+    }
+    if (rec) return tree_launch<LOGN, K, ELL, false, true>(A, leaves, blocks, C, inv, st);
+    return tree_launch<LOGN, K, ELL, false, false>(A, leaves, blocks, C, inv, st);
 }
 
 template <int LOGN, int K>
