@@ -179,6 +179,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--tree-steps", type=int, default=10,
                     help="steps of the tree-packing (respond_seeded_packed) batch leg; 0 skips it")
+    ap.add_argument("--extract-steps", type=int, default=20,
+                    help="launches of the batched client-side extract leg (0 skips it)")
+    ap.add_argument("--extract-batch", type=int, default=4096, help="responses per extract launch")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (a known-size kernel for "
@@ -414,6 +417,8 @@ def main():
     }
     if args.tree_steps:
         line["tree_packing"] = tree_leg(args, M, ctx, db, dev, qsets, log)
+    if args.extract_steps:
+        line["extract"] = extract_leg(args, M, ctx, log)
     if not args.no_cpu:
         shards, b, seeds, y = qsets[(args.steps - 1) % n_sets]
         step(args.steps - 1)  # the batch of this query set, responses in pout
@@ -536,6 +541,90 @@ def tree_leg(args, M, ctx, db, dev, qsets, log):
         }
         log(f"tree cpu baseline: {n / t_cpu:.1f} queries/s, bit-exact {ok}/{n}")
     tout.free()
+    return out
+
+
+def extract_leg(args, M, ctx, log):
+    """Batched client-side extract (extract_inspiring, src/pir/extract.rs:
+    163-192) of InspiRING responses of the headline shape ([2][K][d] u32, 32 B
+    entries: 16 coefficients) with one secret key per response, all resident
+    in HBM: responses/s, the kernel's modmul and HBM rooflines, and a sample
+    checked bit for bit against the oracle (also the CPU baseline)."""
+    E = args.extract_batch
+    es, nvals = 32, 16
+    rng = np.random.default_rng(2028)
+    moduli = np.array([Q0, Q1], dtype=np.uint64)[:, None]
+    resp = (rng.integers(0, 2**63, (E, 2, K, D), dtype=np.uint64) % moduli).astype(np.uint32)
+    # RLWE secret keys: small signed coefficients as residues
+    e = rng.integers(-8, 9, (E, 1, D))
+    sk = np.where(e >= 0, e, np.array([Q0, Q1], dtype=np.int64)[None, :, None] + e).astype(np.uint32)
+    dresp, dsk, dent = M.DeviceArray.from_numpy(resp), M.DeviceArray.from_numpy(sk), M.DeviceArray(E * es)
+
+    def run():
+        ctx.extract_batch_device(E, M.EXTRACT_INSPIRING, es, dresp.ptr, dsk.ptr, dent.ptr)
+
+    for _ in range(args.warmup):
+        run()
+    ctx.set_profiling_kernels(["extract"])
+    ctx.set_profiling(True)
+    ctx.profile("extract", reset=True)
+    t0 = time.perf_counter()
+    for _ in range(args.extract_steps):
+        run()
+    elapsed = time.perf_counter() - t0
+    ms, n = ctx.profile("extract", reset=True)
+    ctx.set_profiling(False)
+    ctx.set_profiling_kernels(None)
+    avg = ms / max(n, 1) / 1e3
+    rps = E * args.extract_steps / elapsed
+    mm = nvals * K * D  # one product per term of each coefficient's dot product
+    nbytes = K * D * 4 + K * D * 4 + nvals * K * 4 + es  # a, the key, the b words read, the entry
+    mm_t = mm * E / avg / 1e12
+    gbs = nbytes * E / avg / 1e9
+    log(f"extract: {rps:.0f} responses/s, kernel {avg * 1e3:.4f} ms per {E} ({mm_t:.2f} Tmodmul/s, {gbs:.0f} GB/s)")
+    out = {
+        "workload": f"extract_inspiring of {E} device-resident InspiRING responses per launch (d=2048, CRT, "
+                    "32 B entries), one secret key each",
+        "value": round(rps, 1),
+        "unit": "responses/s",
+        "steps": args.extract_steps,
+        "avg_launch_ms": round(avg * 1e3, 4),
+        "roofline": {
+            "bound": "valu" if mm_t / MODMUL_PEAK_T >= gbs / HBM_PEAK_GBS else "hbm",
+            "kernel": "k_extract",
+            "achieved_modmul": round(mm_t, 3), "peak_modmul": round(MODMUL_PEAK_T, 2), "unit_modmul": "Tmodmul/s",
+            "frac_modmul": round(mm_t / MODMUL_PEAK_T, 4),
+            "achieved_hbm": round(gbs, 1), "peak_hbm": HBM_PEAK_GBS, "unit_hbm": "GB/s",
+            "frac_hbm": round(gbs / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic("k_extract", E),
+            "algorithmic_per_launch": f"{mm} modmul and {nbytes} B per response x {E} responses",
+        },
+        "cpu_baseline": None,
+    }
+    got = dent.download(np.zeros((E, es), dtype=np.uint8))
+    if not args.no_cpu:
+        sys.path.insert(0, ROOT)
+        import oracle as O
+
+        o = O.Oracle(O.baseline_params())
+        n_cpu, t_cpu, ok = 0, 0.0, 0
+        t0 = time.perf_counter()
+        while n_cpu < E and (time.perf_counter() - t0) < max(args.cpu_seconds / 4, 1.0):
+            want = o.extract_inspiring(resp[n_cpu].astype(np.uint64), sk[n_cpu].astype(np.uint64), es)
+            ok += want == got[n_cpu].tobytes()
+            n_cpu += 1
+        t_cpu = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "value": n_cpu / t_cpu,
+            "unit": "responses/s",
+            "cores": 1,
+            "kind": "port",
+            "sample": f"the first {n_cpu} responses of the launch, one thread, {t_cpu:.1f} s; "
+                      f"bit-exact vs GPU: {ok}/{n_cpu}",
+            "bit_exact_sampled": f"{ok}/{n_cpu}",
+        }
+        log(f"extract cpu baseline: {n_cpu / t_cpu:.1f} responses/s, bit-exact {ok}/{n_cpu}")
+    dresp.free(), dsk.free(), dent.free()
     return out
 
 

@@ -339,13 +339,66 @@ int inspire_query_from_binary(const inspire_params *params, const uint8_t *in, u
                               inspire_wire_query **out, inspire_query *q);
 void inspire_wire_query_free(inspire_wire_query *w);
 
+/* ---- client-side extract (src/pir/extract.rs) ---- */
+
+/* Decrypts the coefficients an entry needs under the client's RLWE secret key
+ * (RlweCiphertext::decrypt, src/rlwe/enc.rs:90-108: round((a s + b) / Delta)
+ * mod p) and rebuilds the entry's bytes (reconstruct_entry,
+ * src/pir/encode_db.rs:186-201), 2 bytes per column value.  secret_key =
+ * ClientState.rlwe_secret_key, uint64_t[K][d] canonical residues; entry =
+ * entry_size bytes out.  The response is read as inspire_response describes
+ * it (ciphertext_b_dim 0 means d). */
+/* extract (extract.rs:31-68): coefficient 0 of each of the first
+ * ceil(entry_size / 2) column ciphertexts; a response without column
+ * ciphertexts uses the ciphertext's coefficient 0 for every column. */
+int inspire_extract(inspire_ctx *ctx, const inspire_response *r, const uint64_t *secret_key, uint32_t entry_size,
+                    uint8_t *entry);
+/* extract_with_variant (extract.rs:78-92): INSPIRE_VARIANT_NO_PACKING is
+ * extract; ONE_PACKING and TWO_PACKING are extract_packed (:119-157),
+ * coefficient c times d^-1 mod p, or times 1 when d has no inverse mod p. */
+int inspire_extract_with_variant(inspire_ctx *ctx, const inspire_response *r, const uint64_t *secret_key,
+                                 uint32_t entry_size, uint32_t variant, uint8_t *entry);
+/* extract_inspiring (extract.rs:163-192): coefficient c, unscaled. */
+int inspire_extract_inspiring(inspire_ctx *ctx, const inspire_response *r, const uint64_t *secret_key,
+                              uint32_t entry_size, uint8_t *entry);
+
+/* The same for a batch of responses already in device memory, one mode for
+ * the batch.  Device layouts are u32 canonical residues, response i at
+ * responses + i * stride words, in the layouts inspire_device_batch.out
+ * gives them:
+ *   a [K][d], b [K][b_dim], then num_column_ciphertexts column ciphertexts
+ *   [2][K][d] (InsPIRe^0: the ciphertext, then its columns).
+ * secret_keys [count][K][d]; entries [count][entry_size] bytes out.  In the
+ * packed modes entry_size may need at most d coefficients (2d bytes): the
+ * reference panics beyond. */
+#define INSPIRE_EXTRACT_COLUMNS 0    /* extract */
+#define INSPIRE_EXTRACT_PACKED 1     /* extract_packed (extract_with_variant, ONE/TWO_PACKING) */
+#define INSPIRE_EXTRACT_INSPIRING 2  /* extract_inspiring */
+typedef struct {
+    uint32_t count;
+    uint32_t mode;                    /* INSPIRE_EXTRACT_* */
+    uint32_t entry_size;              /* bytes per entry */
+    uint32_t num_column_ciphertexts;  /* after the ciphertext; read in INSPIRE_EXTRACT_COLUMNS mode */
+    uint32_t b_dim;                   /* coefficients per residue of the ciphertext's b: d or 2d (0 means d) */
+    uint32_t reserved;
+    uint64_t stride;                  /* words from one response to the next; 0: K d + K b_dim +
+                                         num_column_ciphertexts 2 K d */
+    const uint32_t *responses;
+    const uint32_t *secret_keys;
+    uint8_t *entries;
+} inspire_extract_batch;
+
+/* Returns when the batch has finished. */
+int inspire_extract_batch_device(inspire_ctx *ctx, const inspire_extract_batch *batch);
+
 /* ---- profiling ---- */
 
 /* When enabled, every kernel launch of the respond path is bracketed by HIP
  * events on the context's stream.  inspire_ctx_profile reads the summed
  * duration (ms) and launch count of one kernel by name (rows_ntt,
  * expand_seeds, first_dim, cols_intt, fold, pack_front, pack_b0, pack_step1,
- * y_ntt, pack_chain, db_load) and, with reset != 0, clears it. */
+ * y_ntt, pack_chain, db_load, tree_leaves, tree_combine, extract) and,
+ * with reset != 0, clears it. */
 int inspire_ctx_set_profiling(inspire_ctx *ctx, int enable);
 /* Restricts the bracketing to the named kernels (comma-separated names as
  * above); NULL or "" brackets every kernel (the default).  Fewer events keep

@@ -100,7 +100,7 @@ struct PackTables {
 // HIP-event timing of each kernel category on the context's stream.
 enum ProfId {
     P_ROWS_NTT, P_EXPAND, P_FIRST_DIM, P_COLS_INTT, P_FOLD, P_B0, P_STEP1, P_Y_NTT, P_CHAIN, P_DB, P_FRONT,
-    P_TREE_LEAVES, P_TREE_COMBINE, P_COUNT
+    P_TREE_LEAVES, P_TREE_COMBINE, P_EXTRACT, P_COUNT
 };
 extern const char* const kProfNames[P_COUNT];
 

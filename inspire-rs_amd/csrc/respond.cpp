@@ -28,7 +28,7 @@ namespace inspire {
 
 const char* const kProfNames[P_COUNT] = {"rows_ntt", "expand_seeds", "first_dim", "cols_intt", "fold",
                                          "pack_b0",  "pack_step1",   "y_ntt",     "pack_chain", "db_load",
-                                         "pack_front", "tree_leaves", "tree_combine"};
+                                         "pack_front", "tree_leaves", "tree_combine", "extract"};
 
 ProfScope::ProfScope(inspire_ctx* ctx, int id) : ctx_(ctx), id_(id) {
     if (!ctx_->prof.on || !((ctx_->prof.mask >> id) & 1u)) return;

@@ -10,6 +10,9 @@ argument meaning and errors follow the Rust crate `inspire`:
     respond_with_variant(ctx, db_id, query, v) src/pir/respond.rs:129-183
     respond_one_packing(ctx, db_id, query)     src/pir/respond.rs:213-268
     respond_seeded_packed(ctx, db_id, query)   src/pir/respond.rs:428-435
+    extract(ctx, response, sk, entry_size)     src/pir/extract.rs:31-68
+    extract_with_variant(..., variant)         src/pir/extract.rs:78-92
+    extract_inspiring(ctx, response, sk, n)    src/pir/extract.rs:163-192
 
 Errors raise PirError carrying the reference's message text
 (src/pir/error.rs).  There is no CPU fallback: importing this module without
@@ -45,6 +48,7 @@ ERR_PENDING = 12
 REQUEST_BORROW = 0x1
 
 NO_PACKING, ONE_PACKING, TWO_PACKING = 0, 1, 2  # InspireVariant (src/params.rs:78-103)
+EXTRACT_COLUMNS, EXTRACT_PACKED, EXTRACT_INSPIRING = 0, 1, 2  # extract / extract_packed / extract_inspiring
 PACKING_INSPIRING, PACKING_TREE = 0, 1  # PackingMode (src/pir/query.rs:27-40)
 FLAG_RESIDUE_WISE_AUTOMORPHISM = 0x1
 
@@ -111,6 +115,21 @@ class _Response(C.Structure):
         ("num_column_ciphertexts", C.c_uint32),
         ("ciphertext_capacity", C.c_uint32),
         ("ciphertext_b_dim", C.c_uint32),
+    ]
+
+
+class _ExtractBatch(C.Structure):
+    _fields_ = [
+        ("count", C.c_uint32),
+        ("mode", C.c_uint32),
+        ("entry_size", C.c_uint32),
+        ("num_column_ciphertexts", C.c_uint32),
+        ("b_dim", C.c_uint32),
+        ("reserved", C.c_uint32),
+        ("stride", C.c_uint64),
+        ("responses", C.c_void_p),
+        ("secret_keys", C.c_void_p),
+        ("entries", C.c_void_p),
     ]
 
 
@@ -203,6 +222,11 @@ EXPORTS = [
     ("inspire_query_from_binary", C.c_int,
      [C.POINTER(_Params), C.c_char_p, C.c_uint64, C.c_uint32, C.POINTER(C.c_void_p), C.POINTER(_Query)]),
     ("inspire_wire_query_free", None, [C.c_void_p]),
+    ("inspire_extract", C.c_int, [C.c_void_p, C.POINTER(_Response), _u64p, C.c_uint32, C.c_void_p]),
+    ("inspire_extract_with_variant", C.c_int,
+     [C.c_void_p, C.POINTER(_Response), _u64p, C.c_uint32, C.c_uint32, C.c_void_p]),
+    ("inspire_extract_inspiring", C.c_int, [C.c_void_p, C.POINTER(_Response), _u64p, C.c_uint32, C.c_void_p]),
+    ("inspire_extract_batch_device", C.c_int, [C.c_void_p, C.POINTER(_ExtractBatch)]),
     ("inspire_ntt_forward", C.c_int, [C.c_void_p, _u64p, C.c_uint64]),
     ("inspire_ntt_inverse", C.c_int, [C.c_void_p, _u64p, C.c_uint64]),
     ("inspire_external_product", C.c_int, [C.c_void_p, C.c_uint64, _u64p, _u64p, _u64p]),
@@ -517,6 +541,14 @@ class Context:
         _raise(lib().inspire_respond_batch_device(self._h, C.byref(b)))
         return st
 
+    def extract_batch_device(self, count: int, mode: int, entry_size: int, responses_ptr: int, secret_keys_ptr: int,
+                             entries_ptr: int, num_column_ciphertexts: int = 0, b_dim: int = 0, stride: int = 0):
+        """Client-side extract of `count` device-resident responses (layouts in
+        include/inspire_mi355x.h, inspire_extract_batch) into entries_ptr."""
+        b = _ExtractBatch(count, mode, entry_size, num_column_ciphertexts, b_dim, 0, stride, responses_ptr,
+                          secret_keys_ptr, entries_ptr)
+        _raise(lib().inspire_extract_batch_device(self._h, C.byref(b)))
+
     def set_profiling(self, on: bool):
         _raise(lib().inspire_ctx_set_profiling(self._h, 1 if on else 0))
 
@@ -745,6 +777,34 @@ def respond_one_packing(ctx: Context, db_id: int, query: ClientQuery) -> ServerR
 def respond_seeded_packed(ctx: Context, db_id: int, query: ClientQuery) -> ServerResponse:
     """respond_seeded_packed (src/pir/respond.rs:428-435)."""
     return ctx._call(lib().inspire_respond_seeded_packed, db_id, query)
+
+
+# -- client-side extract (src/pir/extract.rs)
+
+def _extract(fn, ctx: Context, response: ServerResponse, secret_key: np.ndarray, entry_size: int, *extra) -> bytes:
+    keep: list = []
+    r = _response_struct(ctx.params, response, keep)
+    sk = np.ascontiguousarray(secret_key, dtype=np.uint64)
+    assert sk.size == ctx.K * ctx.d
+    out = np.zeros(max(entry_size, 1), dtype=np.uint8)
+    _raise(fn(ctx._h, C.byref(r), _ptr64(sk), entry_size, *extra, out.ctypes.data))
+    return out[:entry_size].tobytes()
+
+
+def extract(ctx: Context, response: ServerResponse, secret_key: np.ndarray, entry_size: int) -> bytes:
+    """extract (src/pir/extract.rs:31-68); secret_key = ClientState.rlwe_secret_key, uint64[K, d]."""
+    return _extract(lib().inspire_extract, ctx, response, secret_key, entry_size)
+
+
+def extract_with_variant(ctx: Context, response: ServerResponse, secret_key: np.ndarray, entry_size: int,
+                         variant: int) -> bytes:
+    """extract_with_variant (src/pir/extract.rs:78-92)."""
+    return _extract(lib().inspire_extract_with_variant, ctx, response, secret_key, entry_size, variant)
+
+
+def extract_inspiring(ctx: Context, response: ServerResponse, secret_key: np.ndarray, entry_size: int) -> bytes:
+    """extract_inspiring (src/pir/extract.rs:163-192)."""
+    return _extract(lib().inspire_extract_inspiring, ctx, response, secret_key, entry_size)
 
 
 # -- wire formats (bincode / serde_json of the reference's types)

@@ -11,6 +11,7 @@ uint64[2, K, d] (a, b); an RGSW ciphertext is uint64[2*ell, 2, K, d].
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 from dataclasses import dataclass
 
@@ -523,6 +524,35 @@ class Oracle:
         out = np.zeros(self.params.d, dtype=np.uint64)
         lib.or_decrypt(C.byref(self.cp), self.ntt, _p64(np.ascontiguousarray(ct, dtype=np.uint64)), _p64(np.ascontiguousarray(sk)), _p64(out))
         return out
+
+    # -- extract (src/pir/extract.rs): the client's side of a response
+    def extract(self, ciphertext, column_ciphertexts, sk, entry_size: int) -> bytes:
+        """extract (extract.rs:31-68): coefficient 0 of the first
+        ceil(entry_size / 2) column ciphertexts; without column ciphertexts,
+        coefficient 0 of the ciphertext stands for every column."""
+        ncols = (entry_size * 8 + 15) // 16
+        cols = [] if column_ciphertexts is None else list(column_ciphertexts)
+        if cols:
+            vals = [int(self.decrypt(c, sk)[0]) for c in cols[:ncols]]
+        else:
+            vals = [int(self.decrypt(ciphertext, sk)[0])] * ncols
+        return reconstruct_entry(vals, entry_size)
+
+    def extract_packed(self, ciphertext, sk, entry_size: int) -> bytes:
+        """extract_packed (extract.rs:119-157), the OnePacking / TwoPacking arms
+        of extract_with_variant (:78-92): coefficient c times d^-1 mod p, with
+        1 in place of d^-1 when gcd(d, p) != 1 (mod_inverse(..).unwrap_or(1))."""
+        d, p = self.params.d, self.params.p
+        ncols = (entry_size * 8 + 15) // 16
+        m = self.decrypt(ciphertext, sk)
+        dinv = pow(d, -1, p) if math.gcd(d, p) == 1 else 1
+        return reconstruct_entry([int(m[c]) * dinv % p for c in range(ncols)], entry_size)
+
+    def extract_inspiring(self, ciphertext, sk, entry_size: int) -> bytes:
+        """extract_inspiring (extract.rs:163-192): coefficient c, no scaling."""
+        ncols = (entry_size * 8 + 15) // 16
+        m = self.decrypt(ciphertext, sk)
+        return reconstruct_entry([int(m[c]) for c in range(ncols)], entry_size)
 
     def encode_shard(self, db: np.ndarray, entry_size: int, shard_idx: int, entries_per_shard=None):
         db = np.ascontiguousarray(db, dtype=np.uint8)
