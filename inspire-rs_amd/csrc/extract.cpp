@@ -58,6 +58,7 @@ static ExtractArgs make_args(const inspire_ctx* ctx, const inspire_extract_batch
     A.half_delta = R->delta / 2;
     A.p = R->params.p;
     A.d_inv = packed_multiplier(N, R->params.p);
+    for (uint32_t k = 0; k < K; k++) A.r64[k] = (uint32_t)pow2_64_mod(ctx->hm[k].q);
     return A;
 }
 

@@ -24,6 +24,7 @@ struct ExtractArgs {
     uint32_t col_off;         // words from a response to its first column ciphertext (K N + K b_dim)
     uint64_t delta, half_delta, p;
     uint64_t d_inv;           // extract_packed's multiplier: d^-1 mod p, or 1
+    uint32_t r64[kMaxK];      // 2^64 mod q_k
 };
 
 hipError_t launch_extract(const ExtractArgs& A, const CtxDev& C, hipStream_t st);
