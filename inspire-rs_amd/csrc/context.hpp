@@ -175,6 +175,22 @@ class ProfScope {
     int id_;
     hipEvent_t a_ = nullptr;
 };
+// Events for one kernel launch that records them itself
+// (hipExtLaunchKernelGGL): the span is the kernel's execution, without the
+// host's submission time that a scope around a launch on an idle stream
+// includes.  start()/stop() are null when profiling is off.
+class ProfLaunch {
+   public:
+    ProfLaunch(inspire_ctx* ctx, int id);
+    ~ProfLaunch();
+    hipEvent_t start() const { return a_; }
+    hipEvent_t stop() const { return b_; }
+
+   private:
+    inspire_ctx* ctx_;
+    int id_;
+    hipEvent_t a_ = nullptr, b_ = nullptr;
+};
 // Folds finished spans into the totals (synchronises the stream).
 void prof_collect(inspire_ctx* ctx);
 

@@ -71,8 +71,8 @@ void extract_batch(inspire_ctx* ctx, const inspire_extract_batch& b) {
     A.sk = b.secret_keys;
     A.entries = b.entries;
     {
-        ProfScope ps(ctx, P_EXTRACT);
-        hip_check(launch_extract(A, ctx->dev, ctx->stream), "extract");
+        ProfLaunch pl(ctx, P_EXTRACT);
+        hip_check(launch_extract(A, ctx->dev, ctx->stream, pl.start(), pl.stop()), "extract");
     }
     hip_check(hipStreamSynchronize(ctx->stream), "extract sync");
 }

@@ -27,7 +27,9 @@ struct ExtractArgs {
     uint32_t r64[kMaxK];      // 2^64 mod q_k
 };
 
-hipError_t launch_extract(const ExtractArgs& A, const CtxDev& C, hipStream_t st);
+// start/stop: events the launch records around the kernel (null: none).
+hipError_t launch_extract(const ExtractArgs& A, const CtxDev& C, hipStream_t st, hipEvent_t start = nullptr,
+                          hipEvent_t stop = nullptr);
 
 // Runs a device batch (inspire_extract_batch_device).
 void extract_batch(inspire_ctx* ctx, const inspire_extract_batch& b);

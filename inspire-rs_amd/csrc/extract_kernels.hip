@@ -17,6 +17,7 @@
 // the residues composed (crt_compose_2, src/math/crt.rs:41-50), rounded,
 // reduced mod p and written as two bytes per value (reconstruct_entry,
 // src/pir/encode_db.rs:186-201).
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "../../include/inspire_mi355x.h"
@@ -205,29 +206,34 @@ __global__ void __launch_bounds__(kThreads, (K << LOGN) <= 4096 ? 4 : 2) k_extra
 }
 
 template <int LOGN, int K>
-hipError_t launch_k(const ExtractArgs& A, const CtxDev& C, hipStream_t st) {
+hipError_t launch_k(const ExtractArgs& A, const CtxDev& C, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
     if (A.use_cols)
-        hipLaunchKernelGGL((k_extract<LOGN, K, true>), dim3(A.count), dim3(kThreads), 0, st, A, C);
+        hipExtLaunchKernelGGL((k_extract<LOGN, K, true>), dim3(A.count), dim3(kThreads), 0, st, e0, e1, 0, A, C);
     else
-        hipLaunchKernelGGL((k_extract<LOGN, K, false>), dim3(A.count), dim3(kThreads), 0, st, A, C);
+        hipExtLaunchKernelGGL((k_extract<LOGN, K, false>), dim3(A.count), dim3(kThreads), 0, st, e0, e1, 0, A, C);
     return hipGetLastError();
 }
 
 template <int LOGN>
-hipError_t launch_t(const ExtractArgs& A, const CtxDev& C, hipStream_t st) {
-    return C.K == 2 ? launch_k<LOGN, 2>(A, C, st) : launch_k<LOGN, 1>(A, C, st);
+hipError_t launch_t(const ExtractArgs& A, const CtxDev& C, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
+    return C.K == 2 ? launch_k<LOGN, 2>(A, C, st, e0, e1) : launch_k<LOGN, 1>(A, C, st, e0, e1);
 }
 
 }  // namespace
 
-hipError_t launch_extract(const ExtractArgs& A, const CtxDev& C, hipStream_t st) {
-    if (!A.count || !A.nvals) return hipSuccess;
+hipError_t launch_extract(const ExtractArgs& A, const CtxDev& C, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
+    if (!A.count || !A.nvals) {
+        // an empty batch still completes the profiler's span
+        if (e0 && hipEventRecord(e0, st) != hipSuccess) return hipGetLastError();
+        if (e1 && hipEventRecord(e1, st) != hipSuccess) return hipGetLastError();
+        return hipSuccess;
+    }
     switch (C.logn) {
-        case 8: return launch_t<8>(A, C, st);
-        case 9: return launch_t<9>(A, C, st);
-        case 10: return launch_t<10>(A, C, st);
-        case 11: return launch_t<11>(A, C, st);
-        case 12: return launch_t<12>(A, C, st);
+        case 8: return launch_t<8>(A, C, st, e0, e1);
+        case 9: return launch_t<9>(A, C, st, e0, e1);
+        case 10: return launch_t<10>(A, C, st, e0, e1);
+        case 11: return launch_t<11>(A, C, st, e0, e1);
+        case 12: return launch_t<12>(A, C, st, e0, e1);
         default: return hipErrorInvalidValue;
     }
 }

@@ -57,6 +57,26 @@ ProfScope::~ProfScope() {
     P.spans.push_back({id_, a_, b});
 }
 
+ProfLaunch::ProfLaunch(inspire_ctx* ctx, int id) : ctx_(ctx), id_(id) {
+    if (!ctx_->prof.on || !((ctx_->prof.mask >> id) & 1u)) return;
+    auto& P = ctx_->prof;
+    hipEvent_t e[2];
+    for (int j = 0; j < 2; j++) {
+        if (P.pool.empty()) {
+            hip_check(hipEventCreate(&e[j]), "event");
+        } else {
+            e[j] = P.pool.back();
+            P.pool.pop_back();
+        }
+    }
+    a_ = e[0];
+    b_ = e[1];
+}
+
+ProfLaunch::~ProfLaunch() {
+    if (a_) ctx_->prof.spans.push_back({id_, a_, b_});
+}
+
 void prof_collect(inspire_ctx* ctx) {
     auto& P = ctx->prof;
     if (P.spans.empty()) return;
