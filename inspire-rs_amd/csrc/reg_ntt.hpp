@@ -30,6 +30,16 @@ namespace inspire {
 // completed, then s_barrier.  Global loads issued earlier are not waited for.
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
+// Ordering point for an LDS exchange among the lanes of one wave: the LDS
+// unit executes a wave's instructions in order, so only the compiler has to
+// keep the stores before the loads (wavefront-scope fences around a
+// scheduling barrier).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 struct Mod {
     uint32_t q, q2, qinv;
 };
@@ -70,11 +80,20 @@ template <int LOGN>
 struct Lay {
     static constexpr int N = 1 << LOGN, T = N / 8, LP = NttShape<LOGN>::LDS;
     static constexpr int R8 = LOGN / 3, REM = LOGN % 3;
-    // first word of the two 4-element groups of a thread's slot positions
-    static __device__ __forceinline__ int group(int t, int g) {
-        if constexpr (REM != 0) return 4 * (t + g * T);
-        else return 8 * t + 4 * g;
-    }
+    // first word of the two 4-element groups of a thread's slot positions:
+    // eight consecutive slots, so the blocks the last forward passes and the
+    // first inverse passes exchange stay inside one wave
+    static __device__ __forceinline__ int group(int t, int g) { return 8 * t + 4 * g; }
+    // Exchange between forward radix-8 pass p and the next read: the block a
+    // pass-(p+1) thread reads was written by N / 8^(p+1) threads of pass p.
+    // With at most 64 of them (one wave) the exchange needs no s_barrier.
+    static constexpr bool fwd_wave_local(int p) { return (N >> (3 * (p + 1))) <= 64; }
+    // Exchange into inverse radix-8 pass p: its blocks of 8 t0 elements, t0 =
+    // 2^REM 8^p, were written by the same t0 threads.
+    static constexpr bool inv_wave_local(int p) { return ((1 << REM) << (3 * p)) <= 64; }
+    // The forward transform passes at least one s_barrier (callers that need
+    // one between earlier LDS reads and later writes rely on it otherwise).
+    static constexpr bool fwd_has_barrier = !fwd_wave_local(0) && R8 >= 2;
 };
 
 template <int LOGN>
@@ -101,7 +120,7 @@ __device__ __forceinline__ void intt1(uint32_t (&x)[8], int t, uint32_t* rb, con
     if constexpr (L::REM != 0) {
 #pragma unroll
         for (int g = 0; g < 2; g++) {
-            const int i = t + g * L::T;
+            const int i = 2 * t + g;  // the 4-slot group at Lay::group(t, g)
             uint32_t& x0 = x[4 * g + 0];
             uint32_t& x1 = x[4 * g + 1];
             uint32_t& x2 = x[4 * g + 2];
@@ -115,7 +134,7 @@ __device__ __forceinline__ void intt1(uint32_t (&x)[8], int t, uint32_t* rb, con
             }
             *reinterpret_cast<uint4*>(&rb[lds_ix(4 * i)]) = make_uint4(x0, x1, x2, x3);
         }
-        lds_barrier();
+        wave_lds_sync();  // pass 0's blocks of 8 2^REM elements belong to 2^REM threads
     }
     int t0 = 1 << L::REM;
 #pragma unroll
@@ -141,7 +160,10 @@ __device__ __forceinline__ void intt1(uint32_t (&x)[8], int t, uint32_t* rb, con
             for (int k = 0; k < 4; k++) gs_m(x[k], x[k + 4], w, M);
 #pragma unroll
             for (int e = 0; e < 8; e++) rb[lds_step(bix, e, t0)] = x[e];
-            lds_barrier();
+            if (L::inv_wave_local(p + 1))
+                wave_lds_sync();
+            else
+                lds_barrier();
         } else {
             // last stage (m = 1) with the output scale: X' = c (X + Y), Y' = cw (X - Y)
 #pragma unroll
@@ -207,14 +229,17 @@ __device__ __forceinline__ void ntt_digits(uint32_t (&x)[ELL][8], int t, uint32_
             for (int j = 0; j < ELL; j++)
 #pragma unroll
                 for (int e = 0; e < 8; e++) db[j * LP + lds_step(bix, e, st)] = x[j][e];
-            lds_barrier();
+            if (L::fwd_wave_local(p))
+                wave_lds_sync();
+            else
+                lds_barrier();
         }
         m0 <<= 3;
     }
     if constexpr (L::REM != 0) {
 #pragma unroll
         for (int g = 0; g < 2; g++) {
-            const int i = t + g * L::T;
+            const int i = 2 * t + g;  // the 4-slot group at Lay::group(t, g)
 #pragma unroll
             for (int j = 0; j < ELL; j++) {
                 const uint4 v = *reinterpret_cast<const uint4*>(&db[j * LP + lds_ix(4 * i)]);

@@ -196,10 +196,14 @@ __device__ __forceinline__ void tree_level(uint32_t (&a0)[8], const uint32_t (&a
             if constexpr (REC) ht[4 * g + c] = hb[sx];
         }
 
-    if constexpr (ND > 0)
+    if constexpr (ND > 0) {
         ntt_digits<LOGN, ND>(reinterpret_cast<uint32_t(&)[ND][8]>(dg), t, L.dbuf + k * DB * LP, L.twf + k * N, M);
-    else
-        lds_barrier();  // no transform: every thread is past its reads of abuf before the INTT reuses it
+        // every thread is past its gathers from abuf before the INTT reuses it: the transform's
+        // own barrier orders them when it has one
+        if constexpr (!Ly::fwd_has_barrier) lds_barrier();
+    } else {
+        lds_barrier();
+    }
     if constexpr (REC) {
         // transform outputs < 4q and Montgomery products < 2q keep the difference in (0, 7q)
 #pragma unroll
@@ -226,7 +230,6 @@ __device__ __forceinline__ void tree_level(uint32_t (&a0)[8], const uint32_t (&a
         h0[e] = addq(h0[e], csub(A[e], q), q);
         b0[e] = addq(addq(b0[e], bt[e], q), csub(mont_reduce(sb, q, M.qinv), q), q);
     }
-    // the digit transform's barriers ordered every read of abuf before this
     intt1<LOGN>(A, t, ab, L.twf + k * N, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
 #pragma unroll
     for (int e = 0; e < 8; e++) a0[e] = addq(a0[e], csub(A[e], q), q);
