@@ -270,7 +270,7 @@ void pack_run(inspire_ctx* ctx, uint32_t gamma, uint32_t Q, const uint32_t* rows
     PackTables& T = pack_tables(ctx, gamma);
     const bool rw = (ctx->params.flags & INSPIRE_FLAG_RESIDUE_WISE_AUTOMORPHISM) != 0;
     const size_t rpre_words = (size_t)Q * gamma * KN;
-    const uint32_t nparts = T.fast_front ? ctx->N / 256 : 1;
+    const uint32_t nparts = T.fast_front ? ctx->N / 64 : 1;  // the front writes one b sum per wave
     const size_t bpart_words = (size_t)Q * nparts * gamma * K * 2;  // u64
     ctx->ws3.reserve((rpre_words + bpart_words) * 4);
     uint32_t* d_rpre = ctx->ws3.get<uint32_t>();

@@ -61,16 +61,125 @@ k_pack_b0(const uint32_t* __restrict__ cols, uint64_t* __restrict__ bpart, uint3
 // gamma.  In the reference's residue handling only residue 0 of r_pre is
 // read later, so residue 1 (a-part) is computed in the corrected mode only.
 // Column values never leave registers: ahat feeds the DFT, bhat feeds the
-// per-block b sums (bpart [Q][n/256][gamma][K] u64).
+// per-wave b sums (bpart [Q][n/64][gamma][K] u64, each part reduced mod q_k).
+// No LDS: the b sums are a reduce-scatter across the wave's lanes, which
+// leaves room on a CU for this kernel beside another lane's packing chain.
 // ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t front_addmod(uint32_t a, uint32_t b, uint32_t q) { return csub(a + b, q); }
+
+// Sums x[v] (v < G <= 64, canonical mod q) over the 64 lanes of the wave;
+// returns on lane l the sum of value l / (64 / G).  While more than one value
+// remains, each step with offset o keeps the upper half of the values on the
+// lanes with bit o set (a masked xor swaps the halves in fixed registers).
+template <int G>
+__device__ __forceinline__ uint32_t wave_sum_scatter(uint32_t (&x)[G], uint32_t lane, uint32_t q) {
+    static_assert(G >= 1 && G <= 64 && (G & (G - 1)) == 0, "G");
+    int off = 32;
+#pragma unroll
+    for (int h = G / 2; h >= 1; h >>= 1, off >>= 1) {
+        const uint32_t low = (lane & off) ? 0u : 0xffffffffu;
+#pragma unroll
+        for (int j = 0; j < h; j++) {
+            const uint32_t d = (x[j] ^ x[j + h]) & low;
+            const uint32_t send = x[j] ^ d, keep = x[j + h] ^ d;
+            x[j] = front_addmod(keep, __shfl_xor(send, off, 64), q);
+        }
+    }
+#pragma unroll
+    for (; off >= 1; off >>= 1) x[0] = front_addmod(x[0], __shfl_xor(x[0], off, 64), q);
+    return x[0];
+}
+// One residue of the fused front end for the thread's slot s: the column
+// values (a only when WANT_A), the twisted DFT of the a values into r_pre,
+// and this wave's b sums.
+template <int LG, bool WANT_A, int ELL>
+__device__ __forceinline__ void front_residue(int k, const uint32_t* R, const uint32_t* Dq, uint32_t* r_pre,
+                                              uint64_t* bpart, const uint32_t* twist, const uint2* dftw,
+                                              const CtxDev& C, uint32_t qi, uint32_t blk, int s, uint32_t cls) {
+    constexpr int G = 1 << LG;
+    constexpr int CH = G < 8 ? G : 8;  // columns whose digits are loaded together
+    const int n = (int)C.n, nb = n / 256;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const size_t KN = (size_t)C.K * n;
+    const uint32_t q = C.m[k].q, q2 = 2 * q, qinv = C.m[k].qinv;
+    uint32_t ca[ELL], cb[ELL];
+#pragma unroll
+    for (int i = 0; i < ELL; i++) {
+        ca[i] = WANT_A ? R[((size_t)(2 * i) * C.K + k) * n] : 0u;
+        cb[i] = R[((size_t)(2 * i + 1) * C.K + k) * n];
+    }
+    uint32_t v[G], bv[G];
+#pragma unroll
+    for (int j0 = 0; j0 < G; j0 += CH) {
+        // every digit load of the chunk is issued before the first use
+        uint32_t dv[CH][ELL];
+#pragma unroll
+        for (int jj = 0; jj < CH; jj++)
+#pragma unroll
+            for (int i = 0; i < ELL; i++) dv[jj][i] = Dq[((((size_t)(j0 + jj)) * ELL + i) * C.K + k) * n];
+#pragma unroll
+        for (int jj = 0; jj < CH; jj++) {
+            const int j = j0 + jj;
+            uint64_t aa = 0, bb = 0;
+#pragma unroll
+            for (int i = 0; i < ELL; i++) {
+                if constexpr (WANT_A) aa += (uint64_t)dv[jj][i] * ca[i];
+                bb += (uint64_t)dv[jj][i] * cb[i];
+            }
+            bv[j] = csub(mont_reduce(bb, q, qinv), q);
+            if constexpr (WANT_A) {
+                const uint32_t a = csub(mont_reduce(aa, q, qinv), q);
+                // twist by gamma^-1 w_s^j (Montgomery form) and place at the bit-reversed index
+                const uint32_t tw = twist[((size_t)k * G + j) * n + s];
+                int p = 0;
+#pragma unroll
+                for (int b = 0; b < LG; b++) p |= ((j >> b) & 1) << (LG - 1 - b);
+                v[p] = csub(mont_reduce((uint64_t)a * tw, q, qinv), q);
+            }
+        }
+    }
+    if constexpr (WANT_A) {
+        // radix-2 DIT over the bit-reversed input: X_i = sum_j v_j W^(ij), W = zeta_s^-1
+        const uint2* W = dftw + ((size_t)k * (G / 2 > 0 ? G / 2 : 1) + cls) * (G / 2 > 0 ? G / 2 : 1);
+#pragma unroll
+        for (int len = 2; len <= G; len <<= 1) {
+            const int half = len / 2, step = G / len;
+#pragma unroll
+            for (int b = 0; b < G; b += len)
+#pragma unroll
+                for (int kk = 0; kk < half; kk++) {
+                    uint32_t& X = v[b + kk];
+                    uint32_t& Y = v[b + kk + half];
+                    if (kk == 0) {
+                        const uint32_t x = csub(X, q2), y = csub(Y, q2);
+                        X = x + y;
+                        Y = x - y + q2;
+                    } else {
+                        const uint2 w = W[kk * step];
+                        ct_bf(X, Y, w.x, w.y, q, q2);
+                    }
+                }
+        }
+        uint32_t* out = r_pre + (size_t)qi * G * KN + (size_t)k * n + s;
+#pragma unroll
+        for (int i = 0; i < G; i++) out[(size_t)i * KN] = csub(csub(v[i], q2), q);
+    }
+    // this wave's sums of bhat_j over its 64 slots, part blk * 4 + wave
+    const uint32_t sum = wave_sum_scatter<G>(bv, lane, q);
+    constexpr uint32_t LPV = 64 / G;  // lanes holding each value
+    if (lane % LPV == 0) {
+        const uint32_t j = lane / LPV, part = blk * 4 + wave;
+        bpart[(((size_t)qi * nb * 4 + part) * G + j) * C.K + k] = sum;
+    }
+}
+
 template <int LG, bool RW, int ELL>
 __global__ void __launch_bounds__(256)
 k_pack_front(const uint64_t* __restrict__ qdig, const uint32_t* __restrict__ rows, uint32_t* __restrict__ r_pre,
              uint64_t* __restrict__ bpart, const uint32_t* __restrict__ twist, const uint2* __restrict__ dftw,
              CtxDev C) {
     constexpr int G = 1 << LG;
-    constexpr int CH = G < 16 ? G : 16;  // columns whose digits are loaded together
-    __shared__ uint32_t red[2 * G][257];
     const int n = (int)C.n, nb = n / 256, tid = threadIdx.x;
     const uint32_t qi = blockIdx.x / nb, blk = blockIdx.x % nb;
     const int s = blk * 256 + tid;
@@ -79,89 +188,10 @@ k_pack_front(const uint64_t* __restrict__ qdig, const uint32_t* __restrict__ row
     // the host points a missing shard at a valid one and reports it
     const uint32_t* Dq = reinterpret_cast<const uint32_t*>(qdig[qi]) + s;
     const uint32_t cls = ((2u * (__brev((uint32_t)s) >> (32 - C.logn)) + 1u) & (G - 1)) >> 1;
-    for (int k = 0; k < (int)C.K; k++) {
-        const uint32_t q = C.m[k].q, q2 = 2 * q, qinv = C.m[k].qinv;
-        const bool want_a = k == 0 || RW;
-        uint32_t ca[ELL], cb[ELL];
-#pragma unroll
-        for (int i = 0; i < ELL; i++) {
-            ca[i] = want_a ? R[((size_t)(2 * i) * C.K + k) * n] : 0u;
-            cb[i] = R[((size_t)(2 * i + 1) * C.K + k) * n];
-        }
-        uint32_t v[G];
-#pragma unroll
-        for (int j0 = 0; j0 < G; j0 += CH) {
-            // every digit load of the chunk is issued before the first use
-            uint32_t dv[CH][ELL];
-#pragma unroll
-            for (int jj = 0; jj < CH; jj++)
-#pragma unroll
-                for (int i = 0; i < ELL; i++) dv[jj][i] = Dq[((((size_t)(j0 + jj)) * ELL + i) * C.K + k) * n];
-#pragma unroll
-            for (int jj = 0; jj < CH; jj++) {
-                const int j = j0 + jj;
-                uint64_t aa = 0, bb = 0;
-#pragma unroll
-                for (int i = 0; i < ELL; i++) {
-                    aa += (uint64_t)dv[jj][i] * ca[i];
-                    bb += (uint64_t)dv[jj][i] * cb[i];
-                }
-                red[k * G + j][tid] = csub(mont_reduce(bb, q, qinv), q);
-                if (want_a) {
-                    const uint32_t a = csub(mont_reduce(aa, q, qinv), q);
-                    // twist by gamma^-1 w_s^j (Montgomery form) and place at the bit-reversed index
-                    const uint32_t tw = twist[((size_t)k * G + j) * n + s];
-                    int p = 0;
-#pragma unroll
-                    for (int b = 0; b < LG; b++) p |= ((j >> b) & 1) << (LG - 1 - b);
-                    v[p] = csub(mont_reduce((uint64_t)a * tw, q, qinv), q);
-                }
-            }
-        }
-        if (want_a) {
-            // radix-2 DIT over the bit-reversed input: X_i = sum_j v_j W^(ij), W = zeta_s^-1
-            const uint2* W = dftw + ((size_t)k * (G / 2 > 0 ? G / 2 : 1) + cls) * (G / 2 > 0 ? G / 2 : 1);
-#pragma unroll
-            for (int len = 2; len <= G; len <<= 1) {
-                const int half = len / 2, step = G / len;
-#pragma unroll
-                for (int b = 0; b < G; b += len)
-#pragma unroll
-                    for (int kk = 0; kk < half; kk++) {
-                        uint32_t& X = v[b + kk];
-                        uint32_t& Y = v[b + kk + half];
-                        if (kk == 0) {
-                            const uint32_t x = csub(X, q2), y = csub(Y, q2);
-                            X = x + y;
-                            Y = x - y + q2;
-                        } else {
-                            const uint2 w = W[kk * step];
-                            ct_bf(X, Y, w.x, w.y, q, q2);
-                        }
-                    }
-            }
-            uint32_t* out = r_pre + (size_t)qi * G * KN + (size_t)k * n + s;
-#pragma unroll
-            for (int i = 0; i < G; i++) out[(size_t)i * KN] = csub(csub(v[i], q2), q);
-        }
-    }
-    __syncthreads();
-    // per-block sums of bhat_j over this block's 256 slots, rows (k, j)
-    constexpr int ROWS_MAX = 2 * G;
-    const int rows_n = (int)C.K * G;
-    const int tpr = 256 / rows_n < 64 ? 256 / rows_n : 64;  // threads per row, within one wavefront
-    const int row = tid / tpr, part = tid % tpr;
-    uint64_t acc = 0;
-    if (row < rows_n) {
-        const int per = 256 / tpr;
-        for (int t = part * per; t < (part + 1) * per; t++) acc += red[row][t];
-    }
-    for (int o = 1; o < tpr; o <<= 1) acc += __shfl_down(acc, o, 64);
-    if (row < rows_n && part == 0) {
-        const int k = row / G, j = row % G;
-        bpart[(((size_t)qi * nb + blk) * G + j) * C.K + k] = acc;
-    }
-    (void)ROWS_MAX;
+    front_residue<LG, true, ELL>(0, R, Dq, r_pre, bpart, twist, dftw, C, qi, blk, s, cls);
+    // in the reference's residue handling only residue 0 of r_pre is read later
+    for (int k = 1; k < (int)C.K; k++)
+        front_residue<LG, RW, ELL>(k, R, Dq, r_pre, bpart, twist, dftw, C, qi, blk, s, cls);
 }
 
 // Step 1 before the automorphism:
