@@ -171,9 +171,11 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--batch", type=int, default=256)
-    ap.add_argument("--lanes", type=int, default=3,
+    ap.add_argument("--lanes", type=int, default=2,
                     help="lanes (inspire_ctx_create_lane) submitting batches from their own threads; "
-                         "1 lane: 0.84 M queries/s at p50 0.30 ms, 2: 1.17 M at 0.43 ms, 3: 1.28 M at 0.58 ms")
+                         "1 lane: 0.84 M queries/s at p50 0.30 ms, 2: 1.17 M at 0.43 ms, 3: 1.28 M at 0.58 ms "
+                         "(with more lanes the chain launches overlap, so their per-launch time no longer "
+                         "measures the kernel alone)")
     ap.add_argument("--db-log2", type=int, default=26)
     ap.add_argument("--single-steps", type=int, default=32, help="steps of the one-query-per-launch run")
     ap.add_argument("--host-steps", type=int, default=3, help="steps of the host-buffer (PCIe-inclusive) run")
