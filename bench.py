@@ -176,6 +176,8 @@ def main():
                          "1 lane: 0.84 M queries/s at p50 0.30 ms, 2: 1.17 M at 0.43 ms, 3: 1.28 M at 0.58 ms "
                          "(with more lanes the chain launches overlap, so their per-launch time no longer "
                          "measures the kernel alone)")
+    ap.add_argument("--lanes-sweep", type=str, default="1,3",
+                    help="other lane counts timed after the headline (reported as lanes_sweep)")
     ap.add_argument("--db-log2", type=int, default=26)
     ap.add_argument("--single-steps", type=int, default=32, help="steps of the one-query-per-launch run")
     ap.add_argument("--host-steps", type=int, default=3, help="steps of the host-buffer (PCIe-inclusive) run")
@@ -259,38 +261,43 @@ def main():
         ln.set_profiling(True)
         for name in kernels:
             ln.profile(name, reset=True)
-    lat = []
-    counter = [0]
-    lock = threading.Lock()
-    errors = []
-    go = threading.Barrier(len(lanes) + 1)
+    def run_lanes(lns, outs, steps):
+        """steps batches taken in turn by the lanes' threads: (elapsed s, per-step latencies ms)"""
+        lat = []
+        counter = [0]
+        lock = threading.Lock()
+        errors = []
+        go = threading.Barrier(len(lns) + 1)
 
-    def lane_loop(j):
+        def lane_loop(j):
+            go.wait()
+            try:
+                while True:
+                    with lock:
+                        i = counter[0]
+                        counter[0] += 1
+                    if i >= steps:
+                        return
+                    t0 = time.perf_counter()
+                    step(i, c=lns[j], out=outs[j])
+                    with lock:
+                        lat.append((time.perf_counter() - t0) * 1e3)
+            except Exception as e:  # surfaced after the timed region
+                errors.append(e)
+
+        threads = [threading.Thread(target=lane_loop, args=(j,)) for j in range(len(lns))]
+        for t in threads:
+            t.start()
         go.wait()
-        try:
-            while True:
-                with lock:
-                    i = counter[0]
-                    counter[0] += 1
-                if i >= args.steps:
-                    return
-                t0 = time.perf_counter()
-                step(i, c=lanes[j], out=louts[j])
-                with lock:
-                    lat.append((time.perf_counter() - t0) * 1e3)
-        except Exception as e:  # surfaced after the timed region
-            errors.append(e)
+        t_start = time.perf_counter()
+        for t in threads:
+            t.join()
+        el = time.perf_counter() - t_start
+        if errors:
+            raise errors[0]
+        return el, lat
 
-    threads = [threading.Thread(target=lane_loop, args=(j,)) for j in range(len(lanes))]
-    for t in threads:
-        t.start()
-    go.wait()
-    t_start = time.perf_counter()
-    for t in threads:
-        t.join()
-    elapsed = time.perf_counter() - t_start
-    if errors:
-        raise errors[0]
+    elapsed, lat = run_lanes(lanes, louts, args.steps)
     prof = {}
     for name in kernels:
         tot = [ln.profile(name, reset=True) for ln in lanes]
@@ -300,6 +307,21 @@ def main():
     qps = B * args.steps / elapsed
     log(f"batched: {qps:.0f} queries/s, {elapsed / args.steps * 1e3:.3f} ms/step over {len(lanes)} lanes, "
         f"p50 {percentile(lat, 50):.3f} ms")
+    # the same batches from other numbers of lanes (more lanes: more throughput, longer latency)
+    sweep = []
+    for nl in [int(x) for x in args.lanes_sweep.split(",") if x.strip()]:
+        if nl < 1 or nl == len(lanes):
+            continue
+        while len(lanes) < nl:
+            lanes.append(ctx.lane())
+            louts.append(M.DeviceArray(B * 2 * K * D * 4))
+            for i in range(args.warmup):
+                step(i, c=lanes[-1], out=louts[-1])
+        el, lt = run_lanes(lanes[:nl], louts[:nl], args.steps)
+        sweep.append({"lanes": nl, "value": round(B * args.steps / el, 1), "ms_per_step": round(el / args.steps * 1e3, 3),
+                      "p50_ms": round(percentile(lt, 50), 3), "p99_ms": round(percentile(lt, 99), 3)})
+        log(f"  {nl} lanes: {sweep[-1]['value']:.0f} queries/s, p50 {sweep[-1]['p50_ms']} ms, "
+            f"p99 {sweep[-1]['p99_ms']} ms")
     # untimed: every kernel of the step, for the breakdown
     ctx.set_profiling(True)
     ctx.set_profiling_kernels(None)
@@ -412,6 +434,7 @@ def main():
         "single_query_p50_ms": round(percentile(single, 50), 3) if single else None,
         "speedup_vs_single_query_launch": round(qps / single_qps, 2) if single_qps else None,
         "host_buffer_api_qps": round(host_qps, 1) if host_qps else None,
+        "lanes_sweep": sweep,
         "modmul_per_query": MODMUL_QUERY,
         "roofline": roofline,
         "roofline_hbm_front": roofline_hbm,
