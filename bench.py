@@ -298,6 +298,7 @@ def main():
         return el, lat
 
     elapsed, lat = run_lanes(lanes, louts, args.steps)
+    n_lanes = len(lanes)
     prof = {}
     for name in kernels:
         tot = [ln.profile(name, reset=True) for ln in lanes]
@@ -305,12 +306,12 @@ def main():
     for ln in lanes:
         ln.set_profiling(False)
     qps = B * args.steps / elapsed
-    log(f"batched: {qps:.0f} queries/s, {elapsed / args.steps * 1e3:.3f} ms/step over {len(lanes)} lanes, "
+    log(f"batched: {qps:.0f} queries/s, {elapsed / args.steps * 1e3:.3f} ms/step over {n_lanes} lanes, "
         f"p50 {percentile(lat, 50):.3f} ms")
     # the same batches from other numbers of lanes (more lanes: more throughput, longer latency)
     sweep = []
     for nl in [int(x) for x in args.lanes_sweep.split(",") if x.strip()]:
-        if nl < 1 or nl == len(lanes):
+        if nl < 1 or nl == n_lanes:
             continue
         while len(lanes) < nl:
             lanes.append(ctx.lane())
@@ -389,7 +390,8 @@ def main():
         "traffic": measured_traffic("k_pack_chain_crt", B),
         "algorithmic_per_launch": f"{MODMUL_CHAIN} modmul/query x {B} queries",
         "avg_launch_ms": round(chain_avg_s * 1e3, 4),
-        "measured": f"HIP events on each lane's stream over the timed region ({len(lanes)} lanes)",
+        "measured": f"HIP events on each lane's stream over the timed region ({n_lanes} lanes, "
+                    "sharing CUs with the other lanes' kernels)",
         "avg_launch_ms_alone": round(iso_chain_ms, 4),
     }
     roofline_hbm = {
@@ -422,7 +424,7 @@ def main():
             "workload": "InsPIRe^2 batched respond: 256 seeded queries + y_body keys per step, "
                         "d=2048, CRT [268369921, 249561089], 2^26 x 32 B DB resident in HBM",
             "batch": B,
-            "lanes": len(lanes),
+            "lanes": n_lanes,
             "db_entries": n_entries,
             "entry_bytes": 32,
             "ring_dim": D,
