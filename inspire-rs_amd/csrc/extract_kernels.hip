@@ -63,7 +63,7 @@ __global__ void __launch_bounds__(kThreads, (K << LOGN) <= 4096 ? 4 : 2) k_extra
     constexpr uint32_t TPL = SPAN / 64;         // terms of one value per lane
     constexpr uint32_t SPT = K * N / kThreads;  // key words staged per thread
     static_assert(N >= kThreads && TPL * 64 == SPAN, "shape");
-    __shared__ uint32_t ext[K * 2 * N];
+    extern __shared__ uint32_t ext[];  // [K][2N], dynamic: 64 KiB at d = 4096 with two moduli
     __shared__ uint32_t part[kWaves * V];
     __shared__ uint32_t res[V * K];
     const uint32_t qi = blockIdx.x;
@@ -205,12 +205,29 @@ __global__ void __launch_bounds__(kThreads, (K << LOGN) <= 4096 ? 4 : 2) k_extra
     }
 }
 
+// Raises a kernel's dynamic LDS limit once (a function-local static is
+// initialised exactly once, also under concurrent callers).
+template <int LOGN, int K, bool COLS>
+hipError_t extract_lds_limit(size_t shmem) {
+    static const hipError_t e =
+        shmem > 48 * 1024 ? hipFuncSetAttribute((const void*)k_extract<LOGN, K, COLS>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem)
+                          : hipSuccess;
+    return e;
+}
+
 template <int LOGN, int K>
 hipError_t launch_k(const ExtractArgs& A, const CtxDev& C, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
-    if (A.use_cols)
-        hipExtLaunchKernelGGL((k_extract<LOGN, K, true>), dim3(A.count), dim3(kThreads), 0, st, e0, e1, 0, A, C);
-    else
-        hipExtLaunchKernelGGL((k_extract<LOGN, K, false>), dim3(A.count), dim3(kThreads), 0, st, e0, e1, 0, A, C);
+    constexpr size_t shmem = (size_t)K * 2 * (1u << LOGN) * 4;
+    if (A.use_cols) {
+        const hipError_t e = extract_lds_limit<LOGN, K, true>(shmem);
+        if (e != hipSuccess) return e;
+        hipExtLaunchKernelGGL((k_extract<LOGN, K, true>), dim3(A.count), dim3(kThreads), shmem, st, e0, e1, 0, A, C);
+    } else {
+        const hipError_t e = extract_lds_limit<LOGN, K, false>(shmem);
+        if (e != hipSuccess) return e;
+        hipExtLaunchKernelGGL((k_extract<LOGN, K, false>), dim3(A.count), dim3(kThreads), shmem, st, e0, e1, 0, A, C);
+    }
     return hipGetLastError();
 }
 

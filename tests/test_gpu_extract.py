@@ -138,6 +138,29 @@ def test_extract_single_modulus(M):
             assert M.extract(ctx, r, cl.sk, es) == o.extract(r.ciphertext, r.column_ciphertexts, cl.sk, es), es
 
 
+@pytest.mark.parametrize("d,moduli", [(512, (O.Q0, O.Q1)), (1024, (O.Q1,)), (4096, (O.Q0, O.Q1))])
+def test_extract_other_degrees(M, d, moduli):
+    """Every ring degree and modulus count of the kernel (the key in LDS is
+    64 KiB at d = 4096 with two moduli): InsPIRe^0 responses through extract,
+    and their ciphertext through the packed modes."""
+    say("extract shape", d, moduli)
+    params = O.Params(d=d, moduli=moduli)
+    o = O.Oracle(params)
+    cl = Client(o, d + 1)
+    db = random_db(d, 32, seed=d)
+    shards = encode_db(o, db, 32)
+    with make_ctx(M, params) as ctx:
+        ctx.load_database(0, shards)
+        r = M.respond(ctx, 0, M.ClientQuery(shard_id=0, rgsw=cl.query_rows(d // 3)))
+        assert M.extract(ctx, r, cl.sk, 32) == o.extract(r.ciphertext, r.column_ciphertexts, cl.sk, 32)
+        for es in (32, 2 * d):
+            assert M.extract_inspiring(ctx, r, cl.sk, es) == o.extract_inspiring(r.ciphertext, cl.sk, es), es
+            assert M.extract_with_variant(ctx, r, cl.sk, es, M.ONE_PACKING) == \
+                o.extract_packed(r.ciphertext, cl.sk, es), es
+        if len(moduli) == 2:  # two moduli decrypt the entry back
+            assert M.extract(ctx, r, cl.sk, 32) == db[(d // 3) * 32:(d // 3 + 1) * 32]
+
+
 def test_extract_batch_device(M, inp):
     """Responses straight from inspire_respond_batch_device (InspiRING
     [count][2][K][d] and InsPIRe^0 [count][1 + cols][2][K][d]) into the
