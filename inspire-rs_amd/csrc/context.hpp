@@ -156,6 +156,9 @@ struct inspire_ctx {
     // scratch reused across calls
     inspire::DevBuf ws0, ws1, ws2, ws3, ws4, ws5, ws6;
     inspire::PinBuf pin;  // small per-batch metadata
+    // pinned staging of the host-buffer calls: inputs (rows, seeds, digit
+    // addresses), client keys, responses
+    inspire::PinBuf hin, hkeys, hout;
     std::vector<uint32_t> hbuf;
     inspire::Profiler prof;
 };

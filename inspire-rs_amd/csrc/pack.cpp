@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "context.hpp"
+#include "host_par.hpp"
 #include "kernels.hpp"
 #include "modarith.hpp"
 #include "pack.hpp"
@@ -391,6 +392,7 @@ void pack_respond_host(inspire_ctx* ctx, uint32_t gamma, const BatchItem* items,
     std::vector<uint64_t> y_off(Q);
     std::vector<uint32_t> y_mode(Q), y_count(Q);
     size_t y_words = 0;
+    bool all_body = true;
     for (uint32_t t = 0; t < Q; t++) {
         const inspire_query& q = qs[items[t].idx];
         y_off[t] = y_words;
@@ -407,29 +409,35 @@ void pack_respond_host(inspire_ctx* ctx, uint32_t gamma, const BatchItem* items,
             y_count[t] = nrot;
             y_words += (size_t)std::max(nrot, 1u) * ell * KN;
         }
+        all_body &= y_mode[t] == 0;
     }
-    std::vector<uint32_t> hy(y_words, 0);
-    for (uint32_t t = 0; t < Q; t++) {
-        const inspire_query& q = qs[items[t].idx];
-        uint32_t* dst = hy.data() + y_off[t];
-        if (q.y_all_len > 0) {
-            for (size_t w = 0; w < (size_t)y_count[t] * ell * KN; w++) dst[w] = (uint32_t)q.y_all[w];
-        } else if (y_mode[t] == 0) {
-            for (size_t w = 0; w < (size_t)ell * KN; w++) dst[w] = (uint32_t)q.y_body[w];
-        } else {
-            for (uint32_t i = 0; i < nrot; i++)
-                for (uint32_t d = 0; d < ell; d++) {
-                    std::vector<uint64_t> r = coeff_automorphism(ctx, q.y_body + (size_t)d * KN, T.gen_pows[i]);
-                    for (size_t w = 0; w < KN; w++) dst[((size_t)i * ell + d) * KN + w] = (uint32_t)r[w];
-                }
+    ctx->hkeys.reserve(std::max<size_t>(y_words, 1) * 4);
+    uint32_t* hy = ctx->hkeys.get<uint32_t>();
+    par_ranges(Q, 4, [&](size_t t0, size_t t1) {
+        for (size_t t = t0; t < t1; t++) {
+            const inspire_query& q = qs[items[t].idx];
+            uint32_t* dst = hy + y_off[t];
+            if (q.y_all_len > 0) {
+                narrow_u64(dst, q.y_all, (size_t)y_count[t] * ell * KN);
+            } else if (y_mode[t] == 0) {
+                narrow_u64(dst, q.y_body, (size_t)ell * KN);
+            } else {
+                for (uint32_t i = 0; i < nrot; i++)
+                    for (uint32_t d = 0; d < ell; d++) {
+                        std::vector<uint64_t> r = coeff_automorphism(ctx, q.y_body + (size_t)d * KN, T.gen_pows[i]);
+                        narrow_u64(dst + ((size_t)i * ell + d) * KN, r.data(), KN);
+                    }
+            }
         }
-    }
+    });
     ctx->ws0.reserve(std::max<size_t>(y_words, 1) * 4);
     uint32_t* d_y = ctx->ws0.get<uint32_t>();
-    hip_check(hipMemcpyAsync(d_y, hy.data(), y_words * 4, hipMemcpyHostToDevice, ctx->stream), "y");
-    uint64_t mont[kMaxK];
-    for (uint32_t k = 0; k < K; k++) mont[k] = (1ull << 32) % ctx->hm[k].q;
-    {
+    hip_check(hipMemcpyAsync(d_y, hy, y_words * 4, hipMemcpyHostToDevice, ctx->stream), "y");
+    // y_body alone: the chain kernel transforms it on the fly
+    const bool y_coeff = all_body && pack_chain_crt_supported(ctx->dev);
+    if (!y_coeff) {
+        uint64_t mont[kMaxK];
+        for (uint32_t k = 0; k < K; k++) mont[k] = (1ull << 32) % ctx->hm[k].q;
         ProfScope ps(ctx, P_Y_NTT);
         hip_check(launch_ntt(d_y, y_words / KN, ctx->dev, scale_fwd(ctx, mont), false, ctx->stream), "y ntt");
     }
@@ -439,23 +447,21 @@ void pack_respond_host(inspire_ctx* ctx, uint32_t gamma, const BatchItem* items,
     const size_t out_words = (size_t)Q * 2 * KN;
     DevBuf& outb = ctx->ws6;
     outb.reserve(out_words * 4);
-    pack_run(ctx, gamma, Q, rows, d_qdig, d_y, d_yoff, d_ymode, d_ycount, outb.get<uint32_t>(), false);
-    std::vector<uint32_t> hout(out_words);
+    pack_run(ctx, gamma, Q, rows, d_qdig, d_y, d_yoff, d_ymode, d_ycount, outb.get<uint32_t>(), y_coeff);
     uint32_t herr = 0;
-    hip_check(hipMemcpyAsync(hout.data(), outb.get(), out_words * 4, hipMemcpyDeviceToHost, ctx->stream), "out");
-    hip_check(hipMemcpyAsync(&herr, d_err, 4, hipMemcpyDeviceToHost, ctx->stream), "err");
-    hip_check(hipStreamSynchronize(ctx->stream), "pack respond");
-    for (uint32_t t = 0; t < Q; t++) {
-        const uint32_t i = items[t].idx;
-        if (herr) {
-            status[i] = INSPIRE_ERR_DEVICE;
-            set_last_error("seed expansion exhausted its draw budget");
-            continue;
-        }
-        inspire_response& r = rs[i];
-        for (size_t w = 0; w < 2 * KN; w++) r.ciphertext[w] = hout[(size_t)t * 2 * KN + w];
-        r.num_column_ciphertexts = 0;  // respond.rs:382-385
+    const uint32_t* hout = fetch_results(ctx, outb.get<uint32_t>(), out_words, d_err, &herr);
+    if (herr) {
+        set_last_error("seed expansion exhausted its draw budget");
+        for (uint32_t t = 0; t < Q; t++) status[items[t].idx] = INSPIRE_ERR_DEVICE;
+        return;
     }
+    par_ranges(Q, 2, [&](size_t t0, size_t t1) {
+        for (size_t t = t0; t < t1; t++) {
+            inspire_response& r = rs[items[t].idx];
+            widen_u32(r.ciphertext, hout + t * 2 * KN, 2 * KN);
+            r.num_column_ciphertexts = 0;  // respond.rs:382-385
+        }
+    });
 }
 
 }  // namespace inspire

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "context.hpp"
+#include "host_par.hpp"
 #include "kernels.hpp"
 #include "modarith.hpp"
 #include "pack.hpp"
@@ -332,26 +333,6 @@ void external_product_batch(inspire_ctx* ctx, uint64_t count, const uint64_t* rl
 // Device stages
 // ---------------------------------------------------------------------------
 
-// rows_ws: [Q][ell][2][K][N] staged coefficient rows (a-slots of seeded
-// queries are filled here from their seeds).  seeds: [Q][ell][8] (device).
-// seeded_runs: [first, end) ranges of seeded queries.  -> NTT, Montgomery.
-void stage_rows(inspire_ctx* ctx, uint32_t Q, uint32_t* rows_ws, const uint32_t* seeds,
-                const std::vector<std::pair<uint32_t, uint32_t>>& seeded_runs, uint32_t* d_err) {
-    const size_t KN = (size_t)ctx->K * ctx->N, ell = ctx->ell;
-    {
-        ProfScope ps(ctx, P_EXPAND);
-        for (auto& r : seeded_runs)
-            hip_check(launch_expand_seeds(seeds + (size_t)r.first * ell * 8, rows_ws + (size_t)r.first * ell * 2 * KN,
-                                          (size_t)(r.second - r.first) * ell, 2 * KN, d_err, ctx->dev, ctx->stream),
-                      "expand seeds");
-    }
-    uint64_t mont[kMaxK];
-    for (uint32_t k = 0; k < ctx->K; k++) mont[k] = mont_of(ctx, k);
-    ProfScope ps(ctx, P_ROWS_NTT);
-    hip_check(launch_ntt(rows_ws, (size_t)Q * ell * 2, ctx->dev, scale_fwd(ctx, mont), false, ctx->stream),
-              "rows ntt");
-}
-
 // Rows from the caller's buffers straight into the workspace: queries [0, S)
 // are seeded (a expanded from seeds [S][ell][8], b read from bsrc
 // [S][ell][K][N]), queries [S, Q) bring full rows fsrc [Q-S][ell][2][K][N].
@@ -413,7 +394,67 @@ void run_respond0(inspire_ctx* ctx, uint32_t cols, uint32_t Q, const uint32_t* r
 // Respond (host API)
 // ---------------------------------------------------------------------------
 
+const uint32_t* fetch_results(inspire_ctx* ctx, const uint32_t* d_out, size_t words, const uint32_t* d_err,
+                              uint32_t* err) {
+    const size_t err_at = (words * 4 + 255) & ~size_t(255);
+    ctx->hout.reserve(err_at + 4);
+    uint8_t* h = ctx->hout.get();
+    if (words) hip_check(hipMemcpyAsync(h, d_out, words * 4, hipMemcpyDeviceToHost, ctx->stream), "results");
+    hip_check(hipMemcpyAsync(h + err_at, d_err, 4, hipMemcpyDeviceToHost, ctx->stream), "error word");
+    hip_check(hipStreamSynchronize(ctx->stream), "respond");
+    std::memcpy(err, h + err_at, 4);
+    return reinterpret_cast<const uint32_t*>(h);
+}
+
 namespace {
+
+// Device inputs of one host-buffer group: transformed rows, the per-query
+// digit addresses and the error word of the seed expansion.
+struct HostRows {
+    const uint32_t* rows;
+    const uint64_t* qdig;
+    uint32_t* err;
+};
+
+// Stages the RGSW rows ell..2ell-1 of `items` (seeded ones first, [0, S))
+// from the caller's u64 buffers: narrowed into pinned memory on several
+// threads, one copy to the device, then seed expansion and transforms
+// (stage_rows_from).  Device layout, each part 256-byte aligned: digit
+// addresses [Q] u64, error word, seeds [S][ell][8], b rows [S][ell][K][N],
+// full rows [Q-S][ell][2][K][N].
+HostRows upload_rows(inspire_ctx* ctx, const inspire_query* qs, const std::vector<BatchItem>& items, uint32_t S) {
+    const size_t KN = (size_t)ctx->K * ctx->N, ell = ctx->ell;
+    const uint32_t Q = (uint32_t)items.size();
+    auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+    const size_t o_err = al((size_t)Q * 8), o_seeds = o_err + 256, o_b = al(o_seeds + (size_t)S * ell * 32),
+                 o_full = al(o_b + (size_t)S * ell * KN * 4), bytes = al(o_full + (size_t)(Q - S) * ell * 2 * KN * 4);
+    ctx->hin.reserve(bytes);
+    ctx->ws4.reserve(bytes);
+    uint8_t* h = ctx->hin.get();
+    uint8_t* d = ctx->ws4.get<uint8_t>();
+    uint64_t* hdig = reinterpret_cast<uint64_t*>(h);
+    *reinterpret_cast<uint32_t*>(h + o_err) = 0;
+    par_ranges(Q, 4, [&](size_t t0, size_t t1) {
+        for (size_t t = t0; t < t1; t++) {
+            const inspire_query& q = qs[items[t].idx];
+            hdig[t] = items[t].dig;
+            if (t < S) {  // seeded: rgsw [2 ell][K][N] b words, rgsw_seeds [2 ell][32]
+                narrow_u64(reinterpret_cast<uint32_t*>(h + o_b) + t * ell * KN, q.rgsw + ell * KN, ell * KN);
+                std::memcpy(h + o_seeds + t * ell * 32, q.rgsw_seeds + ell * 32, ell * 32);
+            } else {  // full: rgsw [2 ell][2][K][N]
+                narrow_u64(reinterpret_cast<uint32_t*>(h + o_full) + (t - S) * ell * 2 * KN, q.rgsw + ell * 2 * KN,
+                           ell * 2 * KN);
+            }
+        }
+    });
+    hip_check(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, ctx->stream), "rows upload");
+    ctx->ws1.reserve((size_t)Q * ell * 2 * KN * 4);
+    uint32_t* rows = ctx->ws1.get<uint32_t>();
+    uint32_t* d_err = reinterpret_cast<uint32_t*>(d + o_err);
+    stage_rows_from(ctx, Q, S, rows, reinterpret_cast<const uint32_t*>(d + o_seeds),
+                    reinterpret_cast<const uint32_t*>(d + o_b), reinterpret_cast<const uint32_t*>(d + o_full), d_err);
+    return HostRows{rows, reinterpret_cast<const uint64_t*>(d), d_err};
+}
 
 void fail(int32_t* status, uint32_t i, int code, const std::string& msg) {
     status[i] = code;
@@ -499,7 +540,7 @@ static const Database* admit(inspire_ctx* ctx, const inspire_query& q, uint32_t 
 
 void respond_batch(inspire_ctx* ctx, uint32_t n, const inspire_query* qs, const uint32_t* variants,
                    inspire_response* rs, int32_t* status, Route route) {
-    const size_t KN = (size_t)ctx->K * ctx->N, ell = ctx->ell;
+    const size_t KN = (size_t)ctx->K * ctx->N;
     std::map<uint32_t, std::vector<BatchItem>> groups;  // by column count: one launch spans databases
     for (uint32_t i = 0; i < n; i++) {
         status[i] = INSPIRE_OK;
@@ -521,86 +562,46 @@ void respond_batch(inspire_ctx* ctx, uint32_t n, const inspire_query* qs, const 
             for (const BatchItem& it : kv.second)
                 if (it.kind == pass) items.push_back(it);
             if (items.empty()) continue;
+            // seeded queries first: their rows are expanded and transformed in one launch
+            std::stable_partition(items.begin(), items.end(),
+                                  [&](const BatchItem& it) { return qs[it.idx].seeded != 0; });
             const uint32_t Q = (uint32_t)items.size();
-            // ---- stage rows ell..2ell-1 (the a-digits of the trivial encryption are 0)
-            ctx->ws1.reserve((size_t)Q * ell * 2 * KN * 4);
-            ctx->ws4.reserve((size_t)Q * (ell * 8 + 2) * 4 + 64);
-            uint32_t* rows = ctx->ws1.get<uint32_t>();
-            uint64_t* d_qdig = ctx->ws4.get<uint64_t>();
-            uint32_t* d_seeds = reinterpret_cast<uint32_t*>(d_qdig + Q);
-            uint32_t* d_err = d_seeds + (size_t)Q * ell * 8;
-            std::vector<uint32_t> hrows((size_t)Q * ell * 2 * KN, 0);
-            std::vector<uint32_t> hseeds((size_t)Q * ell * 8, 0);
-            std::vector<uint64_t> hdig(Q);
-            std::vector<std::pair<uint32_t, uint32_t>> runs;
-            for (uint32_t t = 0; t < Q; t++) {
-                const inspire_query& q = qs[items[t].idx];
-                hdig[t] = items[t].dig;
-                if (q.seeded) {
-                    if (!runs.empty() && runs.back().second == t) runs.back().second = t + 1;
-                    else runs.push_back({t, t + 1});
-                }
-                for (size_t i = 0; i < ell; i++) {
-                    const size_t row = ell + i;
-                    uint32_t* dst = hrows.data() + ((size_t)t * ell + i) * 2 * KN;
-                    if (q.seeded) {
-                        const uint64_t* b = q.rgsw + row * KN;
-                        for (size_t w = 0; w < KN; w++) dst[KN + w] = (uint32_t)b[w];
-                        std::memcpy(&hseeds[((size_t)t * ell + i) * 8], q.rgsw_seeds + 32 * row, 32);
+            uint32_t S = 0;
+            while (S < Q && qs[items[S].idx].seeded) S++;
+            const HostRows in = upload_rows(ctx, qs, items, S);
+            if (pass == KIND_INSPIRING) {
+                pack_respond_host(ctx, cols, items.data(), Q, in.rows, in.qdig, in.err, qs, rs, status);
+                continue;
+            }
+            const bool plain = pass == KIND_PLAIN;
+            const size_t per = plain ? (size_t)(1 + cols) * 2 * KN : tree_out_words(ctx, cols);
+            ctx->ws2.reserve(Q * per * 4);
+            uint32_t* out = ctx->ws2.get<uint32_t>();
+            if (plain) run_respond0(ctx, cols, Q, in.rows, in.qdig, out);
+            else tree_run(ctx, cols, Q, in.rows, in.qdig, out);
+            uint32_t herr = 0;
+            const uint32_t* hout = fetch_results(ctx, out, Q * per, in.err, &herr);
+            if (herr) {
+                for (uint32_t t = 0; t < Q; t++)
+                    fail(status, items[t].idx, INSPIRE_ERR_DEVICE, "seed expansion exhausted its draw budget");
+                continue;
+            }
+            const uint32_t b_dim = plain ? ctx->N : tree_b_dim(ctx, cols);
+            par_ranges(Q, 2, [&](size_t t0, size_t t1) {
+                for (size_t t = t0; t < t1; t++) {
+                    inspire_response& r = rs[items[t].idx];
+                    const uint32_t* src = hout + t * per;
+                    if (plain) {
+                        widen_u32(r.ciphertext, src, 2 * KN);
+                        r.num_column_ciphertexts = cols;
+                        if (r.column_ciphertexts) widen_u32(r.column_ciphertexts, src + 2 * KN, (size_t)cols * 2 * KN);
                     } else {
-                        const uint64_t* ab = q.rgsw + row * 2 * KN;
-                        for (size_t w = 0; w < 2 * KN; w++) dst[w] = (uint32_t)ab[w];
+                        widen_u32(r.ciphertext, src, per);
+                        r.ciphertext_b_dim = b_dim;
+                        r.num_column_ciphertexts = 0;  // respond.rs:262-267
                     }
                 }
-            }
-            hip_check(hipMemcpyAsync(rows, hrows.data(), hrows.size() * 4, hipMemcpyHostToDevice, ctx->stream), "rows");
-            hip_check(hipMemcpyAsync(d_qdig, hdig.data(), Q * 8, hipMemcpyHostToDevice, ctx->stream), "digits");
-            hip_check(hipMemcpyAsync(d_seeds, hseeds.data(), hseeds.size() * 4, hipMemcpyHostToDevice, ctx->stream),
-                      "seeds");
-            hip_check(hipMemsetAsync(d_err, 0, 4, ctx->stream), "err");
-            stage_rows(ctx, Q, rows, d_seeds, runs, d_err);
-            if (pass == 0) {
-                const size_t per = (size_t)(1 + cols) * 2 * KN;
-                ctx->ws2.reserve(Q * per * 4);
-                uint32_t* out = ctx->ws2.get<uint32_t>();
-                run_respond0(ctx, cols, Q, rows, d_qdig, out);
-                std::vector<uint32_t> hout(Q * per);
-                uint32_t herr = 0;
-                hip_check(hipMemcpyAsync(hout.data(), out, hout.size() * 4, hipMemcpyDeviceToHost, ctx->stream), "out");
-                hip_check(hipMemcpyAsync(&herr, d_err, 4, hipMemcpyDeviceToHost, ctx->stream), "err");
-                hip_check(hipStreamSynchronize(ctx->stream), "respond");
-                for (uint32_t t = 0; t < Q; t++) {
-                    const uint32_t i = items[t].idx;
-                    if (herr) { fail(status, i, INSPIRE_ERR_DEVICE, "seed expansion exhausted its draw budget"); continue; }
-                    inspire_response& r = rs[i];
-                    const uint32_t* src = hout.data() + t * per;
-                    for (size_t w = 0; w < 2 * KN; w++) r.ciphertext[w] = src[w];
-                    r.num_column_ciphertexts = cols;
-                    if (r.column_ciphertexts)
-                        for (size_t w = 0; w < (size_t)cols * 2 * KN; w++) r.column_ciphertexts[w] = src[2 * KN + w];
-                }
-            } else if (pass == KIND_INSPIRING) {
-                pack_respond_host(ctx, cols, items.data(), Q, rows, d_qdig, d_err, qs, rs, status);
-            } else {
-                const size_t ow = tree_out_words(ctx, cols);
-                ctx->ws2.reserve(Q * ow * 4);
-                uint32_t* out = ctx->ws2.get<uint32_t>();
-                tree_run(ctx, cols, Q, rows, d_qdig, out);
-                std::vector<uint32_t> hout(Q * ow);
-                uint32_t herr = 0;
-                hip_check(hipMemcpyAsync(hout.data(), out, hout.size() * 4, hipMemcpyDeviceToHost, ctx->stream), "out");
-                hip_check(hipMemcpyAsync(&herr, d_err, 4, hipMemcpyDeviceToHost, ctx->stream), "err");
-                hip_check(hipStreamSynchronize(ctx->stream), "respond");
-                for (uint32_t t = 0; t < Q; t++) {
-                    const uint32_t i = items[t].idx;
-                    if (herr) { fail(status, i, INSPIRE_ERR_DEVICE, "seed expansion exhausted its draw budget"); continue; }
-                    inspire_response& r = rs[i];
-                    const uint32_t* src = hout.data() + t * ow;
-                    for (size_t w = 0; w < ow; w++) r.ciphertext[w] = src[w];
-                    r.ciphertext_b_dim = tree_b_dim(ctx, cols);
-                    r.num_column_ciphertexts = 0;  // respond.rs:262-267
-                }
-            }
+            });
         }
     }
     prof_collect(ctx);

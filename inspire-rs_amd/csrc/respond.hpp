@@ -38,16 +38,16 @@ enum class Route {
 void respond_batch(inspire_ctx* ctx, uint32_t n, const inspire_query* qs, const uint32_t* variants,
                    inspire_response* rs, int32_t* status, Route route);
 
-// RGSW rows ell..2ell-1 staged as [Q][ell][2][K][N] coefficient words: the
-// a-slots of the seeded runs [first, end) are expanded from seeds
-// ([Q][ell][8] words), then every row is transformed (NTT, Montgomery form).
-void stage_rows(inspire_ctx* ctx, uint32_t Q, uint32_t* rows_ws, const uint32_t* seeds,
-                const std::vector<std::pair<uint32_t, uint32_t>>& seeded_runs, uint32_t* d_err);
-
-// The same from the caller's buffers: queries [0, S) seeded (b in bsrc
+// RGSW rows ell..2ell-1 into rows_ws [Q][ell][2][K][N] (NTT, Montgomery
+// form): queries [0, S) seeded (a expanded from seeds [S][ell][8], b in bsrc
 // [S][ell][K][N]), [S, Q) full rows fsrc [Q-S][ell][2][K][N].
 void stage_rows_from(inspire_ctx* ctx, uint32_t Q, uint32_t S, uint32_t* rows_ws, const uint32_t* seeds,
                      const uint32_t* bsrc, const uint32_t* fsrc, uint32_t* d_err);
+
+// Copies `words` device words and the 32-bit error word back into pinned
+// staging (ctx->hout), waits for the stream and returns the staged words.
+const uint32_t* fetch_results(inspire_ctx* ctx, const uint32_t* d_out, size_t words, const uint32_t* d_err,
+                              uint32_t* err);
 
 // Device address of the digits of the shard in `slot`.
 uint64_t shard_digits(const Database& db, uint32_t slot);

@@ -509,11 +509,26 @@ class Context:
         qa = (_Query * n)()
         ra = (_Response * n)()
         outs = []
+        # Two allocations per batch, left uninitialised: the library writes every
+        # word a response returns.  Ciphertexts have room for a tree-packed b of
+        # 2d coefficients; InsPIRe^0 needs its columns, the packed variants one
+        # column block at most (an empty shard's zero column).
+        K, d = self.K, self.d
+        caps = [max(self._dbs.get(db, 0), 1) if v == NO_PACKING else 1 for db, v in zip(db_ids, variants)]
+        ct_all = np.empty((n, 3 * K * d), dtype=np.uint64)
+        cc_all = np.empty((sum(caps), 2, K, d), dtype=np.uint64)
+        ct_base, cc_base = ct_all.ctypes.data, cc_all.ctypes.data
+        u64p = _u64p
+        c0 = 0
         for i, (db, q) in enumerate(zip(db_ids, queries)):
             qa[i] = self._query(db, q, keep)
-            r, ct, cc = self._response(self._dbs.get(db, 0), keep)
-            ra[i] = r
-            outs.append((ct, cc))
+            r = ra[i]
+            r.ciphertext = C.cast(ct_base + i * ct_all.strides[0], u64p)
+            r.column_ciphertexts = C.cast(cc_base + c0 * cc_all.strides[0], u64p)
+            r.column_capacity = caps[i]
+            r.ciphertext_capacity = ct_all.shape[1]
+            outs.append((ct_all[i], cc_all[c0:c0 + caps[i]]))
+            c0 += caps[i]
         va = (C.c_uint32 * n)(*variants)
         st = (C.c_int32 * n)()
         _raise(lib().inspire_respond_batch(self._h, n, qa, va, ra, st))
