@@ -45,6 +45,7 @@ struct PackArgs {
     uint32_t one_q1p;            // Shoup companion of 1 mod q1
     uint32_t zpowm[2][4];        // z^j 2^32 mod q_k (gadget base z), j < ell
     uint32_t zinvm[2];           // z^-(ell-1) 2^32 mod q_k
+    uint32_t zrecm[2][4];        // -z^(j-(ell-1)) 2^32 mod q_k, j < ell - 1
 };
 
 // Builds (once per gamma) the resident PackParams/OfflinePackingKeys tables.

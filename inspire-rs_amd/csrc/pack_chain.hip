@@ -58,7 +58,13 @@ struct ChainCrtShape {
 
 // LAZY: ELL * 4q * q < q * 2^32, so the multiply-accumulate takes the
 // transform outputs (< 4q) without reducing them first.
-template <int LOGN, int ELL, bool LAZY, bool REC>
+// SGN: the digit transforms run on two's-complement words with butterflies
+// that reduce nothing (ct_s), the recovered digit and the multiply-accumulate
+// are signed 64-bit sums with one signed Montgomery reduction each, and R, S
+// stay in [0, 2q).  The host enables it only when chain_signed_ok() shows
+// that every transform stage stays inside int32 and every reduced sum inside
+// (-2q, 2q).
+template <int LOGN, int ELL, bool LAZY, bool REC, bool SGN>
 __global__ void __launch_bounds__(2 * NttShape<LOGN>::T, 1)
 k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
     using L = Lay<LOGN>;
@@ -78,7 +84,7 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
     const uint32_t ymode = A.y_mode ? A.y_mode[qi] : 0u;
     const uint32_t ycount = A.y_count ? A.y_count[qi] : gamma - 1;
     const Mod M{C.m[k].q, C.m[k].q2, C.m[k].qinv};
-    const uint32_t q0 = C.m[0].q, q1 = C.m[1].q;
+    const uint32_t q0 = C.m[0].q, q1 = C.m[1].q, q4 = 2 * M.q2;
     uint32_t* db = dbuf + k * DB * LP;
     uint32_t* rb = db;  // the inverse transform's exchange: this residue's first digit buffer
     const uint32_t* tw = twf + k * N;
@@ -197,8 +203,17 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
             }
         }
         lds_barrier();  // the other half is past its reads of this half's exchange buffer
-        if constexpr (ND > 0) ntt_digits<LOGN, ND>(reinterpret_cast<uint32_t(&)[ND][8]>(dg), ts, db, tw, M);
-        if constexpr (REC) {
+        if constexpr (ND > 0) ntt_digits<LOGN, ND, SGN>(reinterpret_cast<uint32_t(&)[ND][8]>(dg), ts, db, tw, M);
+        if constexpr (REC && SGN) {
+            // NTT(D_last) = z^-(ell-1) R + sum_{j < ell-1} -z^(j-(ell-1)) NTT(D_j): one signed sum
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                int64_t sum = (int64_t)(int32_t)Rs[e] * (int32_t)A.zinvm[k];
+#pragma unroll
+                for (int j = 0; j < ND; j++) sum += (int64_t)(int32_t)dg[j][e] * (int32_t)A.zrecm[k][j];
+                dg[ELL - 1][e] = smont(sum, M);
+            }
+        } else if constexpr (REC) {
             // NTT(D_last) = z^-(ell-1) (R - sum_{j < ell-1} z^j NTT(D_j)): transform outputs are
             // below 4q, the Montgomery products below 2q, so the difference stays in (0, 7q)
 #pragma unroll
@@ -227,13 +242,6 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
         }
 #pragma unroll
         for (int e = 0; e < 8; e++) {
-            uint64_t ar = 0, ab = 0;
-#pragma unroll
-            for (int j = 0; j < ELL; j++) {
-                const uint32_t x = LAZY ? dg[j][e] : csub(csub(dg[j][e], M.q2), M.q);
-                ar += (uint64_t)x * w[j][e];
-                if (use_y) ab += (uint64_t)x * yv[j][e];
-            }
             uint32_t ri;
             if (k == 0) ri = rv[e];
             else if (A.residue_wise) ri = rv1[e];
@@ -242,8 +250,28 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
                 const uint32_t v1 = csub(shoup_mul(ref, 1u, A.one_q1p, q1), q1);
                 ri = csub(shoup_mul(v1, A.q1r64i, A.q1r64ip, q1), q1);
             }
-            R[e] = csub(ri + csub(mont_reduce(ar, M.q, M.qinv), M.q), M.q);
-            if (use_y) S[e] = csub(S[e] + csub(mont_reduce(ab, M.q, M.qinv), M.q), M.q);
+            if constexpr (SGN) {
+                int64_t ar = 0, ab = 0;
+#pragma unroll
+                for (int j = 0; j < ELL; j++) {
+                    const int64_t x = (int32_t)dg[j][e];
+                    ar += x * (int32_t)w[j][e];
+                    if (use_y) ab += x * (int32_t)yv[j][e];
+                }
+                // reduced sums in (-2q, 2q): + 2q, then [0, 8q) -> [0, 4q) -> [0, 2q)
+                R[e] = csub(csub(ri + smont(ar, M) + M.q2, q4), M.q2);
+                if (use_y) S[e] = csub(csub(S[e] + smont(ab, M) + M.q2, q4), M.q2);
+            } else {
+                uint64_t ar = 0, ab = 0;
+#pragma unroll
+                for (int j = 0; j < ELL; j++) {
+                    const uint32_t x = LAZY ? dg[j][e] : csub(csub(dg[j][e], M.q2), M.q);
+                    ar += (uint64_t)x * w[j][e];
+                    if (use_y) ab += (uint64_t)x * yv[j][e];
+                }
+                R[e] = csub(ri + csub(mont_reduce(ar, M.q, M.qinv), M.q), M.q);
+                if (use_y) S[e] = csub(S[e] + csub(mont_reduce(ab, M.q, M.qinv), M.q), M.q);
+            }
         }
     }
 
@@ -273,25 +301,52 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
 
 // Raises the kernel's dynamic LDS limit once per instantiation (a function-
 // local static is initialised exactly once, also under concurrent callers).
-template <int LOGN, int ELL, bool LAZY, bool REC>
+template <int LOGN, int ELL, bool LAZY, bool REC, bool SGN>
 static hipError_t chain_crt_lds_limit() {
     constexpr size_t shmem = ChainCrtShape<LOGN, ELL, REC>::kBytes;
     static const hipError_t e =
-        shmem > 64 * 1024 ? hipFuncSetAttribute((const void*)k_pack_chain_crt<LOGN, ELL, LAZY, REC>,
+        shmem > 64 * 1024 ? hipFuncSetAttribute((const void*)k_pack_chain_crt<LOGN, ELL, LAZY, REC, SGN>,
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem)
                           : hipSuccess;
     return e;
 }
 
-template <int LOGN, int ELL, bool LAZY, bool REC>
+template <int LOGN, int ELL, bool LAZY, bool REC, bool SGN = false>
 static hipError_t chain_crt_launch(const PackArgs& A, uint32_t Q, const CtxDev& C, const ScaleDev& inv,
                                    hipStream_t st) {
-    const hipError_t e = chain_crt_lds_limit<LOGN, ELL, LAZY, REC>();
+    const hipError_t e = chain_crt_lds_limit<LOGN, ELL, LAZY, REC, SGN>();
     if (e != hipSuccess) return e;
-    auto kern = k_pack_chain_crt<LOGN, ELL, LAZY, REC>;
+    auto kern = k_pack_chain_crt<LOGN, ELL, LAZY, REC, SGN>;
     constexpr size_t shmem = ChainCrtShape<LOGN, ELL, REC>::kBytes;
     hipLaunchKernelGGL(kern, dim3(Q), dim3(2 * NttShape<LOGN>::T), shmem, st, A, C, inv);
     return hipGetLastError();
+}
+
+// Bounds of the SGN variant, from the moduli and the digit size: the digit
+// transforms' LOGN stages stay inside int32 (a stage widens a bound B by the
+// signed Montgomery product's (B (q - 1) + 2^31 q) / 2^32), the recovered
+// digit's sum (R < 2q) and the multiply-accumulate's (w, y < q) reduce into
+// (-2q, 2q), and 8q fits 32 bits.
+static bool chain_signed_ok(const CtxDev& C, int logn, int ell, bool rec) {
+    using u128 = unsigned __int128;
+    const int nd = rec ? ell - 1 : ell;
+    for (int k = 0; k < 2; k++) {
+        const uint64_t q = C.m[k].q;
+        if (8 * q > 0xffffffffull) return false;
+        auto red = [&](u128 t) { return (uint64_t)((t + ((u128)q << 31)) >> 32); };  // |smont(t)| bound
+        uint64_t B = C.small_digits ? C.gadget_mask : q - 1;
+        for (int s = 0; s < logn; s++) {
+            B += red((u128)B * (q - 1));
+            if (B >= (1ull << 31)) return false;
+        }
+        uint64_t last = 0;
+        if (rec) {
+            last = red((u128)(2 * q) * (q - 1) + (u128)nd * B * (q - 1));
+            if (last >= 2 * q) return false;
+        }
+        if (red(((u128)nd * B + last) * (q - 1)) >= 2 * q) return false;
+    }
+    return true;
 }
 
 template <int LOGN, int ELL>
@@ -300,6 +355,10 @@ static hipError_t chain_crt_t(const PackArgs& A, uint32_t Q, const CtxDev& C, co
     // the recovered digit needs the composite value to fit ell digits: q <= z^ell
     const bool rec = (uint64_t)C.gadget_log * ELL >= 64 || C.q <= (1ull << (C.gadget_log * ELL));
     if ((uint64_t)ELL * 4 * qmax < (1ull << 32)) {
+        if (chain_signed_ok(C, LOGN, ELL, rec)) {
+            if (rec) return chain_crt_launch<LOGN, ELL, true, true, true>(A, Q, C, inv, st);
+            return chain_crt_launch<LOGN, ELL, true, false, true>(A, Q, C, inv, st);
+        }
         if (rec) return chain_crt_launch<LOGN, ELL, true, true>(A, Q, C, inv, st);
         return chain_crt_launch<LOGN, ELL, true, false>(A, Q, C, inv, st);
     }

@@ -63,6 +63,34 @@ __device__ __forceinline__ void ct_m(uint32_t& X, uint32_t& Y, uint32_t wm, cons
     Y = x - t + M.q2;
 }
 
+// Montgomery reduction with a signed quotient: t * 2^-32 mod q for a signed
+// 64-bit t, as a two's-complement word.  |result| <= (|t| + 2^31 q) / 2^32,
+// so |t| < 2^31 q gives a result in (-q, q).
+__device__ __forceinline__ uint32_t smont(int64_t t, const Mod& M) {
+    const int32_t m = (int32_t)((uint32_t)t * M.qinv);
+    return (uint32_t)((t + (int64_t)m * (int32_t)M.q) >> 32);
+}
+
+// a * b * 2^-32 mod q for a two's-complement word a and 0 <= b < q < 2^31.
+__device__ __forceinline__ uint32_t smmul(uint32_t a, uint32_t b, const Mod& M) {
+    return smont((int64_t)(int32_t)a * (int32_t)b, M);
+}
+
+// Signed CT butterfly: X, Y two's-complement words, X + wY and X - wY with no
+// reduction.  A stage widens the bound B of its inputs to B + (B (q - 1) +
+// 2^31 q) / 2^32; callers check on the host that every stage stays in int32.
+__device__ __forceinline__ void ct_s(uint32_t& X, uint32_t& Y, uint32_t wm, const Mod& M) {
+    const uint32_t t = smmul(Y, wm, M);
+    Y = X - t;
+    X = X + t;
+}
+
+template <bool SGN>
+__device__ __forceinline__ void ct_b(uint32_t& X, uint32_t& Y, uint32_t wm, const Mod& M) {
+    if constexpr (SGN) ct_s(X, Y, wm, M);
+    else ct_m(X, Y, wm, M);
+}
+
 // Harvey GS butterfly with the mirrored forward twiddle wm standing for -wm:
 // X, Y < 2q -> X + Y, (-w)(X - Y) = w(Y - X) (< 2q).
 __device__ __forceinline__ void gs_m(uint32_t& X, uint32_t& Y, uint32_t wm, const Mod& M) {
@@ -177,36 +205,38 @@ __device__ __forceinline__ void intt1(uint32_t (&x)[8], int t, uint32_t* rb, con
     }
 }
 
-template <int ELL>
+template <int ELL, bool SGN>
 __device__ __forceinline__ void fwd_r8(uint32_t (&x)[ELL][8], int m0, int i, const uint32_t* tw, const Mod& M) {
     {
         const uint32_t w = tw[m0 + i];
 #pragma unroll
         for (int j = 0; j < ELL; j++)
 #pragma unroll
-            for (int k = 0; k < 4; k++) ct_m(x[j][k], x[j][k + 4], w, M);
+            for (int k = 0; k < 4; k++) ct_b<SGN>(x[j][k], x[j][k + 4], w, M);
     }
 #pragma unroll
     for (int h = 0; h < 2; h++) {
         const uint32_t w = tw[2 * m0 + 2 * i + h];
 #pragma unroll
         for (int j = 0; j < ELL; j++) {
-            ct_m(x[j][4 * h + 0], x[j][4 * h + 2], w, M);
-            ct_m(x[j][4 * h + 1], x[j][4 * h + 3], w, M);
+            ct_b<SGN>(x[j][4 * h + 0], x[j][4 * h + 2], w, M);
+            ct_b<SGN>(x[j][4 * h + 1], x[j][4 * h + 3], w, M);
         }
     }
 #pragma unroll
     for (int h = 0; h < 4; h++) {
         const uint32_t w = tw[4 * m0 + 4 * i + h];
 #pragma unroll
-        for (int j = 0; j < ELL; j++) ct_m(x[j][2 * h], x[j][2 * h + 1], w, M);
+        for (int j = 0; j < ELL; j++) ct_b<SGN>(x[j][2 * h], x[j][2 * h + 1], w, M);
     }
 }
 
 // Forward NTT of ELL polys of one residue: x at the coefficient positions of
 // thread t -> x at its slot positions (< 4q).  db is [ELL][LP] LDS; no thread
-// may still be reading it.
-template <int LOGN, int ELL>
+// may still be reading it.  SGN: the inputs and outputs are two's-complement
+// words and the butterflies reduce nothing (ct_s): the caller guarantees that
+// LOGN stages keep the values inside int32.
+template <int LOGN, int ELL, bool SGN = false>
 __device__ __forceinline__ void ntt_digits(uint32_t (&x)[ELL][8], int t, uint32_t* db, const uint32_t* tw,
                                            const Mod& M) {
     using L = Lay<LOGN>;
@@ -223,7 +253,7 @@ __device__ __forceinline__ void ntt_digits(uint32_t (&x)[ELL][8], int t, uint32_
 #pragma unroll
                 for (int e = 0; e < 8; e++) x[j][e] = db[j * LP + lds_step(bix, e, st)];
         }
-        fwd_r8<ELL>(x, m0, i, tw, M);
+        fwd_r8<ELL, SGN>(x, m0, i, tw, M);
         if (!(L::REM == 0 && p == L::R8 - 1)) {
 #pragma unroll
             for (int j = 0; j < ELL; j++)
@@ -252,17 +282,17 @@ __device__ __forceinline__ void ntt_digits(uint32_t (&x)[ELL][8], int t, uint32_
                 const uint32_t w = tw[N / 4 + i], w0 = tw[N / 2 + 2 * i], w1 = tw[N / 2 + 2 * i + 1];
 #pragma unroll
                 for (int j = 0; j < ELL; j++) {
-                    ct_m(x[j][4 * g + 0], x[j][4 * g + 2], w, M);
-                    ct_m(x[j][4 * g + 1], x[j][4 * g + 3], w, M);
-                    ct_m(x[j][4 * g + 0], x[j][4 * g + 1], w0, M);
-                    ct_m(x[j][4 * g + 2], x[j][4 * g + 3], w1, M);
+                    ct_b<SGN>(x[j][4 * g + 0], x[j][4 * g + 2], w, M);
+                    ct_b<SGN>(x[j][4 * g + 1], x[j][4 * g + 3], w, M);
+                    ct_b<SGN>(x[j][4 * g + 0], x[j][4 * g + 1], w0, M);
+                    ct_b<SGN>(x[j][4 * g + 2], x[j][4 * g + 3], w1, M);
                 }
             } else {
                 const uint32_t w0 = tw[N / 2 + 2 * i], w1 = tw[N / 2 + 2 * i + 1];
 #pragma unroll
                 for (int j = 0; j < ELL; j++) {
-                    ct_m(x[j][4 * g + 0], x[j][4 * g + 1], w0, M);
-                    ct_m(x[j][4 * g + 2], x[j][4 * g + 3], w1, M);
+                    ct_b<SGN>(x[j][4 * g + 0], x[j][4 * g + 1], w0, M);
+                    ct_b<SGN>(x[j][4 * g + 2], x[j][4 * g + 3], w1, M);
                 }
             }
         }
