@@ -334,6 +334,8 @@ void pack_run(inspire_ctx* ctx, uint32_t gamma, uint32_t Q, const uint32_t* rows
             A.zrecm[k][j] = (uint32_t)(c ? q - c : 0);
             zc = mulmod64(zc, z, q);
         }
+        A.smul[k] = (uint32_t)r32;
+        A.rmul[k] = (uint32_t)(k == 1 && !rw ? powmod64(r32, q - 2, q) : r32);
     }
     const uint64_t q0 = ctx->hm[0].q;
     A.q0r64 = (uint32_t)pow2_64_mod(q0);

@@ -46,6 +46,9 @@ struct PackArgs {
     uint32_t zpowm[2][4];        // z^j 2^32 mod q_k (gadget base z), j < ell
     uint32_t zinvm[2];           // z^-(ell-1) 2^32 mod q_k
     uint32_t zrecm[2][4];        // -z^(j-(ell-1)) 2^32 mod q_k, j < ell - 1
+    // signed chain: R[i] enters its multiply-accumulate as r * rmul[k] (2^32 mod q_k,
+    // or 2^-32 mod q1 for the rebuilt residue 1, whose value carries 2^64), S as S * 2^32
+    uint32_t rmul[2], smul[2];
 };
 
 // Builds (once per gamma) the resident PackParams/OfflinePackingKeys tables.

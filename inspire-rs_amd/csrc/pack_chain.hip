@@ -60,10 +60,10 @@ struct ChainCrtShape {
 // transform outputs (< 4q) without reducing them first.
 // SGN: the digit transforms run on two's-complement words with butterflies
 // that reduce nothing (ct_s), the recovered digit and the multiply-accumulate
-// are signed 64-bit sums with one signed Montgomery reduction each, and R, S
-// stay in [0, 2q).  The host enables it only when chain_signed_ok() shows
-// that every transform stage stays inside int32 and every reduced sum inside
-// (-2q, 2q).
+// (into which R's and S's previous values are folded) are signed 64-bit sums
+// with one signed Montgomery reduction each, and R, S stay in [0, 2q).  The
+// host enables it only when chain_signed_ok() shows that every transform
+// stage stays inside int32, every reduced sum inside (-2q, 2q), and q0 < 2 q1.
 template <int LOGN, int ELL, bool LAZY, bool REC, bool SGN>
 __global__ void __launch_bounds__(2 * NttShape<LOGN>::T, 1)
 k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
@@ -84,7 +84,7 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
     const uint32_t ymode = A.y_mode ? A.y_mode[qi] : 0u;
     const uint32_t ycount = A.y_count ? A.y_count[qi] : gamma - 1;
     const Mod M{C.m[k].q, C.m[k].q2, C.m[k].qinv};
-    const uint32_t q0 = C.m[0].q, q1 = C.m[1].q, q4 = 2 * M.q2;
+    const uint32_t q0 = C.m[0].q, q1 = C.m[1].q;
     uint32_t* db = dbuf + k * DB * LP;
     uint32_t* rb = db;  // the inverse transform's exchange: this residue's first digit buffer
     const uint32_t* tw = twf + k * N;
@@ -192,7 +192,8 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
         for (int e = 0; e < 8; e++) {
             const uint32_t other = ro[lds_step(cix, e, T)];
             const uint32_t c0 = k == 0 ? R[e] : other, c1 = k == 0 ? other : R[e];
-            const uint32_t a0m = csub(shoup_mul(c0, 1u, C.one_q1p, q1), q1);
+            // c0 mod q1 (SGN also guarantees q0 < 2 q1)
+            const uint32_t a0m = SGN ? csub(c0, q1) : csub(shoup_mul(c0, 1u, C.one_q1p, q1), q1);
             const uint32_t u = csub(shoup_mul(c1 + q1 - a0m, C.q0_inv_q1, C.q0_inv_q1p, q1), q1);
             const uint64_t x = (uint64_t)c0 + (uint64_t)q0 * u;
 #pragma unroll
@@ -242,26 +243,33 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
         }
 #pragma unroll
         for (int e = 0; e < 8; e++) {
-            uint32_t ri;
-            if (k == 0) ri = rv[e];
-            else if (A.residue_wise) ri = rv1[e];
-            else {
-                const uint32_t ref = csub(shoup_mul(rv[e], A.q0r64, A.q0r64p, q0), q0);
-                const uint32_t v1 = csub(shoup_mul(ref, 1u, A.one_q1p, q1), q1);
-                ri = csub(shoup_mul(v1, A.q1r64i, A.q1r64ip, q1), q1);
-            }
             if constexpr (SGN) {
-                int64_t ar = 0, ab = 0;
+                // r_pre's value enters the sum as r * rmul: residue 1 of the reference's automorphism
+                // is residue 0's 2^64 Montgomery value mod q1, times 2^-64 (rmul = 2^-32 mod q1)
+                uint32_t ri;
+                if (k == 0) ri = rv[e];
+                else if (A.residue_wise) ri = rv1[e];
+                else ri = csub(csub(shoup_mul(rv[e], A.q0r64, A.q0r64p, q0), q0), q1);
+                int64_t ar = (int64_t)(int32_t)ri * (int32_t)A.rmul[k];
+                int64_t ab = (int64_t)(int32_t)S[e] * (int32_t)A.smul[k];
 #pragma unroll
                 for (int j = 0; j < ELL; j++) {
                     const int64_t x = (int32_t)dg[j][e];
                     ar += x * (int32_t)w[j][e];
                     if (use_y) ab += x * (int32_t)yv[j][e];
                 }
-                // reduced sums in (-2q, 2q): + 2q, then [0, 8q) -> [0, 4q) -> [0, 2q)
-                R[e] = csub(csub(ri + smont(ar, M) + M.q2, q4), M.q2);
-                if (use_y) S[e] = csub(csub(S[e] + smont(ab, M) + M.q2, q4), M.q2);
+                // reduced sums in (-2q, 2q): + 2q, one conditional subtraction -> [0, 2q)
+                R[e] = csub(smont(ar, M) + M.q2, M.q2);
+                if (use_y) S[e] = csub(smont(ab, M) + M.q2, M.q2);
             } else {
+                uint32_t ri;
+                if (k == 0) ri = rv[e];
+                else if (A.residue_wise) ri = rv1[e];
+                else {
+                    const uint32_t ref = csub(shoup_mul(rv[e], A.q0r64, A.q0r64p, q0), q0);
+                    const uint32_t v1 = csub(shoup_mul(ref, 1u, A.one_q1p, q1), q1);
+                    ri = csub(shoup_mul(v1, A.q1r64i, A.q1r64ip, q1), q1);
+                }
                 uint64_t ar = 0, ab = 0;
 #pragma unroll
                 for (int j = 0; j < ELL; j++) {
@@ -325,14 +333,16 @@ static hipError_t chain_crt_launch(const PackArgs& A, uint32_t Q, const CtxDev& 
 // Bounds of the SGN variant, from the moduli and the digit size: the digit
 // transforms' LOGN stages stay inside int32 (a stage widens a bound B by the
 // signed Montgomery product's (B (q - 1) + 2^31 q) / 2^32), the recovered
-// digit's sum (R < 2q) and the multiply-accumulate's (w, y < q) reduce into
-// (-2q, 2q), and 8q fits 32 bits.
+// digit's sum (R < 2q) and the multiply-accumulates' (w, y < q, plus the
+// folded r < q or S < 2q) reduce into (-2q, 2q), 4q fits 31 bits, and
+// q0 < 2 q1 (a residue mod q0 is reduced mod q1 by one subtraction).
 static bool chain_signed_ok(const CtxDev& C, int logn, int ell, bool rec) {
     using u128 = unsigned __int128;
     const int nd = rec ? ell - 1 : ell;
+    if (C.m[0].q >= 2ull * C.m[1].q) return false;
     for (int k = 0; k < 2; k++) {
         const uint64_t q = C.m[k].q;
-        if (8 * q > 0xffffffffull) return false;
+        if (4 * q >= (1ull << 31)) return false;
         auto red = [&](u128 t) { return (uint64_t)((t + ((u128)q << 31)) >> 32); };  // |smont(t)| bound
         uint64_t B = C.small_digits ? C.gadget_mask : q - 1;
         for (int s = 0; s < logn; s++) {
@@ -344,7 +354,7 @@ static bool chain_signed_ok(const CtxDev& C, int logn, int ell, bool rec) {
             last = red((u128)(2 * q) * (q - 1) + (u128)nd * B * (q - 1));
             if (last >= 2 * q) return false;
         }
-        if (red(((u128)nd * B + last) * (q - 1)) >= 2 * q) return false;
+        if (red(((u128)nd * B + last + 2 * q) * (q - 1)) >= 2 * q) return false;
     }
     return true;
 }
