@@ -330,31 +330,16 @@ static hipError_t chain_crt_launch(const PackArgs& A, uint32_t Q, const CtxDev& 
     return hipGetLastError();
 }
 
-// Bounds of the SGN variant, from the moduli and the digit size: the digit
-// transforms' LOGN stages stay inside int32 (a stage widens a bound B by the
-// signed Montgomery product's (B (q - 1) + 2^31 q) / 2^32), the recovered
-// digit's sum (R < 2q) and the multiply-accumulates' (w, y < q, plus the
-// folded r < q or S < 2q) reduce into (-2q, 2q), 4q fits 31 bits, and
+// The SGN variant's bounds (signed_digits_ok): R < 2q enters the recovered
+// digit, r < q and S < 2q are folded into the multiply-accumulates, and
 // q0 < 2 q1 (a residue mod q0 is reduced mod q1 by one subtraction).
 static bool chain_signed_ok(const CtxDev& C, int logn, int ell, bool rec) {
-    using u128 = unsigned __int128;
-    const int nd = rec ? ell - 1 : ell;
     if (C.m[0].q >= 2ull * C.m[1].q) return false;
     for (int k = 0; k < 2; k++) {
         const uint64_t q = C.m[k].q;
-        if (4 * q >= (1ull << 31)) return false;
-        auto red = [&](u128 t) { return (uint64_t)((t + ((u128)q << 31)) >> 32); };  // |smont(t)| bound
-        uint64_t B = C.small_digits ? C.gadget_mask : q - 1;
-        for (int s = 0; s < logn; s++) {
-            B += red((u128)B * (q - 1));
-            if (B >= (1ull << 31)) return false;
-        }
-        uint64_t last = 0;
-        if (rec) {
-            last = red((u128)(2 * q) * (q - 1) + (u128)nd * B * (q - 1));
-            if (last >= 2 * q) return false;
-        }
-        if (red(((u128)nd * B + last + 2 * q) * (q - 1)) >= 2 * q) return false;
+        if (!signed_digits_ok(q, C.small_digits ? C.gadget_mask : q - 1, logn, rec ? ell - 1 : ell, rec ? 2 * q : 0,
+                              2 * q))
+            return false;
     }
     return true;
 }

@@ -85,6 +85,30 @@ __device__ __forceinline__ void ct_s(uint32_t& X, uint32_t& Y, uint32_t wm, cons
     X = X + t;
 }
 
+// Host: bounds of a signed digit path for modulus q (< 2^29): gadget digits
+// below digit_max through logn ct_s stages, nd transformed digits; rec_in > 0:
+// a recovered digit is the signed Montgomery sum of a value below rec_in times
+// a constant below q and the nd transforms times constants below q; the
+// multiply-accumulate sums the transformed and recovered digits times operands
+// below q, plus terms below `extra` times constants below q.  True when every
+// stage stays inside int32 and every reduced sum inside (-2q, 2q).
+inline bool signed_digits_ok(uint64_t q, uint64_t digit_max, int logn, int nd, uint64_t rec_in, uint64_t extra) {
+    using u128 = unsigned __int128;
+    if (4 * q >= (1ull << 31)) return false;
+    auto red = [&](u128 t) { return (uint64_t)((t + ((u128)q << 31)) >> 32); };  // bound of |smont(t)|
+    uint64_t B = digit_max;
+    for (int s = 0; s < logn; s++) {
+        B += red((u128)B * (q - 1));
+        if (B >= (1ull << 31)) return false;
+    }
+    uint64_t last = 0;
+    if (rec_in) {
+        last = red((u128)rec_in * (q - 1) + (u128)nd * B * (q - 1));
+        if (last >= 2 * q) return false;
+    }
+    return red(((u128)nd * B + last + extra) * (q - 1)) < 2 * q;
+}
+
 template <bool SGN>
 __device__ __forceinline__ void ct_b(uint32_t& X, uint32_t& Y, uint32_t wm, const Mod& M) {
     if constexpr (SGN) ct_s(X, Y, wm, M);

@@ -109,7 +109,15 @@ void tree_run(inspire_ctx* ctx, uint32_t cols, uint32_t Q, const uint32_t* rows,
             A.zpowm[k][j] = (uint32_t)mulmod64(zj, r32, q);
             if (j + 1 < ctx->ell) zj = mulmod64(zj, z, q);
         }
-        A.zinvm[k] = (uint32_t)mulmod64(powmod64(zj, q - 2, q), r32, q);  // zj = z^(ell-1)
+        const uint64_t zinv = powmod64(zj, q - 2, q);  // zj = z^(ell-1)
+        A.zinvm[k] = (uint32_t)mulmod64(zinv, r32, q);
+        // the recovered digit as one sum: z^-(ell-1) R + sum_j -z^(j-(ell-1)) NTT(D_j)
+        uint64_t zc = zinv;
+        for (uint32_t j = 0; j + 1 < ctx->ell && j < 4; j++) {
+            const uint64_t c = mulmod64(zc, r32, q);
+            A.zrecm[k][j] = (uint32_t)(c ? q - c : 0);
+            zc = mulmod64(zc, z, q);
+        }
     }
     const size_t node_words = (size_t)Q * m * 3 * KN, lweb_words = (size_t)Q * m * K;
     ctx->ws3.reserve((node_words + lweb_words) * 4);

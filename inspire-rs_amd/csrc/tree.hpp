@@ -28,6 +28,7 @@ struct TreeArgs {
     uint32_t dmod[2], dmodp[2];    // N mod q_k (Shoup pairs)
     uint32_t zpowm[2][4];          // z^j 2^32 mod q_k (gadget base z), j < ell
     uint32_t zinvm[2];             // z^-(ell-1) 2^32 mod q_k
+    uint32_t zrecm[2][4];          // -z^(j-(ell-1)) 2^32 mod q_k, j < ell - 1
 };
 
 bool tree_supported(const CtxDev& C);

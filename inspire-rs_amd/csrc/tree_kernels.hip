@@ -117,7 +117,7 @@ __device__ __forceinline__ uint32_t aut_slot(uint32_t p, uint32_t t) {
 // h0/h1: s0.a^/s1.a^ and b0/b1: s0.b/s1.b at its slot positions (canonical).
 // On return a0, h0, b0 hold the new node (canonical).  key: [ell][2][K][N]
 // NTT-domain Montgomery rows.
-template <int LOGN, int K, int ELL, bool LAZY, bool REC>
+template <int LOGN, int K, int ELL, bool LAZY, bool REC, bool SGN>
 __device__ __forceinline__ void tree_level(uint32_t (&a0)[8], const uint32_t (&a1)[8], uint32_t (&h0)[8],
                                            const uint32_t (&h1)[8], uint32_t (&b0)[8], const uint32_t (&b1)[8],
                                            uint32_t level, const uint32_t* key, const TreeLds<LOGN, K, ELL, REC>& L,
@@ -197,14 +197,24 @@ __device__ __forceinline__ void tree_level(uint32_t (&a0)[8], const uint32_t (&a
         }
 
     if constexpr (ND > 0) {
-        ntt_digits<LOGN, ND>(reinterpret_cast<uint32_t(&)[ND][8]>(dg), t, L.dbuf + k * DB * LP, L.twf + k * N, M);
+        ntt_digits<LOGN, ND, SGN>(reinterpret_cast<uint32_t(&)[ND][8]>(dg), t, L.dbuf + k * DB * LP, L.twf + k * N,
+                                  M);
         // every thread is past its gathers from abuf before the INTT reuses it: the transform's
         // own barrier orders them when it has one
         if constexpr (!Ly::fwd_has_barrier) lds_barrier();
     } else {
         lds_barrier();
     }
-    if constexpr (REC) {
+    if constexpr (REC && SGN) {
+        // NTT(D_last) = z^-(ell-1) tau(s1.a^) + sum_{j < ell-1} -z^(j-(ell-1)) NTT(D_j): one signed sum
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            int64_t sum = (int64_t)(int32_t)ht[e] * (int32_t)TA.zinvm[k];
+#pragma unroll
+            for (int j = 0; j < ND; j++) sum += (int64_t)(int32_t)dg[j][e] * (int32_t)TA.zrecm[k][j];
+            dg[ELL - 1][e] = smont(sum, M);
+        }
+    } else if constexpr (REC) {
         // transform outputs < 4q and Montgomery products < 2q keep the difference in (0, 7q)
 #pragma unroll
         for (int e = 0; e < 8; e++) {
@@ -219,23 +229,37 @@ __device__ __forceinline__ void tree_level(uint32_t (&a0)[8], const uint32_t (&a
     uint32_t A[8];
 #pragma unroll
     for (int e = 0; e < 8; e++) {
-        uint64_t sa = 0, sb = 0;
+        if constexpr (SGN) {
+            int64_t sa = 0, sb = 0;
 #pragma unroll
-        for (int j = 0; j < ELL; j++) {
-            const uint32_t x = LAZY ? dg[j][e] : csub(csub(dg[j][e], M.q2), q);
-            sa += (uint64_t)x * ka[j][e];
-            sb += (uint64_t)x * kb[j][e];
+            for (int j = 0; j < ELL; j++) {
+                const int64_t x = (int32_t)dg[j][e];
+                sa += x * (int32_t)ka[j][e];
+                sb += x * (int32_t)kb[j][e];
+            }
+            // reduced sums in (-2q, 2q): + 2q and one conditional subtraction give [0, 2q)
+            A[e] = csub(smont(sa, M) + M.q2, M.q2);
+            h0[e] = addq(h0[e], csub(A[e], q), q);
+            b0[e] = addq(addq(b0[e], bt[e], q), csub(csub(smont(sb, M) + M.q2, M.q2), q), q);
+        } else {
+            uint64_t sa = 0, sb = 0;
+#pragma unroll
+            for (int j = 0; j < ELL; j++) {
+                const uint32_t x = LAZY ? dg[j][e] : csub(csub(dg[j][e], M.q2), q);
+                sa += (uint64_t)x * ka[j][e];
+                sb += (uint64_t)x * kb[j][e];
+            }
+            A[e] = mont_reduce(sa, q, M.qinv);  // < 2q, the inverse transform's input range
+            h0[e] = addq(h0[e], csub(A[e], q), q);
+            b0[e] = addq(addq(b0[e], bt[e], q), csub(mont_reduce(sb, q, M.qinv), q), q);
         }
-        A[e] = mont_reduce(sa, q, M.qinv);  // < 2q, the inverse transform's input range
-        h0[e] = addq(h0[e], csub(A[e], q), q);
-        b0[e] = addq(addq(b0[e], bt[e], q), csub(mont_reduce(sb, q, M.qinv), q), q);
     }
     intt1<LOGN>(A, t, ab, L.twf + k * N, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
 #pragma unroll
     for (int e = 0; e < 8; e++) a0[e] = addq(a0[e], csub(A[e], q), q);
 }
 
-template <int LOGN, int K, int ELL, bool LAZY, bool REC>
+template <int LOGN, int K, int ELL, bool LAZY, bool REC, bool SGN>
 __global__ void __launch_bounds__(K * NttShape<LOGN>::T, 1)
 k_tree_leaves(TreeArgs A, CtxDev C, ScaleDev inv) {
     using Ly = Lay<LOGN>;
@@ -312,7 +336,7 @@ k_tree_leaves(TreeArgs A, CtxDev C, ScaleDev inv) {
     if (t == 0) A.lweb[((size_t)qi * A.m + s) * K + k] = lwe_b;
 
     for (uint32_t lv = A.level_lo; lv <= A.level_hi; lv++)
-        tree_level<LOGN, K, ELL, LAZY, REC>(a, a, h, h, b, b, lv, A.ks + (size_t)(A.logn - lv) * ELL * 2 * KN, L, A,
+        tree_level<LOGN, K, ELL, LAZY, REC, SGN>(a, a, h, h, b, b, lv, A.ks + (size_t)(A.logn - lv) * ELL * 2 * KN, L, A,
                                             C, inv, k, t);
 
     uint32_t* nd = A.node + ((size_t)qi * A.m + s) * 3 * KN + (size_t)k * N;
@@ -330,7 +354,7 @@ k_tree_leaves(TreeArgs A, CtxDev C, ScaleDev inv) {
 // s + 2^(log d - level) of the level below (in place: outputs s <
 // 2^(log d - level) never hold an odd input).  At the root the packed
 // ciphertext is finished into out.
-template <int LOGN, int K, int ELL, bool LAZY, bool REC>
+template <int LOGN, int K, int ELL, bool LAZY, bool REC, bool SGN>
 __global__ void __launch_bounds__(K * NttShape<LOGN>::T, 1)
 k_tree_combine(TreeArgs A, CtxDev C, ScaleDev inv) {
     using Ly = Lay<LOGN>;
@@ -381,7 +405,7 @@ k_tree_combine(TreeArgs A, CtxDev C, ScaleDev inv) {
         }
     }
     stage_twiddles(L, C);  // the level's first barrier orders these stores
-    tree_level<LOGN, K, ELL, LAZY, REC>(a0, a1, h0, h1, b0, b1, level, A.ks + (size_t)(A.logn - level) * ELL * 2 * KN,
+    tree_level<LOGN, K, ELL, LAZY, REC, SGN>(a0, a1, h0, h1, b0, b1, level, A.ks + (size_t)(A.logn - level) * ELL * 2 * KN,
                                         L, A, C, inv, k, t);
 
     if (level < (uint32_t)LOGN) {
@@ -435,19 +459,19 @@ k_tree_combine(TreeArgs A, CtxDev C, ScaleDev inv) {
     }
 }
 
-template <int LOGN, int K, int ELL, bool LAZY, bool REC>
+template <int LOGN, int K, int ELL, bool LAZY, bool REC, bool SGN = false>
 static hipError_t tree_launch(const TreeArgs& A, bool leaves, uint32_t blocks, const CtxDev& C, const ScaleDev& inv,
                               hipStream_t st) {
     constexpr size_t shmem = TreeShape<LOGN, K, ELL, REC>::kBytes;
     static_assert(shmem <= 160 * 1024, "tree kernel LDS");
-    auto kern = leaves ? k_tree_leaves<LOGN, K, ELL, LAZY, REC> : k_tree_combine<LOGN, K, ELL, LAZY, REC>;
+    auto kern = leaves ? k_tree_leaves<LOGN, K, ELL, LAZY, REC, SGN> : k_tree_combine<LOGN, K, ELL, LAZY, REC, SGN>;
     // raise the dynamic LDS limit once per instantiation
     static const hipError_t e0 =
-        shmem > 64 * 1024 ? hipFuncSetAttribute((const void*)k_tree_leaves<LOGN, K, ELL, LAZY, REC>,
+        shmem > 64 * 1024 ? hipFuncSetAttribute((const void*)k_tree_leaves<LOGN, K, ELL, LAZY, REC, SGN>,
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem)
                           : hipSuccess;
     static const hipError_t e1 =
-        shmem > 64 * 1024 ? hipFuncSetAttribute((const void*)k_tree_combine<LOGN, K, ELL, LAZY, REC>,
+        shmem > 64 * 1024 ? hipFuncSetAttribute((const void*)k_tree_combine<LOGN, K, ELL, LAZY, REC, SGN>,
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem)
                           : hipSuccess;
     if (e0 != hipSuccess) return e0;
@@ -463,7 +487,17 @@ static hipError_t tree_lazy(const TreeArgs& A, bool leaves, uint32_t blocks, con
     for (int kk = 0; kk < K; kk++) qmax = C.m[kk].q > qmax ? C.m[kk].q : qmax;
     // the recovered digit needs the composite value to fit ell digits: q <= z^ell
     const bool rec = (uint64_t)C.gadget_log * ELL >= 64 || C.q <= (1ull << (C.gadget_log * ELL));
+    // signed digit path (ct_s transforms, signed Montgomery sums): tau(s1.a^) < q enters the
+    // recovered digit, nothing is folded into the multiply-accumulates
+    bool sgn = true;
+    for (int kk = 0; kk < K; kk++) {
+        const uint64_t q = C.m[kk].q;
+        sgn = sgn && signed_digits_ok(q, C.small_digits ? C.gadget_mask : q - 1, LOGN, rec ? ELL - 1 : ELL,
+                                      rec ? q : 0, 0);
+    }
     if ((uint64_t)ELL * 4 * qmax < (1ull << 32)) {
+        if (sgn && rec) return tree_launch<LOGN, K, ELL, true, true, true>(A, leaves, blocks, C, inv, st);
+        if (sgn) return tree_launch<LOGN, K, ELL, true, false, true>(A, leaves, blocks, C, inv, st);
         if (rec) return tree_launch<LOGN, K, ELL, true, true>(A, leaves, blocks, C, inv, st);
         return tree_launch<LOGN, K, ELL, true, false>(A, leaves, blocks, C, inv, st);
     }
