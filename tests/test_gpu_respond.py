@@ -139,6 +139,39 @@ def test_inspiring_matches_oracle_other_shapes(M, d, entry_size):
             assert (r.ciphertext == ct).all(), (d, entry_size, use_all)
 
 
+# Moduli above 2^28 take the packing chain's unsigned (Harvey) arithmetic: its
+# signed variant needs the digit transforms to stay inside int32, which holds
+# for q below about 2.8e8.  4 ell q < 2^32 then selects the lazy
+# multiply-accumulate (3.0e8) or the reduced one (4.5e8).
+WIDE_MODULI = [(300122113, 300244993), (450011137, 450109441)]
+
+
+@pytest.mark.parametrize("moduli", WIDE_MODULI)
+def test_inspiring_matches_oracle_wide_moduli(M, moduli):
+    say("inspiring moduli", moduli)
+    params = O.Params(d=2048, moduli=moduli)
+    o = O.Oracle(params)
+    db = random_db(2048, 32, seed=moduli[0] % 1000)
+    shards = encode_db(o, db, 32)
+    gamma = shards.shape[1]
+    cl = Client(o, 77)
+    yb, ya = cl.packing_keys(gamma, W_SEED)
+    rows = cl.query_rows(1234)
+    seeds, sb = cl.seeded_query(99)
+    with make_ctx(M, params) as ctx:
+        ctx.load_database(0, shards)
+        for use_all in (False, True):
+            q = M.ClientQuery(shard_id=0, rgsw=rows, has_packing_keys=True, y_body=yb,
+                              y_all=ya if use_all else None)
+            r = M.respond_inspiring(ctx, 0, q)
+            _, ct = o.respond_inspiring(W_SEED, shards[0], rows, y_body=yb, y_all=ya if use_all else None)
+            assert (r.ciphertext == ct).all(), (moduli, use_all)
+        q = M.ClientQuery(shard_id=0, rgsw=sb, seeds=seeds, has_packing_keys=True, y_body=yb)
+        r = M.respond_seeded_inspiring(ctx, 0, q)
+        _, ct = o.respond_inspiring(W_SEED, shards[0], o.seeded_rgsw_expand(seeds, sb), y_body=yb)
+        assert (r.ciphertext == ct).all(), (moduli, "seeded")
+
+
 def test_error_paths_match_reference(M, inp):
     say("error paths")
     crt = inp["crt"]

@@ -137,8 +137,12 @@ def test_tree_every_column_count(M, inp, entry_size):
         assert np.array_equal(r.a, a) and np.array_equal(r.b, b), entry_size
 
 
+# the last two: moduli above 2^28 take the unsigned (Harvey) arithmetic, lazy
+# (3.0e8) and reduced (4.5e8) multiply-accumulates
 @pytest.mark.parametrize("d,moduli,entry_size", [(256, (O.Q0, O.Q1), 32), (1024, (O.Q0, O.Q1), 12),
-                                                 (256, (O.Q0,), 32), (2048, (O.Q1,), 8)])
+                                                 (256, (O.Q0,), 32), (2048, (O.Q1,), 8),
+                                                 (2048, (300122113, 300244993), 32),
+                                                 (2048, (450011137, 450109441), 32)])
 def test_tree_other_degrees_and_one_modulus(M, d, moduli, entry_size):
     say("tree shape", d, moduli, entry_size)
     params = O.Params(d=d, moduli=moduli)
