@@ -77,7 +77,7 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
     uint32_t* dbuf = yl + ELL * 2 * N;   // [2][DB][LP]
 
     const int tid = threadIdx.x;
-    const int k = tid / T, t = tid - k * T;  // residue of this half, thread within it
+    const int k = residue_of<T>(tid), t = tid - k * T;  // residue of this half, thread within it
     const uint32_t qi = blockIdx.x, gamma = A.gamma;
     const uint32_t* rpre = A.r_pre + (size_t)qi * gamma * KN;
     const uint32_t* Y = A.y + (A.y_off ? A.y_off[qi] : (uint64_t)qi * ELL * KN);

@@ -44,6 +44,17 @@ struct Mod {
     uint32_t q, q2, qinv;
 };
 
+// Residue handled by thread tid when each residue has T threads.  With T a
+// multiple of the wave size every wave works on one residue, and the index is
+// made wave-uniform so that per-residue constants are scalar loads and scalar
+// registers rather than per-lane ones.
+template <int T>
+__device__ __forceinline__ int residue_of(int tid) {
+    const int k = tid / T;
+    if constexpr (T % 64 == 0) return __builtin_amdgcn_readfirstlane(k);
+    return k;
+}
+
 // Padded LDS index of element base + e*st, from the padded index of base.
 // For the radix-8 passes (st a power of two, base = an aligned block of 8*st
 // elements plus r < st) the padding of the eight elements differs from that

@@ -267,7 +267,7 @@ k_tree_leaves(TreeArgs A, CtxDev C, ScaleDev inv) {
     constexpr size_t KN = (size_t)K * N;
     extern __shared__ __align__(16) uint32_t lds[];
     const TreeLds<LOGN, K, ELL, REC> L(lds);
-    const int tid = threadIdx.x, k = tid / T, t = tid - k * T;
+    const int tid = threadIdx.x, k = residue_of<T>(tid), t = tid - k * T;
     const uint32_t qi = blockIdx.x / A.m, s = blockIdx.x - qi * A.m;
     const Mod M{C.m[k].q, C.m[k].q2, C.m[k].qinv};
     const uint32_t q = M.q;
@@ -362,7 +362,7 @@ k_tree_combine(TreeArgs A, CtxDev C, ScaleDev inv) {
     constexpr size_t KN = (size_t)K * N;
     extern __shared__ __align__(16) uint32_t lds[];
     const TreeLds<LOGN, K, ELL, REC> L(lds);
-    const int tid = threadIdx.x, k = tid / T, t = tid - k * T;
+    const int tid = threadIdx.x, k = residue_of<T>(tid), t = tid - k * T;
     const uint32_t level = A.level_lo;
     const uint32_t half = 1u << (LOGN - level);  // nodes at this level: s < min(m, half)
     const uint32_t nn = A.m < half ? A.m : half;
