@@ -36,6 +36,7 @@
 
 #include "abi.hpp"
 #include "context.hpp"
+#include "host_par.hpp"
 #include "kernels.hpp"
 #include "modarith.hpp"
 #include "pack.hpp"
@@ -65,7 +66,7 @@ struct SReq {
     uint64_t arrival = 0, done_ns = 0;
     int32_t status = INSPIRE_OK;
     std::string err;
-    std::vector<uint32_t> out;    // packed: [2][K][N]; InsPIRe^0: [1 + cols][2][K][N]
+    std::unique_ptr<uint32_t[]> out;  // packed: [2][K][N]; InsPIRe^0: [1 + cols][2][K][N]
     uint32_t cols = 0;            // column count of the database (gamma)
     uint64_t dig = 0;             // device address of the shard's digits
     uint64_t yoff = 0;            // word offset of the client's keys in the pool
@@ -173,22 +174,24 @@ void run_batch(inspire_server* s, Lane& L, std::vector<std::shared_ptr<SReq>>& b
     L.rows.reserve(rows_words * 4);
     L.out.reserve(out_words * 4);
 
-    // ---- staging (host) and one upload
+    // ---- staging (host, on several threads) and one upload
     uint8_t* pin = L.pin_in.get();
     for (auto& kv : gm) {
         Group& g = kv.second;
         const size_t Q = g.reqs.size();
-        for (size_t t = 0; t < Q; t++) {
-            const SReq* r = g.reqs[t];
-            reinterpret_cast<uint64_t*>(pin + g.o_dig)[t] = r->dig;
-            reinterpret_cast<uint64_t*>(pin + g.o_yoff)[t] = r->yoff;
-            if (t < g.S) {
-                std::memcpy(pin + g.o_seeds + t * ell * 32, r->seeds_ptr(), ell * 32);
-                std::memcpy(pin + g.o_b + t * ell * KN * 4, r->rows_ptr(), ell * KN * 4);
-            } else {
-                std::memcpy(pin + g.o_full + (t - g.S) * ell * 2 * KN * 4, r->rows_ptr(), ell * 2 * KN * 4);
+        par_ranges(Q, 8, [&](size_t t0, size_t t1) {
+            for (size_t t = t0; t < t1; t++) {
+                const SReq* r = g.reqs[t];
+                reinterpret_cast<uint64_t*>(pin + g.o_dig)[t] = r->dig;
+                reinterpret_cast<uint64_t*>(pin + g.o_yoff)[t] = r->yoff;
+                if (t < g.S) {
+                    std::memcpy(pin + g.o_seeds + t * ell * 32, r->seeds_ptr(), ell * 32);
+                    std::memcpy(pin + g.o_b + t * ell * KN * 4, r->rows_ptr(), ell * KN * 4);
+                } else {
+                    std::memcpy(pin + g.o_full + (t - g.S) * ell * 2 * KN * 4, r->rows_ptr(), ell * 2 * KN * 4);
+                }
             }
-        }
+        });
         *reinterpret_cast<uint32_t*>(pin + g.o_err) = 0;
     }
     hipStream_t st = ctx->stream;
@@ -227,17 +230,19 @@ void run_batch(inspire_server* s, Lane& L, std::vector<std::shared_ptr<SReq>>& b
     for (auto& kv : gm) {
         Group& g = kv.second;
         const bool err = pout[out_words + ei++] != 0;
-        for (size_t t = 0; t < g.reqs.size(); t++) {
-            SReq* r = g.reqs[t];
-            if (err) {
+        if (err) {
+            for (SReq* r : g.reqs) {
                 r->status = INSPIRE_ERR_DEVICE;
                 r->err = "seed expansion exhausted its draw budget";
-            } else {
-                const uint32_t* src = pout + g.o_out + t * g.out_words_per;
-                r->out.assign(src, src + g.out_words_per);
             }
-            r->done_ns = t_done;
+            continue;
         }
+        // each response out of the lane's pinned buffer (allocated here, copied on several threads)
+        for (SReq* r : g.reqs) r->out.reset(new uint32_t[g.out_words_per]);
+        par_ranges(g.reqs.size(), 8, [&](size_t t0, size_t t1) {
+            for (size_t t = t0; t < t1; t++)
+                std::memcpy(g.reqs[t]->out.get(), pout + g.o_out + t * g.out_words_per, g.out_words_per * 4);
+        });
     }
 }
 
