@@ -195,7 +195,7 @@ def main():
                     help="measure the serving loop instead: open-loop Poisson arrivals over 64 databases")
     ap.add_argument("--serve-log2", type=int, default=28, help="total entries of the serving databases (log2)")
     ap.add_argument("--serve-dbs", type=int, default=64)
-    ap.add_argument("--serve-rates", type=str, default="100,300,1000,3000,10000,30000,100000")
+    ap.add_argument("--serve-rates", type=str, default="100,300,1000,3000,10000,30000,100000,200000,300000")
     ap.add_argument("--serve-lanes", type=int, default=2)
     ap.add_argument("--serve-max-batch", type=int, default=256)
     ap.add_argument("--serve-pool", type=int, default=256, help="distinct queries (with CPU answers) drawn from")
