@@ -1,24 +1,25 @@
 // Packing chain for two CRT moduli: packing_offline step 2 and the online sum
 // (src/inspiring/inspiring2.rs:926-1041, :1185-1222), one query per
-// workgroup of 2 x N/8 threads; the first half works on residue 0, the second
-// on residue 1.
+// workgroup of 2 x N/V threads (V values each, kChainV below); the first half
+// works on residue 0, the second on residue 1.
 //
 // Per step i = gamma-2 .. 0, in the NTT domain:
 //   T[i] = G^-1(INTT(R[i+1]));  R[i] = tau_{g^i}(r_pre[i]) + sum_j NTT(T[i][j]) * w_all[i][j]
 //   S   += sum_j NTT(T[i][j]) * y_all[i][j]          (online accumulator)
 // then a_hat = INTT(R[0]) and b = b_poly + INTT(S).
 //
-// Thread layout.  The radix-8 NTT passes leave every thread with eight fixed
-// positions.  The inverse transform's last pass and the forward transform's
-// first pass touch the same positions {t + k N/8} (the "coefficient
-// positions"), and the forward transform's last pass and the inverse
-// transform's first pass touch the same positions (the "slot positions").
-// So, per step, the CRT composition and the gadget digits of T[i] are formed
-// in registers between the two transforms (after one LDS exchange of the
-// coefficients between the halves), and the multiply-accumulate that produces
-// R[i] runs in registers on the slots the next inverse transform starts from.
-// R, the online accumulator S and the NTT of the client's y_body (rotated by
-// the automorphism table on the fly, staged in LDS) never leave the chip.
+// Thread layout.  The register-resident NTT passes (radix 4 or 8) leave every
+// thread with V fixed positions.  The inverse transform's last pass and the
+// forward transform's first pass touch the same positions {t + k N/V} (the
+// "coefficient positions"), and the forward transform's last pass and the
+// inverse transform's first pass touch the same positions (the "slot
+// positions", V t .. V t + V - 1).  So, per step, the CRT composition and the
+// gadget digits of T[i] are formed in registers between the two transforms
+// (after one LDS exchange of the coefficients between the halves), and the
+// multiply-accumulate that produces R[i] runs in registers on the slots the
+// next inverse transform starts from.  R, the online accumulator S and the
+// NTT of the client's y_body (rotated by the automorphism table on the fly,
+// staged in LDS) never leave the chip.
 //
 // Twiddles are kept in LDS in Montgomery form (w 2^32 mod q, one word each),
 // the forward table only (the inverse transforms read it mirrored,
@@ -64,11 +65,11 @@ struct ChainCrtShape {
 // with one signed Montgomery reduction each, and R, S stay in [0, 2q).  The
 // host enables it only when chain_signed_ok() shows that every transform
 // stage stays inside int32, every reduced sum inside (-2q, 2q), and q0 < 2 q1.
-template <int LOGN, int ELL, bool LAZY, bool REC, bool SGN>
-__global__ void __launch_bounds__(2 * NttShape<LOGN>::T, 1)
+template <int LOGN, int ELL, bool LAZY, bool REC, bool SGN, int V>
+__global__ void __launch_bounds__(2 * ((1 << LOGN) / V), 1)
 k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
     using L = Lay<LOGN>;
-    constexpr int N = L::N, T = L::T, LP = L::LP;
+    constexpr int N = L::N, T = N / V, LP = L::LP;  // T threads per residue, V values each
     constexpr int ND = ChainDigits<ELL, REC>::ND, DB = ChainDigits<ELL, REC>::DB;
     constexpr size_t KN = 2 * (size_t)N;
     extern __shared__ __align__(16) uint32_t lds[];
@@ -101,23 +102,23 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
         // DB polys at a time through the DB exchange buffers
 #pragma unroll
         for (int j0 = 0; j0 < ELL; j0 += DB) {
-            uint32_t yv[DB][8];
+            uint32_t yv[DB][V];
 #pragma unroll
             for (int j = 0; j < DB; j++)
 #pragma unroll
-                for (int e = 0; e < 8; e++) yv[j][e] = j0 + j < ELL ? Y[((j0 + j) * 2 + k) * N + t + e * T] : 0u;
+                for (int e = 0; e < V; e++) yv[j][e] = j0 + j < ELL ? Y[((j0 + j) * 2 + k) * N + t + e * T] : 0u;
             if (j0 > 0) lds_barrier();  // every thread is past the previous transform's reads
-            ntt_digits<LOGN, DB>(yv, t, db, tw, M);
+            ntt_v<LOGN, V, DB>(yv, t, db, tw, M);
 #pragma unroll
             for (int j = 0; j < DB; j++) {
                 if (j0 + j >= ELL) break;
 #pragma unroll
-                for (int g = 0; g < 2; g++) {
+                for (int g = 0; g < V / 4; g++) {
                     uint32_t v[4];
 #pragma unroll
                     for (int c = 0; c < 4; c++)
                         v[c] = csub(shoup_mul(yv[j][4 * g + c], C.mont[k], C.montp[k], M.q), M.q);
-                    *reinterpret_cast<uint4*>(&yl[((j0 + j) * 2 + k) * N + L::group(t, g)]) =
+                    *reinterpret_cast<uint4*>(&yl[((j0 + j) * 2 + k) * N + V * t + 4 * g]) =
                         make_uint4(v[0], v[1], v[2], v[3]);
                 }
             }
@@ -140,14 +141,14 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
         return csub(shoup_mul(v1, A.q1r64i, A.q1r64ip, q1), q1);
     };
 
-    uint32_t R[8], S[8], pm[8], pmn[8];
-    load_slots<LOGN>(A.perm + (size_t)(gamma - 1) * N, t, pm);
+    uint32_t R[V], S[V], pm[V], pmn[V];
+    load_slots_v<V>(A.perm + (size_t)(gamma - 1) * N, t, pm);
 #pragma unroll
-    for (int e = 0; e < 8; e++) {
+    for (int e = 0; e < V; e++) {
         R[e] = r_init(rpre + (size_t)(gamma - 1) * KN, pm[e]);
         S[e] = 0;
     }
-    if (gamma >= 2) load_slots<LOGN>(A.perm + (size_t)(gamma - 2) * N, t, pmn);
+    if (gamma >= 2) load_slots_v<V>(A.perm + (size_t)(gamma - 2) * N, t, pmn);
     __syncthreads();
 
     for (int i = (int)gamma - 2; i >= 0; i--) {
@@ -156,40 +157,40 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
         int ts = t;
         asm volatile("" : "+v"(ts));
 #pragma unroll
-        for (int e = 0; e < 8; e++) pm[e] = pmn[e];
-        if (i > 0) load_slots<LOGN>(A.perm + (size_t)(i - 1) * N, ts, pmn);
+        for (int e = 0; e < V; e++) pm[e] = pmn[e];
+        if (i > 0) load_slots_v<V>(A.perm + (size_t)(i - 1) * N, ts, pmn);
         // operands of this step's multiply-accumulate, loaded ahead
         const uint32_t* Ri = rpre + (size_t)i * KN;
-        uint32_t rv[8], w[ELL][8];
+        uint32_t rv[V], w[ELL][V];
 #pragma unroll
-        for (int e = 0; e < 8; e++) rv[e] = Ri[pm[e]];
-        uint32_t rv1[8];
+        for (int e = 0; e < V; e++) rv[e] = Ri[pm[e]];
+        uint32_t rv1[V];
         if (k == 1 && A.residue_wise) {
 #pragma unroll
-            for (int e = 0; e < 8; e++) rv1[e] = Ri[N + pm[e]];
+            for (int e = 0; e < V; e++) rv1[e] = Ri[N + pm[e]];
         }
         const uint32_t* Wi = A.w_all + (size_t)i * ELL * KN + (size_t)k * N;
 #pragma unroll
-        for (int j = 0; j < ELL; j++) load_slots<LOGN>(Wi + (size_t)j * KN, ts, w[j]);
+        for (int j = 0; j < ELL; j++) load_slots_v<V>(Wi + (size_t)j * KN, ts, w[j]);
 
         // T[i] = G^-1(INTT(R[i+1])): this residue's coefficients, then the other's via LDS
-        uint32_t Rs[8];  // R[i+1] itself, for the recovered digit
+        uint32_t Rs[V];  // R[i+1] itself, for the recovered digit
         if constexpr (REC) {
 #pragma unroll
-            for (int e = 0; e < 8; e++) Rs[e] = R[e];
+            for (int e = 0; e < V; e++) Rs[e] = R[e];
         }
-        intt1<LOGN>(R, ts, rb, tw, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
+        intt_v<LOGN, V>(R, ts, rb, tw, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
         const int cix = lds_ix(ts);
 #pragma unroll
-        for (int e = 0; e < 8; e++) {
+        for (int e = 0; e < V; e++) {
             R[e] = csub(R[e], M.q);
             rb[lds_step(cix, e, T)] = R[e];
         }
         lds_barrier();
         const uint32_t* ro = dbuf + (1 - k) * DB * LP;  // the other residue's exchange buffer
-        uint32_t dg[ELL][8];
+        uint32_t dg[ELL][V];
 #pragma unroll
-        for (int e = 0; e < 8; e++) {
+        for (int e = 0; e < V; e++) {
             const uint32_t other = ro[lds_step(cix, e, T)];
             const uint32_t c0 = k == 0 ? R[e] : other, c1 = k == 0 ? other : R[e];
             // c0 mod q1 (SGN also guarantees q0 < 2 q1)
@@ -204,11 +205,11 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
             }
         }
         lds_barrier();  // the other half is past its reads of this half's exchange buffer
-        if constexpr (ND > 0) ntt_digits<LOGN, ND, SGN>(reinterpret_cast<uint32_t(&)[ND][8]>(dg), ts, db, tw, M);
+        if constexpr (ND > 0) ntt_v<LOGN, V, ND, SGN>(reinterpret_cast<uint32_t(&)[ND][V]>(dg), ts, db, tw, M);
         if constexpr (REC && SGN) {
             // NTT(D_last) = z^-(ell-1) R + sum_{j < ell-1} -z^(j-(ell-1)) NTT(D_j): one signed sum
 #pragma unroll
-            for (int e = 0; e < 8; e++) {
+            for (int e = 0; e < V; e++) {
                 int64_t sum = (int64_t)(int32_t)Rs[e] * (int32_t)A.zinvm[k];
 #pragma unroll
                 for (int j = 0; j < ND; j++) sum += (int64_t)(int32_t)dg[j][e] * (int32_t)A.zrecm[k][j];
@@ -218,7 +219,7 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
             // NTT(D_last) = z^-(ell-1) (R - sum_{j < ell-1} z^j NTT(D_j)): transform outputs are
             // below 4q, the Montgomery products below 2q, so the difference stays in (0, 7q)
 #pragma unroll
-            for (int e = 0; e < 8; e++) {
+            for (int e = 0; e < V; e++) {
                 uint32_t d = Rs[e];
                 if constexpr (ND >= 1) d += 4 * M.q - dg[0][e];
 #pragma unroll
@@ -228,21 +229,21 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
         }
 
         const bool use_y = (uint32_t)i < ycount;
-        uint32_t yv[ELL][8];
+        uint32_t yv[ELL][V];
         if (use_y) {
             if (ymode == 0) {
 #pragma unroll
                 for (int j = 0; j < ELL; j++)
 #pragma unroll
-                    for (int e = 0; e < 8; e++) yv[j][e] = yl[(j * 2 + k) * N + pm[e]];
+                    for (int e = 0; e < V; e++) yv[j][e] = yl[(j * 2 + k) * N + pm[e]];
             } else {
 #pragma unroll
                 for (int j = 0; j < ELL; j++)
-                    load_slots<LOGN>(Y + (((size_t)i * ELL + j) * 2 + k) * N, ts, yv[j]);
+                    load_slots_v<V>(Y + (((size_t)i * ELL + j) * 2 + k) * N, ts, yv[j]);
             }
         }
 #pragma unroll
-        for (int e = 0; e < 8; e++) {
+        for (int e = 0; e < V; e++) {
             if constexpr (SGN) {
                 // r_pre's value enters the sum as r * rmul: residue 1 of the reference's automorphism
                 // is residue 0's 2^64 Montgomery value mod q1, times 2^-64 (rmul = 2^-32 mod q1)
@@ -285,13 +286,13 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
 
     // a_hat = INTT(R[0]); b = b_poly + INTT(S)
     uint32_t* out = A.out + (size_t)qi * 2 * KN + (size_t)k * N;
-    intt1<LOGN>(R, t, rb, tw, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
+    intt_v<LOGN, V>(R, t, rb, tw, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
 #pragma unroll
-    for (int e = 0; e < 8; e++) out[t + e * T] = csub(R[e], M.q);
+    for (int e = 0; e < V; e++) out[t + e * T] = csub(R[e], M.q);
     lds_barrier();  // every thread is past its last read of rb
-    intt1<LOGN>(S, t, rb, tw, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
+    intt_v<LOGN, V>(S, t, rb, tw, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
 #pragma unroll
-    for (int e = 0; e < 8; e++) {
+    for (int e = 0; e < V; e++) {
         const uint32_t pos = t + e * T;
         uint32_t v = csub(S[e], M.q);
         if (pos < gamma) {
@@ -309,25 +310,37 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
 
 // Raises the kernel's dynamic LDS limit once per instantiation (a function-
 // local static is initialised exactly once, also under concurrent callers).
-template <int LOGN, int ELL, bool LAZY, bool REC, bool SGN>
+template <int LOGN, int ELL, bool LAZY, bool REC, bool SGN, int V>
 static hipError_t chain_crt_lds_limit() {
     constexpr size_t shmem = ChainCrtShape<LOGN, ELL, REC>::kBytes;
     static const hipError_t e =
-        shmem > 64 * 1024 ? hipFuncSetAttribute((const void*)k_pack_chain_crt<LOGN, ELL, LAZY, REC, SGN>,
+        shmem > 64 * 1024 ? hipFuncSetAttribute((const void*)k_pack_chain_crt<LOGN, ELL, LAZY, REC, SGN, V>,
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem)
                           : hipSuccess;
     return e;
 }
 
+template <int LOGN, int ELL, bool LAZY, bool REC, bool SGN, int V>
+static hipError_t chain_crt_launch_v(const PackArgs& A, uint32_t Q, const CtxDev& C, const ScaleDev& inv,
+                                     hipStream_t st) {
+    const hipError_t e = chain_crt_lds_limit<LOGN, ELL, LAZY, REC, SGN, V>();
+    if (e != hipSuccess) return e;
+    auto kern = k_pack_chain_crt<LOGN, ELL, LAZY, REC, SGN, V>;
+    constexpr size_t shmem = ChainCrtShape<LOGN, ELL, REC>::kBytes;
+    hipLaunchKernelGGL(kern, dim3(Q), dim3(2 * ((1 << LOGN) / V)), shmem, st, A, C, inv);
+    return hipGetLastError();
+}
+
+// Four values per thread: 2 N/4 threads per query, four waves per SIMD at
+// d = 2048 (84 VGPRs), where eight values per thread (the radix-8 layout,
+// 138 VGPRs) leave two: the extra waves hide the latency of each step's LDS
+// exchanges and barriers (0.133 against 0.152 ms per 256-query launch).
+constexpr int kChainV = 4;
+
 template <int LOGN, int ELL, bool LAZY, bool REC, bool SGN = false>
 static hipError_t chain_crt_launch(const PackArgs& A, uint32_t Q, const CtxDev& C, const ScaleDev& inv,
                                    hipStream_t st) {
-    const hipError_t e = chain_crt_lds_limit<LOGN, ELL, LAZY, REC, SGN>();
-    if (e != hipSuccess) return e;
-    auto kern = k_pack_chain_crt<LOGN, ELL, LAZY, REC, SGN>;
-    constexpr size_t shmem = ChainCrtShape<LOGN, ELL, REC>::kBytes;
-    hipLaunchKernelGGL(kern, dim3(Q), dim3(2 * NttShape<LOGN>::T), shmem, st, A, C, inv);
-    return hipGetLastError();
+    return chain_crt_launch_v<LOGN, ELL, LAZY, REC, SGN, kChainV>(A, Q, C, inv, st);
 }
 
 // The SGN variant's bounds (signed_digits_ok): R < 2q enters the recovered

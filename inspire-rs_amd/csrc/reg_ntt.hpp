@@ -333,6 +333,170 @@ __device__ __forceinline__ void ntt_digits(uint32_t (&x)[ELL][8], int t, uint32_
         }
     }
 }
+// V values per thread (8: the radix-8 layout above, 4: the radix-4 one
+// below): the slot positions of thread t are V t .. V t + V - 1.
+template <int V>
+__device__ __forceinline__ void load_slots_v(const uint32_t* p, int t, uint32_t (&v)[V]) {
+#pragma unroll
+    for (int g = 0; g < V / 4; g++) {
+        const uint4 a = *reinterpret_cast<const uint4*>(p + V * t + 4 * g);
+        v[4 * g + 0] = a.x;
+        v[4 * g + 1] = a.y;
+        v[4 * g + 2] = a.z;
+        v[4 * g + 3] = a.w;
+    }
+}
 
+template <int LOGN>
+struct Lay4;
+template <int LOGN, int ELL, bool SGN>
+__device__ __forceinline__ void ntt4_digits(uint32_t (&x)[ELL][4], int t, uint32_t* db, const uint32_t* tw,
+                                            const Mod& M);
+template <int LOGN>
+__device__ __forceinline__ void intt4(uint32_t (&x)[4], int t, uint32_t* rb, const uint32_t* tw, const Mod& M,
+                                      uint32_t c, uint32_t cp, uint32_t cw, uint32_t cwp);
+
+template <int LOGN, int V, int ELL, bool SGN = false>
+__device__ __forceinline__ void ntt_v(uint32_t (&x)[ELL][V], int t, uint32_t* db, const uint32_t* tw, const Mod& M) {
+    if constexpr (V == 8) ntt_digits<LOGN, ELL, SGN>(x, t, db, tw, M);
+    else ntt4_digits<LOGN, ELL, SGN>(x, t, db, tw, M);
+}
+
+template <int LOGN, int V>
+__device__ __forceinline__ void intt_v(uint32_t (&x)[V], int t, uint32_t* rb, const uint32_t* tw, const Mod& M,
+                                       uint32_t c, uint32_t cp, uint32_t cw, uint32_t cwp) {
+    if constexpr (V == 8) intt1<LOGN>(x, t, rb, tw, M, c, cp, cw, cwp);
+    else intt4<LOGN>(x, t, rb, tw, M, c, cp, cw, cwp);
+}
+
+// ---------------------------------------------------------------------------
+// Four values per thread (N/4 threads per residue, twice the waves of the
+// radix-8 layout): radix-4 passes plus one radix-2 stage when LOGN is odd.
+// Coefficient positions {t + e N/4}; slot positions 4t .. 4t + 3.
+// ---------------------------------------------------------------------------
+template <int LOGN>
+struct Lay4 {
+    static constexpr int N = 1 << LOGN, T = N / 4, LP = NttShape<LOGN>::LDS;
+    static constexpr int R4 = LOGN / 2, REM = LOGN % 2;
+    // The block a forward pass-(p+1) thread reads was written by the N/4^(p+1)
+    // threads of its pass-p block, the block an inverse pass-p thread reads by
+    // the same 2^REM 4^p threads: with at most 64 of them (one wave) the
+    // exchange needs no s_barrier.
+    static constexpr bool fwd_wave_local(int p) { return (N >> (2 * (p + 1))) <= 64; }
+    static constexpr bool inv_wave_local(int p) { return ((1 << REM) << (2 * p)) <= 64; }
+};
+
+// Forward NTT of ELL polys of one residue, four values per thread:
+// coefficient positions -> slot positions (< 4q; SGN as for ntt_digits).
+// db is [ELL][LP] LDS; no thread may still be reading it.
+template <int LOGN, int ELL, bool SGN>
+__device__ __forceinline__ void ntt4_digits(uint32_t (&x)[ELL][4], int t, uint32_t* db, const uint32_t* tw,
+                                            const Mod& M) {
+    using L = Lay4<LOGN>;
+    constexpr int N = L::N, LP = L::LP;
+    int m0 = 1;
+#pragma unroll
+    for (int p = 0; p < L::R4; p++) {
+        const int t0 = N / (2 * m0), st = t0 >> 1;
+        const int i = t / st, r = t - i * st;
+        const int bix = lds_ix(2 * i * t0 + r);
+        if (p > 0) {
+#pragma unroll
+            for (int j = 0; j < ELL; j++)
+#pragma unroll
+                for (int e = 0; e < 4; e++) x[j][e] = db[j * LP + lds_step(bix, e, st)];
+        }
+        {
+            const uint32_t w = tw[m0 + i];
+#pragma unroll
+            for (int j = 0; j < ELL; j++) {
+                ct_b<SGN>(x[j][0], x[j][2], w, M);
+                ct_b<SGN>(x[j][1], x[j][3], w, M);
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const uint32_t w = tw[2 * m0 + 2 * i + h];
+#pragma unroll
+            for (int j = 0; j < ELL; j++) ct_b<SGN>(x[j][2 * h], x[j][2 * h + 1], w, M);
+        }
+        if (!(L::REM == 0 && p == L::R4 - 1)) {
+#pragma unroll
+            for (int j = 0; j < ELL; j++)
+#pragma unroll
+                for (int e = 0; e < 4; e++) db[j * LP + lds_step(bix, e, st)] = x[j][e];
+            if (L::fwd_wave_local(p))
+                wave_lds_sync();
+            else
+                lds_barrier();
+        }
+        m0 <<= 2;
+    }
+    if constexpr (L::REM != 0) {
+        const uint32_t w0 = tw[N / 2 + 2 * t], w1 = tw[N / 2 + 2 * t + 1];
+#pragma unroll
+        for (int j = 0; j < ELL; j++) {
+            const uint4 v = *reinterpret_cast<const uint4*>(&db[j * LP + lds_ix(4 * t)]);
+            x[j][0] = v.x;
+            x[j][1] = v.y;
+            x[j][2] = v.z;
+            x[j][3] = v.w;
+            ct_b<SGN>(x[j][0], x[j][1], w0, M);
+            ct_b<SGN>(x[j][2], x[j][3], w1, M);
+        }
+    }
+}
+
+// Inverse NTT of one residue, four values per thread, scaled by (c, cw) in
+// the last stage: slot positions -> coefficient positions (< 2q).  rb is the
+// residue's [LP] LDS buffer; no thread may still be reading it.
+template <int LOGN>
+__device__ __forceinline__ void intt4(uint32_t (&x)[4], int t, uint32_t* rb, const uint32_t* tw, const Mod& M,
+                                      uint32_t c, uint32_t cp, uint32_t cw, uint32_t cwp) {
+    using L = Lay4<LOGN>;
+    constexpr int N = L::N;
+    if constexpr (L::REM != 0) {
+        gs_m(x[0], x[1], inv_tw(tw, N / 2, 2 * t), M);
+        gs_m(x[2], x[3], inv_tw(tw, N / 2, 2 * t + 1), M);
+        *reinterpret_cast<uint4*>(&rb[lds_ix(4 * t)]) = make_uint4(x[0], x[1], x[2], x[3]);
+        if (L::inv_wave_local(0))
+            wave_lds_sync();
+        else
+            lds_barrier();
+    }
+    int t0 = 1 << L::REM;
+#pragma unroll
+    for (int p = 0; p < L::R4; p++) {
+        const int i2 = t / t0, r = t - i2 * t0;
+        const int bix = lds_ix(4 * t0 * i2 + r);
+        if (!(L::REM == 0 && p == 0)) {
+#pragma unroll
+            for (int e = 0; e < 4; e++) x[e] = rb[lds_step(bix, e, t0)];
+        }
+        const int mA = N / (2 * t0), mB = N / (4 * t0);
+#pragma unroll
+        for (int h = 0; h < 2; h++) gs_m(x[2 * h], x[2 * h + 1], inv_tw(tw, mA, 2 * i2 + h), M);
+        if (p < L::R4 - 1) {
+            const uint32_t w = inv_tw(tw, mB, i2);
+            gs_m(x[0], x[2], w, M);
+            gs_m(x[1], x[3], w, M);
+#pragma unroll
+            for (int e = 0; e < 4; e++) rb[lds_step(bix, e, t0)] = x[e];
+            if (L::inv_wave_local(p + 1))
+                wave_lds_sync();
+            else
+                lds_barrier();
+        } else {
+            // last stage (m = 1) with the output scale: X' = c (X + Y), Y' = cw (X - Y)
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                const uint32_t X = x[k], Y = x[k + 2];
+                x[k] = shoup_mul(X + Y, c, cp, M.q);
+                x[k + 2] = shoup_mul(X - Y + M.q2, cw, cwp, M.q);
+            }
+        }
+        t0 <<= 2;
+    }
+}
 
 }  // namespace inspire
