@@ -386,6 +386,15 @@ struct Lay4 {
     static constexpr bool inv_wave_local(int p) { return ((1 << REM) << (2 * p)) <= 64; }
 };
 
+// Either layout by its number of values per thread.
+template <int LOGN, int V>
+struct LayV {
+    static constexpr int N = 1 << LOGN, T = N / V, LP = NttShape<LOGN>::LDS;
+    static __device__ __forceinline__ int group(int t, int g) { return V * t + 4 * g; }
+    // the forward transform passes at least one s_barrier
+    static constexpr bool fwd_has_barrier = V == 8 ? Lay<LOGN>::fwd_has_barrier : !Lay4<LOGN>::fwd_wave_local(0);
+};
+
 // Forward NTT of ELL polys of one residue, four values per thread:
 // coefficient positions -> slot positions (< 4q; SGN as for ntt_digits).
 // db is [ELL][LP] LDS; no thread may still be reading it.

@@ -14,8 +14,8 @@
 // automorphism need it) and in the NTT domain (a^), b in the NTT domain (b
 // only ever enters linearly: the automorphism is a slot gather and y =
 // X^(d/2^l) a pointwise product), all canonical u32 residues, [3][K][N] per
-// node.  One workgroup of K x N/8 threads works on one node, the k-th group
-// of N/8 threads on residue k.
+// node.  One workgroup of K x N/4 threads works on one node, the k-th group
+// of N/4 threads (four values each, kTreeV) on residue k.
 //
 // One level (pack_lwes_inner with t = 2^l + 1 and the galois key of index
 // log d - l, whose rows are kept as NTT-domain Montgomery words):
@@ -46,6 +46,10 @@ namespace inspire {
 
 namespace {
 
+// Values per thread of the register-resident transforms: four (N/4 threads
+// per residue, four waves per SIMD at d = 2048) rather than eight.
+constexpr int kTreeV = 4;
+
 template <int ELL, bool REC>
 struct TreeDigits {
     static constexpr int ND = REC ? ELL - 1 : ELL;  // digits transformed
@@ -54,7 +58,7 @@ struct TreeDigits {
 
 template <int LOGN, int K, int ELL, bool REC>
 struct TreeShape {
-    static constexpr int N = 1 << LOGN, T = N / 8, LP = NttShape<LOGN>::LDS;
+    static constexpr int N = 1 << LOGN, T = N / kTreeV, LP = NttShape<LOGN>::LDS;
     static constexpr size_t kTw = (size_t)K * N;          // forward Montgomery twiddles (the inverse reads them mirrored)
     static constexpr size_t kA = (size_t)K * LP;          // s1.a (automorphism source), then INTT exchange
     static constexpr size_t kB = (size_t)K * LP;          // s1.b (automorphism source)
@@ -118,11 +122,12 @@ __device__ __forceinline__ uint32_t aut_slot(uint32_t p, uint32_t t) {
 // On return a0, h0, b0 hold the new node (canonical).  key: [ell][2][K][N]
 // NTT-domain Montgomery rows.
 template <int LOGN, int K, int ELL, bool LAZY, bool REC, bool SGN>
-__device__ __forceinline__ void tree_level(uint32_t (&a0)[8], const uint32_t (&a1)[8], uint32_t (&h0)[8],
-                                           const uint32_t (&h1)[8], uint32_t (&b0)[8], const uint32_t (&b1)[8],
+__device__ __forceinline__ void tree_level(uint32_t (&a0)[kTreeV], const uint32_t (&a1)[kTreeV], uint32_t (&h0)[kTreeV],
+                                           const uint32_t (&h1)[kTreeV], uint32_t (&b0)[kTreeV], const uint32_t (&b1)[kTreeV],
                                            uint32_t level, const uint32_t* key, const TreeLds<LOGN, K, ELL, REC>& L,
                                            const TreeArgs& TA, const CtxDev& C, const ScaleDev& inv, int k, int t) {
-    using Ly = Lay<LOGN>;
+    using Ly = LayV<LOGN, kTreeV>;
+    constexpr int V = kTreeV;
     constexpr int N = Ly::N, T = Ly::T, LP = Ly::LP;
     constexpr int ND = TreeDigits<ELL, REC>::ND, DB = TreeDigits<ELL, REC>::DB;
     constexpr size_t KN = (size_t)K * N;
@@ -132,11 +137,11 @@ __device__ __forceinline__ void tree_level(uint32_t (&a0)[8], const uint32_t (&a
     const uint32_t tinv = inv_odd(taut) & (2u * N - 1u);
 
     // the key rows this residue multiplies by, loaded ahead of the transforms
-    uint32_t ka[ELL][8], kb[ELL][8];
+    uint32_t ka[ELL][kTreeV], kb[ELL][kTreeV];
 #pragma unroll
     for (int j = 0; j < ELL; j++) {
-        load_slots<LOGN>(key + ((size_t)j * 2 + 0) * KN + (size_t)k * N, t, ka[j]);
-        load_slots<LOGN>(key + ((size_t)j * 2 + 1) * KN + (size_t)k * N, t, kb[j]);
+        load_slots_v<V>(key + ((size_t)j * 2 + 0) * KN + (size_t)k * N, t, ka[j]);
+        load_slots_v<V>(key + ((size_t)j * 2 + 1) * KN + (size_t)k * N, t, kb[j]);
     }
 
     lds_barrier();  // every thread is past its reads of the previous level
@@ -145,9 +150,9 @@ __device__ __forceinline__ void tree_level(uint32_t (&a0)[8], const uint32_t (&a
     uint32_t* bb = L.bbuf + k * LP;
     uint32_t* hb = L.hbuf + k * LP;
 #pragma unroll
-    for (int e = 0; e < 8; e++) ab[lds_step(cix, e, T)] = a1[e];
+    for (int e = 0; e < V; e++) ab[lds_step(cix, e, T)] = a1[e];
 #pragma unroll
-    for (int g = 0; g < 2; g++) {
+    for (int g = 0; g < V / 4; g++) {
         *reinterpret_cast<uint4*>(&bb[lds_ix(Ly::group(t, g))]) =
             make_uint4(b1[4 * g + 0], b1[4 * g + 1], b1[4 * g + 2], b1[4 * g + 3]);
         if constexpr (REC)
@@ -158,9 +163,9 @@ __device__ __forceinline__ void tree_level(uint32_t (&a0)[8], const uint32_t (&a
 
     // gadget digits of tau_t(s1.a) at the coefficient positions:
     // tau_t(x)_j = x_i for t i = j mod 2N, and -x_i for t i = j + N mod 2N
-    uint32_t dg[ELL][8];
+    uint32_t dg[ELL][kTreeV];
 #pragma unroll
-    for (int e = 0; e < 8; e++) {
+    for (int e = 0; e < V; e++) {
         const uint32_t j = (uint32_t)(t + e * T);
         const uint32_t src = (tinv * j) & (2u * N - 1u);
         const bool neg = src >= (uint32_t)N;
@@ -186,9 +191,9 @@ __device__ __forceinline__ void tree_level(uint32_t (&a0)[8], const uint32_t (&a
         }
     }
     // tau_t(s1.b), and tau_t(s1.a^) for the recovered digit, at the slot positions
-    uint32_t bt[8], ht[8];
+    uint32_t bt[kTreeV], ht[kTreeV];
 #pragma unroll
-    for (int g = 0; g < 2; g++)
+    for (int g = 0; g < V / 4; g++)
 #pragma unroll
         for (int c = 0; c < 4; c++) {
             const int sx = lds_ix((int)aut_slot<LOGN>((uint32_t)(Ly::group(t, g) + c), taut));
@@ -197,7 +202,7 @@ __device__ __forceinline__ void tree_level(uint32_t (&a0)[8], const uint32_t (&a
         }
 
     if constexpr (ND > 0) {
-        ntt_digits<LOGN, ND, SGN>(reinterpret_cast<uint32_t(&)[ND][8]>(dg), t, L.dbuf + k * DB * LP, L.twf + k * N,
+        ntt_v<LOGN, V, ND, SGN>(reinterpret_cast<uint32_t(&)[ND][V]>(dg), t, L.dbuf + k * DB * LP, L.twf + k * N,
                                   M);
         // every thread is past its gathers from abuf before the INTT reuses it: the transform's
         // own barrier orders them when it has one
@@ -208,7 +213,7 @@ __device__ __forceinline__ void tree_level(uint32_t (&a0)[8], const uint32_t (&a
     if constexpr (REC && SGN) {
         // NTT(D_last) = z^-(ell-1) tau(s1.a^) + sum_{j < ell-1} -z^(j-(ell-1)) NTT(D_j): one signed sum
 #pragma unroll
-        for (int e = 0; e < 8; e++) {
+        for (int e = 0; e < V; e++) {
             int64_t sum = (int64_t)(int32_t)ht[e] * (int32_t)TA.zinvm[k];
 #pragma unroll
             for (int j = 0; j < ND; j++) sum += (int64_t)(int32_t)dg[j][e] * (int32_t)TA.zrecm[k][j];
@@ -217,7 +222,7 @@ __device__ __forceinline__ void tree_level(uint32_t (&a0)[8], const uint32_t (&a
     } else if constexpr (REC) {
         // transform outputs < 4q and Montgomery products < 2q keep the difference in (0, 7q)
 #pragma unroll
-        for (int e = 0; e < 8; e++) {
+        for (int e = 0; e < V; e++) {
             uint32_t d = ht[e];
             if constexpr (ND >= 1) d += 4 * M.q - dg[0][e];
 #pragma unroll
@@ -226,9 +231,9 @@ __device__ __forceinline__ void tree_level(uint32_t (&a0)[8], const uint32_t (&a
         }
     }
 
-    uint32_t A[8];
+    uint32_t A[kTreeV];
 #pragma unroll
-    for (int e = 0; e < 8; e++) {
+    for (int e = 0; e < V; e++) {
         if constexpr (SGN) {
             int64_t sa = 0, sb = 0;
 #pragma unroll
@@ -254,15 +259,16 @@ __device__ __forceinline__ void tree_level(uint32_t (&a0)[8], const uint32_t (&a
             b0[e] = addq(addq(b0[e], bt[e], q), csub(mont_reduce(sb, q, M.qinv), q), q);
         }
     }
-    intt1<LOGN>(A, t, ab, L.twf + k * N, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
+    intt_v<LOGN, V>(A, t, ab, L.twf + k * N, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
 #pragma unroll
-    for (int e = 0; e < 8; e++) a0[e] = addq(a0[e], csub(A[e], q), q);
+    for (int e = 0; e < V; e++) a0[e] = addq(a0[e], csub(A[e], q), q);
 }
 
 template <int LOGN, int K, int ELL, bool LAZY, bool REC, bool SGN>
-__global__ void __launch_bounds__(K * NttShape<LOGN>::T, 1)
+__global__ void __launch_bounds__(K * ((1 << LOGN) / kTreeV), 1)
 k_tree_leaves(TreeArgs A, CtxDev C, ScaleDev inv) {
-    using Ly = Lay<LOGN>;
+    using Ly = LayV<LOGN, kTreeV>;
+    constexpr int V = kTreeV;
     constexpr int N = Ly::N, T = Ly::T, LP = Ly::LP;
     constexpr size_t KN = (size_t)K * N;
     extern __shared__ __align__(16) uint32_t lds[];
@@ -276,18 +282,18 @@ k_tree_leaves(TreeArgs A, CtxDev C, ScaleDev inv) {
     // rows ell..2ell-1; the a-digits are 0): NTT domain, slot positions
     const uint32_t* dig = reinterpret_cast<const uint32_t*>(A.qdig[qi]) + (size_t)s * ELL * KN + (size_t)k * N;
     const uint32_t* rw = A.rows + (size_t)qi * ELL * 2 * KN + (size_t)k * N;
-    uint32_t dv[ELL][8], ra[ELL][8], rb[ELL][8];
+    uint32_t dv[ELL][kTreeV], ra[ELL][kTreeV], rb[ELL][kTreeV];
 #pragma unroll
     for (int j = 0; j < ELL; j++) {
-        load_slots<LOGN>(dig + (size_t)j * KN, t, dv[j]);
-        load_slots<LOGN>(rw + (size_t)j * 2 * KN, t, ra[j]);
-        load_slots<LOGN>(rw + (size_t)j * 2 * KN + KN, t, rb[j]);
+        load_slots_v<V>(dig + (size_t)j * KN, t, dv[j]);
+        load_slots_v<V>(rw + (size_t)j * 2 * KN, t, ra[j]);
+        load_slots_v<V>(rw + (size_t)j * 2 * KN + KN, t, rb[j]);
     }
     stage_twiddles(L, C);
-    uint32_t ah[8];
+    uint32_t ah[kTreeV];
     uint64_t bsum = 0;
 #pragma unroll
-    for (int e = 0; e < 8; e++) {
+    for (int e = 0; e < V; e++) {
         uint64_t sa = 0, sb = 0;
 #pragma unroll
         for (int j = 0; j < ELL; j++) {
@@ -312,21 +318,21 @@ k_tree_leaves(TreeArgs A, CtxDev C, ScaleDev inv) {
 
     // the leaf: (a, 0) with a the column's a (invert_sample_extract undoes
     // sample_extract_coeff0, automorph_pack.rs:349-405)
-    uint32_t a[8], h[8], b[8];
+    uint32_t a[kTreeV], h[kTreeV], b[kTreeV];
 #pragma unroll
-    for (int e = 0; e < 8; e++) {
+    for (int e = 0; e < V; e++) {
         h[e] = csub(ah[e], q);
         a[e] = h[e];
         b[e] = 0;
     }
-    intt1<LOGN>(a, t, L.abuf + k * LP, L.twf + k * N, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
+    intt_v<LOGN, V>(a, t, L.abuf + k * LP, L.twf + k * N, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
 #pragma unroll
-    for (int e = 0; e < 8; e++) a[e] = csub(a[e], q);
+    for (int e = 0; e < V; e++) a[e] = csub(a[e], q);
 
     if (A.m == 1) {  // a single LWE is not tree-packed (automorph_pack.rs:436-443)
         uint32_t* o = A.out + (size_t)qi * A.out_words;
 #pragma unroll
-        for (int e = 0; e < 8; e++) {
+        for (int e = 0; e < V; e++) {
             const int j = t + e * T;
             o[(size_t)k * N + j] = a[e];
             o[KN + (size_t)k * N + j] = j == 0 ? lwe_b : 0u;
@@ -341,9 +347,9 @@ k_tree_leaves(TreeArgs A, CtxDev C, ScaleDev inv) {
 
     uint32_t* nd = A.node + ((size_t)qi * A.m + s) * 3 * KN + (size_t)k * N;
 #pragma unroll
-    for (int e = 0; e < 8; e++) nd[t + e * T] = a[e];
+    for (int e = 0; e < V; e++) nd[t + e * T] = a[e];
 #pragma unroll
-    for (int g = 0; g < 2; g++) {
+    for (int g = 0; g < V / 4; g++) {
         *reinterpret_cast<uint4*>(nd + KN + Ly::group(t, g)) = make_uint4(h[4 * g], h[4 * g + 1], h[4 * g + 2], h[4 * g + 3]);
         *reinterpret_cast<uint4*>(nd + 2 * KN + Ly::group(t, g)) =
             make_uint4(b[4 * g], b[4 * g + 1], b[4 * g + 2], b[4 * g + 3]);
@@ -355,9 +361,10 @@ k_tree_leaves(TreeArgs A, CtxDev C, ScaleDev inv) {
 // 2^(log d - level) never hold an odd input).  At the root the packed
 // ciphertext is finished into out.
 template <int LOGN, int K, int ELL, bool LAZY, bool REC, bool SGN>
-__global__ void __launch_bounds__(K * NttShape<LOGN>::T, 1)
+__global__ void __launch_bounds__(K * ((1 << LOGN) / kTreeV), 1)
 k_tree_combine(TreeArgs A, CtxDev C, ScaleDev inv) {
-    using Ly = Lay<LOGN>;
+    using Ly = LayV<LOGN, kTreeV>;
+    constexpr int V = kTreeV;
     constexpr int N = Ly::N, T = Ly::T, LP = Ly::LP;
     constexpr size_t KN = (size_t)K * N;
     extern __shared__ __align__(16) uint32_t lds[];
@@ -372,10 +379,10 @@ k_tree_combine(TreeArgs A, CtxDev C, ScaleDev inv) {
     const uint32_t* ev = A.node + ((size_t)qi * A.m + s) * 3 * KN + (size_t)k * N;
     const uint32_t* od = A.node + ((size_t)qi * A.m + s + half) * 3 * KN + (size_t)k * N;
 
-    uint32_t a0[8], a1[8], h0[8], h1[8], b0[8], b1[8];
+    uint32_t a0[kTreeV], a1[kTreeV], h0[kTreeV], h1[kTreeV], b0[kTreeV], b1[kTreeV];
     const uint32_t ysh = (uint32_t)N >> level;  // y = X^(N / 2^level)
 #pragma unroll
-    for (int e = 0; e < 8; e++) {
+    for (int e = 0; e < V; e++) {
         const uint32_t j = (uint32_t)(t + e * T);
         const uint32_t x = ev[j];
         uint32_t v = 0;
@@ -385,17 +392,17 @@ k_tree_combine(TreeArgs A, CtxDev C, ScaleDev inv) {
     }
     {
         // a^ and b: s0 = even + y^ odd, s1 = even - y^ odd, slot by slot
-        uint32_t eh[8], oh[8], eb[8], ob[8], yh[8];
-        load_slots<LOGN>(ev + KN, t, eh);
-        load_slots<LOGN>(ev + 2 * KN, t, eb);
+        uint32_t eh[kTreeV], oh[kTreeV], eb[kTreeV], ob[kTreeV], yh[kTreeV];
+        load_slots_v<V>(ev + KN, t, eh);
+        load_slots_v<V>(ev + 2 * KN, t, eb);
         if (odd) {
-            load_slots<LOGN>(od + KN, t, oh);
-            load_slots<LOGN>(od + 2 * KN, t, ob);
-            load_slots<LOGN>(A.yhat + ((size_t)(level - 1) * K + k) * N, t, yh);
+            load_slots_v<V>(od + KN, t, oh);
+            load_slots_v<V>(od + 2 * KN, t, ob);
+            load_slots_v<V>(A.yhat + ((size_t)(level - 1) * K + k) * N, t, yh);
         }
         const Mod M{q, C.m[k].q2, C.m[k].qinv};
 #pragma unroll
-        for (int e = 0; e < 8; e++) {
+        for (int e = 0; e < V; e++) {
             const uint32_t vh = odd ? csub(mmul(oh[e], yh[e], M), q) : 0u;
             h0[e] = addq(eh[e], vh, q);
             h1[e] = subq(eh[e], vh, q);
@@ -412,9 +419,9 @@ k_tree_combine(TreeArgs A, CtxDev C, ScaleDev inv) {
         // each thread rewrites exactly the words of node s it read
         uint32_t* nd = A.node + ((size_t)qi * A.m + s) * 3 * KN + (size_t)k * N;
 #pragma unroll
-        for (int e = 0; e < 8; e++) nd[t + e * T] = a0[e];
+        for (int e = 0; e < V; e++) nd[t + e * T] = a0[e];
 #pragma unroll
-        for (int g = 0; g < 2; g++) {
+        for (int g = 0; g < V / 4; g++) {
             *reinterpret_cast<uint4*>(nd + KN + Ly::group(t, g)) =
                 make_uint4(h0[4 * g], h0[4 * g + 1], h0[4 * g + 2], h0[4 * g + 3]);
             *reinterpret_cast<uint4*>(nd + 2 * KN + Ly::group(t, g)) =
@@ -426,12 +433,12 @@ k_tree_combine(TreeArgs A, CtxDev C, ScaleDev inv) {
     // root: b to the coefficient domain, then b_val * d at coefficient z < m
     const Mod M{q, C.m[k].q2, C.m[k].qinv};
     lds_barrier();  // every thread is past the level's last reads of abuf
-    intt1<LOGN>(b0, t, L.abuf + k * LP, L.twf + k * N, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
+    intt_v<LOGN, V>(b0, t, L.abuf + k * LP, L.twf + k * N, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
     uint32_t* o = A.out + (size_t)qi * A.out_words;
     uint32_t* ob = o + KN;
     const uint32_t* lw = A.lweb + (size_t)qi * A.m * K;
 #pragma unroll
-    for (int e = 0; e < 8; e++) {
+    for (int e = 0; e < V; e++) {
         const uint32_t j = (uint32_t)(t + e * T);
         o[(size_t)k * N + j] = a0[e];
         const uint32_t bv = csub(b0[e], q);
@@ -476,7 +483,7 @@ static hipError_t tree_launch(const TreeArgs& A, bool leaves, uint32_t blocks, c
                           : hipSuccess;
     if (e0 != hipSuccess) return e0;
     if (e1 != hipSuccess) return e1;
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(K * NttShape<LOGN>::T), shmem, st, A, C, inv);
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(K * ((1 << LOGN) / kTreeV)), shmem, st, A, C, inv);
     return hipGetLastError();
 }
 
