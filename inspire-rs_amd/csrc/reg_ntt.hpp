@@ -1,12 +1,13 @@
 // Register-resident NTT building blocks for kernels that keep a residue's
-// polynomial in registers (eight values per thread, N/8 threads per residue)
-// and exchange through LDS only between radix-8 passes.
+// polynomial in registers and exchange through LDS only between passes: V =
+// 8 values per thread (N/8 threads per residue, radix-8 passes, Lay) or V = 4
+// (N/4 threads, radix-4 passes, Lay4: twice the waves for the same work).
 //
-// Position layout.  The radix-8 passes leave every thread on eight fixed
-// positions.  The inverse transform's last pass and the forward transform's
-// first pass touch the same positions {t + k N/8} (the "coefficient
-// positions"); the forward transform's last pass and the inverse transform's
-// first pass touch the same positions (the "slot positions", Lay::group).
+// Position layout.  The passes leave every thread on V fixed positions.  The
+// inverse transform's last pass and the forward transform's first pass touch
+// the same positions {t + k N/V} (the "coefficient positions"); the forward
+// transform's last pass and the inverse transform's first pass touch the same
+// positions (the "slot positions", V t .. V t + V - 1).
 // Coefficient-domain work (CRT composition, gadget digits, automorphisms)
 // therefore runs in registers between an inverse and a forward transform, and
 // NTT-domain work (multiply-accumulate) between a forward and an inverse one.
