@@ -53,8 +53,9 @@ struct ChainCrtShape {
     static constexpr int N = 1 << LOGN, LP = NttShape<LOGN>::LDS;
     static constexpr size_t kBytes = (size_t)2 * N * 4      // forward Montgomery twiddles of both residues
                                      + (size_t)ELL * 2 * N * 4  // NTT of the client's y_body
-                                     + (size_t)2 * ChainDigits<ELL, REC>::DB * LP * 4;  // transform exchange,
+                                     + (size_t)2 * ChainDigits<ELL, REC>::DB * LP * 4  // transform exchange,
                                      // per residue (the inverse uses the first digit buffer)
+                                     + (size_t)2 * LP * 4;  // coefficient exchange between the halves
 };
 
 // LAZY: ELL * 4q * q < q * 2^32, so the multiply-accumulate takes the
@@ -76,6 +77,7 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
     uint32_t* twf = lds;                 // [2][N] psi^br(i) 2^32 mod q_k
     uint32_t* yl = twf + 2 * N;          // [ELL][2][N]
     uint32_t* dbuf = yl + ELL * 2 * N;   // [2][DB][LP]
+    uint32_t* xbuf = dbuf + 2 * DB * LP;  // [2][LP]
 
     const int tid = threadIdx.x;
     const int k = residue_of<T>(tid), t = tid - k * T;  // residue of this half, thread within it
@@ -181,13 +183,16 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
         }
         intt_v<LOGN, V>(R, ts, rb, tw, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
         const int cix = lds_ix(ts);
+        // the halves swap coefficients through buffers of their own, so the digit
+        // transforms may write their exchange buffers while the other half still reads
+        uint32_t* xb = xbuf + k * LP;
 #pragma unroll
         for (int e = 0; e < V; e++) {
             R[e] = csub(R[e], M.q);
-            rb[lds_step(cix, e, T)] = R[e];
+            xb[lds_step(cix, e, T)] = R[e];
         }
         lds_barrier();
-        const uint32_t* ro = dbuf + (1 - k) * DB * LP;  // the other residue's exchange buffer
+        const uint32_t* ro = xbuf + (1 - k) * LP;  // the other residue's coefficients
         uint32_t dg[ELL][V];
 #pragma unroll
         for (int e = 0; e < V; e++) {
@@ -204,7 +209,6 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
                 dg[j][e] = C.small_digits ? v : reduce32(v, M.q, C.onep[k]);
             }
         }
-        lds_barrier();  // the other half is past its reads of this half's exchange buffer
         if constexpr (ND > 0) ntt_v<LOGN, V, ND, SGN>(reinterpret_cast<uint32_t(&)[ND][V]>(dg), ts, db, tw, M);
         if constexpr (REC && SGN) {
             // NTT(D_last) = z^-(ell-1) R + sum_{j < ell-1} -z^(j-(ell-1)) NTT(D_j): one signed sum
