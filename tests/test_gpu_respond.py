@@ -307,3 +307,54 @@ def test_device_batch_matches_host_api(M, variant):
             else:
                 want = ref.ciphertext.reshape(-1)
             assert np.array_equal(out[i].astype(np.uint64), want), i
+
+
+@pytest.mark.parametrize("variant,n", [("two", 48), ("zero", 24)])
+def test_host_batch_many_queries_matches_device_batch(M, variant, n):
+    """A host-buffer batch large enough to be staged and handed back on several
+    host threads equals the same queries as a device-resident batch, and a
+    sample equals the oracle's responses."""
+    say("host batch", variant, n)
+    o = O.Oracle(O.baseline_params())
+    d, K = o.params.d, o.params.K
+    db = random_db(4 * d, 32, seed=61)
+    shards = encode_db(o, db, 32)
+    cl = Client(o, 62)
+    yb, _ = cl.packing_keys(16, W_SEED, with_y_all=False)
+    g = np.random.default_rng(63)
+    sids = g.integers(0, 4, n).astype(np.uint32)
+    locals_ = g.integers(0, d, n)
+    qs, rows, seeds = [], [], []
+    for i in range(n):
+        s, b = cl.seeded_query(int(locals_[i]))
+        rows.append(b[3:6].astype(np.uint32))
+        seeds.append(np.frombuffer(s[3:6].tobytes(), dtype="<u4").reshape(3, 8))
+        qs.append(M.ClientQuery(shard_id=int(sids[i]), rgsw=b, seeds=s, has_packing_keys=True, y_body=yb))
+    v = {"two": M.TWO_PACKING, "zero": M.NO_PACKING}[variant]
+    cols = 16
+    per = (2 if v != M.NO_PACKING else 2 * (1 + cols)) * K * d
+    with make_ctx(M, o.params) as ctx:
+        ctx.load_database(0, shards)
+        res = ctx.respond_batch([0] * n, qs, [v] * n)
+        drows = M.DeviceArray.from_numpy(np.stack(rows))
+        dseeds = M.DeviceArray.from_numpy(np.stack(seeds))
+        dy = M.DeviceArray.from_numpy(np.stack([yb.astype(np.uint32)] * n))
+        dout = M.DeviceArray(n * per * 4)
+        st = ctx.respond_batch_device(0, v, sids, drows.ptr, dout.ptr, seeds_ptr=dseeds.ptr,
+                                      y_body_ptr=dy.ptr if v != M.NO_PACKING else 0, seeded=True)
+        out = dout.download(np.zeros((n, per), dtype=np.uint32))
+    for i in range(n):
+        assert st[i] == M.OK and not isinstance(res[i], M.PirError), i
+        if v == M.NO_PACKING:
+            got = np.concatenate([res[i].ciphertext.reshape(-1), res[i].column_ciphertexts.reshape(-1)])
+        else:
+            got = res[i].ciphertext.reshape(-1)
+        assert np.array_equal(out[i].astype(np.uint64), got), i
+    for i in (0, n // 2, n - 1):
+        full = o.seeded_rgsw_expand(qs[i].seeds, qs[i].rgsw)
+        if v == M.NO_PACKING:
+            ct, colct = o.respond(shards[qs[i].shard_id], full)
+            assert np.array_equal(res[i].ciphertext, ct), i
+        else:
+            _, ct = o.respond_inspiring(W_SEED, shards[qs[i].shard_id], full, y_body=yb)
+            assert np.array_equal(res[i].ciphertext, ct), i
