@@ -29,6 +29,8 @@
 // the transforms.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "kernels.hpp"
 #include "ntt.hpp"
 #include "pack.hpp"
@@ -101,19 +103,19 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
         // NTT of this query's y_body (both halves, one residue each), into
         // LDS at the slot positions in Montgomery form
         __syncthreads();
-        // DB polys at a time through the DB exchange buffers
+        // DB polys at a time through the DB exchange buffers, the last group
+        // only as many as are left
+        auto y_group = [&](auto cnt, int j0) {
+            constexpr int NP = decltype(cnt)::value;
+            uint32_t yv[NP][V];
 #pragma unroll
-        for (int j0 = 0; j0 < ELL; j0 += DB) {
-            uint32_t yv[DB][V];
+            for (int j = 0; j < NP; j++)
 #pragma unroll
-            for (int j = 0; j < DB; j++)
-#pragma unroll
-                for (int e = 0; e < V; e++) yv[j][e] = j0 + j < ELL ? Y[((j0 + j) * 2 + k) * N + t + e * T] : 0u;
+                for (int e = 0; e < V; e++) yv[j][e] = Y[((j0 + j) * 2 + k) * N + t + e * T];
             if (j0 > 0) lds_barrier();  // every thread is past the previous transform's reads
-            ntt_v<LOGN, V, DB>(yv, t, db, tw, M);
+            ntt_v<LOGN, V, NP>(yv, t, db, tw, M);
 #pragma unroll
-            for (int j = 0; j < DB; j++) {
-                if (j0 + j >= ELL) break;
+            for (int j = 0; j < NP; j++)
 #pragma unroll
                 for (int g = 0; g < V / 4; g++) {
                     uint32_t v[4];
@@ -123,8 +125,10 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
                     *reinterpret_cast<uint4*>(&yl[((j0 + j) * 2 + k) * N + V * t + 4 * g]) =
                         make_uint4(v[0], v[1], v[2], v[3]);
                 }
-            }
-        }
+        };
+#pragma unroll
+        for (int j0 = 0; j0 + DB <= ELL; j0 += DB) y_group(std::integral_constant<int, DB>{}, j0);
+        if constexpr (ELL % DB != 0) y_group(std::integral_constant<int, ELL % DB>{}, ELL - ELL % DB);
     } else if (ymode == 0) {
         for (int e = tid; e < ELL * 2 * N / 4; e += 2 * T)
             reinterpret_cast<uint4*>(yl)[e] = reinterpret_cast<const uint4*>(Y)[e];
