@@ -88,7 +88,10 @@ struct Database {
 // OfflinePackingKeys, inspiring2.rs:107-182, 448-506), resident.
 struct PackTables {
     uint32_t gamma = 0;
-    DevBuf step1;      // [gamma][gamma][K][N] u32: gamma^-1 * NTT(X^e_ij), Montgomery form
+    DevBuf stepu;      // [gamma][K][N] u32: w_s^(g^(n-i)) (step 1 by Horner), Montgomery form;
+                       // built only when the fused front end does not apply
+    uint32_t ginvm[2] = {0, 0};  // gamma^-1 2^32 mod q_k
+    bool odd_gens = true;      // every g^i (i < gamma - 1) odd: rotations of y_body are NTT-slot permutations
     DevBuf perm;       // [gamma][N] u32: automorphism table for g^i
     DevBuf w_all;      // [(gamma-1)][ell][K][N] u32 NTT domain, Montgomery form
     std::vector<uint32_t> gen_pows;  // g^i mod 2n, i < n

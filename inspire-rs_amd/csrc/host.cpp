@@ -327,7 +327,7 @@ uint64_t inspire_ctx_device_bytes(const inspire_ctx* ctx) {
                  ctx->ws4.bytes() + ctx->ws5.bytes() + ctx->ws6.bytes() + ctx->galois.bytes() + ctx->yhat.bytes();
     for (auto& kv : ctx->dbs) b += kv.second.digits.bytes();
     std::lock_guard<std::mutex> lk(const_cast<inspire_ctx*>(ctx)->packs_mu);
-    for (auto& kv : ctx->packs) b += kv.second->step1.bytes() + kv.second->perm.bytes() + kv.second->w_all.bytes();
+    for (auto& kv : ctx->packs) b += kv.second->stepu.bytes() + kv.second->perm.bytes() + kv.second->w_all.bytes();
     return b;
 }
 

@@ -140,6 +140,11 @@ PackTables& pack_tables(inspire_ctx* ctx, uint32_t gamma) {
     if (it != packs.end()) return *it->second;
     const uint32_t n = ctx->N, K = ctx->K, ell = ctx->ell, logn = ctx->logn;
     const size_t KN = (size_t)K * n;
+    // PackParams::new keeps n generator powers and packing_offline indexes them
+    // by i < num_to_pack (inspiring2.rs:122-127, 926-1041): gamma <= n
+    if (gamma == 0 || gamma > n)
+        throw Error{INSPIRE_ERR_UNSUPPORTED, "InspiRING packs between 1 and d column LWEs (got " +
+                                                 std::to_string(gamma) + " columns at d = " + std::to_string(n) + ")"};
     std::unique_ptr<PackTables> T(new PackTables());
     T->gamma = gamma;
     const GenInfo G = gen_info(n, gamma);
@@ -152,22 +157,10 @@ PackTables& pack_tables(inspire_ctx* ctx, uint32_t gamma) {
             perm[(size_t)i * n + s] = bitrev((uint32_t)((e - 1) / 2), logn);
         }
     }
-    // step-1 table: gamma^-1 * NTT(X^e_ij), e_ij = j * g^((n-i) mod n) mod 2n, Montgomery form
     const uint64_t ginv = mod_inverse_u64(gamma, ctx->q);
-    std::vector<uint32_t> step1((size_t)gamma * gamma * KN);
     for (uint32_t k = 0; k < K; k++) {
-        const uint64_t q = ctx->hm[k].q, psi = ctx->hm[k].psi;
-        const uint64_t scale = mulmod64(ginv % q, (1ull << 32) % q, q);
-        for (uint32_t s = 0; s < n; s++) {
-            const uint64_t w = powmod64(psi, 2ull * bitrev(s, logn) + 1, q);  // NTT slot s evaluates at w
-            for (uint32_t i = 0; i < gamma; i++) {
-                const uint64_t gp = G.gen_pows[(n - i) % n];
-                for (uint32_t j = 0; j < gamma; j++) {
-                    const uint64_t e = ((uint64_t)j * gp) % (2ull * n);
-                    step1[(((size_t)i * gamma + j) * K + k) * n + s] = (uint32_t)mulmod64(powmod64(w, e, q), scale, q);
-                }
-            }
-        }
+        const uint64_t q = ctx->hm[k].q;
+        T->ginvm[k] = (uint32_t)mulmod64(ginv % q, (1ull << 32) % q, q);
     }
     // w_mask (inspiring2.rs:816-835) -> NTT on the device -> rotations with the
     // reference's residue handling -> Montgomery form.
@@ -248,13 +241,28 @@ PackTables& pack_tables(inspire_ctx* ctx, uint32_t gamma) {
             hip_check(hipMemcpy(T->dftw.get(), dftw.data(), dftw.size() * 4, hipMemcpyHostToDevice), "dftw");
         }
     }
+    if (!T->fast_front) {
+        // step 1 by Horner's rule: u[i][k][s] = w_s^(g^((n-i) mod n) mod 2n), Montgomery form
+        // (NTT(X^e)[s] = w_s^e, and e_ij = j g^(n-i) mod 2n is j times that exponent mod 2n)
+        std::vector<uint32_t> stepu((size_t)gamma * KN);
+        for (uint32_t k = 0; k < K; k++) {
+            const uint64_t q = ctx->hm[k].q, psi = ctx->hm[k].psi, r32 = (1ull << 32) % q;
+            for (uint32_t s = 0; s < n; s++) {
+                const uint64_t w = powmod64(psi, 2ull * bitrev(s, logn) + 1, q);  // NTT slot s evaluates at w
+                for (uint32_t i = 0; i < gamma; i++)
+                    stepu[((size_t)i * K + k) * n + s] =
+                        (uint32_t)mulmod64(powmod64(w, G.gen_pows[(n - i) % n], q), r32, q);
+            }
+        }
+        T->stepu.alloc(stepu.size() * 4);
+        hip_check(hipMemcpy(T->stepu.get(), stepu.data(), stepu.size() * 4, hipMemcpyHostToDevice), "step1 u");
+    }
     T->perm.alloc(perm.size() * 4);
-    T->step1.alloc(step1.size() * 4);
     T->w_all.alloc(w_all.size() * 4);
     hip_check(hipMemcpy(T->perm.get(), perm.data(), perm.size() * 4, hipMemcpyHostToDevice), "perm");
-    hip_check(hipMemcpy(T->step1.get(), step1.data(), step1.size() * 4, hipMemcpyHostToDevice), "step1");
     hip_check(hipMemcpy(T->w_all.get(), w_all.data(), w_all.size() * 4, hipMemcpyHostToDevice), "w_all");
     T->gen_pows = G.gen_pows;
+    for (uint32_t i = 0; i + 1 < gamma; i++) T->odd_gens &= (G.gen_pows[i] & 1) != 0;
     auto& ref = *T;
     packs[gamma] = std::move(T);
     return ref;
@@ -296,7 +304,8 @@ void pack_run(inspire_ctx* ctx, uint32_t gamma, uint32_t Q, const uint32_t* rows
         }
         ProfScope ps(ctx, P_STEP1);
         const uint32_t kres = (rw || K == 1) ? K : 1;  // residue 1 is rebuilt from residue 0 otherwise
-        hip_check(launch_pack_step1(d_cols, T.step1.get<uint32_t>(), d_rpre, Q, gamma, kres, ctx->dev, ctx->stream),
+        hip_check(launch_pack_step1(d_cols, T.stepu.get<uint32_t>(), d_rpre, Q, gamma, kres,
+                                    make_uint2(T.ginvm[0], T.ginvm[1]), ctx->dev, ctx->stream),
                   "step1");
     }
     PackArgs A{};
@@ -367,9 +376,7 @@ void pack_run_device_ybody(inspire_ctx* ctx, uint32_t gamma, uint32_t Q, const u
     const uint32_t ell = ctx->ell;
     const size_t KN = (size_t)ctx->K * ctx->N;
     PackTables& T = pack_tables(ctx, gamma);
-    for (uint32_t i = 0; i + 1 < gamma; i++)
-        if (!(T.gen_pows[i] & 1))
-            throw Error{INSPIRE_ERR_UNSUPPORTED, "device batches need an odd InspiRING generator (gamma | 2d)"};
+    if (!T.odd_gens) throw Error{INSPIRE_ERR_UNSUPPORTED, kEvenGeneratorMsg};
     // y layout: query t at t*ell*K*N, rotated from y_body, gamma-1 rotations (the kernel's default)
     if (pack_chain_crt_supported(ctx->dev)) {  // the chain kernel transforms y_body itself
         pack_run(ctx, gamma, Q, rows, d_qdig, y_body, nullptr, nullptr, nullptr, d_out, true);
@@ -395,8 +402,7 @@ void pack_respond_host(inspire_ctx* ctx, uint32_t gamma, const BatchItem* items,
     const size_t KN = (size_t)K * ctx->N;
     PackTables& T = pack_tables(ctx, gamma);
     const uint32_t nrot = gamma - 1;
-    bool odd_gens = true;
-    for (uint32_t i = 0; i < nrot; i++) odd_gens &= (T.gen_pows[i] & 1) != 0;
+    const bool odd_gens = T.odd_gens;
 
     // ---- client keys: y_body (rotated on the device) or explicit y_all
     std::vector<uint64_t> y_off(Q);

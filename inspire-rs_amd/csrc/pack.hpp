@@ -51,6 +51,15 @@ struct PackArgs {
     uint32_t rmul[2], smul[2];
 };
 
+// Keys given as y_body alone are rotated in the NTT domain by the slot
+// permutation of tau_{g^i}; generate_rotations (inspiring2.rs:1309-1323)
+// applies the coefficient map of g^i, which is that permutation only for odd
+// g^i.  Paths that rotate y_body on the device refuse even generators with
+// this message (the host-buffer API computes those rotations as written).
+constexpr const char* kEvenGeneratorMsg =
+    "y_body keys rotated on the device need an odd InspiRING generator (gamma | 2d); "
+    "send y_all or use the host-buffer API for this column count";
+
 // Builds (once per gamma) the resident PackParams/OfflinePackingKeys tables.
 PackTables& pack_tables(inspire_ctx* ctx, uint32_t gamma);
 
@@ -77,8 +86,9 @@ hipError_t launch_pack_b0(const uint32_t* cols, uint64_t* bpart, uint32_t Q, uin
 hipError_t launch_pack_front(const uint64_t* qdig, const uint32_t* rows, uint32_t* r_pre, uint64_t* bpart,
                              const uint32_t* twist, const uint2* dftw, uint32_t Q, uint32_t gamma, bool rw,
                              const CtxDev& C, hipStream_t st);
-hipError_t launch_pack_step1(const uint32_t* cols, const uint32_t* step1, uint32_t* r_pre, uint32_t Q,
-                             uint32_t gamma, uint32_t kres, const CtxDev& C, hipStream_t st);
+// stepu: [gamma][K][N] u = w_s^(g^(n-i)) in Montgomery form; ginvm: gamma^-1 2^32 mod q_0, q_1.
+hipError_t launch_pack_step1(const uint32_t* cols, const uint32_t* stepu, uint32_t* r_pre, uint32_t Q,
+                             uint32_t gamma, uint32_t kres, uint2 ginvm, const CtxDev& C, hipStream_t st);
 // Two-modulus packing chain with every thread on both residues
 // (pack_chain.hip), for d <= 2048 and ell <= 3; launch_pack_chain_scaled
 // uses it whenever pack_chain_crt_supported holds.

@@ -195,12 +195,16 @@ k_pack_front(const uint64_t* __restrict__ qdig, const uint32_t* __restrict__ row
 }
 
 // Step 1 before the automorphism:
-//   r_pre[q][i][k][s] = sum_j step1[i][j][k][s] * ahat_j[k][s]
-// step1 = gamma^-1 NTT(X^(j g^(n-i))) in Montgomery form; kres residues (1
-// when the automorphism rebuilds residue 1 from residue 0).
+//   r_pre[q][i][k][s] = gamma^-1 sum_j NTT(X^(j g^(n-i)))[s] ahat_j[k][s]
+// NTT(X^e)[s] = w_s^e (w_s the slot's evaluation point, w_s^(2n) = 1), so with
+// u = w_s^(g^(n-i)) the sum is the polynomial sum_j ahat_j u^j, evaluated by
+// Horner's rule: gamma - 1 products per (i, s) from one table word u
+// (stepu [gamma][K][N], Montgomery form), where a table of every monomial
+// would hold gamma^2 words per slot.  kres residues (1 when the automorphism
+// rebuilds residue 1 from residue 0).
 __global__ void __launch_bounds__(256)
-k_pack_step1(const uint32_t* __restrict__ cols, const uint32_t* __restrict__ step1, uint32_t* __restrict__ r_pre,
-             uint32_t Q, uint32_t gamma, uint32_t kres, CtxDev C) {
+k_pack_step1(const uint32_t* __restrict__ cols, const uint32_t* __restrict__ stepu, uint32_t* __restrict__ r_pre,
+             uint32_t Q, uint32_t gamma, uint32_t kres, uint2 ginvm, CtxDev C) {
     const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= (size_t)Q * kres * C.n) return;
     const uint32_t s = (uint32_t)(gid % C.n);
@@ -208,18 +212,15 @@ k_pack_step1(const uint32_t* __restrict__ cols, const uint32_t* __restrict__ ste
     const uint32_t qi = (uint32_t)(gid / ((size_t)C.n * kres));
     const size_t KN = (size_t)C.K * C.n;
     const uint32_t q = C.m[k].q, qinv = C.m[k].qinv;
+    const uint32_t gm = k ? ginvm.y : ginvm.x;  // gamma^-1 2^32 mod q_k
     const uint32_t* a = cols + (size_t)qi * 2 * gamma * KN + (size_t)k * C.n + s;
     uint32_t* out = r_pre + (size_t)qi * gamma * KN + (size_t)k * C.n + s;
     for (uint32_t i = 0; i < gamma; i++) {
-        const uint32_t* m = step1 + (size_t)i * gamma * KN + (size_t)k * C.n + s;
-        uint32_t sum = 0;
-        for (uint32_t j0 = 0; j0 < gamma; j0 += 4) {
-            uint64_t acc = 0;
-            const uint32_t je = j0 + 4 < gamma ? j0 + 4 : gamma;
-            for (uint32_t j = j0; j < je; j++) acc += (uint64_t)m[(size_t)j * KN] * a[(size_t)2 * j * KN];
-            sum = csub(sum + csub(mont_reduce(acc, q, qinv), q), q);
-        }
-        out[(size_t)i * KN] = sum;
+        const uint32_t u = stepu[(size_t)i * KN + (size_t)k * C.n + s];
+        // acc < 3q < 2^32 throughout, so acc * u < q 2^32 for the reduction
+        uint32_t acc = a[(size_t)2 * (gamma - 1) * KN];
+        for (uint32_t j = gamma - 1; j-- > 0;) acc = mont_reduce((uint64_t)acc * u, q, qinv) + a[(size_t)2 * j * KN];
+        out[(size_t)i * KN] = csub(mont_reduce((uint64_t)acc * gm, q, qinv), q);
     }
 }
 
@@ -438,11 +439,12 @@ hipError_t launch_pack_front(const uint64_t* qdig, const uint32_t* rows, uint32_
     }
 }
 
-hipError_t launch_pack_step1(const uint32_t* cols, const uint32_t* step1, uint32_t* r_pre, uint32_t Q,
-                             uint32_t gamma, uint32_t kres, const CtxDev& C, hipStream_t st) {
+hipError_t launch_pack_step1(const uint32_t* cols, const uint32_t* stepu, uint32_t* r_pre, uint32_t Q,
+                             uint32_t gamma, uint32_t kres, uint2 ginvm, const CtxDev& C, hipStream_t st) {
     const size_t total = (size_t)Q * kres * C.n;
     if (!total) return hipSuccess;
-    hipLaunchKernelGGL(k_pack_step1, dim3(nblk(total, 256)), dim3(256), 0, st, cols, step1, r_pre, Q, gamma, kres, C);
+    hipLaunchKernelGGL(k_pack_step1, dim3(nblk(total, 256)), dim3(256), 0, st, cols, stepu, r_pre, Q, gamma, kres,
+                       ginvm, C);
     return hipGetLastError();
 }
 

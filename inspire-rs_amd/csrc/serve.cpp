@@ -342,15 +342,21 @@ void resolve(inspire_server* s, SReq& r) {
         r.err = "the serving loop needs a database with columns";
         return;
     }
-    bool have_tables = true;
     if (r.packed()) {
+        // the lanes rotate the registered y_body by the NTT-slot permutation of
+        // tau_{g^i}, which is generate_rotations only for odd g^i
         std::lock_guard<std::mutex> lk(ctx->packs_mu);
-        have_tables = ctx->packs.count(db.cols) != 0;
-    }
-    if (!have_tables) {
-        r.status = INSPIRE_ERR_UNSUPPORTED;
-        r.err = "no packing tables for this database (loaded after the server started)";
-        return;
+        auto t = ctx->packs.find(db.cols);
+        if (t == ctx->packs.end()) {
+            r.status = INSPIRE_ERR_UNSUPPORTED;
+            r.err = "no packing tables for this database (loaded after the server started)";
+            return;
+        }
+        if (!t->second->odd_gens) {
+            r.status = INSPIRE_ERR_UNSUPPORTED;
+            r.err = kEvenGeneratorMsg;
+            return;
+        }
     }
     r.cols = db.cols;
     r.dig = shard_digits(db, (uint32_t)slot);

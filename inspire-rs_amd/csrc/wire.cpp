@@ -64,6 +64,9 @@ WireParams wire_params(const inspire_params* p) {
     if (p->crt_count < 1 || p->crt_count > 2) throw Error{INSPIRE_ERR_INVALID_PARAMS, "crt_count must be 1 or 2"};
     if (!p->ring_dim || (p->ring_dim & (p->ring_dim - 1)) || p->ring_dim > (1u << 20))
         throw Error{INSPIRE_ERR_INVALID_PARAMS, "ring_dim must be a power of two"};
+    // the readers index rows[0] and size every row array by 2 * gadget_len
+    if (p->gadget_len == 0 || p->gadget_len > (uint32_t)kMaxEll)
+        throw Error{INSPIRE_ERR_UNSUPPORTED, "gadget_len must be in [1, 4]"};
     WireParams w{};
     w.K = p->crt_count;
     w.N = p->ring_dim;

@@ -358,3 +358,61 @@ def test_host_batch_many_queries_matches_device_batch(M, variant, n):
         else:
             _, ct = o.respond_inspiring(W_SEED, shards[qs[i].shard_id], full, y_body=yb)
             assert np.array_equal(res[i].ciphertext, ct), i
+
+
+@pytest.mark.parametrize("entry_size", [6, 34, 256])
+def test_inspiring_general_column_counts(M, entry_size):
+    """Column counts off the fused front end's shape, through the host-buffer
+    batch API: gamma = 3 (generator 1366, even: the y_body rotations are
+    coefficient maps computed as generate_rotations writes them), 17 (odd
+    generator, not a power of two) and 128 (gamma^2 does not divide 2d).
+    These run the general first dimension, b sums and Horner step 1.  y_body,
+    y_all and a seeded query, each against the oracle; the device-resident
+    batch matches for the odd generators and refuses the even one."""
+    o = O.Oracle(O.baseline_params())
+    d, ell = o.params.d, o.params.gadget_len
+    db = random_db(d + 40, entry_size, seed=entry_size)
+    shards = encode_db(o, db, entry_size)
+    gamma = shards.shape[1]
+    say("inspiring general gamma", gamma)
+    cl = Client(o, 40 + entry_size)
+    yb, ya = cl.packing_keys(gamma, W_SEED)
+    rows = cl.query_rows(d // 5)
+    seeds, b = cl.seeded_query(33)
+    full_seeded = o.seeded_rgsw_expand(seeds, b)
+    cases = [
+        (0, M.ClientQuery(shard_id=0, rgsw=rows, has_packing_keys=True, y_body=yb), rows, None),
+        (1, M.ClientQuery(shard_id=1, rgsw=rows, has_packing_keys=True, y_body=yb, y_all=ya), rows, ya),
+        (0, M.ClientQuery(shard_id=0, rgsw=b, seeds=seeds, has_packing_keys=True, y_body=yb), full_seeded, None),
+    ]
+    wants = []
+    for sh, _, rg, y_all in cases:
+        rc, ct = o.respond_inspiring(W_SEED, shards[sh], rg, y_body=yb, y_all=y_all)
+        assert rc == 0
+        wants.append(ct)
+    with make_ctx(M, o.params) as ctx:
+        ctx.load_database(0, shards)
+        res = ctx.respond_batch([0] * len(cases), [c[1] for c in cases], [M.TWO_PACKING] * len(cases))
+        for r, want in zip(res, wants):
+            assert not isinstance(r, M.PirError), r
+            assert (r.ciphertext == want).all(), gamma
+        # device-resident batch of the seeded query (y_body rotated on the device)
+        drows = M.DeviceArray.from_numpy(np.ascontiguousarray(b[ell:], dtype=np.uint32))
+        dseeds = M.DeviceArray.from_numpy(np.ascontiguousarray(seeds[ell:]).view(np.uint32))
+        dy = M.DeviceArray.from_numpy(yb.astype(np.uint32))
+        dout = M.DeviceArray(2 * 2 * d * 4)
+        try:
+            if gamma == 3:
+                with pytest.raises(M.PirError) as e:
+                    ctx.respond_batch_device(0, M.TWO_PACKING, [0], drows.ptr, dout.ptr, seeds_ptr=dseeds.ptr,
+                                             y_body_ptr=dy.ptr)
+                assert e.value.code == M.ERR_UNSUPPORTED
+            else:
+                st = ctx.respond_batch_device(0, M.TWO_PACKING, [0], drows.ptr, dout.ptr, seeds_ptr=dseeds.ptr,
+                                              y_body_ptr=dy.ptr)
+                assert (st == 0).all()
+                got = dout.download(np.zeros((2, 2, d), dtype=np.uint32))
+                assert (got == wants[2]).all(), gamma
+        finally:
+            for a in (drows, dseeds, dy, dout):
+                a.free()

@@ -174,3 +174,52 @@ def test_serving_errors_match_reference(M, world):
         check(M, q, srv.wait(int(t[3])))
         with pytest.raises(M.PirError):
             srv.wait(int(t[3]))  # released
+
+
+def test_serving_other_column_counts(M):
+    """Databases whose column count is off the fused front end's shape:
+    entry sizes 34 (gamma 17) and 256 (gamma 128) serve packed queries equal
+    to the oracle's; entry size 6 (gamma 3) has an even generator, whose
+    rotations of the registered y_body are not NTT-slot permutations, so its
+    packed queries are refused (INSPIRE_ERR_UNSUPPORTED) while its InsPIRe^0
+    queries are served."""
+    o = O.Oracle(O.baseline_params())
+    d, ell = o.params.d, o.params.gadget_len
+    sizes = {0: 34, 1: 256, 2: 6}
+    dbs = {k: random_db(d + 3, es, seed=100 + k) for k, es in sizes.items()}
+    shards = {k: encode_db(o, v, sizes[k]) for k, v in dbs.items()}
+    cl = Client(o, 120)
+    yb = cl.packing_keys(16, W_SEED, with_y_all=False)[0]
+    p = o.params
+    reqs, wants = [], []
+    for k in sizes:
+        for variant, sh, local in ((2, 1, 5), (0, 0, 9)):
+            seeds, b = cl.seeded_query(local)
+            full = o.seeded_rgsw_expand(seeds, b)
+            reqs.append(M.Request(k, sh, variant, rows=b[ell:].astype(np.uint32), seeds=seeds[ell:], client_id=3))
+            if variant == 0:
+                wants.append(o.respond(shards[k][sh], full))
+            elif sizes[k] == 6:
+                wants.append(None)
+            else:
+                rc, ct = o.respond_inspiring(W_SEED, shards[k][sh], full, y_body=yb)
+                assert rc == 0
+                wants.append((ct, None))
+    say("other column counts: oracle answers ready")
+    ctx = M.Context(M.InspireParams(ring_dim=p.d, crt_moduli=p.moduli, p=p.p, gadget_base=p.gadget_base,
+                                    gadget_len=p.gadget_len), W_SEED)
+    with ctx:
+        for k, v in dbs.items():
+            ctx.load_entries(k, v, sizes[k])
+        ctx.register_client_keys(3, yb)
+        with M.Server(ctx, lanes=2, max_batch=4) as srv:
+            tickets = srv.submit(reqs)
+            for r, t, want in zip(reqs, tickets, wants):
+                st, res, _ = srv.wait(int(t), cols=128)
+                if want is None:
+                    assert st == M.ERR_UNSUPPORTED
+                    continue
+                assert st == M.OK
+                assert (res.ciphertext == want[0]).all()
+                if want[1] is not None:
+                    assert (res.column_ciphertexts == want[1]).all()

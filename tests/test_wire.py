@@ -225,3 +225,24 @@ def test_single_modulus_poly_has_no_crt_inverse():
     o = W.read_bin_response(b)
     assert o["ciphertext"]["a"]["crt_q0_inv_mod_q1"] == 0 and o["ciphertext"]["a"]["q"] == Q0
     assert b == W.bin_response(W.response_obj(ct[0], ct[1], [], (Q0,), 256))
+
+
+@pytest.mark.parametrize("ell", [0, 5])
+def test_decoders_refuse_gadget_len_out_of_range(P, ell):
+    """A parameter set with gadget_len outside [1, 4] is refused before any
+    row is read: with gadget_len = 0 an empty row array would otherwise pass
+    the 2 x gadget_len row count check."""
+    bad = M.InspireParams(gadget_len=ell)
+    empty = json.dumps({"shard_id": 1, "rgsw_ciphertext": {"rows": [], "gadget": {"base": 1 << 20, "len": ell,
+                                                                                  "q": Q0 * Q1}}}).encode()
+    with pytest.raises(M.PirError) as e:
+        M.query_from_json(bad, empty)
+    assert e.value.code == M.ERR_UNSUPPORTED
+    b = np.zeros(8 + 8, dtype=np.uint8).tobytes()  # shard_id, then an empty row vector
+    with pytest.raises(M.PirError) as e:
+        M.query_from_binary(bad, b, False)
+    assert e.value.code == M.ERR_UNSUPPORTED
+    g = np.random.default_rng(13)
+    with pytest.raises(M.PirError) as e:
+        M.response_to_binary(bad, M.ServerResponse(rnd_polys(g, 2), np.zeros((0, 2, K, D), np.uint64), D))
+    assert e.value.code == M.ERR_UNSUPPORTED
