@@ -109,6 +109,7 @@ extern const char* const kProfNames[P_COUNT];
 
 struct Profiler {
     bool on = false;
+    bool suspended = false;  // no events recorded (the no-allocation entry point)
     uint32_t mask = ~0u;  // kernels (bit = ProfId) bracketed while on
     struct Span { int id; hipEvent_t a, b; };
     std::vector<Span> spans;
@@ -138,6 +139,7 @@ struct inspire_ctx {
     uint64_t q = 0, delta = 0;
     uint8_t w_seed[32]{};
     inspire::HostModulus hm[inspire::kMaxK];
+    int device = 0;           // the HIP device current at creation (owns the stream and tables)
     inspire::DevBuf tables;   // twiddles of every modulus
     inspire::CtxDev dev{};
     hipStream_t stream = nullptr;
@@ -199,6 +201,23 @@ class ProfLaunch {
 };
 // Folds finished spans into the totals (synchronises the stream).
 void prof_collect(inspire_ctx* ctx);
+
+// Shared hold on the owner's resident state for a call on a lane (the owner's
+// own calls are serialised by its mutex; writers hold the lock exclusively).
+struct LaneRead {
+    std::shared_lock<std::shared_mutex> lk;
+    explicit LaneRead(inspire_ctx* c) {
+        if (c->owner) lk = std::shared_lock<std::shared_mutex>(c->owner->rw);
+    }
+};
+
+// Records no profiling events while in scope.
+struct ProfSuspend {
+    inspire_ctx* ctx;
+    bool was;
+    explicit ProfSuspend(inspire_ctx* c) : ctx(c), was(c->prof.suspended) { c->prof.suspended = true; }
+    ~ProfSuspend() { ctx->prof.suspended = was; }
+};
 
 // A lane of `owner`: its own stream and workspaces over the owner's resident
 // state (databases, tables, keys).

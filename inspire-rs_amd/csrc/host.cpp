@@ -155,6 +155,7 @@ inspire_ctx* make_lane_ctx(inspire_ctx* owner) {
     std::memcpy(c->w_seed, owner->w_seed, 32);
     for (uint32_t k = 0; k < owner->K; k++) c->hm[k] = owner->hm[k];
     c->dev = owner->dev;  // device tables belong to the owner
+    c->device = owner->device;
     c->owner = owner;
     hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "lane stream");
     return c.release();
@@ -166,13 +167,6 @@ namespace {
 
 // A lane reads its owner's databases, tables and keys: it holds the owner's
 // reader lock for the whole call (loads and registrations hold it exclusively).
-struct LaneRead {
-    std::shared_lock<std::shared_mutex> lk;
-    explicit LaneRead(inspire_ctx* c) {
-        if (c->owner) lk = std::shared_lock<std::shared_mutex>(c->owner->rw);
-    }
-};
-
 void owner_only(const inspire_ctx* ctx, const char* what) {
     if (ctx->owner)
         throw Error{INSPIRE_ERR_INVALID_ARGUMENT, std::string(what) + " goes through the lane's owner context"};
@@ -245,6 +239,7 @@ int inspire_ctx_create(const inspire_params* params, const uint8_t w_seed[32], i
             throw Error{INSPIRE_ERR_DEVICE, "no HIP device available"};
         }
         std::unique_ptr<inspire_ctx> ctx(new inspire_ctx());
+        hip_check(hipGetDevice(&ctx->device), "current device");
         ctx->params = *params;
         ctx->N = params->ring_dim;
         ctx->K = params->crt_count;

@@ -268,31 +268,55 @@ PackTables& pack_tables(inspire_ctx* ctx, uint32_t gamma) {
     return ref;
 }
 
+PackTables* find_pack_tables(inspire_ctx* ctx, uint32_t gamma) {
+    std::lock_guard<std::mutex> lk(root(ctx)->packs_mu);
+    auto& packs = root(ctx)->packs;
+    auto it = packs.find(gamma);
+    return it == packs.end() ? nullptr : it->second.get();
+}
+
+PackWsSizes pack_ws_sizes(const inspire_ctx* ctx, const PackTables& T, uint32_t Q) {
+    const size_t KN = (size_t)ctx->K * ctx->N;
+    const uint32_t nparts = T.fast_front ? ctx->N / 64 : 1;  // the fused front writes one b sum per wave
+    PackWsSizes z;
+    z.rpre = (size_t)Q * T.gamma * KN;
+    z.bpart = (size_t)Q * nparts * T.gamma * ctx->K;
+    z.cols = T.fast_front ? 0 : (size_t)Q * 2 * T.gamma * KN;
+    return z;
+}
+
+bool pack_ybody_in_chain(const inspire_ctx* ctx) { return pack_chain_crt_supported(ctx->dev); }
+
 // Device-only packing of a staged group: first dimension, b_poly, step 1 and
 // the chain.  y: NTT-domain (Montgomery) client keys, per query y_off/y_mode/
 // y_count (device arrays).  out: [Q][2][K][N].
 void pack_run(inspire_ctx* ctx, uint32_t gamma, uint32_t Q, const uint32_t* rows, const uint64_t* d_qdig,
               const uint32_t* d_y, const uint64_t* d_yoff, const uint32_t* d_ymode, const uint32_t* d_ycount,
-              uint32_t* d_out, bool y_coeff) {
+              uint32_t* d_out, bool y_coeff, const PackWs& ws) {
     const uint32_t K = ctx->K;
-    const size_t KN = (size_t)K * ctx->N;
     PackTables& T = pack_tables(ctx, gamma);
     const bool rw = (ctx->params.flags & INSPIRE_FLAG_RESIDUE_WISE_AUTOMORPHISM) != 0;
-    const size_t rpre_words = (size_t)Q * gamma * KN;
-    const uint32_t nparts = T.fast_front ? ctx->N / 64 : 1;  // the front writes one b sum per wave
-    const size_t bpart_words = (size_t)Q * nparts * gamma * K * 2;  // u64
-    ctx->ws3.reserve((rpre_words + bpart_words) * 4);
-    uint32_t* d_rpre = ctx->ws3.get<uint32_t>();
-    uint64_t* d_bpart = reinterpret_cast<uint64_t*>(d_rpre + rpre_words + (rpre_words & 1));
+    const PackWsSizes z = pack_ws_sizes(ctx, T, Q);
+    const uint32_t nparts = T.fast_front ? ctx->N / 64 : 1;
+    uint32_t* d_rpre = ws.rpre;
+    uint64_t* d_bpart = ws.bpart;
+    if (!d_rpre || !d_bpart) {
+        const size_t rpre_words = z.rpre + (z.rpre & 1);
+        ctx->ws3.reserve((rpre_words + 2 * z.bpart) * 4);
+        d_rpre = ctx->ws3.get<uint32_t>();
+        d_bpart = reinterpret_cast<uint64_t*>(d_rpre + rpre_words);
+    }
     if (T.fast_front) {
         ProfScope ps(ctx, P_FRONT);
         hip_check(launch_pack_front(d_qdig, rows, d_rpre, d_bpart, T.twist.get<uint32_t>(), T.dftw.get<uint2>(), Q,
                                     gamma, rw || K == 1, ctx->dev, ctx->stream),
                   "pack front");
     } else {
-        const size_t cols_words = (size_t)Q * 2 * gamma * KN;
-        ctx->ws2.reserve(cols_words * 4);
-        uint32_t* d_cols = ctx->ws2.get<uint32_t>();
+        uint32_t* d_cols = ws.cols;
+        if (!d_cols) {
+            ctx->ws2.reserve(z.cols * 4);
+            d_cols = ctx->ws2.get<uint32_t>();
+        }
         {
             ProfScope ps(ctx, P_FIRST_DIM);
             hip_check(launch_first_dim(d_qdig, rows, d_cols, Q, gamma, 2 * gamma, 0, ctx->dev, ctx->stream),
@@ -372,18 +396,22 @@ static void upload_meta(inspire_ctx* ctx, uint32_t Q, const std::vector<uint64_t
 }
 
 void pack_run_device_ybody(inspire_ctx* ctx, uint32_t gamma, uint32_t Q, const uint32_t* rows,
-                           const uint64_t* d_qdig, const uint32_t* y_body, uint32_t* d_out) {
+                           const uint64_t* d_qdig, const uint32_t* y_body, uint32_t* d_out, const PackWs& ws,
+                           uint32_t* y_ws) {
     const uint32_t ell = ctx->ell;
     const size_t KN = (size_t)ctx->K * ctx->N;
     PackTables& T = pack_tables(ctx, gamma);
     if (!T.odd_gens) throw Error{INSPIRE_ERR_UNSUPPORTED, kEvenGeneratorMsg};
     // y layout: query t at t*ell*K*N, rotated from y_body, gamma-1 rotations (the kernel's default)
-    if (pack_chain_crt_supported(ctx->dev)) {  // the chain kernel transforms y_body itself
-        pack_run(ctx, gamma, Q, rows, d_qdig, y_body, nullptr, nullptr, nullptr, d_out, true);
+    if (pack_ybody_in_chain(ctx)) {  // the chain kernel transforms y_body itself
+        pack_run(ctx, gamma, Q, rows, d_qdig, y_body, nullptr, nullptr, nullptr, d_out, true, ws);
         return;
     }
-    ctx->ws0.reserve((size_t)Q * ell * KN * 4);
-    uint32_t* d_y = ctx->ws0.get<uint32_t>();
+    uint32_t* d_y = y_ws;
+    if (!d_y) {
+        ctx->ws0.reserve((size_t)Q * ell * KN * 4);
+        d_y = ctx->ws0.get<uint32_t>();
+    }
     uint64_t mont[kMaxK];
     for (uint32_t k = 0; k < ctx->K; k++) mont[k] = (1ull << 32) % ctx->hm[k].q;
     {
@@ -392,7 +420,7 @@ void pack_run_device_ybody(inspire_ctx* ctx, uint32_t gamma, uint32_t Q, const u
                                      ctx->stream),
                   "y ntt");
     }
-    pack_run(ctx, gamma, Q, rows, d_qdig, d_y, nullptr, nullptr, nullptr, d_out, false);
+    pack_run(ctx, gamma, Q, rows, d_qdig, d_y, nullptr, nullptr, nullptr, d_out, false, ws);
 }
 
 void pack_respond_host(inspire_ctx* ctx, uint32_t gamma, const BatchItem* items, uint32_t Q, const uint32_t* rows,

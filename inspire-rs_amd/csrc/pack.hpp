@@ -62,6 +62,21 @@ constexpr const char* kEvenGeneratorMsg =
 
 // Builds (once per gamma) the resident PackParams/OfflinePackingKeys tables.
 PackTables& pack_tables(inspire_ctx* ctx, uint32_t gamma);
+// The tables of gamma when already built, else nullptr (builds nothing).
+PackTables* find_pack_tables(inspire_ctx* ctx, uint32_t gamma);
+
+// Device scratch of a packing batch.  Null members are taken from the
+// context's workspaces (grown on demand).
+struct PackWs {
+    uint32_t* rpre = nullptr;   // [Q][gamma][K][N] step-1 sums before the automorphism
+    uint64_t* bpart = nullptr;  // [Q][nparts][gamma][K] b sums
+    uint32_t* cols = nullptr;   // [Q][2 gamma][K][N] column ciphertexts (general front end only)
+};
+// Words of each PackWs part for Q queries (u32 words; bpart in u64 words).
+struct PackWsSizes {
+    size_t rpre = 0, bpart = 0, cols = 0;
+};
+PackWsSizes pack_ws_sizes(const inspire_ctx* ctx, const PackTables& T, uint32_t Q);
 
 // Device-only packing of a staged group (rows in NTT/Montgomery form):
 // out [Q][2][K][N].  y: NTT-domain Montgomery client keys with per-query
@@ -70,10 +85,16 @@ PackTables& pack_tables(inspire_ctx* ctx, uint32_t gamma);
 // y_coeff: d_y holds coefficient-domain y_body (only when pack_chain_crt_supported).
 void pack_run(inspire_ctx* ctx, uint32_t gamma, uint32_t Q, const uint32_t* rows, const uint64_t* d_qdig,
               const uint32_t* d_y, const uint64_t* d_yoff, const uint32_t* d_ymode, const uint32_t* d_ycount,
-              uint32_t* d_out, bool y_coeff);
-// Device-resident y_body [Q][ell][K][N] (coefficient domain).
+              uint32_t* d_out, bool y_coeff, const PackWs& ws = PackWs{});
+// True when the packing chain transforms coefficient-domain y_body itself;
+// otherwise pack_run_device_ybody needs y_ws for the transformed keys.
+bool pack_ybody_in_chain(const inspire_ctx* ctx);
+// Device-resident y_body [Q][ell][K][N] (coefficient domain).  y_ws: [Q][ell][K][N]
+// scratch for the transformed keys when the chain does not transform them
+// (nullptr: a context workspace).
 void pack_run_device_ybody(inspire_ctx* ctx, uint32_t gamma, uint32_t Q, const uint32_t* rows,
-                           const uint64_t* d_qdig, const uint32_t* y_body, uint32_t* d_out);
+                           const uint64_t* d_qdig, const uint32_t* y_body, uint32_t* d_out,
+                           const PackWs& ws = PackWs{}, uint32_t* y_ws = nullptr);
 // Host-side keys of the queries `items` (y_body or y_all), responses copied back.
 void pack_respond_host(inspire_ctx* ctx, uint32_t gamma, const BatchItem* items, uint32_t Q, const uint32_t* rows,
                        const uint64_t* d_qdig, const uint32_t* d_err, const inspire_query* qs, inspire_response* rs,

@@ -32,7 +32,7 @@ const char* const kProfNames[P_COUNT] = {"rows_ntt", "expand_seeds", "first_dim"
                                          "pack_front", "tree_leaves", "tree_combine", "extract"};
 
 ProfScope::ProfScope(inspire_ctx* ctx, int id) : ctx_(ctx), id_(id) {
-    if (!ctx_->prof.on || !((ctx_->prof.mask >> id) & 1u)) return;
+    if (!ctx_->prof.on || ctx_->prof.suspended || !((ctx_->prof.mask >> id) & 1u)) return;
     auto& P = ctx_->prof;
     if (P.pool.empty()) {
         hipEvent_t e;
@@ -59,7 +59,7 @@ ProfScope::~ProfScope() {
 }
 
 ProfLaunch::ProfLaunch(inspire_ctx* ctx, int id) : ctx_(ctx), id_(id) {
-    if (!ctx_->prof.on || !((ctx_->prof.mask >> id) & 1u)) return;
+    if (!ctx_->prof.on || ctx_->prof.suspended || !((ctx_->prof.mask >> id) & 1u)) return;
     auto& P = ctx_->prof;
     hipEvent_t e[2];
     for (int j = 0; j < 2; j++) {
@@ -611,51 +611,100 @@ void respond_batch(inspire_ctx* ctx, uint32_t n, const inspire_query* qs, const 
 // Respond (device-resident batch)
 // ---------------------------------------------------------------------------
 
-void respond_batch_device(inspire_ctx* ctx, const inspire_device_batch& b) {
-    const size_t KN = (size_t)ctx->K * ctx->N, ell = ctx->ell;
-    const uint32_t Q = b.count;
-    if (!Q) return;
+// The database a device batch reads, after the checks of the entry point.
+static const Database& device_batch_db(inspire_ctx* ctx, const inspire_device_batch& b, bool* tree, bool* packed) {
     if (!b.shard_ids || !b.rows || !b.out || !b.status || (b.seeded && !b.seeds))
         throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "null argument in device batch"};
     if (b.variant > INSPIRE_VARIANT_TWO_PACKING) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "unknown variant"};
     if (b.variant != INSPIRE_VARIANT_NO_PACKING && b.packing_mode > INSPIRE_PACKING_TREE)
         throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "unknown packing mode"};
-    const bool tree = b.variant != INSPIRE_VARIANT_NO_PACKING && b.packing_mode == INSPIRE_PACKING_TREE;
-    const bool packed = b.variant != INSPIRE_VARIANT_NO_PACKING && !tree;  // InspiRING
-    if (packed && !b.y_body)
+    *tree = b.variant != INSPIRE_VARIANT_NO_PACKING && b.packing_mode == INSPIRE_PACKING_TREE;
+    *packed = b.variant != INSPIRE_VARIANT_NO_PACKING && !*tree;  // InspiRING
+    if (*packed && !b.y_body)
         throw Error{INSPIRE_ERR_PACKING_KEYS_MISSING, "InspiRING client packing keys missing from query"};
-    if (tree && !root(ctx)->galois_count)
+    if (*tree && !root(ctx)->galois_count)
         throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "tree packing needs the CRS galois keys (inspire_ctx_set_galois_keys)"};
     auto& dbs = root(ctx)->dbs;
     auto it = dbs.find(b.db_id);
     if (it == dbs.end()) throw Error{INSPIRE_ERR_UNKNOWN_DATABASE, "unknown database " + std::to_string(b.db_id)};
     const Database& db = it->second;
     if (db.cols == 0) throw Error{INSPIRE_ERR_UNSUPPORTED, "device batches need a database with columns"};
-    // slots (a query whose shard is missing gets slot -1 and status SHARD_NOT_FOUND)
-    // per-query digit addresses and the error word, staged in pinned memory
-    const size_t meta = (size_t)Q * 8 + 8;
-    ctx->ws4.reserve(meta + 64);
-    ctx->pin.reserve(meta + 8);
-    uint64_t* d_qdig = ctx->ws4.get<uint64_t>();
+    return db;
+}
+
+DeviceBatchPlan device_batch_plan(inspire_ctx* ctx, const inspire_device_batch& b) {
+    DeviceBatchPlan P{};
+    const Database& db = device_batch_db(ctx, b, &P.tree, &P.packed);
+    P.cols = db.cols;
+    const size_t KN = (size_t)ctx->K * ctx->N, Q = b.count;
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    P.o_qdig = 0;
+    P.o_rows = al((Q + 1) * 8);
+    P.bytes = P.o_rows + al(Q * ctx->ell * 2 * KN * 4);
+    if (P.packed) {
+        P.tables = find_pack_tables(ctx, db.cols);
+        if (P.tables) {
+            const PackWsSizes z = pack_ws_sizes(ctx, *P.tables, b.count);
+            P.o_rpre = P.bytes;
+            P.o_bpart = P.o_rpre + al(z.rpre * 4);
+            P.o_cols = P.o_bpart + al(z.bpart * 8);
+            P.o_y = P.o_cols + al(z.cols * 4);
+            P.bytes = P.o_y + (pack_ybody_in_chain(ctx) ? 0 : al(Q * ctx->ell * KN * 4));
+        }
+    }
+    return P;
+}
+
+void respond_batch_device(inspire_ctx* ctx, const inspire_device_batch& b, uint8_t* scratch, uint64_t* host_meta) {
+    const size_t KN = (size_t)ctx->K * ctx->N, ell = ctx->ell;
+    const uint32_t Q = b.count;
+    if (!Q) return;
+    bool tree = false, packed = false;
+    const Database& db = device_batch_db(ctx, b, &tree, &packed);
+    // per-query digit addresses and the error word, staged on the host (a
+    // missing shard reads shard 0's digits and reports SHARD_NOT_FOUND)
+    DeviceBatchPlan P{};
+    if (scratch) P = device_batch_plan(ctx, b);
+    uint64_t* d_qdig;
+    uint64_t* hdig;
+    uint32_t* rows;
+    PackWs pws;
+    uint32_t* y_ws = nullptr;
+    if (scratch) {
+        d_qdig = reinterpret_cast<uint64_t*>(scratch + P.o_qdig);
+        rows = reinterpret_cast<uint32_t*>(scratch + P.o_rows);
+        hdig = host_meta;
+        if (packed) {
+            pws.rpre = reinterpret_cast<uint32_t*>(scratch + P.o_rpre);
+            pws.bpart = reinterpret_cast<uint64_t*>(scratch + P.o_bpart);
+            pws.cols = reinterpret_cast<uint32_t*>(scratch + P.o_cols);
+            y_ws = reinterpret_cast<uint32_t*>(scratch + P.o_y);
+        }
+    } else {
+        const size_t meta = (size_t)Q * 8 + 8;
+        ctx->ws4.reserve(meta + 64);
+        ctx->pin.reserve(meta + 8);
+        d_qdig = ctx->ws4.get<uint64_t>();
+        hdig = ctx->pin.get<uint64_t>();
+        ctx->ws1.reserve((size_t)Q * ell * 2 * KN * 4);
+        rows = ctx->ws1.get<uint32_t>();
+    }
     uint32_t* d_err = reinterpret_cast<uint32_t*>(d_qdig + Q);
-    uint64_t* hdig = ctx->pin.get<uint64_t>();
     for (uint32_t t = 0; t < Q; t++) {
         const int64_t sl = db.slot(b.shard_ids[t]);
         const bool found = sl >= 0;
-        hdig[t] = shard_digits(db, found ? (uint32_t)sl : 0);  // a missing shard reads shard 0, reported below
+        hdig[t] = shard_digits(db, found ? (uint32_t)sl : 0);
         b.status[t] = found ? INSPIRE_OK : INSPIRE_ERR_SHARD_NOT_FOUND;
     }
     hdig[Q] = 0;  // the error word
-    hip_check(hipMemcpyAsync(d_qdig, hdig, meta, hipMemcpyHostToDevice, ctx->stream), "digits");
+    hip_check(hipMemcpyAsync(d_qdig, hdig, (size_t)Q * 8 + 8, hipMemcpyHostToDevice, ctx->stream), "digits");
     // rows: transformed from the caller's buffers into the workspace layout
-    ctx->ws1.reserve((size_t)Q * ell * 2 * KN * 4);
-    uint32_t* rows = ctx->ws1.get<uint32_t>();
     if (b.seeded) stage_rows_from(ctx, Q, Q, rows, b.seeds, b.rows, nullptr, d_err);
     else stage_rows_from(ctx, Q, 0, rows, nullptr, nullptr, b.rows, d_err);
     if (tree) tree_run(ctx, db.cols, Q, rows, d_qdig, b.out);
     else if (!packed) run_respond0(ctx, db.cols, Q, rows, d_qdig, b.out);
-    else pack_run_device_ybody(ctx, db.cols, Q, rows, d_qdig, b.y_body, b.out);
-    uint32_t* herr = reinterpret_cast<uint32_t*>(ctx->pin.get<uint64_t>() + Q + 1);
+    else pack_run_device_ybody(ctx, db.cols, Q, rows, d_qdig, b.y_body, b.out, pws, y_ws);
+    uint32_t* herr = reinterpret_cast<uint32_t*>(hdig + Q + 1);
     hip_check(hipMemcpyAsync(herr, d_err, 4, hipMemcpyDeviceToHost, ctx->stream), "err");
     hip_check(hipStreamSynchronize(ctx->stream), "device batch");
     prof_collect(ctx);

@@ -56,7 +56,27 @@ uint64_t shard_digits(const Database& db, uint32_t slot);
 void run_respond0(inspire_ctx* ctx, uint32_t cols, uint32_t Q, const uint32_t* rows, const uint64_t* d_qdig,
                   uint32_t* out);
 
-// Same work for a batch whose inputs are resident in device memory.
-void respond_batch_device(inspire_ctx* ctx, const inspire_device_batch& b);
+// Scratch layout of a device-resident batch in one buffer (byte offsets,
+// each part 256-byte aligned): digit addresses and error word, transformed
+// rows, and for InspiRING the packing scratch (PackWs) and transformed keys.
+// tables is null when the InspiRING tables of the column count are not built
+// yet (then the packing parts are not sized).
+struct PackTables;
+struct DeviceBatchPlan {
+    bool tree = false, packed = false;
+    uint32_t cols = 0;
+    const PackTables* tables = nullptr;
+    size_t o_qdig = 0, o_rows = 0, o_rpre = 0, o_bpart = 0, o_cols = 0, o_y = 0, bytes = 0;
+};
+// Checks the batch as the entry point does (throws its errors) and lays out its scratch.
+DeviceBatchPlan device_batch_plan(inspire_ctx* ctx, const inspire_device_batch& b);
+
+// Same work for a batch whose inputs are resident in device memory.  With
+// scratch == nullptr the device scratch comes from the context's workspaces
+// and the host staging from its pinned buffer; otherwise scratch holds
+// device_batch_plan(ctx, b).bytes and host_meta count + 2 words of host
+// memory, and nothing is allocated.
+void respond_batch_device(inspire_ctx* ctx, const inspire_device_batch& b, uint8_t* scratch = nullptr,
+                          uint64_t* host_meta = nullptr);
 
 }  // namespace inspire
