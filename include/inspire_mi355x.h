@@ -227,6 +227,22 @@ typedef struct {
 
 int inspire_respond_batch_device(inspire_ctx *ctx, const inspire_device_batch *batch);
 
+/* inspire_respond_batch_device without any allocation: no hipMalloc,
+ * hipFree or pinned host allocation in the call.  This entry point has no
+ * counterpart in the reference; it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Its device scratch is one 24 MiB array
+ * per GPU, held by the library until the process ends: enough for 64
+ * InsPIRe^2 queries at d = 2048 with 16 columns, or 256 InsPIRe^0 queries;
+ * a batch needing more fails with INSPIRE_ERR_CAPACITY and the size in the
+ * message, and more than 1024 queries are refused the same way.  Tree
+ * packing is refused (INSPIRE_ERR_UNSUPPORTED), and so is an InspiRING batch
+ * whose column count has no tables yet (any allocating respond call on the
+ * context builds them).  Calls on one GPU share the scratch: they are run one
+ * at a time (the call returns when its batch has finished).  No profiling
+ * events are recorded. */
+int inspire_respond_batch_device_noalloc(inspire_ctx *ctx, const inspire_device_batch *batch);
+
 /* ---- serving loop ---- */
 
 /* Registers a client's InspiRING packing keys once: y_body of its

@@ -219,6 +219,10 @@ struct ProfSuspend {
     ~ProfSuspend() { ctx->prof.suspended = was; }
 };
 
+// Looks up the no-allocation entry point's device scratch on `device` (the
+// current device) once per device; called when a context is created.
+void noalloc_scratch_setup(int device);
+
 // A lane of `owner`: its own stream and workspaces over the owner's resident
 // state (databases, tables, keys).
 inspire_ctx* make_lane_ctx(inspire_ctx* owner);

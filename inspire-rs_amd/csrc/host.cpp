@@ -240,6 +240,7 @@ int inspire_ctx_create(const inspire_params* params, const uint8_t w_seed[32], i
         }
         std::unique_ptr<inspire_ctx> ctx(new inspire_ctx());
         hip_check(hipGetDevice(&ctx->device), "current device");
+        noalloc_scratch_setup(ctx->device);
         ctx->params = *params;
         ctx->N = params->ring_dim;
         ctx->K = params->crt_count;

@@ -195,6 +195,7 @@ EXPORTS = [
         [C.c_void_p, C.c_uint32, C.POINTER(_Query), C.POINTER(C.c_uint32), C.POINTER(_Response), C.POINTER(C.c_int32)],
     ),
     ("inspire_respond_batch_device", C.c_int, [C.c_void_p, C.POINTER(_DeviceBatch)]),
+    ("inspire_respond_batch_device_noalloc", C.c_int, [C.c_void_p, C.POINTER(_DeviceBatch)]),
     ("inspire_register_client_keys", C.c_int, [C.c_void_p, C.c_uint32, _u64p]),
     ("inspire_server_create", C.c_int, [C.c_void_p, C.POINTER(_ServerConfig), C.POINTER(C.c_void_p)]),
     ("inspire_server_submit", C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(_Request), _u64p]),
@@ -544,16 +545,19 @@ class Context:
     # -- device-resident batches and profiling
     def respond_batch_device(self, db_id: int, variant: int, shard_ids, rows_ptr: int, out_ptr: int,
                              seeds_ptr: int = 0, y_body_ptr: int = 0, seeded: bool = True,
-                             packing_mode: int = PACKING_INSPIRING):
+                             packing_mode: int = PACKING_INSPIRING, noalloc: bool = False):
         """Runs one batch whose inputs live in device memory (pointers from e.g.
         torch tensors; layouts in include/inspire_mi355x.h).  Returns the
-        per-query status array."""
+        per-query status array.  noalloc: through
+        inspire_respond_batch_device_noalloc (library-held scratch, no
+        allocation in the call)."""
         ids = np.ascontiguousarray(shard_ids, dtype=np.uint32)
         st = np.zeros(ids.size, dtype=np.int32)
         b = _DeviceBatch(ids.size, db_id, variant, 1 if seeded else 0, ids.ctypes.data_as(C.POINTER(C.c_uint32)),
                          rows_ptr, seeds_ptr or None, y_body_ptr or None, out_ptr,
                          st.ctypes.data_as(C.POINTER(C.c_int32)), packing_mode, 0)
-        _raise(lib().inspire_respond_batch_device(self._h, C.byref(b)))
+        fn = lib().inspire_respond_batch_device_noalloc if noalloc else lib().inspire_respond_batch_device
+        _raise(fn(self._h, C.byref(b)))
         return st
 
     def extract_batch_device(self, count: int, mode: int, entry_size: int, responses_ptr: int, secret_keys_ptr: int,
