@@ -128,6 +128,8 @@ def _load():
         "or_chacha_next_u32": (C.c_uint32, [C.POINTER(OrChacha)]),
         "or_chacha_next_u64": (C.c_uint64, [C.POINTER(OrChacha)]),
         "or_gen_range": (C.c_uint64, [C.POINTER(OrChacha), C.c_uint64]),
+        "or_zone_accept": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint, C.POINTER(C.c_uint64)]),
+        "or_zone_counts": (None, [C.c_uint64, C.c_uint, C.POINTER(C.c_uint64)]),
         "or_poly_from_seed": (None, [P, _u8p, _u64p]),
         "or_seeded_rgsw_expand": (None, [P, _u8p, _u64p, _u64p]),
         "or_sample_gaussian_poly": (None, [P, C.POINTER(OrChacha), C.c_double, _u64p]),

@@ -552,16 +552,42 @@ uint64_t or_chacha_next_u64(or_chacha *r) { /* rand_core 0.6 BlockRng::next_u64 
     }
 }
 
-/* rand 0.8.5 UniformInt<u64>::sample_single(0, range) -> sample_single_inclusive(0, range-1):
- * widening multiply with the "conservative zone" rejection. */
-uint64_t or_gen_range(or_chacha *r, uint64_t range) {
-    uint64_t zone = (range << __builtin_clzll(range)) - 1;
-    for (;;) {
-        uint64_t v = or_chacha_next_u64(r);
-        u128 m = (u128)v * range;
-        uint64_t hi = (uint64_t)(m >> 64), lo = (uint64_t)m;
-        if (lo <= zone) return hi;
+/* One draw of rand 0.8.5's UniformInt sample_single_inclusive for the
+ * integer width `bits` (the u64 instance has bits = 64): v uniform in
+ * [0, 2^bits), 1 <= range < 2^bits.  The widening product v * range splits
+ * into hi (the candidate output) and lo (the low bits); the draw is accepted
+ * when lo <= zone, the "conservative zone" (range << leading_zeros(range)) - 1
+ * at that width.  Returns 1 and sets *out = hi when accepted.  Written for any
+ * width so that the same code can be checked exhaustively at 8 and 16 bits. */
+int or_zone_accept(uint64_t v, uint64_t range, unsigned bits, uint64_t *out) {
+    const u128 mask = ((u128)1 << bits) - 1;
+    const unsigned lz = (unsigned)__builtin_clzll(range) - (64u - bits);
+    const uint64_t zone = (uint64_t)((((u128)range << lz) - 1) & mask);
+    const u128 m = (u128)v * range;
+    const uint64_t hi = (uint64_t)(m >> bits), lo = (uint64_t)(m & mask);
+    if (lo <= zone) {
+        *out = hi;
+        return 1;
     }
+    return 0;
+}
+
+/* Test support: for every v in [0, 2^bits) (bits <= 24), the accepted
+ * outputs of or_zone_accept counted per value: counts[range]. */
+void or_zone_counts(uint64_t range, unsigned bits, uint64_t *counts) {
+    for (uint64_t i = 0; i < range; i++) counts[i] = 0;
+    for (uint64_t v = 0; v < (1ull << bits); v++) {
+        uint64_t out;
+        if (or_zone_accept(v, range, bits, &out)) counts[out]++;
+    }
+}
+
+/* rand 0.8.5 UniformInt<u64>::sample_single(0, range) -> sample_single_inclusive(0, range-1):
+ * u64 draws until one is accepted. */
+uint64_t or_gen_range(or_chacha *r, uint64_t range) {
+    uint64_t out;
+    for (;;)
+        if (or_zone_accept(or_chacha_next_u64(r), range, 64, &out)) return out;
 }
 
 /* Poly::random_with_rng_moduli (poly.rs:251-268): residue-major draws. */

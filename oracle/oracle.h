@@ -152,6 +152,8 @@ void     or_chacha_seed(or_chacha *r, const uint8_t seed[32]);
 uint32_t or_chacha_next_u32(or_chacha *r);
 uint64_t or_chacha_next_u64(or_chacha *r);
 uint64_t or_gen_range(or_chacha *r, uint64_t range);                    /* rand 0.8.5 UniformInt<u64> */
+int or_zone_accept(uint64_t v, uint64_t range, unsigned bits, uint64_t *out); /* one draw at width bits */
+void or_zone_counts(uint64_t range, unsigned bits, uint64_t *counts);        /* accepted preimages per output */
 void     or_poly_from_seed(const or_params *pr, const uint8_t seed[32], uint64_t *out); /* poly.rs:278-281 */
 /* SeededRgswCiphertext::expand (rgsw/types.rs:315-318): seeds [2ell][32], b [2ell][K][d] -> rows */
 void     or_seeded_rgsw_expand(const or_params *pr, const uint8_t *seeds, const uint64_t *b,
