@@ -50,15 +50,25 @@ struct ChainDigits {
     static constexpr int DB = ND > 0 ? ND : 1;      // LDS exchange buffers per residue
 };
 
-template <int LOGN, int ELL, bool REC>
+template <int LOGN, int ELL, bool REC, int V>
 struct ChainCrtShape {
-    static constexpr int N = 1 << LOGN, LP = NttShape<LOGN>::LDS;
+    static constexpr int N = 1 << LOGN, LP = LayV<LOGN, V>::LP;
     static constexpr size_t kBytes = (size_t)2 * N * 4      // forward Montgomery twiddles of both residues
                                      + (size_t)ELL * 2 * N * 4  // NTT of the client's y_body
                                      + (size_t)2 * ChainDigits<ELL, REC>::DB * LP * 4  // transform exchange,
                                      // per residue (the inverse uses the first digit buffer)
                                      + (size_t)2 * LP * 4;  // coefficient exchange between the halves
 };
+
+// Position of NTT slot s of the client's keys in LDS.  The rotation
+// tau_{g^i} with g = 2n/gamma + 1 moves a slot only within its aligned group
+// of 16 (it changes the top bits of the slot's bit-reversed index), by the
+// same map for every group of a wave's 256 slots, so in the plain layout the
+// gathers of a wave's groups all hit the same 4 banks of their parity.  The
+// XOR of bits 0-1 with bits 5-6 and 7-8 gives each such group its own banks
+// (32 banks per half wave or 64 per wave, checked by simulating every
+// rotation's gathers); it is a bijection, so any permutation stays correct.
+__device__ __forceinline__ int yl_ix(int s) { return s ^ (((s >> 5) ^ (s >> 7)) & 3); }
 
 // LAZY: ELL * 4q * q < q * 2^32, so the multiply-accumulate takes the
 // transform outputs (< 4q) without reducing them first.
@@ -71,7 +81,7 @@ struct ChainCrtShape {
 template <int LOGN, int ELL, bool LAZY, bool REC, bool SGN, int V>
 __global__ void __launch_bounds__(2 * ((1 << LOGN) / V), 1)
 k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
-    using L = Lay<LOGN>;
+    using L = LayV<LOGN, V>;
     constexpr int N = L::N, T = N / V, LP = L::LP;  // T threads per residue, V values each
     constexpr int ND = ChainDigits<ELL, REC>::ND, DB = ChainDigits<ELL, REC>::DB;
     constexpr size_t KN = 2 * (size_t)N;
@@ -117,21 +127,15 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
 #pragma unroll
             for (int j = 0; j < NP; j++)
 #pragma unroll
-                for (int g = 0; g < V / 4; g++) {
-                    uint32_t v[4];
-#pragma unroll
-                    for (int c = 0; c < 4; c++)
-                        v[c] = csub(shoup_mul(yv[j][4 * g + c], C.mont[k], C.montp[k], M.q), M.q);
-                    *reinterpret_cast<uint4*>(&yl[((j0 + j) * 2 + k) * N + V * t + 4 * g]) =
-                        make_uint4(v[0], v[1], v[2], v[3]);
-                }
+                for (int e = 0; e < V; e++)
+                    yl[((j0 + j) * 2 + k) * N + yl_ix(V * t + e)] =
+                        csub(shoup_mul(yv[j][e], C.mont[k], C.montp[k], M.q), M.q);
         };
 #pragma unroll
         for (int j0 = 0; j0 + DB <= ELL; j0 += DB) y_group(std::integral_constant<int, DB>{}, j0);
         if constexpr (ELL % DB != 0) y_group(std::integral_constant<int, ELL % DB>{}, ELL - ELL % DB);
     } else if (ymode == 0) {
-        for (int e = tid; e < ELL * 2 * N / 4; e += 2 * T)
-            reinterpret_cast<uint4*>(yl)[e] = reinterpret_cast<const uint4*>(Y)[e];
+        for (int e = tid; e < ELL * 2 * N; e += 2 * T) yl[(e & ~(N - 1)) + yl_ix(e & (N - 1))] = Y[e];
     }
 
     // this residue's value of the reference's automorphism of r_pre[i] at a
@@ -240,10 +244,13 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
         uint32_t yv[ELL][V];
         if (use_y) {
             if (ymode == 0) {
+                int ym[V];  // the gathered slots' positions in the swizzled layout
+#pragma unroll
+                for (int e = 0; e < V; e++) ym[e] = yl_ix((int)pm[e]);
 #pragma unroll
                 for (int j = 0; j < ELL; j++)
 #pragma unroll
-                    for (int e = 0; e < V; e++) yv[j][e] = yl[(j * 2 + k) * N + pm[e]];
+                    for (int e = 0; e < V; e++) yv[j][e] = yl[(j * 2 + k) * N + ym[e]];
             } else {
 #pragma unroll
                 for (int j = 0; j < ELL; j++)
@@ -320,7 +327,7 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
 // local static is initialised exactly once, also under concurrent callers).
 template <int LOGN, int ELL, bool LAZY, bool REC, bool SGN, int V>
 static hipError_t chain_crt_lds_limit() {
-    constexpr size_t shmem = ChainCrtShape<LOGN, ELL, REC>::kBytes;
+    constexpr size_t shmem = ChainCrtShape<LOGN, ELL, REC, V>::kBytes;
     static const hipError_t e =
         shmem > 64 * 1024 ? hipFuncSetAttribute((const void*)k_pack_chain_crt<LOGN, ELL, LAZY, REC, SGN, V>,
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem)
@@ -334,7 +341,7 @@ static hipError_t chain_crt_launch_v(const PackArgs& A, uint32_t Q, const CtxDev
     const hipError_t e = chain_crt_lds_limit<LOGN, ELL, LAZY, REC, SGN, V>();
     if (e != hipSuccess) return e;
     auto kern = k_pack_chain_crt<LOGN, ELL, LAZY, REC, SGN, V>;
-    constexpr size_t shmem = ChainCrtShape<LOGN, ELL, REC>::kBytes;
+    constexpr size_t shmem = ChainCrtShape<LOGN, ELL, REC, V>::kBytes;
     hipLaunchKernelGGL(kern, dim3(Q), dim3(2 * ((1 << LOGN) / V)), shmem, st, A, C, inv);
     return hipGetLastError();
 }
@@ -397,7 +404,7 @@ bool pack_chain_crt_supported(const CtxDev& C) {
     if (C.K != 2 || C.ell < 1 || C.ell > 3 || C.logn < 8 || C.logn > 11) return false;
     // every modulus must leave room for the lazy ranges: 4q < 2^32
     if (C.m[0].q >= (1u << 30) || C.m[1].q >= (1u << 30)) return false;
-    return ChainCrtShape<11, 3, false>::kBytes <= 160 * 1024;
+    return ChainCrtShape<11, 3, false, kChainV>::kBytes <= 160 * 1024;
 }
 
 hipError_t launch_pack_chain_crt(const PackArgs& A, uint32_t Q, const CtxDev& C, const ScaleDev& inv,

@@ -374,27 +374,118 @@ __device__ __forceinline__ void intt_v(uint32_t (&x)[V], int t, uint32_t* rb, co
 // Four values per thread (N/4 threads per residue, twice the waves of the
 // radix-8 layout): radix-4 passes plus one radix-2 stage when LOGN is odd.
 // Coefficient positions {t + e N/4}; slot positions 4t .. 4t + 3.
+//
+// LDS layouts of the exchanges.  Forward pass p (stride st = N / 4^(p+1))
+// has thread t touch logical words 4 st i + r + e st (i = t / st, r = t %
+// st); inverse pass p (t0 = 2^REM 4^p) touches 4 t0 i + r + e t0.  With 32
+// banks of 4 bytes (lanes of a half wave served together), a stride-8 pass
+// puts 4 rows of 32 words under one half wave with 8-word runs at column 8e,
+// a stride-2 pass puts the same 4 rows under it with 2-word runs at columns
+// 8c + 2e: no padding by rows serves both (the first needs the row offsets
+// equal mod 8, the second distinct).  With 64 banks served a whole wave at a
+// time, a stride-32 pass puts two rows 128 words apart under one wave, which
+// must land 32 banks apart, while the wide passes read 64 consecutive words.
+// So each exchange takes a layout its writer and reader are both
+// conflict-free in, under either bank model (odd LOGN; checked by
+// simulating every access of every pass):
+//   kind 1: x + 8 (x >> 5), the exchanges between strides 32 and 8;
+//   kind 2: x + 2 (x >> 3), between strides 8 and 2 and around the radix-2
+//           stage (read and written there as two 8-byte pairs, which stay
+//           contiguous and aligned);
+//   kind 3: x + 32 (x >> 7), between strides of 32 and more;
+//   kind 0: x + 4 (x >> 5) (lds_ix), for even LOGN, whose stride sets differ.
+// In every kind an element's address is the thread's base address plus an
+// immediate.
 // ---------------------------------------------------------------------------
+template <int KIND>
+__device__ __forceinline__ int xl_ix(int x) {
+    if constexpr (KIND == 1) return x + 8 * (x >> 5);
+    else if constexpr (KIND == 2) return x + 2 * (x >> 3);
+    else if constexpr (KIND == 3) return x + 32 * (x >> 7);
+    else return lds_ix(x);
+}
+// Index of logical base + e st from the base's index: e st is a multiple of
+// the layout's block, or stays inside the base's block.
+template <int KIND>
+__device__ __forceinline__ int xl_step(int base_ix, int e, int st) {
+    if constexpr (KIND == 1) return base_ix + e * st + 8 * ((e * st) >> 5);
+    else if constexpr (KIND == 2) return base_ix + e * st + 2 * ((e * st) >> 3);
+    else if constexpr (KIND == 3) return base_ix + e * st + 32 * ((e * st) >> 7);
+    else return lds_step(base_ix, e, st);
+}
+
 template <int LOGN>
 struct Lay4 {
-    static constexpr int N = 1 << LOGN, T = N / 4, LP = NttShape<LOGN>::LDS;
+    static constexpr int N = 1 << LOGN, T = N / 4;
     static constexpr int R4 = LOGN / 2, REM = LOGN % 2;
+    // exchange words per buffer (kinds 1 and 2 hold N + N/4)
+    static constexpr int LP = REM ? N + N / 4 : NttShape<LOGN>::LDS;
     // The block a forward pass-(p+1) thread reads was written by the N/4^(p+1)
     // threads of its pass-p block, the block an inverse pass-p thread reads by
     // the same 2^REM 4^p threads: with at most 64 of them (one wave) the
     // exchange needs no s_barrier.
     static constexpr bool fwd_wave_local(int p) { return (N >> (2 * (p + 1))) <= 64; }
     static constexpr bool inv_wave_local(int p) { return ((1 << REM) << (2 * p)) <= 64; }
+    // layout of the exchange written by forward pass p, stride st = N / 4^(p+1)
+    // (the last one feeds the radix-2 stage)
+    static constexpr int fwd_kind(int p) {
+        return !REM ? 0 : (N >> (2 * (p + 1))) <= 8 ? 2 : (N >> (2 * (p + 1))) == 32 ? 1 : 3;
+    }
+    // layout of the exchange written by inverse pass p, t0 = 2 4^p (p = -1: the radix-2 stage)
+    static constexpr int inv_kind(int p) { return !REM ? 0 : p <= 0 ? 2 : p == 1 ? 1 : 3; }
 };
 
-// Either layout by its number of values per thread.
-template <int LOGN, int V>
-struct LayV {
-    static constexpr int N = 1 << LOGN, T = N / V, LP = NttShape<LOGN>::LDS;
-    static __device__ __forceinline__ int group(int t, int g) { return V * t + 4 * g; }
-    // the forward transform passes at least one s_barrier
-    static constexpr bool fwd_has_barrier = V == 8 ? Lay<LOGN>::fwd_has_barrier : !Lay4<LOGN>::fwd_wave_local(0);
-};
+template <int LOGN, int ELL, bool SGN, int KR, int KW>
+__device__ __forceinline__ void ntt4_pass(uint32_t (&x)[ELL][4], int t, int p, int m0, uint32_t* db,
+                                          const uint32_t* tw, const Mod& M) {
+    using L = Lay4<LOGN>;
+    constexpr int N = L::N, LP = L::LP;
+    const int t0 = N / (2 * m0), st = t0 >> 1;
+    const int i = t / st, r = t - i * st;
+    if (p > 0) {
+        const int bix = xl_ix<KR>(2 * i * t0 + r);
+#pragma unroll
+        for (int j = 0; j < ELL; j++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) x[j][e] = db[j * LP + xl_step<KR>(bix, e, st)];
+    }
+    {
+        const uint32_t w = tw[m0 + i];
+#pragma unroll
+        for (int j = 0; j < ELL; j++) {
+            ct_b<SGN>(x[j][0], x[j][2], w, M);
+            ct_b<SGN>(x[j][1], x[j][3], w, M);
+        }
+    }
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const uint32_t w = tw[2 * m0 + 2 * i + h];
+#pragma unroll
+        for (int j = 0; j < ELL; j++) ct_b<SGN>(x[j][2 * h], x[j][2 * h + 1], w, M);
+    }
+    if (!(L::REM == 0 && p == L::R4 - 1)) {
+        const int bix = xl_ix<KW>(2 * i * t0 + r);
+#pragma unroll
+        for (int j = 0; j < ELL; j++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) db[j * LP + xl_step<KW>(bix, e, st)] = x[j][e];
+        if (L::fwd_wave_local(p))
+            wave_lds_sync();
+        else
+            lds_barrier();
+    }
+}
+
+template <int LOGN, int ELL, bool SGN, int P>
+__device__ __forceinline__ void ntt4_passes(uint32_t (&x)[ELL][4], int t, uint32_t* db, const uint32_t* tw,
+                                            const Mod& M) {
+    using L = Lay4<LOGN>;
+    if constexpr (P < L::R4) {
+        constexpr int KR = P > 0 ? L::fwd_kind(P - 1) : 0, KW = L::fwd_kind(P);
+        ntt4_pass<LOGN, ELL, SGN, KR, KW>(x, t, P, 1 << (2 * P), db, tw, M);
+        ntt4_passes<LOGN, ELL, SGN, P + 1>(x, t, db, tw, M);
+    }
+}
 
 // Forward NTT of ELL polys of one residue, four values per thread:
 // coefficient positions -> slot positions (< 4q; SGN as for ntt_digits).
@@ -404,55 +495,70 @@ __device__ __forceinline__ void ntt4_digits(uint32_t (&x)[ELL][4], int t, uint32
                                             const Mod& M) {
     using L = Lay4<LOGN>;
     constexpr int N = L::N, LP = L::LP;
-    int m0 = 1;
+    ntt4_passes<LOGN, ELL, SGN, 0>(x, t, db, tw, M);
+    if constexpr (L::REM != 0) {
+        constexpr int KR = L::fwd_kind(L::R4 - 1);
+        const uint32_t w0 = tw[N / 2 + 2 * t], w1 = tw[N / 2 + 2 * t + 1];
+        const int ix = xl_ix<KR>(4 * t);
 #pragma unroll
-    for (int p = 0; p < L::R4; p++) {
-        const int t0 = N / (2 * m0), st = t0 >> 1;
-        const int i = t / st, r = t - i * st;
-        const int bix = lds_ix(2 * i * t0 + r);
-        if (p > 0) {
-#pragma unroll
-            for (int j = 0; j < ELL; j++)
-#pragma unroll
-                for (int e = 0; e < 4; e++) x[j][e] = db[j * LP + lds_step(bix, e, st)];
-        }
-        {
-            const uint32_t w = tw[m0 + i];
-#pragma unroll
-            for (int j = 0; j < ELL; j++) {
-                ct_b<SGN>(x[j][0], x[j][2], w, M);
-                ct_b<SGN>(x[j][1], x[j][3], w, M);
+        for (int j = 0; j < ELL; j++) {
+            if constexpr (KR == 2) {  // two 8-byte pairs (the group is 8-byte aligned only)
+                const uint2 a = *reinterpret_cast<const uint2*>(&db[j * LP + ix]);
+                const uint2 b = *reinterpret_cast<const uint2*>(&db[j * LP + ix + 2]);
+                x[j][0] = a.x;
+                x[j][1] = a.y;
+                x[j][2] = b.x;
+                x[j][3] = b.y;
+            } else {
+                const uint4 v = *reinterpret_cast<const uint4*>(&db[j * LP + ix]);
+                x[j][0] = v.x;
+                x[j][1] = v.y;
+                x[j][2] = v.z;
+                x[j][3] = v.w;
             }
+            ct_b<SGN>(x[j][0], x[j][1], w0, M);
+            ct_b<SGN>(x[j][2], x[j][3], w1, M);
         }
+    }
+}
+
+template <int LOGN, int P>
+__device__ __forceinline__ void intt4_passes(uint32_t (&x)[4], int t, uint32_t* rb, const uint32_t* tw, const Mod& M,
+                                             uint32_t c, uint32_t cp, uint32_t cw, uint32_t cwp) {
+    using L = Lay4<LOGN>;
+    constexpr int N = L::N;
+    if constexpr (P < L::R4) {
+        constexpr int KR = L::inv_kind(P - 1), KW = L::inv_kind(P);
+        const int t0 = (1 << L::REM) << (2 * P);
+        const int i2 = t / t0, r = t - i2 * t0;
+        if (!(L::REM == 0 && P == 0)) {
+            const int bix = xl_ix<KR>(4 * t0 * i2 + r);
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
-            const uint32_t w = tw[2 * m0 + 2 * i + h];
-#pragma unroll
-            for (int j = 0; j < ELL; j++) ct_b<SGN>(x[j][2 * h], x[j][2 * h + 1], w, M);
+            for (int e = 0; e < 4; e++) x[e] = rb[xl_step<KR>(bix, e, t0)];
         }
-        if (!(L::REM == 0 && p == L::R4 - 1)) {
+        const int mA = N / (2 * t0), mB = N / (4 * t0);
 #pragma unroll
-            for (int j = 0; j < ELL; j++)
+        for (int h = 0; h < 2; h++) gs_m(x[2 * h], x[2 * h + 1], inv_tw(tw, mA, 2 * i2 + h), M);
+        if (P < L::R4 - 1) {
+            const uint32_t w = inv_tw(tw, mB, i2);
+            gs_m(x[0], x[2], w, M);
+            gs_m(x[1], x[3], w, M);
+            const int bix = xl_ix<KW>(4 * t0 * i2 + r);
 #pragma unroll
-                for (int e = 0; e < 4; e++) db[j * LP + lds_step(bix, e, st)] = x[j][e];
-            if (L::fwd_wave_local(p))
+            for (int e = 0; e < 4; e++) rb[xl_step<KW>(bix, e, t0)] = x[e];
+            if (L::inv_wave_local(P + 1))
                 wave_lds_sync();
             else
                 lds_barrier();
-        }
-        m0 <<= 2;
-    }
-    if constexpr (L::REM != 0) {
-        const uint32_t w0 = tw[N / 2 + 2 * t], w1 = tw[N / 2 + 2 * t + 1];
+            intt4_passes<LOGN, P + 1>(x, t, rb, tw, M, c, cp, cw, cwp);
+        } else {
+            // last stage (m = 1) with the output scale: X' = c (X + Y), Y' = cw (X - Y)
 #pragma unroll
-        for (int j = 0; j < ELL; j++) {
-            const uint4 v = *reinterpret_cast<const uint4*>(&db[j * LP + lds_ix(4 * t)]);
-            x[j][0] = v.x;
-            x[j][1] = v.y;
-            x[j][2] = v.z;
-            x[j][3] = v.w;
-            ct_b<SGN>(x[j][0], x[j][1], w0, M);
-            ct_b<SGN>(x[j][2], x[j][3], w1, M);
+            for (int k = 0; k < 2; k++) {
+                const uint32_t X = x[k], Y = x[k + 2];
+                x[k] = shoup_mul(X + Y, c, cp, M.q);
+                x[k + 2] = shoup_mul(X - Y + M.q2, cw, cwp, M.q);
+            }
         }
     }
 }
@@ -466,47 +572,32 @@ __device__ __forceinline__ void intt4(uint32_t (&x)[4], int t, uint32_t* rb, con
     using L = Lay4<LOGN>;
     constexpr int N = L::N;
     if constexpr (L::REM != 0) {
+        constexpr int KW = L::inv_kind(-1);
         gs_m(x[0], x[1], inv_tw(tw, N / 2, 2 * t), M);
         gs_m(x[2], x[3], inv_tw(tw, N / 2, 2 * t + 1), M);
-        *reinterpret_cast<uint4*>(&rb[lds_ix(4 * t)]) = make_uint4(x[0], x[1], x[2], x[3]);
+        const int ix = xl_ix<KW>(4 * t);
+        if constexpr (KW == 2) {
+            *reinterpret_cast<uint2*>(&rb[ix]) = make_uint2(x[0], x[1]);
+            *reinterpret_cast<uint2*>(&rb[ix + 2]) = make_uint2(x[2], x[3]);
+        } else {
+            *reinterpret_cast<uint4*>(&rb[ix]) = make_uint4(x[0], x[1], x[2], x[3]);
+        }
         if (L::inv_wave_local(0))
             wave_lds_sync();
         else
             lds_barrier();
     }
-    int t0 = 1 << L::REM;
-#pragma unroll
-    for (int p = 0; p < L::R4; p++) {
-        const int i2 = t / t0, r = t - i2 * t0;
-        const int bix = lds_ix(4 * t0 * i2 + r);
-        if (!(L::REM == 0 && p == 0)) {
-#pragma unroll
-            for (int e = 0; e < 4; e++) x[e] = rb[lds_step(bix, e, t0)];
-        }
-        const int mA = N / (2 * t0), mB = N / (4 * t0);
-#pragma unroll
-        for (int h = 0; h < 2; h++) gs_m(x[2 * h], x[2 * h + 1], inv_tw(tw, mA, 2 * i2 + h), M);
-        if (p < L::R4 - 1) {
-            const uint32_t w = inv_tw(tw, mB, i2);
-            gs_m(x[0], x[2], w, M);
-            gs_m(x[1], x[3], w, M);
-#pragma unroll
-            for (int e = 0; e < 4; e++) rb[lds_step(bix, e, t0)] = x[e];
-            if (L::inv_wave_local(p + 1))
-                wave_lds_sync();
-            else
-                lds_barrier();
-        } else {
-            // last stage (m = 1) with the output scale: X' = c (X + Y), Y' = cw (X - Y)
-#pragma unroll
-            for (int k = 0; k < 2; k++) {
-                const uint32_t X = x[k], Y = x[k + 2];
-                x[k] = shoup_mul(X + Y, c, cp, M.q);
-                x[k + 2] = shoup_mul(X - Y + M.q2, cw, cwp, M.q);
-            }
-        }
-        t0 <<= 2;
-    }
+    intt4_passes<LOGN, 0>(x, t, rb, tw, M, c, cp, cw, cwp);
 }
+
+// Either layout by its number of values per thread.
+template <int LOGN, int V>
+struct LayV {
+    // exchange words per LDS buffer of the transforms (and of any buffer addressed with lds_ix)
+    static constexpr int N = 1 << LOGN, T = N / V, LP = V == 8 ? NttShape<LOGN>::LDS : Lay4<LOGN>::LP;
+    static __device__ __forceinline__ int group(int t, int g) { return V * t + 4 * g; }
+    // the forward transform passes at least one s_barrier
+    static constexpr bool fwd_has_barrier = V == 8 ? Lay<LOGN>::fwd_has_barrier : !Lay4<LOGN>::fwd_wave_local(0);
+};
 
 }  // namespace inspire

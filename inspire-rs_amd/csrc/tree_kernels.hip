@@ -58,7 +58,7 @@ struct TreeDigits {
 
 template <int LOGN, int K, int ELL, bool REC>
 struct TreeShape {
-    static constexpr int N = 1 << LOGN, T = N / kTreeV, LP = NttShape<LOGN>::LDS;
+    static constexpr int N = 1 << LOGN, T = N / kTreeV, LP = LayV<LOGN, kTreeV>::LP;
     static constexpr size_t kTw = (size_t)K * N;          // forward Montgomery twiddles (the inverse reads them mirrored)
     static constexpr size_t kA = (size_t)K * LP;          // s1.a (automorphism source), then INTT exchange
     static constexpr size_t kB = (size_t)K * LP;          // s1.b (automorphism source)
