@@ -199,6 +199,11 @@ def main():
     ap.add_argument("--serve-lanes", type=int, default=2)
     ap.add_argument("--serve-max-batch", type=int, default=256)
     ap.add_argument("--serve-pool", type=int, default=256, help="distinct queries (with CPU answers) drawn from")
+    ap.add_argument("--serve-leg-rates", type=str, default="10000,100000,400000",
+                    help="offered rates of the serving leg of the default run (the last one above saturation); "
+                         "empty skips the leg")
+    ap.add_argument("--serve-leg-queries", type=str, default="10000,50000,50000",
+                    help="queries per rate of the serving leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the respond path shards by database across GPUs with no collective, "
@@ -460,6 +465,15 @@ def main():
     for ln in lanes[1:]:
         ln.close()
     ctx.close()
+    if args.serve_leg_rates:
+        # the serving configuration: continuous batching over 64 uneven databases (2^28 x 32 B
+        # resident), Poisson arrivals, the ^0/^1/^2 mix; every response checked against the oracle
+        rates = [float(x) for x in args.serve_leg_rates.split(",") if x]
+        counts = [int(x) for x in args.serve_leg_queries.split(",") if x]
+        sv = serve_main(args, M, log, w_seed, rates=rates, counts=counts)
+        line["serving"] = {k: sv[k] for k in ("value", "unit", "config", "rates", "bit_exact", "server",
+                                              "cpu_baseline")}
+        line["serving"]["metric"] = "served queries/s at saturation (highest achieved rate), p50/p99 per offered rate"
     print(json.dumps(line))
 
 
@@ -670,35 +684,27 @@ def serve_sizes(rng, total_log2: int, ndb: int):
     return shards * D
 
 
-def serve_main(args, M, log, w_seed: bytes):
-    """Open-loop Poisson arrivals into the serving loop (inspire_server_*)
-    over databases of uneven size: throughput and p50/p99 latency per offered
-    rate.  Queries are drawn from a pool of synthetic queries of the real
-    shape; the CPU leg computes the oracle's answer for each pool query (and
-    times it), and every served response is compared with that answer."""
-    import threading
-
-    rng = np.random.default_rng(2028)
-    sizes = serve_sizes(rng, args.serve_log2, args.serve_dbs)
+def serve_world(M, rng, total_log2: int, ndb: int, w_seed: bytes, log, n_clients: int = 16, pool_size: int = 256):
+    """The serving configuration: ndb databases of log-uniform size totalling
+    2^total_log2 x 32 B of uniform bytes, resident; n_clients clients with
+    registered packing keys; a pool of synthetic queries of the real shape
+    mixing InsPIRe^0/^1/^2 at 10/30/60 % over uniform databases and shards."""
+    sizes = serve_sizes(rng, total_log2, ndb)
     total = int(sizes.sum())
     log(f"serving databases: {len(sizes)} of {sizes.min()}..{sizes.max()} entries, total {total}")
-    raw = np.frombuffer(rng.bytes(total * 32), dtype=np.uint8)
+    raw = rng.integers(0, 2**64 - 1, total * 32 // 8, dtype=np.uint64, endpoint=True).view(np.uint8)
     offs = np.concatenate([[0], np.cumsum(sizes)]) * 32
     ctx = M.Context(M.InspireParams(), w_seed)
     t0 = time.perf_counter()
     for i in range(len(sizes)):
         ctx.load_entries(i, raw[offs[i]:offs[i + 1]], 32)
     log(f"resident: {ctx.device_bytes / 2**30:.1f} GiB, load {time.perf_counter() - t0:.1f} s")
-
-    # clients with resident packing keys, and the query pool (synthetic, real shapes)
     moduli = np.array([Q0, Q1], dtype=np.uint64)[:, None]
-    n_clients = 16
     ybodies = [(rng.integers(0, 2**63, (ELL, K, D), dtype=np.uint64) % moduli) for _ in range(n_clients)]
     for c, yb in enumerate(ybodies):
         ctx.register_client_keys(c, yb)
-    P = args.serve_pool
     pool = []
-    for i in range(P):
+    for i in range(pool_size):
         u = rng.random()
         variant = 0 if u < 0.1 else (1 if u < 0.4 else 2)
         db = int(rng.integers(0, len(sizes)))
@@ -711,76 +717,98 @@ def serve_main(args, M, log, w_seed: bytes):
             rows = (rng.integers(0, 2**63, (ELL, K, D), dtype=np.uint64) % moduli).astype(np.uint32)
             seeds = rng.integers(0, 256, (ELL, 32), dtype=np.uint8)
         pool.append(dict(db=db, shard=shard, variant=variant, client=client, rows=rows, seeds=seeds))
+    return dict(ctx=ctx, sizes=sizes, raw=raw, offs=offs, ybodies=ybodies, pool=pool, n_clients=n_clients)
 
-    # CPU leg: the oracle's respond / respond_inspiring for every pool query
-    cpu = serve_cpu_answers(pool, raw, offs, ybodies, w_seed, log)
 
+def request_structs(M, pool):
     structs, keep = [], []
     for q in pool:
         st, k = M.Server.request_struct(M.Request(q["db"], q["shard"], q["variant"], rows=q["rows"],
                                                   seeds=q["seeds"], client_id=q["client"]), borrow=True)
         structs.append(st)
         keep.append(k)
+    return structs, keep
+
+
+def poisson_run(M, srv, pool, structs, answers, lam: float, n: int, rng):
+    """n queries drawn from the pool, arriving open-loop at Poisson rate lam
+    (arrival times fixed in advance, submitted when due); every response is
+    compared with the pool query's answer.  -> (result dict, tickets' picks,
+    their responses' ciphertexts are not kept)."""
+    import threading
+
+    pick = rng.integers(0, len(pool), n)
+    gaps = rng.exponential(1.0 / lam, n)
+    t_start = time.monotonic_ns() + 20_000_000
+    arrival = (t_start + np.cumsum(gaps) * 1e9).astype(np.int64)
+    tickets = np.zeros(n, dtype=np.uint64)
+    done = np.zeros(n, dtype=np.int64)
+    bad = [0]
+    submitted = [0]
+
+    def waiter():
+        for j in range(n):
+            while submitted[0] <= j:
+                time.sleep(50e-6)
+            q = pool[pick[j]]
+            st, r, dn = srv.wait(int(tickets[j]), cols=16 if q["variant"] == 0 else 0)
+            done[j] = dn
+            want = answers[pick[j]]
+            if st != M.OK or not np.array_equal(r.ciphertext, want[0]) or (
+                    q["variant"] == 0 and not np.array_equal(r.column_ciphertexts, want[1])):
+                bad[0] += 1
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    i = 0
+    while i < n:
+        now = time.monotonic_ns()
+        k = int(np.searchsorted(arrival, now, side="right"))
+        if k > i:
+            arr = (M.RequestStruct * (k - i))()
+            for j in range(i, k):
+                arr[j - i] = structs[pick[j]]
+                arr[j - i].arrival_ns = int(arrival[j])
+            tickets[i:k] = srv.submit_structs(arr, k - i)
+            i = k
+            submitted[0] = k
+        else:
+            dt = (arrival[i] - now) / 1e9
+            if dt > 300e-6:
+                time.sleep(dt - 200e-6)
+    th.join()
+    lat = (done - arrival) / 1e6
+    span = (done.max() - arrival[0]) / 1e9
+    return {
+        "offered_qps": lam,
+        "queries": n,
+        "achieved_qps": round(n / span, 1),
+        "p50_ms": round(float(np.percentile(lat, 50)), 3),
+        "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "bit_exact": f"{n - bad[0]}/{n}",
+    }, bad[0]
+
+
+def serve_main(args, M, log, w_seed: bytes, rates=None, counts=None):
+    """Open-loop Poisson arrivals into the serving loop (inspire_server_*)
+    over databases of uneven size: throughput and p50/p99 latency per offered
+    rate.  Queries are drawn from a pool of synthetic queries of the real
+    shape; the CPU leg computes the oracle's answer for each pool query (and
+    times it), and every served response is compared with that answer."""
+    rng = np.random.default_rng(2028)
+    W = serve_world(M, rng, args.serve_log2, args.serve_dbs, w_seed, log, pool_size=args.serve_pool)
+    ctx, sizes, pool = W["ctx"], W["sizes"], W["pool"]
+    cpu = serve_cpu_answers(pool, W["raw"], W["offs"], W["ybodies"], w_seed, log)
+    structs, keep = request_structs(M, pool)
     results = []
-    rates = [float(x) for x in args.serve_rates.split(",") if x]
+    if rates is None:
+        rates = [float(x) for x in args.serve_rates.split(",") if x]
     mismatches_total, checked_total = 0, 0
     with M.Server(ctx, lanes=args.serve_lanes, max_batch=args.serve_max_batch) as srv:
-        for lam in rates:
-            n = int(min(50000, max(200, lam * 1.0)))
-            pick = rng.integers(0, P, n)
-            gaps = rng.exponential(1.0 / lam, n)
-            t_start = time.monotonic_ns() + 20_000_000
-            arrival = (t_start + np.cumsum(gaps) * 1e9).astype(np.int64)
-            tickets = np.zeros(n, dtype=np.uint64)
-            done = np.zeros(n, dtype=np.int64)
-            status = np.zeros(n, dtype=np.int32)
-            bad = [0]
-            submitted = [0]
-
-            def waiter():
-                for j in range(n):
-                    while submitted[0] <= j:
-                        time.sleep(50e-6)
-                    q = pool[pick[j]]
-                    st, r, dn = srv.wait(int(tickets[j]), cols=16 if q["variant"] == 0 else 0)
-                    status[j] = st
-                    done[j] = dn
-                    want = cpu[pick[j]]
-                    if st != M.OK or not np.array_equal(r.ciphertext, want[0]) or (
-                            q["variant"] == 0 and not np.array_equal(r.column_ciphertexts, want[1])):
-                        bad[0] += 1
-
-            th = threading.Thread(target=waiter)
-            th.start()
-            i = 0
-            while i < n:
-                now = time.monotonic_ns()
-                k = int(np.searchsorted(arrival, now, side="right"))
-                if k > i:
-                    arr = (M.RequestStruct * (k - i))()
-                    for j in range(i, k):
-                        s = structs[pick[j]]
-                        arr[j - i] = s
-                        arr[j - i].arrival_ns = int(arrival[j])
-                    tickets[i:k] = srv.submit_structs(arr, k - i)
-                    i = k
-                    submitted[0] = k
-                else:
-                    dt = (arrival[i] - now) / 1e9
-                    if dt > 300e-6:
-                        time.sleep(dt - 200e-6)
-            th.join()
-            lat = (done - arrival) / 1e6
-            span = (done.max() - arrival[0]) / 1e9
-            res = {
-                "offered_qps": lam,
-                "queries": n,
-                "achieved_qps": round(n / span, 1),
-                "p50_ms": round(float(np.percentile(lat, 50)), 3),
-                "p99_ms": round(float(np.percentile(lat, 99)), 3),
-                "bit_exact": f"{n - bad[0]}/{n}",
-            }
-            mismatches_total += bad[0]
+        for j, lam in enumerate(rates):
+            n = counts[j] if counts else int(min(50000, max(200, lam * 1.0)))
+            res, bad = poisson_run(M, srv, pool, structs, cpu, lam, n, rng)
+            mismatches_total += bad
             checked_total += n
             results.append(res)
             log(f"offered {lam:>8.0f} q/s: achieved {res['achieved_qps']:>9.0f} q/s, p50 {res['p50_ms']} ms, "
@@ -788,6 +816,7 @@ def serve_main(args, M, log, w_seed: bytes):
         stats = srv.stats()
     ctx.close()
     peak = max(results, key=lambda r: r["achieved_qps"])
+    total = int(sizes.sum())
     return {
         "metric": SERVE_METRIC,
         "value": peak["achieved_qps"],
@@ -799,7 +828,7 @@ def serve_main(args, M, log, w_seed: bytes):
         "config": {
             "workload": "serving loop: open-loop Poisson arrivals, InsPIRe^0/^1/^2 at 10/30/60 %, "
                         f"{len(sizes)} databases of log-uniform size totalling 2^{args.serve_log2} x 32 B, "
-                        f"{n_clients} clients with resident packing keys",
+                        f"{W['n_clients']} clients with resident packing keys",
             "db_entries_min": int(sizes.min()),
             "db_entries_max": int(sizes.max()),
             "db_entries_total": total,
