@@ -170,6 +170,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--prime-ms", type=float, default=300.0,
+                    help="untimed run of the workload on every lane before the warmup steps (clocks at load)")
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--lanes", type=int, default=2,
                     help="lanes (inspire_ctx_create_lane) submitting batches from their own threads; "
@@ -255,17 +257,6 @@ def main():
     # transfers overlap another's kernels.  A step is one batch on one lane.
     lanes = [ctx] + [ctx.lane() for _ in range(max(args.lanes, 1) - 1)]
     louts = [dout] + [M.DeviceArray(B * 2 * K * D * 4) for _ in lanes[1:]]
-    for j, ln in enumerate(lanes):
-        for i in range(args.warmup):
-            step(i + j, c=ln, out=louts[j])
-    # Timed region: HIP events bracket only the two kernels the rooflines are
-    # computed for (every bracket costs host time inside the step).
-    kernels = ("pack_chain", "pack_front", "first_dim", "pack_step1", "pack_b0", "expand_seeds", "rows_ntt", "y_ntt")
-    for ln in lanes:
-        ln.set_profiling_kernels(["pack_chain", "pack_front", "first_dim"])
-        ln.set_profiling(True)
-        for name in kernels:
-            ln.profile(name, reset=True)
     def run_lanes(lns, outs, steps):
         """steps batches taken in turn by the lanes' threads: (elapsed s, per-step latencies ms)"""
         lat = []
@@ -302,6 +293,22 @@ def main():
             raise errors[0]
         return el, lat
 
+    # Priming (untimed): about --prime-ms of the workload on every lane, so that
+    # the GPU's clocks are at their loaded state when the warmup steps run.
+    t_prime = time.perf_counter()
+    while (time.perf_counter() - t_prime) * 1e3 < args.prime_ms:
+        run_lanes(lanes, louts, 2 * len(lanes))
+    for j, ln in enumerate(lanes):
+        for i in range(args.warmup):
+            step(i + j, c=ln, out=louts[j])
+    # Timed region: HIP events bracket only the two kernels the rooflines are
+    # computed for (every bracket costs host time inside the step).
+    kernels = ("pack_chain", "pack_front", "first_dim", "pack_step1", "pack_b0", "expand_seeds", "rows_ntt", "y_ntt")
+    for ln in lanes:
+        ln.set_profiling_kernels(["pack_chain", "pack_front"])
+        ln.set_profiling(True)
+        for name in kernels:
+            ln.profile(name, reset=True)
     elapsed, lat = run_lanes(lanes, louts, args.steps)
     n_lanes = len(lanes)
     prof = {}
@@ -313,6 +320,37 @@ def main():
     qps = B * args.steps / elapsed
     log(f"batched: {qps:.0f} queries/s, {elapsed / args.steps * 1e3:.3f} ms/step over {n_lanes} lanes, "
         f"p50 {percentile(lat, 50):.3f} ms")
+    # The same timed region without profiling events, with graph replay of the
+    # repeated batches on and then off (a profiled call always runs eagerly).
+    graphs_ab = {}
+    for on in (True, False):
+        for j, ln in enumerate(lanes):
+            ln.set_graphs(on)
+            for i in range(max(args.warmup, 2 * n_sets)):  # every query set seen twice: captured
+                step(i + j, c=ln, out=louts[j])
+        caps0 = [ln.graph_stats() for ln in lanes]
+        el, lt = run_lanes(lanes, louts, args.steps)
+        caps1 = [ln.graph_stats() for ln in lanes]
+        graphs_ab["graphs_on" if on else "graphs_off"] = {
+            "value": round(B * args.steps / el, 1), "ms_per_step": round(el / args.steps * 1e3, 3),
+            "p50_ms": round(percentile(lt, 50), 3), "p99_ms": round(percentile(lt, 99), 3),
+            "graph_replays": int(sum(b[1] - a[1] for a, b in zip(caps0, caps1)))}
+        log(f"  no profiling, graphs {'on' if on else 'off'}: {graphs_ab['graphs_on' if on else 'graphs_off']}")
+    for ln in lanes:
+        ln.set_graphs(True)
+    # the headline region once more (profiled, same K steps) after the others:
+    # the spread between two measurements of the same thing in one process
+    for ln in lanes:
+        ln.set_profiling_kernels(["pack_chain", "pack_front"])
+        ln.set_profiling(True)
+    el, lt = run_lanes(lanes, louts, args.steps)
+    for ln in lanes:
+        for name in kernels:
+            ln.profile(name, reset=True)
+        ln.set_profiling(False)
+    graphs_ab["profiled_repeat"] = {"value": round(B * args.steps / el, 1), "ms_per_step": round(el / args.steps * 1e3, 3),
+                                    "p50_ms": round(percentile(lt, 50), 3), "p99_ms": round(percentile(lt, 99), 3)}
+    log(f"  profiled repeat: {graphs_ab['profiled_repeat']}")
     # the same batches from other numbers of lanes (more lanes: more throughput, longer latency)
     sweep = []
     for nl in [int(x) for x in args.lanes_sweep.split(",") if x.strip()]:
@@ -442,6 +480,7 @@ def main():
         "speedup_vs_single_query_launch": round(qps / single_qps, 2) if single_qps else None,
         "host_buffer_api_qps": round(host_qps, 1) if host_qps else None,
         "lanes_sweep": sweep,
+        "graphs_ab": graphs_ab,
         "modmul_per_query": MODMUL_QUERY,
         "roofline": roofline,
         "roofline_hbm_front": roofline_hbm,

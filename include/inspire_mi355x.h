@@ -226,6 +226,21 @@ typedef struct {
 } inspire_device_batch;
 
 int inspire_respond_batch_device(inspire_ctx *ctx, const inspire_device_batch *batch);
+/* Graph replay of device batches (on by default).  The launch sequence of
+ * inspire_respond_batch_device is captured into a HIP graph the second time
+ * the same batch shape and buffers (device pointers, count, database column
+ * count, variant) arrive within eight calls, and replayed whenever they
+ * recur (up to eight graphs per context or lane, least recently used
+ * dropped first), so a
+ * serving loop that reuses its buffers submits one graph per batch instead
+ * of each kernel and copy.  The buffers' contents may change between calls;
+ * results are those of the eager launches.  While profiling is enabled
+ * (inspire_ctx_set_profiling) batches run eagerly, so that the HIP events
+ * time each kernel.  enable = 0 runs every batch eagerly and drops the
+ * captured graphs.  Neither call has a counterpart in
+ * the reference (which has no device). */
+int inspire_ctx_set_graphs(inspire_ctx *ctx, int enable);
+int inspire_ctx_graph_stats(const inspire_ctx *ctx, uint64_t *captures, uint64_t *replays);
 
 /* inspire_respond_batch_device without any allocation: no hipMalloc,
  * hipFree or pinned host allocation in the call.  This entry point has no

@@ -126,6 +126,22 @@ struct ClientPool {
     std::unordered_map<uint32_t, uint32_t> slot_of;  // client id -> slot
 };
 
+// A device batch's launch sequence captured once and replayed (hipGraph):
+// the key is every value its launches take as arguments (batch shape,
+// caller buffers, workspaces, staging, resident tables).
+struct GraphEntry {
+    std::vector<uint64_t> key;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+struct GraphCache {
+    bool enabled = true;
+    std::vector<std::vector<uint64_t>> last_eager;  // keys of the last calls run eagerly
+    std::vector<GraphEntry> entries;    // least recently used first
+    uint64_t replays = 0, captures = 0;
+};
+void graph_cache_clear(GraphCache& G);
+
 struct HostModulus {
     uint64_t q = 0, psi = 0;
     std::vector<uint32_t> tw, twp, itw, itwp;
@@ -166,6 +182,7 @@ struct inspire_ctx {
     inspire::PinBuf hin, hkeys, hout;
     std::vector<uint32_t> hbuf;
     inspire::Profiler prof;
+    inspire::GraphCache graphs;
 };
 
 namespace inspire {

@@ -308,6 +308,7 @@ int inspire_ctx_create(const inspire_params* params, const uint8_t w_seed[32], i
 void inspire_ctx_destroy(inspire_ctx* ctx) {
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    graph_cache_clear(ctx->graphs);
     for (auto& s : ctx->prof.spans) {
         (void)hipEventDestroy(s.a);
         (void)hipEventDestroy(s.b);
@@ -343,6 +344,25 @@ int inspire_respond_batch_device(inspire_ctx* ctx, const inspire_device_batch* b
         std::lock_guard<std::recursive_mutex> g(ctx->mu);
         LaneRead lr(ctx);
         respond_batch_device(ctx, *batch);
+        return INSPIRE_OK;
+    })
+}
+
+int inspire_ctx_set_graphs(inspire_ctx* ctx, int enable) {
+    INSPIRE_TRY({
+        if (!ctx) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "null argument"};
+        std::lock_guard<std::recursive_mutex> g(ctx->mu);
+        graph_cache_clear(ctx->graphs);
+        ctx->graphs.enabled = enable != 0;
+        return INSPIRE_OK;
+    })
+}
+
+int inspire_ctx_graph_stats(const inspire_ctx* ctx, uint64_t* captures, uint64_t* replays) {
+    INSPIRE_TRY({
+        if (!ctx || !captures || !replays) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "null argument"};
+        *captures = ctx->graphs.captures;
+        *replays = ctx->graphs.replays;
         return INSPIRE_OK;
     })
 }

@@ -7,6 +7,7 @@
 //   DB digits       [shard][col][ell][K][N] NTT of the gadget digits of Delta*h
 //   RGSW b-rows     [query][ell][2][K][N]   rows ell..2ell-1 in NTT domain, Montgomery
 //                                            form (value * 2^32 mod q)
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -544,10 +545,11 @@ hipError_t launch_sum_cols(uint32_t* resp, uint32_t Q, uint32_t cols, const CtxD
 }
 
 hipError_t launch_rows_seeded(const uint32_t* seeds, const uint32_t* bsrc, uint32_t* rows, size_t count,
-                              uint32_t* err, const CtxDev& C, const ScaleDev& sc, hipStream_t st) {
+                              uint32_t* err, const CtxDev& C, const ScaleDev& sc, hipStream_t st, LaunchEv ev) {
     if (!count) return hipSuccess;
     if (C.logn != 11 || count > 0x7fffffffu) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_rows_seeded, dim3((unsigned)count), dim3(256), 0, st, seeds, bsrc, rows, err, C, sc);
+    hipExtLaunchKernelGGL(k_rows_seeded, dim3((unsigned)count), dim3(256), 0, st, ev.start, ev.stop, 0, seeds, bsrc,
+                          rows, err, C, sc);
     return hipGetLastError();
 }
 

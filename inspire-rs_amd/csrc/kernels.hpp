@@ -34,6 +34,13 @@ struct ScaleDev {
     uint32_t c[kMaxK], cp[kMaxK], cw[kMaxK], cwp[kMaxK];
 };
 
+// Events a launch records around its kernel's execution (hipExtLaunchKernelGGL):
+// the start and stop of the dispatch itself, with no extra stream operations.
+// Null events: a plain launch.
+struct LaunchEv {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+
 // In-place transform of count contiguous [K][N] polys.
 hipError_t launch_ntt(uint32_t* data, size_t count, const CtxDev& C, const ScaleDev& sc, bool inverse,
                       hipStream_t st);
@@ -57,6 +64,7 @@ hipError_t launch_sum_cols(uint32_t* resp, uint32_t Q, uint32_t cols, const CtxD
 // seeds + 8i, b at bsrc + i*K*N; out rows + i*2*K*N (a, then b), Montgomery
 // form via sc.  Returns hipErrorInvalidValue for other degrees.
 hipError_t launch_rows_seeded(const uint32_t* seeds, const uint32_t* bsrc, uint32_t* rows, size_t count,
-                              uint32_t* err, const CtxDev& C, const ScaleDev& sc, hipStream_t st);
+                              uint32_t* err, const CtxDev& C, const ScaleDev& sc, hipStream_t st,
+                              LaunchEv ev = LaunchEv{});
 
 }  // namespace inspire

@@ -307,9 +307,9 @@ void pack_run(inspire_ctx* ctx, uint32_t gamma, uint32_t Q, const uint32_t* rows
         d_bpart = reinterpret_cast<uint64_t*>(d_rpre + rpre_words);
     }
     if (T.fast_front) {
-        ProfScope ps(ctx, P_FRONT);
+        ProfLaunch pl(ctx, P_FRONT);
         hip_check(launch_pack_front(d_qdig, rows, d_rpre, d_bpart, T.twist.get<uint32_t>(), T.dftw.get<uint2>(), Q,
-                                    gamma, rw || K == 1, ctx->dev, ctx->stream),
+                                    gamma, rw || K == 1, ctx->dev, ctx->stream, LaunchEv{pl.start(), pl.stop()}),
                   "pack front");
     } else {
         uint32_t* d_cols = ws.cols;
@@ -379,8 +379,10 @@ void pack_run(inspire_ctx* ctx, uint32_t gamma, uint32_t Q, const uint32_t* rows
         A.q1r64ip = shoup_pre(A.q1r64i, (uint32_t)q1);
         A.one_q1p = shoup_pre(1, (uint32_t)q1);
     }
-    ProfScope ps(ctx, P_CHAIN);
-    hip_check(launch_pack_chain_scaled(A, Q, ctx->dev, scale_inv(ctx, nullptr), ctx->stream), "pack chain");
+    ProfLaunch pl(ctx, P_CHAIN);
+    hip_check(launch_pack_chain_scaled(A, Q, ctx->dev, scale_inv(ctx, nullptr), ctx->stream,
+                                       LaunchEv{pl.start(), pl.stop()}),
+              "pack chain");
 }
 
 static void upload_meta(inspire_ctx* ctx, uint32_t Q, const std::vector<uint64_t>& y_off,

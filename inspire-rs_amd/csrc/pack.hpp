@@ -106,7 +106,7 @@ hipError_t launch_pack_b0(const uint32_t* cols, uint64_t* bpart, uint32_t Q, uin
 // qdig[q]: device address of query q's shard digits (a valid shard also for a missing one).
 hipError_t launch_pack_front(const uint64_t* qdig, const uint32_t* rows, uint32_t* r_pre, uint64_t* bpart,
                              const uint32_t* twist, const uint2* dftw, uint32_t Q, uint32_t gamma, bool rw,
-                             const CtxDev& C, hipStream_t st);
+                             const CtxDev& C, hipStream_t st, LaunchEv ev = LaunchEv{});
 // stepu: [gamma][K][N] u = w_s^(g^(n-i)) in Montgomery form; ginvm: gamma^-1 2^32 mod q_0, q_1.
 hipError_t launch_pack_step1(const uint32_t* cols, const uint32_t* stepu, uint32_t* r_pre, uint32_t Q,
                              uint32_t gamma, uint32_t kres, uint2 ginvm, const CtxDev& C, hipStream_t st);
@@ -115,9 +115,9 @@ hipError_t launch_pack_step1(const uint32_t* cols, const uint32_t* stepu, uint32
 // uses it whenever pack_chain_crt_supported holds.
 bool pack_chain_crt_supported(const CtxDev& C);
 hipError_t launch_pack_chain_crt(const PackArgs& A, uint32_t Q, const CtxDev& C, const ScaleDev& inv,
-                                 hipStream_t st);
+                                 hipStream_t st, LaunchEv ev = LaunchEv{});
 // inv: inverse-NTT scaling (plain inverse: n^-1 folded into the last stage).
 hipError_t launch_pack_chain_scaled(const PackArgs& A, uint32_t Q, const CtxDev& C, const ScaleDev& inv,
-                                    hipStream_t st);
+                                    hipStream_t st, LaunchEv ev = LaunchEv{});
 
 }  // namespace inspire

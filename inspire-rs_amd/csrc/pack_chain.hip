@@ -27,6 +27,7 @@
 // traffic only, so the global loads of the next multiply-accumulate's
 // operands (w_all, the r_pre gathers, the permutation) stay in flight across
 // the transforms.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
@@ -337,12 +338,12 @@ static hipError_t chain_crt_lds_limit() {
 
 template <int LOGN, int ELL, bool LAZY, bool REC, bool SGN, int V>
 static hipError_t chain_crt_launch_v(const PackArgs& A, uint32_t Q, const CtxDev& C, const ScaleDev& inv,
-                                     hipStream_t st) {
+                                     hipStream_t st, LaunchEv ev) {
     const hipError_t e = chain_crt_lds_limit<LOGN, ELL, LAZY, REC, SGN, V>();
     if (e != hipSuccess) return e;
     auto kern = k_pack_chain_crt<LOGN, ELL, LAZY, REC, SGN, V>;
     constexpr size_t shmem = ChainCrtShape<LOGN, ELL, REC, V>::kBytes;
-    hipLaunchKernelGGL(kern, dim3(Q), dim3(2 * ((1 << LOGN) / V)), shmem, st, A, C, inv);
+    hipExtLaunchKernelGGL(kern, dim3(Q), dim3(2 * ((1 << LOGN) / V)), shmem, st, ev.start, ev.stop, 0, A, C, inv);
     return hipGetLastError();
 }
 
@@ -354,8 +355,8 @@ constexpr int kChainV = 4;
 
 template <int LOGN, int ELL, bool LAZY, bool REC, bool SGN = false>
 static hipError_t chain_crt_launch(const PackArgs& A, uint32_t Q, const CtxDev& C, const ScaleDev& inv,
-                                   hipStream_t st) {
-    return chain_crt_launch_v<LOGN, ELL, LAZY, REC, SGN, kChainV>(A, Q, C, inv, st);
+                                   hipStream_t st, LaunchEv ev) {
+    return chain_crt_launch_v<LOGN, ELL, LAZY, REC, SGN, kChainV>(A, Q, C, inv, st, ev);
 }
 
 // The SGN variant's bounds (signed_digits_ok): R < 2q enters the recovered
@@ -373,29 +374,30 @@ static bool chain_signed_ok(const CtxDev& C, int logn, int ell, bool rec) {
 }
 
 template <int LOGN, int ELL>
-static hipError_t chain_crt_t(const PackArgs& A, uint32_t Q, const CtxDev& C, const ScaleDev& inv, hipStream_t st) {
+static hipError_t chain_crt_t(const PackArgs& A, uint32_t Q, const CtxDev& C, const ScaleDev& inv, hipStream_t st,
+                              LaunchEv ev) {
     const uint64_t qmax = C.m[0].q > C.m[1].q ? C.m[0].q : C.m[1].q;
     // the recovered digit needs the composite value to fit ell digits: q <= z^ell
     const bool rec = (uint64_t)C.gadget_log * ELL >= 64 || C.q <= (1ull << (C.gadget_log * ELL));
     if ((uint64_t)ELL * 4 * qmax < (1ull << 32)) {
         if (chain_signed_ok(C, LOGN, ELL, rec)) {
-            if (rec) return chain_crt_launch<LOGN, ELL, true, true, true>(A, Q, C, inv, st);
-            return chain_crt_launch<LOGN, ELL, true, false, true>(A, Q, C, inv, st);
+            if (rec) return chain_crt_launch<LOGN, ELL, true, true, true>(A, Q, C, inv, st, ev);
+            return chain_crt_launch<LOGN, ELL, true, false, true>(A, Q, C, inv, st, ev);
         }
-        if (rec) return chain_crt_launch<LOGN, ELL, true, true>(A, Q, C, inv, st);
-        return chain_crt_launch<LOGN, ELL, true, false>(A, Q, C, inv, st);
+        if (rec) return chain_crt_launch<LOGN, ELL, true, true>(A, Q, C, inv, st, ev);
+        return chain_crt_launch<LOGN, ELL, true, false>(A, Q, C, inv, st, ev);
     }
-    if (rec) return chain_crt_launch<LOGN, ELL, false, true>(A, Q, C, inv, st);
-    return chain_crt_launch<LOGN, ELL, false, false>(A, Q, C, inv, st);
+    if (rec) return chain_crt_launch<LOGN, ELL, false, true>(A, Q, C, inv, st, ev);
+    return chain_crt_launch<LOGN, ELL, false, false>(A, Q, C, inv, st, ev);
 }
 
 template <int LOGN>
 static hipError_t chain_crt_ell(const PackArgs& A, uint32_t Q, const CtxDev& C, const ScaleDev& inv,
-                                hipStream_t st) {
+                                hipStream_t st, LaunchEv ev) {
     switch (C.ell) {
-        case 1: return chain_crt_t<LOGN, 1>(A, Q, C, inv, st);
-        case 2: return chain_crt_t<LOGN, 2>(A, Q, C, inv, st);
-        case 3: return chain_crt_t<LOGN, 3>(A, Q, C, inv, st);
+        case 1: return chain_crt_t<LOGN, 1>(A, Q, C, inv, st, ev);
+        case 2: return chain_crt_t<LOGN, 2>(A, Q, C, inv, st, ev);
+        case 3: return chain_crt_t<LOGN, 3>(A, Q, C, inv, st, ev);
         default: return hipErrorInvalidValue;
     }
 }
@@ -408,13 +410,13 @@ bool pack_chain_crt_supported(const CtxDev& C) {
 }
 
 hipError_t launch_pack_chain_crt(const PackArgs& A, uint32_t Q, const CtxDev& C, const ScaleDev& inv,
-                                 hipStream_t st) {
+                                 hipStream_t st, LaunchEv ev) {
     if (!Q) return hipSuccess;
     switch (C.logn) {
-        case 8: return chain_crt_ell<8>(A, Q, C, inv, st);
-        case 9: return chain_crt_ell<9>(A, Q, C, inv, st);
-        case 10: return chain_crt_ell<10>(A, Q, C, inv, st);
-        case 11: return chain_crt_ell<11>(A, Q, C, inv, st);
+        case 8: return chain_crt_ell<8>(A, Q, C, inv, st, ev);
+        case 9: return chain_crt_ell<9>(A, Q, C, inv, st, ev);
+        case 10: return chain_crt_ell<10>(A, Q, C, inv, st, ev);
+        case 11: return chain_crt_ell<11>(A, Q, C, inv, st, ev);
         default: return hipErrorInvalidValue;
     }
 }

@@ -16,6 +16,7 @@
 // the two residues of every transform are processed by the two halves of the
 // block at once, the digits live in LDS, R and the online accumulator live in
 // registers (8 elements per thread), so T[i] is never written out.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -399,42 +400,42 @@ hipError_t launch_pack_b0(const uint32_t* cols, uint64_t* bpart, uint32_t Q, uin
 template <int LG, int ELL>
 static hipError_t front_t(const uint64_t* qdig, const uint32_t* rows, uint32_t* r_pre, uint64_t* bpart,
                           const uint32_t* twist, const uint2* dftw, uint32_t Q, bool rw, const CtxDev& C,
-                          hipStream_t st) {
+                          hipStream_t st, LaunchEv ev) {
     const dim3 grid(Q * (C.n / 256));
     if (rw)
-        hipLaunchKernelGGL((k_pack_front<LG, true, ELL>), grid, dim3(256), 0, st, qdig, rows, r_pre, bpart, twist,
-                           dftw, C);
+        hipExtLaunchKernelGGL((k_pack_front<LG, true, ELL>), grid, dim3(256), 0, st, ev.start, ev.stop, 0, qdig, rows,
+                              r_pre, bpart, twist, dftw, C);
     else
-        hipLaunchKernelGGL((k_pack_front<LG, false, ELL>), grid, dim3(256), 0, st, qdig, rows, r_pre, bpart, twist,
-                           dftw, C);
+        hipExtLaunchKernelGGL((k_pack_front<LG, false, ELL>), grid, dim3(256), 0, st, ev.start, ev.stop, 0, qdig,
+                              rows, r_pre, bpart, twist, dftw, C);
     return hipGetLastError();
 }
 
 template <int LG>
 static hipError_t front_lg(const uint64_t* qdig, const uint32_t* rows, uint32_t* r_pre, uint64_t* bpart,
                            const uint32_t* twist, const uint2* dftw, uint32_t Q, bool rw, const CtxDev& C,
-                           hipStream_t st) {
+                           hipStream_t st, LaunchEv ev) {
     switch (C.ell) {
-        case 1: return front_t<LG, 1>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st);
-        case 2: return front_t<LG, 2>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st);
-        case 3: return front_t<LG, 3>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st);
-        case 4: return front_t<LG, 4>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st);
+        case 1: return front_t<LG, 1>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st, ev);
+        case 2: return front_t<LG, 2>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st, ev);
+        case 3: return front_t<LG, 3>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st, ev);
+        case 4: return front_t<LG, 4>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st, ev);
         default: return hipErrorInvalidValue;
     }
 }
 
 hipError_t launch_pack_front(const uint64_t* qdig, const uint32_t* rows, uint32_t* r_pre, uint64_t* bpart,
                              const uint32_t* twist, const uint2* dftw, uint32_t Q, uint32_t gamma, bool rw,
-                             const CtxDev& C, hipStream_t st) {
+                             const CtxDev& C, hipStream_t st, LaunchEv ev) {
     if (!Q) return hipSuccess;
     switch (gamma) {
-        case 1: return front_lg<0>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st);
-        case 2: return front_lg<1>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st);
-        case 4: return front_lg<2>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st);
-        case 8: return front_lg<3>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st);
-        case 16: return front_lg<4>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st);
-        case 32: return front_lg<5>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st);
-        case 64: return front_lg<6>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st);
+        case 1: return front_lg<0>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st, ev);
+        case 2: return front_lg<1>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st, ev);
+        case 4: return front_lg<2>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st, ev);
+        case 8: return front_lg<3>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st, ev);
+        case 16: return front_lg<4>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st, ev);
+        case 32: return front_lg<5>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st, ev);
+        case 64: return front_lg<6>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st, ev);
         default: return hipErrorInvalidValue;
     }
 }
@@ -466,9 +467,9 @@ static hipError_t chain_k(const PackArgs& A, uint32_t Q, const CtxDev& C, const 
 }
 
 hipError_t launch_pack_chain_scaled(const PackArgs& A, uint32_t Q, const CtxDev& C, const ScaleDev& inv,
-                                    hipStream_t st) {
+                                    hipStream_t st, LaunchEv ev) {
     if (!Q) return hipSuccess;
-    if (pack_chain_crt_supported(C)) return launch_pack_chain_crt(A, Q, C, inv, st);
+    if (pack_chain_crt_supported(C)) return launch_pack_chain_crt(A, Q, C, inv, st, ev);
     switch (C.logn) {
         case 8: return chain_k<8>(A, Q, C, inv, st);
         case 9: return chain_k<9>(A, Q, C, inv, st);

@@ -96,6 +96,82 @@ void prof_collect(inspire_ctx* ctx) {
     P.spans.clear();
 }
 
+void graph_cache_clear(GraphCache& G) {
+    for (auto& e : G.entries) {
+        if (e.exec) (void)hipGraphExecDestroy(e.exec);
+        if (e.graph) (void)hipGraphDestroy(e.graph);
+    }
+    G.entries.clear();
+    G.last_eager.clear();
+}
+
+namespace {
+
+constexpr size_t kGraphCacheEntries = 8;
+
+// Runs `enqueue` (every stream operation of one device batch) eagerly, or
+// replays the graph captured for `key`.  A key seen on one of the last
+// eager calls is captured now, so one-off shapes never pay for a capture.  When
+// capture or instantiation is refused the call runs eagerly.  Profiling is
+// off here (profiled calls run eagerly), so no events enter a graph.
+template <typename F>
+void run_with_graphs(inspire_ctx* ctx, const std::vector<uint64_t>& key, F&& enqueue) {
+    GraphCache& G = ctx->graphs;
+    for (size_t i = 0; i < G.entries.size(); i++) {
+        if (G.entries[i].key != key) continue;
+        GraphEntry e = std::move(G.entries[i]);  // most recently used goes last
+        G.entries.erase(G.entries.begin() + (ptrdiff_t)i);
+        const hipError_t le = hipGraphLaunch(e.exec, ctx->stream);
+        G.entries.push_back(std::move(e));
+        hip_check(le, "graph launch");
+        G.replays++;
+        return;
+    }
+    auto seen = std::find(G.last_eager.begin(), G.last_eager.end(), key);
+    if (seen == G.last_eager.end()) {
+        enqueue();
+        G.last_eager.push_back(key);
+        if (G.last_eager.size() > kGraphCacheEntries) G.last_eager.erase(G.last_eager.begin());
+        return;
+    }
+    G.last_eager.erase(seen);
+    if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipGetLastError();
+        G.enabled = false;  // this runtime cannot capture: stay eager
+        enqueue();
+        return;
+    }
+    hipGraph_t g = nullptr;
+    try {
+        enqueue();
+    } catch (...) {
+        (void)hipStreamEndCapture(ctx->stream, &g);
+        (void)hipGetLastError();
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    hip_check(hipStreamEndCapture(ctx->stream, &g), "end capture");
+    GraphEntry e;
+    e.key = key;
+    e.graph = g;
+    if (hipGraphInstantiate(&e.exec, g, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipGraphDestroy(g);
+        enqueue();
+        return;
+    }
+    if (G.entries.size() >= kGraphCacheEntries) {  // evict the least recently used
+        (void)hipGraphExecDestroy(G.entries.front().exec);
+        (void)hipGraphDestroy(G.entries.front().graph);
+        G.entries.erase(G.entries.begin());
+    }
+    G.entries.push_back(std::move(e));
+    G.captures++;
+    hip_check(hipGraphLaunch(G.entries.back().exec, ctx->stream), "graph launch");
+}
+
+}  // namespace
+
 void upload_u64(inspire_ctx* ctx, const uint64_t* src, size_t words, uint32_t* dst) {
     if (!words) return;
     ctx->hbuf.resize(words);
@@ -345,8 +421,9 @@ void stage_rows_from(inspire_ctx* ctx, uint32_t Q, uint32_t S, uint32_t* rows_ws
     for (uint32_t k = 0; k < ctx->K; k++) mont[k] = mont_of(ctx, k);
     const ScaleDev sc = scale_fwd(ctx, mont);
     if (S && ctx->logn == 11) {  // expansion and both transforms in one pass
-        ProfScope ps(ctx, P_EXPAND);
-        hip_check(launch_rows_seeded(seeds, bsrc, rows_ws, (size_t)S * ell, d_err, ctx->dev, sc, ctx->stream),
+        ProfLaunch pl(ctx, P_EXPAND);
+        hip_check(launch_rows_seeded(seeds, bsrc, rows_ws, (size_t)S * ell, d_err, ctx->dev, sc, ctx->stream,
+                                     LaunchEv{pl.start(), pl.stop()}),
                   "seeded rows");
     } else if (S) {
         {
@@ -697,17 +774,41 @@ void respond_batch_device(inspire_ctx* ctx, const inspire_device_batch& b, uint8
         b.status[t] = found ? INSPIRE_OK : INSPIRE_ERR_SHARD_NOT_FOUND;
     }
     hdig[Q] = 0;  // the error word
-    hip_check(hipMemcpyAsync(d_qdig, hdig, (size_t)Q * 8 + 8, hipMemcpyHostToDevice, ctx->stream), "digits");
-    // rows: transformed from the caller's buffers into the workspace layout
-    if (b.seeded) stage_rows_from(ctx, Q, Q, rows, b.seeds, b.rows, nullptr, d_err);
-    else stage_rows_from(ctx, Q, 0, rows, nullptr, nullptr, b.rows, d_err);
-    if (tree) tree_run(ctx, db.cols, Q, rows, d_qdig, b.out);
-    else if (!packed) run_respond0(ctx, db.cols, Q, rows, d_qdig, b.out);
-    else pack_run_device_ybody(ctx, db.cols, Q, rows, d_qdig, b.y_body, b.out, pws, y_ws);
     uint32_t* herr = reinterpret_cast<uint32_t*>(hdig + Q + 1);
-    hip_check(hipMemcpyAsync(herr, d_err, 4, hipMemcpyDeviceToHost, ctx->stream), "err");
+    auto enqueue = [&] {
+        hip_check(hipMemcpyAsync(d_qdig, hdig, (size_t)Q * 8 + 8, hipMemcpyHostToDevice, ctx->stream), "digits");
+        // rows: transformed from the caller's buffers into the workspace layout
+        if (b.seeded) stage_rows_from(ctx, Q, Q, rows, b.seeds, b.rows, nullptr, d_err);
+        else stage_rows_from(ctx, Q, 0, rows, nullptr, nullptr, b.rows, d_err);
+        if (tree) tree_run(ctx, db.cols, Q, rows, d_qdig, b.out);
+        else if (!packed) run_respond0(ctx, db.cols, Q, rows, d_qdig, b.out);
+        else pack_run_device_ybody(ctx, db.cols, Q, rows, d_qdig, b.y_body, b.out, pws, y_ws);
+        hip_check(hipMemcpyAsync(herr, d_err, 4, hipMemcpyDeviceToHost, ctx->stream), "err");
+    };
+    // Events recorded inside a replayed graph do not time its kernels on this
+    // runtime, so a profiled call runs eagerly.
+    if (!scratch && ctx->graphs.enabled && !(ctx->prof.on && !ctx->prof.suspended)) {
+        // every value the launches take: the batch's shape and buffers, the
+        // workspaces (grown by an earlier call of this shape), the resident
+        // tables they read, and which kernels are bracketed by profiling events
+        const inspire_ctx* R = root(ctx);
+        const std::vector<uint64_t> key = {
+            Q, db.cols, (uint64_t)tree | (uint64_t)packed << 1 | (uint64_t)(b.seeded != 0) << 2,
+            reinterpret_cast<uint64_t>(b.rows), reinterpret_cast<uint64_t>(b.seeds),
+            reinterpret_cast<uint64_t>(b.y_body), reinterpret_cast<uint64_t>(b.out),
+            reinterpret_cast<uint64_t>(ctx->ws0.get()), reinterpret_cast<uint64_t>(ctx->ws1.get()),
+            reinterpret_cast<uint64_t>(ctx->ws2.get()), reinterpret_cast<uint64_t>(ctx->ws3.get()),
+            reinterpret_cast<uint64_t>(ctx->ws4.get()), reinterpret_cast<uint64_t>(ctx->ws5.get()),
+            reinterpret_cast<uint64_t>(ctx->ws6.get()), reinterpret_cast<uint64_t>(ctx->pin.get()),
+            reinterpret_cast<uint64_t>(R->galois.get()), reinterpret_cast<uint64_t>(R->yhat.get())};
+        run_with_graphs(ctx, key, enqueue);
+    } else {
+        enqueue();
+    }
     hip_check(hipStreamSynchronize(ctx->stream), "device batch");
-    prof_collect(ctx);
+    // the batch's events are complete; their times are read when they are
+    // queried (or once enough accumulate), not inside every batch call
+    if (ctx->prof.spans.size() >= 256) prof_collect(ctx);
     if (*herr) throw Error{INSPIRE_ERR_DEVICE, "seed expansion exhausted its draw budget"};
 }
 

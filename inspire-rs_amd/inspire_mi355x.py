@@ -196,6 +196,8 @@ EXPORTS = [
     ),
     ("inspire_respond_batch_device", C.c_int, [C.c_void_p, C.POINTER(_DeviceBatch)]),
     ("inspire_respond_batch_device_noalloc", C.c_int, [C.c_void_p, C.POINTER(_DeviceBatch)]),
+    ("inspire_ctx_set_graphs", C.c_int, [C.c_void_p, C.c_int]),
+    ("inspire_ctx_graph_stats", C.c_int, [C.c_void_p, _u64p, _u64p]),
     ("inspire_register_client_keys", C.c_int, [C.c_void_p, C.c_uint32, _u64p]),
     ("inspire_server_create", C.c_int, [C.c_void_p, C.POINTER(_ServerConfig), C.POINTER(C.c_void_p)]),
     ("inspire_server_submit", C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(_Request), _u64p]),
@@ -567,6 +569,16 @@ class Context:
         b = _ExtractBatch(count, mode, entry_size, num_column_ciphertexts, b_dim, 0, stride, responses_ptr,
                           secret_keys_ptr, entries_ptr)
         _raise(lib().inspire_extract_batch_device(self._h, C.byref(b)))
+
+    def set_graphs(self, on: bool):
+        """Graph replay of repeated device batches (on by default)."""
+        _raise(lib().inspire_ctx_set_graphs(self._h, 1 if on else 0))
+
+    def graph_stats(self):
+        """-> (captures, replays) of device-batch graphs on this handle."""
+        a, b = C.c_uint64(), C.c_uint64()
+        _raise(lib().inspire_ctx_graph_stats(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def set_profiling(self, on: bool):
         _raise(lib().inspire_ctx_set_profiling(self._h, 1 if on else 0))
