@@ -40,7 +40,7 @@ struct NttShape {
     static constexpr int REM = LOGN % 3;       // leftover stages (0, 1 or 2)
 };
 
-__device__ __forceinline__ int lds_ix(int x) { return x + 4 * (x >> 5); }
+__device__ __forceinline__ int lds_ix(int x) { return (int)((unsigned)x + 4 * ((unsigned)x >> 5)); }  // x >= 0
 
 // Twiddle sources: tw(i) -> (psi^br(i), Shoup companion), itw(i) likewise for
 // psi^-1.  GlobalTw reads the tables of a ModDev; LdsTw reads (w, w') pairs

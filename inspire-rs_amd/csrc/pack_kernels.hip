@@ -175,8 +175,11 @@ __device__ __forceinline__ void front_residue(int k, const uint32_t* R, const ui
     }
 }
 
+// At most 64 VGPRs (eight waves per SIMD): a 256-query launch is 8192 waves,
+// which then all fit on the 256 CUs at once instead of leaving a second,
+// nearly empty round of workgroups.
 template <int LG, bool RW, int ELL>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
 k_pack_front(const uint64_t* __restrict__ qdig, const uint32_t* __restrict__ rows, uint32_t* __restrict__ r_pre,
              uint64_t* __restrict__ bpart, const uint32_t* __restrict__ twist, const uint2* __restrict__ dftw,
              CtxDev C) {

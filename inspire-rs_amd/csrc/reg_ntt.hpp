@@ -203,7 +203,8 @@ __device__ __forceinline__ void intt1(uint32_t (&x)[8], int t, uint32_t* rb, con
     int t0 = 1 << L::REM;
 #pragma unroll
     for (int p = 0; p < L::R8; p++) {
-        const int i3 = t / t0, r = t - i3 * t0;
+        // thread indices are non-negative: unsigned division is a shift
+        const int i3 = (int)((unsigned)t / (unsigned)t0), r = (int)((unsigned)t % (unsigned)t0);
         const int bix = lds_ix(8 * t0 * i3 + r);
         if (!(L::REM == 0 && p == 0)) {
 #pragma unroll
@@ -281,7 +282,7 @@ __device__ __forceinline__ void ntt_digits(uint32_t (&x)[ELL][8], int t, uint32_
 #pragma unroll
     for (int p = 0; p < L::R8; p++) {
         const int t0 = N / (2 * m0), st = t0 >> 2;
-        const int i = t / st, r = t - i * st;
+        const int i = (int)((unsigned)t / (unsigned)st), r = (int)((unsigned)t % (unsigned)st);
         const int bix = lds_ix(2 * i * t0 + r);
         if (p > 0) {
 #pragma unroll
@@ -398,11 +399,12 @@ __device__ __forceinline__ void intt_v(uint32_t (&x)[V], int t, uint32_t* rb, co
 // immediate.
 // ---------------------------------------------------------------------------
 template <int KIND>
-__device__ __forceinline__ int xl_ix(int x) {
-    if constexpr (KIND == 1) return x + 8 * (x >> 5);
-    else if constexpr (KIND == 2) return x + 2 * (x >> 3);
-    else if constexpr (KIND == 3) return x + 32 * (x >> 7);
-    else return lds_ix(x);
+__device__ __forceinline__ int xl_ix(int x) {  // x >= 0
+    const unsigned u = (unsigned)x;
+    if constexpr (KIND == 1) return (int)(u + 8 * (u >> 5));
+    else if constexpr (KIND == 2) return (int)(u + 2 * (u >> 3));
+    else if constexpr (KIND == 3) return (int)(u + 32 * (u >> 7));
+    else return (int)(u + 4 * (u >> 5));
 }
 // Index of logical base + e st from the base's index: e st is a multiple of
 // the layout's block, or stays inside the base's block.
@@ -441,7 +443,7 @@ __device__ __forceinline__ void ntt4_pass(uint32_t (&x)[ELL][4], int t, int p, i
     using L = Lay4<LOGN>;
     constexpr int N = L::N, LP = L::LP;
     const int t0 = N / (2 * m0), st = t0 >> 1;
-    const int i = t / st, r = t - i * st;
+    const int i = (int)((unsigned)t / (unsigned)st), r = (int)((unsigned)t % (unsigned)st);
     if (p > 0) {
         const int bix = xl_ix<KR>(2 * i * t0 + r);
 #pragma unroll
@@ -530,7 +532,7 @@ __device__ __forceinline__ void intt4_passes(uint32_t (&x)[4], int t, uint32_t* 
     if constexpr (P < L::R4) {
         constexpr int KR = L::inv_kind(P - 1), KW = L::inv_kind(P);
         const int t0 = (1 << L::REM) << (2 * P);
-        const int i2 = t / t0, r = t - i2 * t0;
+        const int i2 = (int)((unsigned)t / (unsigned)t0), r = (int)((unsigned)t % (unsigned)t0);
         if (!(L::REM == 0 && P == 0)) {
             const int bix = xl_ix<KR>(4 * t0 * i2 + r);
 #pragma unroll
