@@ -437,6 +437,28 @@ int inspire_ctx_set_profiling(inspire_ctx *ctx, int enable);
 int inspire_ctx_set_profiling_kernels(inspire_ctx *ctx, const char *kernels);
 int inspire_ctx_profile(inspire_ctx *ctx, const char *kernel, double *total_ms, uint64_t *launches, int reset);
 
+/* ---- kernel-variant tuning (no counterpart in the reference) ----
+ *
+ * Two kernels of the respond path have variants with the same results:
+ *   "ntt_fwd_2048"            forward NTT at d = 2048 (database digits, keys,
+ *                             the stand-alone transforms): 0 = the LDS
+ *                             radix-8 kernel on the vector ALUs, 1 = a
+ *                             four-step transform on the matrix cores (int8
+ *                             MFMA over 7-bit limbs; moduli below 2^28);
+ *   "chain_values_per_thread" the two-modulus packing chain: 4 or 8.
+ * inspire_ctx_autotune times both variants of each on the device (2048
+ * transforms; a 256-query InsPIRe^2 batch over a throwaway database), checks
+ * that they agree word for word, keeps the faster, and returns (and, with
+ * path != NULL, writes) a JSON report of the choices and times.
+ * inspire_ctx_load_tuning applies a report written earlier.  Choices are
+ * per context and are copied into lanes created afterwards: tune the owner
+ * before creating lanes.  The report functions follow the wire encoders'
+ * buffer convention (out == NULL sizes it). */
+int inspire_ctx_autotune(inspire_ctx *ctx, const char *path, char *report, uint64_t capacity, uint64_t *len);
+int inspire_ctx_load_tuning(inspire_ctx *ctx, const char *path);
+int inspire_ctx_set_kernel_choice(inspire_ctx *ctx, const char *name, int value);
+int inspire_ctx_get_tuning(const inspire_ctx *ctx, char *out, uint64_t capacity, uint64_t *len);
+
 /* ---- math entry points (NttContext, external_product) ---- */
 
 /* NttContext::forward / inverse (src/math/ntt.rs:223-246, 308-325) over

@@ -157,6 +157,7 @@ struct inspire_ctx {
     inspire::HostModulus hm[inspire::kMaxK];
     int device = 0;           // the HIP device current at creation (owns the stream and tables)
     inspire::DevBuf tables;   // twiddles of every modulus
+    inspire::DevBuf mfma_tables;  // matrix-core NTT constants (d = 2048, moduli below 2^28)
     inspire::CtxDev dev{};
     hipStream_t stream = nullptr;
     std::recursive_mutex mu;

@@ -22,6 +22,7 @@
 #include "abi.hpp"
 #include "respond.hpp"
 #include "tree.hpp"
+#include "ntt_mfma.hpp"
 
 using namespace inspire;
 
@@ -298,6 +299,27 @@ int inspire_ctx_create(const inspire_params* params, const uint8_t w_seed[32], i
             D.q0_inv_q1 = (uint32_t)powmod64(ctx->hm[0].q % q1, q1 - 2, q1);
             D.q0_inv_q1p = shoup_pre(D.q0_inv_q1, q1);
             D.one_q1p = shoup_pre(1, q1);
+        }
+        if (ctx->N == 2048) {  // the matrix-core forward transform's tables (chosen by tuning)
+            bool small = true;
+            uint64_t qs[kMaxK], psis[kMaxK];
+            for (uint32_t k = 0; k < ctx->K; k++) {
+                qs[k] = ctx->hm[k].q;
+                psis[k] = ctx->hm[k].psi;
+                small = small && qs[k] < (1ull << 28);
+            }
+            if (small) {
+                const MfmaNttHost H = ntt_mfma_tables(ctx->K, qs, psis);
+                ctx->mfma_tables.alloc(H.bytes.size());
+                hip_check(hipMemcpy(ctx->mfma_tables.get(), H.bytes.data(), H.bytes.size(), hipMemcpyHostToDevice),
+                          "mfma tables");
+                const uint8_t* b = ctx->mfma_tables.get<uint8_t>();
+                D.mfma.tw1 = reinterpret_cast<const uint32_t*>(b + H.o_tw1);
+                D.mfma.tw2 = reinterpret_cast<const uint32_t*>(b + H.o_tw2);
+                D.mfma.w64 = reinterpret_cast<const int8_t*>(b + H.o_w64);
+                D.mfma.w32 = reinterpret_cast<const int8_t*>(b + H.o_w32);
+                D.mfma.c1 = reinterpret_cast<const uint32_t*>(b + H.o_c1);
+            }
         }
         hip_check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "stream create");
         *out = ctx.release();

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
+#include "ntt_mfma.hpp"
 #include "ntt.hpp"
 
 namespace inspire {
@@ -490,6 +491,8 @@ static hipError_t launch_ntt_t(const uint32_t* src, size_t ss, uint32_t* dst, si
 
 hipError_t launch_ntt_strided(const uint32_t* src, size_t src_stride, uint32_t* dst, size_t dst_stride, size_t count,
                               const CtxDev& C, const ScaleDev& sc, bool inverse, hipStream_t st) {
+    if (!inverse && C.ntt_mfma && ntt_mfma_supported(C))
+        return launch_ntt_fwd_mfma(src, src_stride, dst, dst_stride, count, C.mfma, C, sc, st);
     switch (C.logn) {
         case 8: return launch_ntt_t<8>(src, src_stride, dst, dst_stride, count, C, sc, inverse, st);
         case 9: return launch_ntt_t<9>(src, src_stride, dst, dst_stride, count, C, sc, inverse, st);

@@ -12,6 +12,15 @@ namespace inspire {
 constexpr int kMaxK = 2;      // CRT moduli
 constexpr int kMaxEll = 4;    // gadget digits
 
+// Constant tables of the matrix-core forward NTT at d = 2048 (ntt_mfma.hip).
+struct MfmaNttTables {
+    const uint32_t* tw1;  // [K][2048] psi^j 2^32 mod q (the negacyclic twist, Montgomery form)
+    const uint32_t* tw2;  // [K][32][64] w^(j1 m2) 2^32 mod q, w = psi^2
+    const int8_t* w64;    // [K][4][64][64] 7-bit limbs of w^(32 j2 m2)
+    const int8_t* w32;    // [K][4][32][32] 7-bit limbs of w^(64 m1 j1)
+    const uint32_t* c1;   // [K][8] 2^(7s) 2^32 mod q, s < 7 (limb-sum recombination)
+};
+
 // Everything a kernel needs about the parameter set.
 struct CtxDev {
     ModDev m[kMaxK];
@@ -25,7 +34,13 @@ struct CtxDev {
     uint64_t q;                   // composite modulus
     uint32_t delta[kMaxK], deltap[kMaxK];  // Delta mod q_k (Shoup pairs)
     uint32_t mont[kMaxK], montp[kMaxK];    // 2^32 mod q_k (Montgomery-form conversion)
+    MfmaNttTables mfma;           // null when the matrix-core transform does not apply
+    uint32_t ntt_mfma;            // 1: forward d = 2048 transforms run on the matrix cores
+    uint32_t chain_v;             // values per thread of the two-modulus packing chain: 4 (0) or 8
 };
+
+// d = 2048, every modulus below 2^28 and the tables built.
+bool ntt_mfma_supported(const CtxDev& C);
 
 // Per-residue output scaling.  Forward: x -> c x when enable.  Inverse: the
 // last stage computes c (X + Y) and cw (X - Y).

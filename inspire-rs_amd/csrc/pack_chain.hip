@@ -347,15 +347,16 @@ static hipError_t chain_crt_launch_v(const PackArgs& A, uint32_t Q, const CtxDev
     return hipGetLastError();
 }
 
-// Four values per thread: 2 N/4 threads per query, four waves per SIMD at
-// d = 2048 (84 VGPRs), where eight values per thread (the radix-8 layout,
-// 138 VGPRs) leave two: the extra waves hide the latency of each step's LDS
-// exchanges and barriers (0.133 against 0.152 ms per 256-query launch).
+// Values per thread: four by default (2 N/4 threads per query, four waves
+// per SIMD at d = 2048), where eight (the radix-8 layout) leave two waves per
+// SIMD to hide each step's LDS exchanges and barriers; CtxDev.chain_v selects
+// (the tuner measures both).
 constexpr int kChainV = 4;
 
 template <int LOGN, int ELL, bool LAZY, bool REC, bool SGN = false>
 static hipError_t chain_crt_launch(const PackArgs& A, uint32_t Q, const CtxDev& C, const ScaleDev& inv,
                                    hipStream_t st, LaunchEv ev) {
+    if (C.chain_v == 8) return chain_crt_launch_v<LOGN, ELL, LAZY, REC, SGN, 8>(A, Q, C, inv, st, ev);
     return chain_crt_launch_v<LOGN, ELL, LAZY, REC, SGN, kChainV>(A, Q, C, inv, st, ev);
 }
 
@@ -406,7 +407,7 @@ bool pack_chain_crt_supported(const CtxDev& C) {
     if (C.K != 2 || C.ell < 1 || C.ell > 3 || C.logn < 8 || C.logn > 11) return false;
     // every modulus must leave room for the lazy ranges: 4q < 2^32
     if (C.m[0].q >= (1u << 30) || C.m[1].q >= (1u << 30)) return false;
-    return ChainCrtShape<11, 3, false, kChainV>::kBytes <= 160 * 1024;
+    return ChainCrtShape<11, 3, false, kChainV>::kBytes <= 160 * 1024 && ChainCrtShape<11, 3, false, 8>::kBytes <= 160 * 1024;
 }
 
 hipError_t launch_pack_chain_crt(const PackArgs& A, uint32_t Q, const CtxDev& C, const ScaleDev& inv,

@@ -197,6 +197,10 @@ EXPORTS = [
     ("inspire_respond_batch_device", C.c_int, [C.c_void_p, C.POINTER(_DeviceBatch)]),
     ("inspire_respond_batch_device_noalloc", C.c_int, [C.c_void_p, C.POINTER(_DeviceBatch)]),
     ("inspire_ctx_set_graphs", C.c_int, [C.c_void_p, C.c_int]),
+    ("inspire_ctx_autotune", C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p, C.c_uint64, _u64p]),
+    ("inspire_ctx_load_tuning", C.c_int, [C.c_void_p, C.c_char_p]),
+    ("inspire_ctx_set_kernel_choice", C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
+    ("inspire_ctx_get_tuning", C.c_int, [C.c_void_p, C.c_char_p, C.c_uint64, _u64p]),
     ("inspire_ctx_graph_stats", C.c_int, [C.c_void_p, _u64p, _u64p]),
     ("inspire_register_client_keys", C.c_int, [C.c_void_p, C.c_uint32, _u64p]),
     ("inspire_server_create", C.c_int, [C.c_void_p, C.POINTER(_ServerConfig), C.POINTER(C.c_void_p)]),
@@ -569,6 +573,30 @@ class Context:
         b = _ExtractBatch(count, mode, entry_size, num_column_ciphertexts, b_dim, 0, stride, responses_ptr,
                           secret_keys_ptr, entries_ptr)
         _raise(lib().inspire_extract_batch_device(self._h, C.byref(b)))
+
+    def autotune(self, path: Optional[str] = None) -> dict:
+        """Times the kernel variants (inspire_ctx_autotune), keeps the faster
+        of each and returns the report; with path, also writes it there."""
+        import json
+
+        n = C.c_uint64()
+        buf = C.create_string_buffer(4096)
+        _raise(lib().inspire_ctx_autotune(self._h, path.encode() if path else None, buf, 4096, C.byref(n)))
+        return json.loads(buf.value.decode())
+
+    def load_tuning(self, path: str):
+        _raise(lib().inspire_ctx_load_tuning(self._h, path.encode()))
+
+    def set_kernel_choice(self, name: str, value: int):
+        _raise(lib().inspire_ctx_set_kernel_choice(self._h, name.encode(), int(value)))
+
+    def tuning(self) -> dict:
+        import json
+
+        n = C.c_uint64()
+        buf = C.create_string_buffer(1024)
+        _raise(lib().inspire_ctx_get_tuning(self._h, buf, 1024, C.byref(n)))
+        return json.loads(buf.value.decode())
 
     def set_graphs(self, on: bool):
         """Graph replay of repeated device batches (on by default)."""
