@@ -14,7 +14,7 @@
 // as sum_s S_s 2^(7s), reduced once by a Montgomery step with the constants
 // 2^(7s) 2^32 mod q (times the output scale in the second product).
 //
-// One workgroup of 256 threads (4 waves) per polynomial residue.  The
+// One workgroup of 256 threads (4 waves) per four polynomials of one residue.  The
 // matrix-core operand layouts of v_mfma_i32_16x16x{64,32}_i8 (checked on the
 // device): lane l holds row (A) or column (B) l % 16 and the k-group l / 16;
 // the 16 x 16 result holds column l % 16, rows 4 (l / 16) .. + 3.
@@ -35,58 +35,92 @@ constexpr int kYRow = 40;   // bytes per row of the transposed Y' limbs (32 used
 
 __device__ __forceinline__ uint32_t limb(uint32_t v, int a) { return (v >> (7 * a)) & 127u; }
 
-// Sum of the limb-sum accumulators of one result lane: sum_s acc[s][r] c[s], reduced.
+// Sum of the limb-sum accumulators of one result lane, sum_s S_s 2^(7s),
+// reduced: with S_s below (pairs a + b = s) * K * 127^2 (S_3 < 2^22 at
+// K = 64), the pairs S_2i + 2^7 S_2i+1 stay below 2^29.1, so four 64-bit
+// products with c[2i] = 2^(14 i) 2^32 mod q sum to under 2^58.1 < q 2^32,
+// the bound of one Montgomery reduction.
 __device__ __forceinline__ uint32_t recombine(const v4i (&acc)[7], int r, const uint32_t* c, uint32_t q,
                                               uint32_t qinv) {
-    uint64_t t = 0;
-#pragma unroll
-    for (int s = 0; s < 7; s++) t += (uint64_t)(uint32_t)acc[s][r] * c[s];
+    const uint32_t lo = (uint32_t)acc[0][r] + ((uint32_t)acc[1][r] << 7);
+    const uint32_t mid = (uint32_t)acc[2][r] + ((uint32_t)acc[3][r] << 7);
+    const uint32_t hi = (uint32_t)acc[4][r] + ((uint32_t)acc[5][r] << 7);
+    uint64_t t = (uint64_t)lo * c[0] + (uint64_t)mid * c[2];
+    t += (uint64_t)hi * c[4] + (uint64_t)(uint32_t)acc[6][r] * c[6];
     return csub(mont_reduce(t, q, qinv), q);
 }
 
+// Polynomials per workgroup: the constant operands (w64 columns, w32 rows)
+// are loaded once into registers and reused, and each next polynomial's
+// input is loaded while the current one is transformed.
+constexpr int kPolysPerGroup = 4;
+
 __global__ void __launch_bounds__(kThreads)
-k_ntt_fwd_mfma(const uint32_t* src, size_t src_stride, uint32_t* dst, size_t dst_stride, MfmaNttTables T, CtxDev C,
-               ScaleDev post) {
+k_ntt_fwd_mfma(const uint32_t* src, size_t src_stride, uint32_t* dst, size_t dst_stride, size_t count,
+               MfmaNttTables T, CtxDev C, ScaleDev post) {
     __shared__ __align__(16) uint8_t xl[4 * 32 * kXRow];   // x' limbs [a][j1][j2]
     __shared__ __align__(16) uint8_t yl[4 * 64 * kYRow];   // Y' limbs transposed [a][m2][j1]
     __shared__ __align__(16) uint32_t ob[kN];               // output staging by slot
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int k = blockIdx.x % C.K;
-    const size_t poly = blockIdx.x / C.K;
+    const size_t poly0 = (size_t)(blockIdx.x / C.K) * kPolysPerGroup;
     const uint32_t q = C.m[k].q, qinv = C.m[k].qinv;
-    const uint32_t* p = src + poly * src_stride + (size_t)k * kN;
-    uint32_t* o = dst + poly * dst_stride + (size_t)k * kN;
-
-    // ---- twist and split: thread holds row j1 = tid % 32, columns j2 = 8 (tid / 32) .. + 7
-    {
-        const int j1 = tid & 31, j2b = (tid >> 5) * 8;
-        uint32_t v[8];
-#pragma unroll
-        for (int e = 0; e < 8; e++) {
-            const int j = j1 + 32 * (j2b + e);
-            v[e] = csub(mont_reduce((uint64_t)p[j] * T.tw1[k * kN + j], q, qinv), q);
-        }
-#pragma unroll
-        for (int a = 0; a < 4; a++) {
-            uint32_t lo = 0, hi = 0;
-#pragma unroll
-            for (int e = 0; e < 4; e++) {
-                lo |= limb(v[e], a) << (8 * e);
-                hi |= limb(v[e + 4], a) << (8 * e);
-            }
-            *reinterpret_cast<uint2*>(&xl[(a * 32 + j1) * kXRow + j2b]) = make_uint2(lo, hi);
-        }
-    }
-    __syncthreads();
-
     const int col = lane & 15, kg = lane >> 4;
-    // ---- first product: wave w computes columns m2 = 16 w + col, rows j1 = 16 tm + 4 kg + r
-    {
-        const int m2 = 16 * w + col;
-        v4i bw[4];  // w64 limbs, column m2, k = j2 in 16 kg .. + 15 (the matrix is symmetric)
+    const int m2 = 16 * w + col;  // this wave's result columns in both products
+    const int j1 = tid & 31, j2b = (tid >> 5) * 8;  // this thread's input row and columns
+
+    // constant operands: w64 limbs (column m2, k = j2 in 16 kg .. + 15; the matrix is
+    // symmetric) and w32 limbs (rows m1 = 16 tm + col, k = j1 in 8 kg .. + 7)
+    v4i bw[4];
+    long aw[2][4];
 #pragma unroll
-        for (int b = 0; b < 4; b++)
-            bw[b] = *reinterpret_cast<const v4i*>(T.w64 + (((size_t)k * 4 + b) * 64 + m2) * 64 + 16 * kg);
+    for (int b = 0; b < 4; b++)
+        bw[b] = *reinterpret_cast<const v4i*>(T.w64 + (((size_t)k * 4 + b) * 64 + m2) * 64 + 16 * kg);
+#pragma unroll
+    for (int tm = 0; tm < 2; tm++)
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+            aw[tm][a] = *reinterpret_cast<const long*>(T.w32 + (((size_t)k * 4 + a) * 32 + 16 * tm + col) * 32 + 8 * kg);
+    uint32_t c3[7];  // the recombination constants times the output scale
+#pragma unroll
+    for (int s = 0; s < 7; s++) {
+        const uint32_t c = T.c1[k * 8 + s];
+        c3[s] = post.enable ? csub(shoup_mul(c, post.c[k], post.cp[k], q), q) : c;
+    }
+    uint32_t tw1[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) tw1[e] = T.tw1[k * kN + j1 + 32 * (j2b + e)];
+
+    uint32_t in[8];
+    auto load_in = [&](size_t poly) {
+        const uint32_t* p = src + poly * src_stride + (size_t)k * kN;
+#pragma unroll
+        for (int e = 0; e < 8; e++) in[e] = p[j1 + 32 * (j2b + e)];
+    };
+    if (poly0 < count) load_in(poly0);
+    for (int pi = 0; pi < kPolysPerGroup; pi++) {
+        const size_t poly = poly0 + pi;
+        if (poly >= count) break;
+        // ---- twist and split: row j1, columns j2b .. j2b + 7
+        {
+            uint32_t v[8];
+#pragma unroll
+            for (int e = 0; e < 8; e++) v[e] = csub(mont_reduce((uint64_t)in[e] * tw1[e], q, qinv), q);
+#pragma unroll
+            for (int a = 0; a < 4; a++) {
+                uint32_t lo = 0, hi = 0;
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    lo |= limb(v[e], a) << (8 * e);
+                    hi |= limb(v[e + 4], a) << (8 * e);
+                }
+                *reinterpret_cast<uint2*>(&xl[(a * 32 + j1) * kXRow + j2b]) = make_uint2(lo, hi);
+            }
+        }
+        if (poly + 1 < count && pi + 1 < kPolysPerGroup) load_in(poly + 1);  // in flight across the products
+        __syncthreads();
+
+        // ---- first product: columns m2, rows j1 = 16 tm + 4 kg + r
 #pragma unroll
         for (int tm = 0; tm < 2; tm++) {
             v4i acc[7];
@@ -102,9 +136,9 @@ k_ntt_fwd_mfma(const uint32_t* src, size_t src_stride, uint32_t* dst, size_t dst
             uint32_t yv[4];
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int j1 = 16 * tm + 4 * kg + r;
+                const int jr = 16 * tm + 4 * kg + r;
                 const uint32_t y = recombine(acc, r, T.c1 + k * 8, q, qinv);
-                yv[r] = csub(mont_reduce((uint64_t)y * T.tw2[(k * 32 + j1) * 64 + m2], q, qinv), q);
+                yv[r] = csub(mont_reduce((uint64_t)y * T.tw2[(k * 32 + jr) * 64 + m2], q, qinv), q);
             }
 #pragma unroll
             for (int a = 0; a < 4; a++) {
@@ -114,43 +148,38 @@ k_ntt_fwd_mfma(const uint32_t* src, size_t src_stride, uint32_t* dst, size_t dst
                 *reinterpret_cast<uint32_t*>(&yl[(a * 64 + m2) * kYRow + 16 * tm + 4 * kg]) = word;
             }
         }
-    }
-    __syncthreads();
+        __syncthreads();
 
-    // ---- second product: wave w computes columns m2 = 16 w + col, rows m1 = 16 tm + 4 kg + r
-    {
-        const int m2 = 16 * w + col;
-        long by[4];  // Y' limbs, column m2, k = j1 in 8 kg .. + 7
+        // ---- second product: columns m2, rows m1 = 16 tm + 4 kg + r
+        {
+            long by[4];  // Y' limbs, column m2, k = j1 in 8 kg .. + 7
 #pragma unroll
-        for (int b = 0; b < 4; b++) by[b] = *reinterpret_cast<const long*>(&yl[(b * 64 + m2) * kYRow + 8 * kg]);
-        uint32_t c3[7];  // the recombination constants times the output scale
+            for (int b = 0; b < 4; b++) by[b] = *reinterpret_cast<const long*>(&yl[(b * 64 + m2) * kYRow + 8 * kg]);
 #pragma unroll
-        for (int s = 0; s < 7; s++) {
-            const uint32_t c = T.c1[k * 8 + s];
-            c3[s] = post.enable ? csub(shoup_mul(c, post.c[k], post.cp[k], q), q) : c;
-        }
+            for (int tm = 0; tm < 2; tm++) {
+                v4i acc[7];
 #pragma unroll
-        for (int tm = 0; tm < 2; tm++) {
-            v4i acc[7];
+                for (int s = 0; s < 7; s++) acc[s] = v4i{0, 0, 0, 0};
 #pragma unroll
-            for (int s = 0; s < 7; s++) acc[s] = v4i{0, 0, 0, 0};
+                for (int a = 0; a < 4; a++)
 #pragma unroll
-            for (int a = 0; a < 4; a++) {
-                const long aw = *reinterpret_cast<const long*>(T.w32 + (((size_t)k * 4 + a) * 32 + 16 * tm + col) * 32 + 8 * kg);
+                    for (int b = 0; b < 4; b++)
+                        acc[a + b] = __builtin_amdgcn_mfma_i32_16x16x32_i8(aw[tm][a], by[b], acc[a + b], 0, 0, 0);
 #pragma unroll
-                for (int b = 0; b < 4; b++) acc[a + b] = __builtin_amdgcn_mfma_i32_16x16x32_i8(aw, by[b], acc[a + b], 0, 0, 0);
-            }
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int m1 = 16 * tm + 4 * kg + r;
-                const uint32_t m = (uint32_t)(m2 + 64 * m1);
-                ob[__brev(m) >> 21] = recombine(acc, r, c3, q, qinv);  // slot br11(m)
+                for (int r = 0; r < 4; r++) {
+                    const uint32_t m = (uint32_t)(m2 + 64 * (16 * tm + 4 * kg + r));
+                    ob[__brev(m) >> 21] = recombine(acc, r, c3, q, qinv);  // slot br11(m)
+                }
             }
         }
+        __syncthreads();
+        uint32_t* o = dst + poly * dst_stride + (size_t)k * kN;
+#pragma unroll
+        for (int e = 0; e < kN / kThreads; e++) o[tid + e * kThreads] = ob[tid + e * kThreads];
+        // the next polynomial's split writes xl (read before the second barrier) and its
+        // products write yl and ob only after its first barrier, which every thread passes
+        // after these reads of ob
     }
-    __syncthreads();
-#pragma unroll
-    for (int e = 0; e < kN / kThreads; e++) o[tid + e * kThreads] = ob[tid + e * kThreads];
 }
 
 }  // namespace
@@ -164,11 +193,11 @@ bool ntt_mfma_supported(const CtxDev& C) {
 
 hipError_t launch_ntt_fwd_mfma(const uint32_t* src, size_t src_stride, uint32_t* dst, size_t dst_stride, size_t count,
                                const MfmaNttTables& T, const CtxDev& C, const ScaleDev& sc, hipStream_t st) {
-    const size_t blocks = count * C.K;
+    const size_t blocks = (count + kPolysPerGroup - 1) / kPolysPerGroup * C.K;
     if (!blocks) return hipSuccess;
     if (!ntt_mfma_supported(C) || blocks > 0x7fffffffu) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_ntt_fwd_mfma, dim3((unsigned)blocks), dim3(kThreads), 0, st, src, src_stride, dst, dst_stride, T,
-                       C, sc);
+    hipLaunchKernelGGL(k_ntt_fwd_mfma, dim3((unsigned)blocks), dim3(kThreads), 0, st, src, src_stride, dst, dst_stride,
+                       count, T, C, sc);
     return hipGetLastError();
 }
 
