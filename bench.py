@@ -173,11 +173,15 @@ def main():
     ap.add_argument("--prime-ms", type=float, default=300.0,
                     help="untimed run of the workload on every lane before the warmup steps (clocks at load)")
     ap.add_argument("--batch", type=int, default=256)
-    ap.add_argument("--lanes", type=int, default=2,
-                    help="lanes (inspire_ctx_create_lane) submitting batches from their own threads; "
-                         "1 lane: 0.84 M queries/s at p50 0.30 ms, 2: 1.17 M at 0.43 ms, 3: 1.28 M at 0.58 ms "
-                         "(with more lanes the chain launches overlap, so their per-launch time no longer "
-                         "measures the kernel alone)")
+    ap.add_argument("--lanes", type=int, default=0,
+                    help="lanes (inspire_ctx_create_lane) submitting batches from their own threads; 0 (default): "
+                         "tuned, the count among --tune-lanes with the highest rate over --tune-steps untimed steps")
+    ap.add_argument("--tune-lanes", type=str, default="1,2,3", help="lane counts the lane tuning tries")
+    ap.add_argument("--tune-steps", type=int, default=96, help="untimed steps per lane count when tuning lanes")
+    ap.add_argument("--tuning-file", type=str, default="",
+                    help="where the tuning report (kernel variants and lanes) is written; with --reuse-tuning, "
+                         "read from instead of measuring")
+    ap.add_argument("--reuse-tuning", action="store_true")
     ap.add_argument("--lanes-sweep", type=str, default="1,3",
                     help="other lane counts timed after the headline (reported as lanes_sweep)")
     ap.add_argument("--db-log2", type=int, default=26)
@@ -225,6 +229,17 @@ def main():
     log(f"generating {n_entries} x 32 B database")
     db = np.frombuffer(rng.bytes(n_entries * 32), dtype=np.uint8)
     ctx = M.Context(M.InspireParams(), w_seed)
+    # Kernel variants (matrix-core vs VALU forward NTT, packing-chain width):
+    # timed on the device by the library's tuner, the faster kept.
+    prior = None
+    if args.reuse_tuning and args.tuning_file and os.path.exists(args.tuning_file):
+        prior = json.load(open(args.tuning_file))
+        ctx.load_tuning(args.tuning_file)
+        tuning = dict(prior)
+    else:
+        tuning = ctx.autotune()
+    log(f"tuning: ntt_fwd_2048 {tuning['ntt_fwd_2048']}, chain {tuning['chain_values_per_thread']} values/thread "
+        f"{tuning.get('measured', {})}")
     t0 = time.perf_counter()
     ctx.load_entries(0, db, 32)
     log(f"database resident: {nshards} shards, {ctx.device_bytes / 2**30:.1f} GiB on device, "
@@ -255,7 +270,8 @@ def main():
     # Lanes: each has its own stream and workspaces over the resident state,
     # and submits batches from its own thread, so one batch's host work and
     # transfers overlap another's kernels.  A step is one batch on one lane.
-    lanes = [ctx] + [ctx.lane() for _ in range(max(args.lanes, 1) - 1)]
+    cand = sorted({int(x) for x in args.tune_lanes.split(",") if x.strip()} | ({args.lanes} if args.lanes else set()))
+    lanes = [ctx] + [ctx.lane() for _ in range(max(cand + [1]) - 1)]
     louts = [dout] + [M.DeviceArray(B * 2 * K * D * 4) for _ in lanes[1:]]
     def run_lanes(lns, outs, steps):
         """steps batches taken in turn by the lanes' threads: (elapsed s, per-step latencies ms)"""
@@ -298,6 +314,28 @@ def main():
     t_prime = time.perf_counter()
     while (time.perf_counter() - t_prime) * 1e3 < args.prime_ms:
         run_lanes(lanes, louts, 2 * len(lanes))
+    # Lanes: the headline's lane count is tuned (untimed) unless given.
+    lane_rates = {}
+    if args.lanes:
+        n_lanes = args.lanes
+    elif prior and "lanes" in prior:
+        n_lanes = int(prior["lanes"])
+    else:
+        for nl in cand:
+            el, _ = run_lanes(lanes[:nl], louts[:nl], args.tune_steps)
+            lane_rates[nl] = round(B * args.tune_steps / el, 1)
+        # the fewest lanes within 3 % of the best rate: more lanes only add latency beyond that
+        best = max(lane_rates.values())
+        n_lanes = min(nl for nl, r in lane_rates.items() if r >= 0.97 * best)
+    tuning["lanes"] = n_lanes
+    if lane_rates:
+        tuning.setdefault("measured", {})["lanes_qps"] = {str(k): v for k, v in lane_rates.items()}
+    log(f"lanes: {n_lanes} {lane_rates}")
+    if args.tuning_file and not args.reuse_tuning:
+        with open(args.tuning_file, "w") as f:
+            json.dump(tuning, f)
+    spare = lanes[n_lanes:]
+    lanes, louts, spare_outs = lanes[:n_lanes], louts[:n_lanes], louts[n_lanes:]
     for j, ln in enumerate(lanes):
         for i in range(args.warmup):
             step(i + j, c=ln, out=louts[j])
@@ -353,7 +391,8 @@ def main():
     log(f"  profiled repeat: {graphs_ab['profiled_repeat']}")
     # the same batches from other numbers of lanes (more lanes: more throughput, longer latency)
     sweep = []
-    for nl in [int(x) for x in args.lanes_sweep.split(",") if x.strip()]:
+    lanes, louts = lanes + spare, louts + spare_outs
+    for nl in sorted({int(x) for x in args.lanes_sweep.split(",") if x.strip()} | set(cand)):
         if nl < 1 or nl == n_lanes:
             continue
         while len(lanes) < nl:
@@ -481,6 +520,7 @@ def main():
         "host_buffer_api_qps": round(host_qps, 1) if host_qps else None,
         "lanes_sweep": sweep,
         "graphs_ab": graphs_ab,
+        "tuning": tuning,
         "modmul_per_query": MODMUL_QUERY,
         "roofline": roofline,
         "roofline_hbm_front": roofline_hbm,
@@ -772,8 +812,9 @@ def request_structs(M, pool):
 def poisson_run(M, srv, pool, structs, answers, lam: float, n: int, rng):
     """n queries drawn from the pool, arriving open-loop at Poisson rate lam
     (arrival times fixed in advance, submitted when due); every response is
-    compared with the pool query's answer.  -> (result dict, tickets' picks,
-    their responses' ciphertexts are not kept)."""
+    compared with the pool query's answer, by waiter threads that drain the
+    completions in chunks (inspire_server_wait_many).  -> (result dict,
+    mismatches)."""
     import threading
 
     pick = rng.integers(0, len(pool), n)
@@ -785,20 +826,32 @@ def poisson_run(M, srv, pool, structs, answers, lam: float, n: int, rng):
     bad = [0]
     submitted = [0]
 
-    def waiter():
-        for j in range(n):
-            while submitted[0] <= j:
-                time.sleep(50e-6)
-            q = pool[pick[j]]
-            st, r, dn = srv.wait(int(tickets[j]), cols=16 if q["variant"] == 0 else 0)
-            done[j] = dn
-            want = answers[pick[j]]
-            if st != M.OK or not np.array_equal(r.ciphertext, want[0]) or (
-                    q["variant"] == 0 and not np.array_equal(r.column_ciphertexts, want[1])):
-                bad[0] += 1
+    ans_ct = np.stack([answers[i][0] for i in range(len(pool))])  # [P, 2, K, D] u64
+    chunk, n_waiters = 64, 4
 
-    th = threading.Thread(target=waiter)
-    th.start()
+    def waiter(wi):
+        # drains completions chunk by chunk (chunk c on waiter c % n_waiters) and checks every response
+        cts = np.empty((chunk,) + ans_ct.shape[1:], dtype=np.uint64)
+        colbufs = [np.empty((16, 2, K, D), dtype=np.uint64) for _ in range(chunk)]
+        for c0 in range(wi * chunk, n, n_waiters * chunk):
+            c = min(chunk, n - c0)
+            with progress:
+                progress.wait_for(lambda: submitted[0] >= c0 + c)
+            pk = pick[c0:c0 + c]
+            zero = [i for i in range(c) if pool[pk[i]]["variant"] == 0]
+            st, dn, _ = srv.wait_many(tickets[c0:c0 + c], cts[:c], {i: colbufs[i] for i in zero})
+            done[c0:c0 + c] = dn.astype(np.int64)
+            ok = (st == M.OK) & (cts[:c].reshape(c, -1) == ans_ct[pk].reshape(c, -1)).all(axis=1)
+            for i in zero:
+                ok[i] &= bool(np.array_equal(colbufs[i], answers[pk[i]][1]))
+            with lock:
+                bad[0] += int(c - ok.sum())
+
+    lock = threading.Lock()
+    progress = threading.Condition()  # the submitter's count of submitted queries
+    ths = [threading.Thread(target=waiter, args=(wi,)) for wi in range(n_waiters)]
+    for th in ths:
+        th.start()
     i = 0
     while i < n:
         now = time.monotonic_ns()
@@ -809,13 +862,18 @@ def poisson_run(M, srv, pool, structs, answers, lam: float, n: int, rng):
                 arr[j - i] = structs[pick[j]]
                 arr[j - i].arrival_ns = int(arrival[j])
             tickets[i:k] = srv.submit_structs(arr, k - i)
+            with progress:
+                crossed = k // chunk != i // chunk or k == n  # a chunk became complete
+                submitted[0] = k
+                if crossed:
+                    progress.notify_all()
             i = k
-            submitted[0] = k
         else:
             dt = (arrival[i] - now) / 1e9
             if dt > 300e-6:
                 time.sleep(dt - 200e-6)
-    th.join()
+    for th in ths:
+        th.join()
     lat = (done - arrival) / 1e6
     span = (done.max() - arrival[0]) / 1e9
     return {

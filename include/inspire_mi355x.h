@@ -324,6 +324,13 @@ int inspire_server_submit(inspire_server *s, uint32_t n, const inspire_request *
  * the time ran out (the ticket stays valid). */
 int inspire_server_wait(inspire_server *s, uint64_t ticket, int32_t timeout_ms, inspire_response *r,
                         int32_t *status, uint64_t *done_ns);
+/* inspire_server_wait for n tickets in turn: rs[i] (may be NULL: status
+ * only), status[i] and done_ns[i] (may be NULL) receive ticket i's results,
+ * and status[i] is INSPIRE_ERR_PENDING for a ticket whose wait timed out
+ * (it stays valid; the call then returns INSPIRE_ERR_PENDING).  One call
+ * drains a run of completions, as a server's completion queue does. */
+int inspire_server_wait_many(inspire_server *s, uint32_t n, const uint64_t *tickets, int32_t timeout_ms,
+                             inspire_response *rs, int32_t *status, uint64_t *done_ns);
 int inspire_server_get_stats(inspire_server *s, inspire_server_stats *out);
 /* Finishes the queries already submitted, then stops the workers. */
 void inspire_server_destroy(inspire_server *s);

@@ -5,6 +5,10 @@
 #include <algorithm>
 #include <cstddef>
 #include <cstdint>
+#include <condition_variable>
+#include <deque>
+#include <functional>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -16,27 +20,79 @@ inline unsigned host_threads() {
     return n;
 }
 
+// A fixed set of worker threads for par_ranges, started on first use and
+// kept for the life of the process (the serving lanes call par_ranges for
+// every batch; starting threads per call would cost more than the copies).
+// Tasks are independent parts of callers' ranges; a caller waits only for
+// its own parts, so concurrent callers share the workers.
+class HostPool {
+   public:
+    static HostPool& get() {
+        static HostPool* p = new HostPool(host_threads() - 1);  // never destroyed: workers outlive main
+        return *p;
+    }
+    void submit(std::function<void()> task) {
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            q_.push_back(std::move(task));
+        }
+        cv_.notify_one();
+    }
+    size_t workers() const { return n_; }
+
+   private:
+    explicit HostPool(size_t n) : n_(n) {
+        for (size_t i = 0; i < n; i++) std::thread([this] { loop(); }).detach();
+    }
+    void loop() {
+        for (;;) {
+            std::function<void()> t;
+            {
+                std::unique_lock<std::mutex> lk(m_);
+                cv_.wait(lk, [&] { return !q_.empty(); });
+                t = std::move(q_.front());
+                q_.pop_front();
+            }
+            t();
+        }
+    }
+    size_t n_;
+    std::mutex m_;
+    std::condition_variable cv_;
+    std::deque<std::function<void()>> q_;
+};
+
 // Runs f(begin, end) over contiguous ranges covering [0, n), each of at least
-// `grain` items, on up to host_threads() threads (the caller runs one range).
-// f must not throw.
+// `grain` items, on up to host_threads() threads (the caller runs one range,
+// the pool's workers the others).  f must not throw.
 template <class F>
 void par_ranges(size_t n, size_t grain, F&& f) {
     if (!n) return;
     grain = std::max<size_t>(grain, 1);
-    const size_t parts = std::min<size_t>(host_threads(), (n + grain - 1) / grain);
+    HostPool& pool = HostPool::get();
+    const size_t parts = std::min<size_t>(pool.workers() + 1, (n + grain - 1) / grain);
     if (parts <= 1) {
         f((size_t)0, n);
         return;
     }
     const size_t step = (n + parts - 1) / parts;
-    std::vector<std::thread> th;
-    th.reserve(parts - 1);
+    std::mutex m;
+    std::condition_variable cv;
+    size_t left = 0;  // parts on the workers, counted before any of them starts
+    for (size_t p = 1; p < parts; p++)
+        if (p * step < n) left++;
     for (size_t p = 1; p < parts; p++) {
         const size_t b = p * step, e = std::min(n, b + step);
-        if (b < e) th.emplace_back([&f, b, e] { f(b, e); });
+        if (b >= e) continue;
+        pool.submit([&, b, e] {
+            f(b, e);
+            std::lock_guard<std::mutex> lk(m);
+            if (--left == 0) cv.notify_one();
+        });
     }
     f((size_t)0, std::min(n, step));
-    for (auto& t : th) t.join();
+    std::unique_lock<std::mutex> lk(m);
+    cv.wait(lk, [&] { return left == 0; });
 }
 
 // dst[i] = (uint32_t)src[i]: residues (< 2^32) from the reference's u64 words.

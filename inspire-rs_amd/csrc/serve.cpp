@@ -515,45 +515,77 @@ int inspire_server_submit(inspire_server* s, uint32_t n, const inspire_request* 
     })
 }
 
+}  // extern "C"
+
+namespace inspire {
+namespace {
+
+// Waits for one ticket and releases it (inspire_server_wait).
+int wait_ticket(inspire_server* s, uint64_t ticket, int32_t timeout_ms, inspire_response* r, int32_t* status,
+                uint64_t* done_ns) {
+    std::shared_ptr<SReq> q;
+    {
+        std::unique_lock<std::mutex> lk(s->m);
+        auto it = s->live.find(ticket);
+        if (it == s->live.end()) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "unknown ticket"};
+        q = it->second;
+        auto ready = [&] { return q->done; };
+        if (timeout_ms < 0) s->cv_done.wait(lk, ready);
+        else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready)) {
+            set_last_error("ticket not finished yet");
+            return INSPIRE_ERR_PENDING;
+        }
+        s->live.erase(ticket);  // the map may have changed while waiting
+    }
+    if (status) *status = q->status;
+    if (done_ns) *done_ns = q->done_ns;
+    if (q->status != INSPIRE_OK) set_last_error(q->err);
+    if (r && q->status == INSPIRE_OK) {
+        const size_t KN = (size_t)s->ctx->K * s->ctx->N;
+        if (!r->ciphertext) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "response ciphertext buffer is null"};
+        r->ciphertext_b_dim = s->ctx->N;
+        for (size_t w = 0; w < 2 * KN; w++) r->ciphertext[w] = q->out[w];
+        if (q->packed()) {
+            r->num_column_ciphertexts = 0;  // respond.rs:382-385
+        } else {
+            r->num_column_ciphertexts = q->cols;
+            if (r->column_ciphertexts) {
+                if (r->column_capacity < q->cols)
+                    throw Error{INSPIRE_ERR_CAPACITY, "column_ciphertexts capacity too small"};
+                for (size_t w = 0; w < (size_t)q->cols * 2 * KN; w++) r->column_ciphertexts[w] = q->out[2 * KN + w];
+            }
+        }
+    }
+    return INSPIRE_OK;
+}
+
+}  // namespace
+}  // namespace inspire
+
+extern "C" {
+
 int inspire_server_wait(inspire_server* s, uint64_t ticket, int32_t timeout_ms, inspire_response* r, int32_t* status,
                         uint64_t* done_ns) {
     INSPIRE_TRY({
         if (!s) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "null argument"};
-        std::shared_ptr<SReq> q;
-        {
-            std::unique_lock<std::mutex> lk(s->m);
-            auto it = s->live.find(ticket);
-            if (it == s->live.end()) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "unknown ticket"};
-            q = it->second;
-            auto ready = [&] { return q->done; };
-            if (timeout_ms < 0) s->cv_done.wait(lk, ready);
-            else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready)) {
-                set_last_error("ticket not finished yet");
-                return INSPIRE_ERR_PENDING;
-            }
-            s->live.erase(ticket);  // the map may have changed while waiting
-        }
-        if (status) *status = q->status;
-        if (done_ns) *done_ns = q->done_ns;
-        if (q->status != INSPIRE_OK) set_last_error(q->err);
-        if (r && q->status == INSPIRE_OK) {
-            const size_t KN = (size_t)s->ctx->K * s->ctx->N;
-            if (!r->ciphertext) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "response ciphertext buffer is null"};
-            r->ciphertext_b_dim = s->ctx->N;
-            if (q->packed()) {
-                for (size_t w = 0; w < 2 * KN; w++) r->ciphertext[w] = q->out[w];
-                r->num_column_ciphertexts = 0;  // respond.rs:382-385
-            } else {
-                for (size_t w = 0; w < 2 * KN; w++) r->ciphertext[w] = q->out[w];
-                r->num_column_ciphertexts = q->cols;
-                if (r->column_ciphertexts) {
-                    if (r->column_capacity < q->cols)
-                        throw Error{INSPIRE_ERR_CAPACITY, "column_ciphertexts capacity too small"};
-                    for (size_t w = 0; w < (size_t)q->cols * 2 * KN; w++) r->column_ciphertexts[w] = q->out[2 * KN + w];
-                }
+        return wait_ticket(s, ticket, timeout_ms, r, status, done_ns);
+    })
+}
+
+int inspire_server_wait_many(inspire_server* s, uint32_t n, const uint64_t* tickets, int32_t timeout_ms,
+                             inspire_response* rs, int32_t* status, uint64_t* done_ns) {
+    INSPIRE_TRY({
+        if (!s || (n && (!tickets || !status))) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "null argument"};
+        int rc = INSPIRE_OK;
+        for (uint32_t i = 0; i < n; i++) {
+            const int c = wait_ticket(s, tickets[i], timeout_ms, rs ? rs + i : nullptr, status + i,
+                                      done_ns ? done_ns + i : nullptr);
+            if (c == INSPIRE_ERR_PENDING) {
+                status[i] = INSPIRE_ERR_PENDING;
+                rc = INSPIRE_ERR_PENDING;
             }
         }
-        return INSPIRE_OK;
+        return rc;
     })
 }
 

@@ -204,6 +204,8 @@ EXPORTS = [
     ("inspire_ctx_graph_stats", C.c_int, [C.c_void_p, _u64p, _u64p]),
     ("inspire_register_client_keys", C.c_int, [C.c_void_p, C.c_uint32, _u64p]),
     ("inspire_server_create", C.c_int, [C.c_void_p, C.POINTER(_ServerConfig), C.POINTER(C.c_void_p)]),
+    ("inspire_server_wait_many", C.c_int,
+     [C.c_void_p, C.c_uint32, _u64p, C.c_int32, C.POINTER(_Response), C.POINTER(C.c_int32), _u64p]),
     ("inspire_server_submit", C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(_Request), _u64p]),
     (
         "inspire_server_wait",
@@ -720,6 +722,32 @@ class Server:
         if st.value != OK:
             return st.value, None, done.value
         return OK, ServerResponse(ct, cc[: r.num_column_ciphertexts]), done.value
+
+    def wait_many(self, tickets, cts: np.ndarray, cols_for=None, timeout_ms: int = -1):
+        """Waits for the tickets in order (inspire_server_wait_many).  cts:
+        uint64[n, 2, K, d] receives the ciphertexts; cols_for maps a position to
+        a uint64[cap, 2, K, d] buffer for that query's column ciphertexts
+        (InsPIRe^0).  -> (status int32[n], done_ns uint64[n], columns uint32[n])."""
+        n = len(tickets)
+        t = np.ascontiguousarray(tickets, dtype=np.uint64)
+        ra = (_Response * n)()
+        base, stride = cts.ctypes.data, cts.strides[0]
+        words = cts[0].size if n else 0
+        for i in range(n):
+            r = ra[i]
+            r.ciphertext = C.cast(base + i * stride, _u64p)
+            r.ciphertext_capacity = words
+            if cols_for and i in cols_for:
+                cb = cols_for[i]
+                r.column_ciphertexts = C.cast(cb.ctypes.data, _u64p)
+                r.column_capacity = cb.shape[0]
+        st = np.zeros(n, dtype=np.int32)
+        dn = np.zeros(n, dtype=np.uint64)
+        rc = lib().inspire_server_wait_many(self._h, n, _ptr64(t), timeout_ms, ra,
+                                            st.ctypes.data_as(C.POINTER(C.c_int32)), _ptr64(dn))
+        if rc not in (OK, ERR_PENDING):
+            _raise(rc)
+        return st, dn, np.array([ra[i].num_column_ciphertexts for i in range(n)], dtype=np.uint32)
 
     def stats(self) -> dict:
         s = _ServerStats()
