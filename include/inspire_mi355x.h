@@ -308,6 +308,9 @@ typedef struct {
     uint64_t max_batch_seen;
     uint64_t peak_inflight_bytes;
     uint64_t peak_lanes_busy;   /* most batches on the device at once */
+    uint64_t launch_groups;     /* launch sequences the batches ran: one per column count
+                                   and batch, InsPIRe^0 sharing the packed variants' when the
+                                   fused front end applies (gamma = 2^k, gamma^2 | 2d, two moduli) */
 } inspire_server_stats;
 
 /* Starts the loop: every lane has a worker thread that takes the queries

@@ -290,11 +290,17 @@ bool pack_ybody_in_chain(const inspire_ctx* ctx) { return pack_chain_crt_support
 // Device-only packing of a staged group: first dimension, b_poly, step 1 and
 // the chain.  y: NTT-domain (Montgomery) client keys, per query y_off/y_mode/
 // y_count (device arrays).  out: [Q][2][K][N].
+bool pack_mixed_supported(inspire_ctx* ctx, uint32_t gamma) {
+    return pack_chain_crt_supported(ctx->dev) && pack_tables(ctx, gamma).fast_front;
+}
+
 void pack_run(inspire_ctx* ctx, uint32_t gamma, uint32_t Q, const uint32_t* rows, const uint64_t* d_qdig,
               const uint32_t* d_y, const uint64_t* d_yoff, const uint32_t* d_ymode, const uint32_t* d_ycount,
-              uint32_t* d_out, bool y_coeff, const PackWs& ws) {
+              uint32_t* d_out, bool y_coeff, const PackWs& ws, const PackRoles& roles) {
     const uint32_t K = ctx->K;
     PackTables& T = pack_tables(ctx, gamma);
+    if (roles.role && !pack_mixed_supported(ctx, gamma))
+        throw Error{INSPIRE_ERR_UNSUPPORTED, "InsPIRe^0 roles in a packing launch need the fused front end"};
     const bool rw = (ctx->params.flags & INSPIRE_FLAG_RESIDUE_WISE_AUTOMORPHISM) != 0;
     const PackWsSizes z = pack_ws_sizes(ctx, T, Q);
     const uint32_t nparts = T.fast_front ? ctx->N / 64 : 1;
@@ -309,7 +315,8 @@ void pack_run(inspire_ctx* ctx, uint32_t gamma, uint32_t Q, const uint32_t* rows
     if (T.fast_front) {
         ProfLaunch pl(ctx, P_FRONT);
         hip_check(launch_pack_front(d_qdig, rows, d_rpre, d_bpart, T.twist.get<uint32_t>(), T.dftw.get<uint2>(), Q,
-                                    gamma, rw || K == 1, ctx->dev, ctx->stream, LaunchEv{pl.start(), pl.stop()}),
+                                    gamma, rw || K == 1, ctx->dev, ctx->stream, LaunchEv{pl.start(), pl.stop()},
+                                    roles, d_out),
                   "pack front");
     } else {
         uint32_t* d_cols = ws.cols;
@@ -346,6 +353,7 @@ void pack_run(inspire_ctx* ctx, uint32_t gamma, uint32_t Q, const uint32_t* rows
     A.out = d_out;
     A.gamma = gamma;
     A.residue_wise = rw ? 1u : 0u;
+    A.roles = roles;
     for (uint32_t k = 0; k < K; k++) {
         const uint64_t q = ctx->hm[k].q;
         (k ? A.ninv1 : A.ninv0) = (uint32_t)powmod64(ctx->N, q - 2, q);

@@ -23,6 +23,17 @@ struct BatchItem {
 };
 
 // Kernel-side description of a packing batch.
+// Per-query roles of a launch that serves InsPIRe^0 beside the packed
+// variants: role[q] 0 packs (InsPIRe^1/^2, block [2][K][N]), 1 answers with
+// InsPIRe^0's combined and column ciphertexts (block [1 + gamma][2][K][N]);
+// qout[q] is the word offset of query q's block in the output.  Null role:
+// every query packs, query q's block at q 2KN.  Roles need the fused front
+// end and the two-modulus chain (pack_mixed_supported).
+struct PackRoles {
+    const uint32_t* role = nullptr;
+    const uint64_t* qout = nullptr;
+};
+
 struct PackArgs {
     const uint32_t* r_pre;       // [Q][gamma][K][N] step-1 sums before the automorphism
     const uint32_t* perm;        // [gamma][N] NTT-domain automorphism tables of g^i
@@ -49,6 +60,7 @@ struct PackArgs {
     // signed chain: R[i] enters its multiply-accumulate as r * rmul[k] (2^32 mod q_k,
     // or 2^-32 mod q1 for the rebuilt residue 1, whose value carries 2^64), S as S * 2^32
     uint32_t rmul[2], smul[2];
+    PackRoles roles;             // per-query roles and output offsets (null role: all packed)
 };
 
 // Keys given as y_body alone are rotated in the NTT domain by the slot
@@ -85,7 +97,11 @@ PackWsSizes pack_ws_sizes(const inspire_ctx* ctx, const PackTables& T, uint32_t 
 // y_coeff: d_y holds coefficient-domain y_body (only when pack_chain_crt_supported).
 void pack_run(inspire_ctx* ctx, uint32_t gamma, uint32_t Q, const uint32_t* rows, const uint64_t* d_qdig,
               const uint32_t* d_y, const uint64_t* d_yoff, const uint32_t* d_ymode, const uint32_t* d_ycount,
-              uint32_t* d_out, bool y_coeff, const PackWs& ws = PackWs{});
+              uint32_t* d_out, bool y_coeff, const PackWs& ws = PackWs{}, const PackRoles& roles = PackRoles{});
+// True when one pack_run launch sequence can also answer InsPIRe^0 queries
+// (PackRoles) for databases of gamma columns: the fused front end applies
+// (gamma = 2^k, gamma^2 | 2d) and the two-modulus chain kernel runs.
+bool pack_mixed_supported(inspire_ctx* ctx, uint32_t gamma);
 // True when the packing chain transforms coefficient-domain y_body itself;
 // otherwise pack_run_device_ybody needs y_ws for the transformed keys.
 bool pack_ybody_in_chain(const inspire_ctx* ctx);
@@ -106,7 +122,8 @@ hipError_t launch_pack_b0(const uint32_t* cols, uint64_t* bpart, uint32_t Q, uin
 // qdig[q]: device address of query q's shard digits (a valid shard also for a missing one).
 hipError_t launch_pack_front(const uint64_t* qdig, const uint32_t* rows, uint32_t* r_pre, uint64_t* bpart,
                              const uint32_t* twist, const uint2* dftw, uint32_t Q, uint32_t gamma, bool rw,
-                             const CtxDev& C, hipStream_t st, LaunchEv ev = LaunchEv{});
+                             const CtxDev& C, hipStream_t st, LaunchEv ev = LaunchEv{},
+                             const PackRoles& roles = PackRoles{}, uint32_t* out = nullptr);
 // stepu: [gamma][K][N] u = w_s^(g^(n-i)) in Montgomery form; ginvm: gamma^-1 2^32 mod q_0, q_1.
 hipError_t launch_pack_step1(const uint32_t* cols, const uint32_t* stepu, uint32_t* r_pre, uint32_t Q,
                              uint32_t gamma, uint32_t kres, uint2 ginvm, const CtxDev& C, hipStream_t st);

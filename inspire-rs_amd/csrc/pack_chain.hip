@@ -110,6 +110,29 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
         const ModDev& D = C.m[kk];
         twf[e] = csub(shoup_mul(D.tw[i], C.mont[kk], C.montp[kk], D.q), D.q);
     }
+    if (A.roles.role && A.roles.role[qi]) {
+        // InsPIRe^0 in a packing launch: the front end left the query's
+        // combined and column ciphertexts [1 + gamma][2][K][N] in the NTT
+        // domain; each half transforms its residue of every poly in place
+        // (the next poly's slots loaded while the current one transforms)
+        __syncthreads();  // the twiddles
+        uint32_t* O = A.out + A.roles.qout[qi] + (size_t)k * N;
+        const uint32_t np = 2 * (1 + gamma);
+        uint32_t v[V], nx[V];
+        load_slots_v<V>(O, t, v);
+        for (uint32_t p = 0; p < np; p++) {
+            if (p + 1 < np) load_slots_v<V>(O + (size_t)(p + 1) * KN, t, nx);
+            if (p) lds_barrier();  // every thread is past the previous transform's reads of rb
+            intt_v<LOGN, V>(v, t, rb, tw, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
+            // the transform's barriers order every thread's slot loads of poly p before these stores
+#pragma unroll
+            for (int e = 0; e < V; e++) {
+                O[(size_t)p * KN + t + e * T] = csub(v[e], M.q);
+                v[e] = nx[e];
+            }
+        }
+        return;
+    }
     if (ymode == 0 && A.y_coeff) {
         // NTT of this query's y_body (both halves, one residue each), into
         // LDS at the slot positions in Montgomery form
@@ -301,7 +324,7 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
     }
 
     // a_hat = INTT(R[0]); b = b_poly + INTT(S)
-    uint32_t* out = A.out + (size_t)qi * 2 * KN + (size_t)k * N;
+    uint32_t* out = A.out + (A.roles.qout ? A.roles.qout[qi] : (uint64_t)qi * 2 * KN) + (size_t)k * N;
     intt_v<LOGN, V>(R, t, rb, tw, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
 #pragma unroll
     for (int e = 0; e < V; e++) out[t + e * T] = csub(R[e], M.q);

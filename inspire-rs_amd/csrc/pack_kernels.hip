@@ -175,6 +175,55 @@ __device__ __forceinline__ void front_residue(int k, const uint32_t* R, const ui
     }
 }
 
+// InsPIRe^0 in the same launch (a query whose role is 1): the column values
+// a_j, b_j of every residue and their sums (the combined ciphertext, summed
+// in the NTT domain: the inverse transform is linear mod q_k, so summing
+// before it gives the words the reference's sum after it gives) into the
+// query's block O [1 + gamma][2][K][N] (combined, then the columns), still
+// in the NTT domain; the chain kernel's role-1 workgroups transform them.
+template <int LG, int ELL>
+__device__ __forceinline__ void front_plain(const uint32_t* R, const uint32_t* Dq, uint32_t* O, const CtxDev& C) {
+    constexpr int G = 1 << LG;
+    constexpr int CH = G < 8 ? G : 8;
+    const int n = (int)C.n;
+    const size_t KN = (size_t)C.K * n;
+    for (int k = 0; k < (int)C.K; k++) {
+        const uint32_t q = C.m[k].q, qinv = C.m[k].qinv;
+        uint32_t ca[ELL], cb[ELL];
+#pragma unroll
+        for (int i = 0; i < ELL; i++) {
+            ca[i] = R[((size_t)(2 * i) * C.K + k) * n];
+            cb[i] = R[((size_t)(2 * i + 1) * C.K + k) * n];
+        }
+        uint32_t sa = 0, sb = 0;
+#pragma unroll
+        for (int j0 = 0; j0 < G; j0 += CH) {
+            uint32_t dv[CH][ELL];
+#pragma unroll
+            for (int jj = 0; jj < CH; jj++)
+#pragma unroll
+                for (int i = 0; i < ELL; i++) dv[jj][i] = Dq[((((size_t)(j0 + jj)) * ELL + i) * C.K + k) * n];
+#pragma unroll
+            for (int jj = 0; jj < CH; jj++) {
+                uint64_t aa = 0, bb = 0;
+#pragma unroll
+                for (int i = 0; i < ELL; i++) {
+                    aa += (uint64_t)dv[jj][i] * ca[i];
+                    bb += (uint64_t)dv[jj][i] * cb[i];
+                }
+                const uint32_t a = csub(mont_reduce(aa, q, qinv), q), b = csub(mont_reduce(bb, q, qinv), q);
+                uint32_t* oc = O + (size_t)(2 + 2 * (j0 + jj)) * KN + (size_t)k * n;
+                oc[0] = a;
+                oc[KN] = b;
+                sa = csub(sa + a, q);
+                sb = csub(sb + b, q);
+            }
+        }
+        O[(size_t)k * n] = sa;
+        O[KN + (size_t)k * n] = sb;
+    }
+}
+
 // At most 64 VGPRs (eight waves per SIMD): a 256-query launch is 8192 waves,
 // which then all fit on the 256 CUs at once instead of leaving a second,
 // nearly empty round of workgroups.
@@ -182,7 +231,7 @@ template <int LG, bool RW, int ELL>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
 k_pack_front(const uint64_t* __restrict__ qdig, const uint32_t* __restrict__ rows, uint32_t* __restrict__ r_pre,
              uint64_t* __restrict__ bpart, const uint32_t* __restrict__ twist, const uint2* __restrict__ dftw,
-             CtxDev C) {
+             PackRoles roles, uint32_t* __restrict__ out, CtxDev C) {
     constexpr int G = 1 << LG;
     const int n = (int)C.n, nb = n / 256, tid = threadIdx.x;
     const uint32_t qi = blockIdx.x / nb, blk = blockIdx.x % nb;
@@ -191,6 +240,10 @@ k_pack_front(const uint64_t* __restrict__ qdig, const uint32_t* __restrict__ row
     const uint32_t* R = rows + (size_t)qi * ELL * 2 * KN + s;
     // the host points a missing shard at a valid one and reports it
     const uint32_t* Dq = reinterpret_cast<const uint32_t*>(qdig[qi]) + s;
+    if (roles.role && roles.role[qi]) {
+        front_plain<LG, ELL>(R, Dq, out + roles.qout[qi] + s, C);
+        return;
+    }
     const uint32_t cls = ((2u * (__brev((uint32_t)s) >> (32 - C.logn)) + 1u) & (G - 1)) >> 1;
     front_residue<LG, true, ELL>(0, R, Dq, r_pre, bpart, twist, dftw, C, qi, blk, s, cls);
     // in the reference's residue handling only residue 0 of r_pre is read later
@@ -402,43 +455,44 @@ hipError_t launch_pack_b0(const uint32_t* cols, uint64_t* bpart, uint32_t Q, uin
 
 template <int LG, int ELL>
 static hipError_t front_t(const uint64_t* qdig, const uint32_t* rows, uint32_t* r_pre, uint64_t* bpart,
-                          const uint32_t* twist, const uint2* dftw, uint32_t Q, bool rw, const CtxDev& C,
-                          hipStream_t st, LaunchEv ev) {
+                          const uint32_t* twist, const uint2* dftw, const PackRoles& roles, uint32_t* out, uint32_t Q,
+                          bool rw, const CtxDev& C, hipStream_t st, LaunchEv ev) {
     const dim3 grid(Q * (C.n / 256));
     if (rw)
         hipExtLaunchKernelGGL((k_pack_front<LG, true, ELL>), grid, dim3(256), 0, st, ev.start, ev.stop, 0, qdig, rows,
-                              r_pre, bpart, twist, dftw, C);
+                              r_pre, bpart, twist, dftw, roles, out, C);
     else
         hipExtLaunchKernelGGL((k_pack_front<LG, false, ELL>), grid, dim3(256), 0, st, ev.start, ev.stop, 0, qdig,
-                              rows, r_pre, bpart, twist, dftw, C);
+                              rows, r_pre, bpart, twist, dftw, roles, out, C);
     return hipGetLastError();
 }
 
 template <int LG>
 static hipError_t front_lg(const uint64_t* qdig, const uint32_t* rows, uint32_t* r_pre, uint64_t* bpart,
-                           const uint32_t* twist, const uint2* dftw, uint32_t Q, bool rw, const CtxDev& C,
-                           hipStream_t st, LaunchEv ev) {
+                           const uint32_t* twist, const uint2* dftw, const PackRoles& roles, uint32_t* out,
+                           uint32_t Q, bool rw, const CtxDev& C, hipStream_t st, LaunchEv ev) {
     switch (C.ell) {
-        case 1: return front_t<LG, 1>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st, ev);
-        case 2: return front_t<LG, 2>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st, ev);
-        case 3: return front_t<LG, 3>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st, ev);
-        case 4: return front_t<LG, 4>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st, ev);
+        case 1: return front_t<LG, 1>(qdig, rows, r_pre, bpart, twist, dftw, roles, out, Q, rw, C, st, ev);
+        case 2: return front_t<LG, 2>(qdig, rows, r_pre, bpart, twist, dftw, roles, out, Q, rw, C, st, ev);
+        case 3: return front_t<LG, 3>(qdig, rows, r_pre, bpart, twist, dftw, roles, out, Q, rw, C, st, ev);
+        case 4: return front_t<LG, 4>(qdig, rows, r_pre, bpart, twist, dftw, roles, out, Q, rw, C, st, ev);
         default: return hipErrorInvalidValue;
     }
 }
 
 hipError_t launch_pack_front(const uint64_t* qdig, const uint32_t* rows, uint32_t* r_pre, uint64_t* bpart,
                              const uint32_t* twist, const uint2* dftw, uint32_t Q, uint32_t gamma, bool rw,
-                             const CtxDev& C, hipStream_t st, LaunchEv ev) {
+                             const CtxDev& C, hipStream_t st, LaunchEv ev, const PackRoles& roles, uint32_t* out) {
     if (!Q) return hipSuccess;
+    if (roles.role && (!roles.qout || !out)) return hipErrorInvalidValue;
     switch (gamma) {
-        case 1: return front_lg<0>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st, ev);
-        case 2: return front_lg<1>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st, ev);
-        case 4: return front_lg<2>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st, ev);
-        case 8: return front_lg<3>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st, ev);
-        case 16: return front_lg<4>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st, ev);
-        case 32: return front_lg<5>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st, ev);
-        case 64: return front_lg<6>(qdig, rows, r_pre, bpart, twist, dftw, Q, rw, C, st, ev);
+        case 1: return front_lg<0>(qdig, rows, r_pre, bpart, twist, dftw, roles, out, Q, rw, C, st, ev);
+        case 2: return front_lg<1>(qdig, rows, r_pre, bpart, twist, dftw, roles, out, Q, rw, C, st, ev);
+        case 4: return front_lg<2>(qdig, rows, r_pre, bpart, twist, dftw, roles, out, Q, rw, C, st, ev);
+        case 8: return front_lg<3>(qdig, rows, r_pre, bpart, twist, dftw, roles, out, Q, rw, C, st, ev);
+        case 16: return front_lg<4>(qdig, rows, r_pre, bpart, twist, dftw, roles, out, Q, rw, C, st, ev);
+        case 32: return front_lg<5>(qdig, rows, r_pre, bpart, twist, dftw, roles, out, Q, rw, C, st, ev);
+        case 64: return front_lg<6>(qdig, rows, r_pre, bpart, twist, dftw, roles, out, Q, rw, C, st, ev);
         default: return hipErrorInvalidValue;
     }
 }
@@ -473,6 +527,7 @@ hipError_t launch_pack_chain_scaled(const PackArgs& A, uint32_t Q, const CtxDev&
                                     hipStream_t st, LaunchEv ev) {
     if (!Q) return hipSuccess;
     if (pack_chain_crt_supported(C)) return launch_pack_chain_crt(A, Q, C, inv, st, ev);
+    if (A.roles.role || A.roles.qout) return hipErrorInvalidValue;  // roles are the two-modulus kernel's
     switch (C.logn) {
         case 8: return chain_k<8>(A, Q, C, inv, st);
         case 9: return chain_k<9>(A, Q, C, inv, st);

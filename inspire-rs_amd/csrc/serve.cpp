@@ -107,24 +107,35 @@ size_t request_footprint(const inspire_ctx* ctx, uint32_t cols, bool packed) {
     const size_t KN = (size_t)ctx->K * ctx->N, ell = ctx->ell;
     const size_t in = 16 + ell * 32 + ell * 2 * KN * 4;  // staging (at most full rows)
     const size_t rows = ell * 2 * KN * 4;
-    if (packed)
-        return in + rows + (size_t)cols * KN * 4 + (size_t)(ctx->N / 256) * cols * ctx->K * 8 + 2 * KN * 4;
-    return in + rows + (size_t)(1 + cols) * 2 * KN * 4;
+    // packing workspaces: step-1 sums [cols][K][N] and one b sum per wave [N/64][cols][K] u64
+    // (an InsPIRe^0 query holds them too when it shares a packing launch)
+    const size_t pack_ws = (size_t)cols * KN * 4 + (size_t)(ctx->N / 64) * cols * ctx->K * 8;
+    if (packed) return in + rows + pack_ws + 2 * KN * 4;
+    return in + rows + pack_ws + (size_t)(1 + cols) * 2 * KN * 4;
 }
 
 void resolve(inspire_server* s, SReq& r);
 
-// One group of a batch: same variant class and column count.
+// One group of a batch: one column count, and one variant class unless
+// the packing launches also answer InsPIRe^0 (mixed: per-query roles).
 struct Group {
-    bool packed = false;
+    bool packed = false, mixed = false;
     uint32_t cols = 0;
     std::vector<SReq*> reqs;  // seeded first
     uint32_t S = 0;           // seeded count
     // byte offsets in the staging area / inbox
-    size_t o_dig = 0, o_yoff = 0, o_seeds = 0, o_b = 0, o_full = 0, o_err = 0, in_end = 0;
+    size_t o_dig = 0, o_yoff = 0, o_role = 0, o_qout = 0, o_seeds = 0, o_b = 0, o_full = 0, o_err = 0, in_end = 0;
     size_t o_rows = 0;        // in the lane's rows buffer
-    size_t o_out = 0, out_words_per = 0;  // in the lane's out buffer
+    size_t o_out = 0;         // in the lane's out buffer
+    std::vector<size_t> off;  // each response's word offset from o_out
+    size_t out_words = 0;
 };
+
+// Words of a request's response block: packed [2][K][N], InsPIRe^0 [1 + cols][2][K][N].
+size_t response_words(const inspire_ctx* ctx, const SReq& r) {
+    const size_t KN = (size_t)ctx->K * ctx->N;
+    return r.packed() ? 2 * KN : (size_t)(1 + r.cols) * 2 * KN;
+}
 
 void run_batch(inspire_server* s, Lane& L, std::vector<std::shared_ptr<SReq>>& batch) {
     inspire_ctx* ctx = L.ctx;
@@ -132,6 +143,7 @@ void run_batch(inspire_server* s, Lane& L, std::vector<std::shared_ptr<SReq>>& b
     const size_t KN = (size_t)ctx->K * ctx->N, ell = ctx->ell;
     std::shared_lock<std::shared_mutex> rd(own->rw);
 
+    // key (variant class, cols): 0 InsPIRe^0, 1 packed, 2 both in one launch sequence
     std::map<std::pair<int, uint32_t>, Group> gm;
     for (auto& r : batch) {
         // bind again under this lock: a database may have been reloaded since submit
@@ -141,6 +153,24 @@ void run_batch(inspire_server* s, Lane& L, std::vector<std::shared_ptr<SReq>>& b
         g.packed = r->packed();
         g.cols = r->cols;
         g.reqs.push_back(r.get());
+    }
+    // a column count with both classes: one packing launch sequence with
+    // per-query roles, when the fused front end and the two-modulus chain apply
+    for (auto it = gm.begin(); it != gm.end();) {
+        const uint32_t cols = it->first.second;
+        auto pk = gm.find({1, cols});
+        if (it->first.first == 0 && pk != gm.end() && pack_mixed_supported(ctx, cols)) {
+            Group& m = gm[{2, cols}];
+            m.packed = true;
+            m.mixed = true;
+            m.cols = cols;
+            m.reqs = std::move(pk->second.reqs);
+            m.reqs.insert(m.reqs.end(), it->second.reqs.begin(), it->second.reqs.end());
+            gm.erase(pk);
+            it = gm.erase(it);
+        } else {
+            ++it;
+        }
     }
     if (gm.empty()) {  // every query of the batch failed its binding
         const uint64_t t = now_ns();
@@ -154,9 +184,16 @@ void run_batch(inspire_server* s, Lane& L, std::vector<std::shared_ptr<SReq>>& b
         std::stable_partition(g.reqs.begin(), g.reqs.end(), [](const SReq* r) { return r->seeded != 0; });
         const size_t Q = g.reqs.size();
         g.S = (uint32_t)std::count_if(g.reqs.begin(), g.reqs.end(), [](const SReq* r) { return r->seeded != 0; });
+        g.off.resize(Q);
+        for (size_t t = 0; t < Q; t++) {
+            g.off[t] = g.out_words;
+            g.out_words += response_words(ctx, *g.reqs[t]);
+        }
         g.o_dig = align(in);
         g.o_yoff = align(g.o_dig + Q * 8);
-        g.o_seeds = align(g.o_yoff + Q * 8);
+        g.o_role = align(g.o_yoff + Q * 8);
+        g.o_qout = align(g.o_role + (g.mixed ? Q * 4 : 0));
+        g.o_seeds = align(g.o_qout + (g.mixed ? Q * 8 : 0));
         g.o_b = align(g.o_seeds + (size_t)g.S * ell * 32);
         g.o_full = align(g.o_b + (size_t)g.S * ell * KN * 4);
         g.o_err = align(g.o_full + (Q - g.S) * ell * 2 * KN * 4);
@@ -164,9 +201,8 @@ void run_batch(inspire_server* s, Lane& L, std::vector<std::shared_ptr<SReq>>& b
         in = g.in_end;
         g.o_rows = rows_words;
         rows_words += Q * ell * 2 * KN;
-        g.out_words_per = g.packed ? 2 * KN : (size_t)(1 + g.cols) * 2 * KN;
         g.o_out = out_words;
-        out_words += Q * g.out_words_per;
+        out_words += g.out_words;
     }
     L.pin_in.reserve(in);
     L.pin_out.reserve(out_words * 4 + 4 * gm.size() + 16);
@@ -183,7 +219,11 @@ void run_batch(inspire_server* s, Lane& L, std::vector<std::shared_ptr<SReq>>& b
             for (size_t t = t0; t < t1; t++) {
                 const SReq* r = g.reqs[t];
                 reinterpret_cast<uint64_t*>(pin + g.o_dig)[t] = r->dig;
-                reinterpret_cast<uint64_t*>(pin + g.o_yoff)[t] = r->yoff;
+                reinterpret_cast<uint64_t*>(pin + g.o_yoff)[t] = r->packed() ? r->yoff : 0;
+                if (g.mixed) {
+                    reinterpret_cast<uint32_t*>(pin + g.o_role)[t] = r->packed() ? 0u : 1u;
+                    reinterpret_cast<uint64_t*>(pin + g.o_qout)[t] = g.off[t];
+                }
                 if (t < g.S) {
                     std::memcpy(pin + g.o_seeds + t * ell * 32, r->seeds_ptr(), ell * 32);
                     std::memcpy(pin + g.o_b + t * ell * KN * 4, r->rows_ptr(), ell * KN * 4);
@@ -210,11 +250,21 @@ void run_batch(inspire_server* s, Lane& L, std::vector<std::shared_ptr<SReq>>& b
                         reinterpret_cast<const uint32_t*>(inbox + g.o_b),
                         reinterpret_cast<const uint32_t*>(inbox + g.o_full), d_err);
         const uint64_t* d_qdig = reinterpret_cast<const uint64_t*>(inbox + g.o_dig);
+        PackRoles roles;
+        if (g.mixed) {
+            roles.role = reinterpret_cast<const uint32_t*>(inbox + g.o_role);
+            roles.qout = reinterpret_cast<const uint64_t*>(inbox + g.o_qout);
+        }
         if (g.packed)
             pack_run(ctx, g.cols, Q, grows, d_qdig, own->clients.keys.get<uint32_t>(),
-                     reinterpret_cast<const uint64_t*>(inbox + g.o_yoff), nullptr, nullptr, out + g.o_out, false);
+                     reinterpret_cast<const uint64_t*>(inbox + g.o_yoff), nullptr, nullptr, out + g.o_out, false,
+                     PackWs{}, roles);
         else
             run_respond0(ctx, g.cols, Q, grows, d_qdig, out + g.o_out);
+    }
+    {
+        std::lock_guard<std::mutex> lk(s->m);
+        s->stats.launch_groups += gm.size();
     }
     uint32_t* pout = L.pin_out.get<uint32_t>();
     hip_check(hipMemcpyAsync(pout, out, out_words * 4, hipMemcpyDeviceToHost, st), "serve download");
@@ -238,10 +288,10 @@ void run_batch(inspire_server* s, Lane& L, std::vector<std::shared_ptr<SReq>>& b
             continue;
         }
         // each response out of the lane's pinned buffer (allocated here, copied on several threads)
-        for (SReq* r : g.reqs) r->out.reset(new uint32_t[g.out_words_per]);
+        for (SReq* r : g.reqs) r->out.reset(new uint32_t[response_words(ctx, *r)]);
         par_ranges(g.reqs.size(), 8, [&](size_t t0, size_t t1) {
             for (size_t t = t0; t < t1; t++)
-                std::memcpy(g.reqs[t]->out.get(), pout + g.o_out + t * g.out_words_per, g.out_words_per * 4);
+                std::memcpy(g.reqs[t]->out.get(), pout + g.o_out + g.off[t], response_words(ctx, *g.reqs[t]) * 4);
         });
     }
 }

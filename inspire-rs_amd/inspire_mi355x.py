@@ -164,6 +164,7 @@ class _ServerStats(C.Structure):
         ("max_batch_seen", C.c_uint64),
         ("peak_inflight_bytes", C.c_uint64),
         ("peak_lanes_busy", C.c_uint64),
+        ("launch_groups", C.c_uint64),
     ]
 
 

@@ -136,6 +136,27 @@ def test_serving_mix_equals_oracle_and_alone(M, world):
         assert st["requests"] == 12 + 2 * len(qs)
         assert st["batches"] >= len(qs) // 8
         assert st["max_batch_seen"] <= 8
+        # one column count (16): every batch, whatever its variants, is one launch sequence
+        assert st["launch_groups"] == st["batches"]
+
+
+def test_serving_variants_share_one_launch(M, world):
+    """InsPIRe^0, ^1 and ^2 queries of one column count in one batch run as
+    one launch sequence (expansion, fused front end with per-query roles,
+    chain kernel whose InsPIRe^0 workgroups only transform), and every
+    response equals the oracle's."""
+    ell = world["o"].params.gadget_len
+    qs = world["queries"]
+    assert {q["variant"] for q in qs} == {0, 1, 2}
+    with make_ctx(M, world) as ctx, M.Server(ctx, lanes=1, max_batch=64) as srv:
+        tickets = srv.submit([request(M, q, ell) for q in qs])
+        for q, t in zip(qs, tickets):
+            check(M, q, srv.wait(int(t), cols=16))
+        st = srv.stats()
+        say("one-launch stats", st)
+        assert st["requests"] == len(qs)
+        assert st["launch_groups"] == st["batches"]
+        assert st["max_batch_seen"] > 1
 
 
 def test_serving_budget_limits_batches(M, world):
@@ -145,7 +166,7 @@ def test_serving_budget_limits_batches(M, world):
     o = world["o"]
     K, d = o.params.K, o.params.d
     KN = K * d
-    per = (16 + ell * 32 + ell * 2 * KN * 4) + ell * 2 * KN * 4 + 16 * KN * 4 + (d // 256) * 16 * K * 8 + 2 * KN * 4
+    per = (16 + ell * 32 + ell * 2 * KN * 4) + ell * 2 * KN * 4 + 16 * KN * 4 + (d // 64) * 16 * K * 8 + 2 * KN * 4
     with make_ctx(M, world) as ctx, M.Server(ctx, lanes=2, max_batch=64, hbm_budget=2 * per) as srv:
         tickets = srv.submit([request(M, q, ell) for q in qs])
         for q, t in zip(qs, tickets):
@@ -223,3 +244,6 @@ def test_serving_other_column_counts(M):
                 assert (res.ciphertext == want[0]).all()
                 if want[1] is not None:
                     assert (res.column_ciphertexts == want[1]).all()
+            # none of these column counts has the fused front end: each keeps a
+            # launch sequence per variant class
+            assert srv.stats()["launch_groups"] >= srv.stats()["batches"]
