@@ -51,6 +51,19 @@ MODMUL_QUERY = (
     + GAMMA * GAMMA * D          # step 1 (residue 0)
     + MODMUL_CHAIN
 )
+# The same count for the reference's algorithm as written, with the database
+# digits transformed per query (respond.rs:316-324 under external_product.rs:
+# 122-150) and step 1 as the reference runs it (inspiring2.rs:952-982): RGSW
+# rows 12 NTT; digit NTTs 16*3*K; column MAC 16*3*2*K*d; step 1 16*16*K*d plus
+# the gamma^-1 scaling 16*K*d; step 2 15*(INTT + 3 NTT)*K plus its MAC
+# 15*3*K*d; a_hat's INTT; online NTT(y_body) 3*K, MAC 45*K*d, INTT K.
+MODMUL_QUERY_REF_ALGORITHM = (
+    2 * ELL * K * NTT + GAMMA * ELL * K * NTT + GAMMA * ELL * 2 * K * D
+    + GAMMA * GAMMA * K * D + GAMMA * K * D
+    + (GAMMA - 1) * (1 + ELL) * K * NTT + (GAMMA - 1) * ELL * K * D
+    + K * NTT
+    + ELL * K * NTT + (GAMMA - 1) * ELL * K * D + K * NTT
+)
 # Algorithmic HBM bytes per query of the fused front end (first dimension +
 # step 1 + b sums): the shard's NTT-domain digits, the RGSW rows it reads (b in
 # both residues, a in residue 0, which is all the reference's residue handling
@@ -62,6 +75,9 @@ FRONT_BYTES = (GAMMA * ELL * K * D * 4 + ELL * (K + 1) * D * 4 + GAMMA * D * 4
 # that rate and a Shoup/Montgomery modular product needs 3 of them.
 HBM_PEAK_GBS = 8000.0
 MODMUL_PEAK_T = 157.3 / 2 / 2 / 3
+# Issue rates measured on the device at start-up (inspire_ctx_measure_rates):
+# mmul_tops / smmul_tops (the transforms' Montgomery products), mad64_tops.
+RATES: dict = {}
 
 
 def tree_modmul(m: int = GAMMA):
@@ -119,10 +135,35 @@ def percentile(xs, p):
     return float(np.percentile(np.asarray(xs), p))
 
 
-def cpu_baseline(db_bytes: np.ndarray, qset, gpu_out: np.ndarray, budget_s: float, w_seed: bytes):
+def host_info():
+    """The host the CPU baseline ran on: CPU model, logical CPUs, the CPUs
+    this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count()
+    return {"cpu_model": model, "nproc": os.cpu_count(), "usable_cpus": usable}
+
+
+def cpu_baseline(db_bytes: np.ndarray, qset, gpu_out: np.ndarray, budget_s: float, w_seed: bytes,
+                 literal_queries: int = -1):
     """The oracle (C restatement of the reference respond) on this host's
     cores: InsPIRe^2 with the query-independent tables hoisted, one query per
-    thread.  Also compares the sampled responses with the GPU's, bit for bit."""
+    thread.  Also compares the sampled responses with the GPU's, bit for bit.
+    Then the literal form (respond_inspiring as the reference runs it, per
+    query: NTT contexts, mul_ntt external products, PackParams::new with the
+    brute-force automorphism table of every odd t < 2d, and
+    OfflinePackingKeys::generate) on `literal_queries` queries, one per
+    thread (-1: one per thread; 0: skipped)."""
     sys.path.insert(0, ROOT)
     import oracle as O
 
@@ -153,7 +194,7 @@ def cpu_baseline(db_bytes: np.ndarray, qset, gpu_out: np.ndarray, budget_s: floa
             if rc[j] != 0 or not np.array_equal(outs[j].astype(np.uint32), gpu_out[done + j]):
                 mismatches += 1
         done += m
-    return {
+    out = {
         "value": done / t_total,
         "unit": "queries/s",
         "cores": threads,
@@ -162,7 +203,30 @@ def cpu_baseline(db_bytes: np.ndarray, qset, gpu_out: np.ndarray, budget_s: floa
                   f"({t_total * threads:.1f} core-s); "
                   f"PackParams/OfflinePackingKeys hoisted; bit-exact vs GPU: {done - mismatches}/{done}",
         "bit_exact_sampled": f"{done - mismatches}/{done}",
+        "host": host_info(),
+        "literal": None,
     }
+    nl = threads if literal_queries < 0 else min(literal_queries, n)
+    if nl > 0:
+        ins = [oracle_inputs(j) for j in range(nl)]
+        t0 = time.perf_counter()
+        rc, outs = o.respond_inspiring_literal_many(w_seed, [x[0] for x in ins], [x[1] for x in ins],
+                                                    [x[2] for x in ins])
+        t_lit = time.perf_counter() - t0
+        ok = sum(1 for j in range(nl) if rc[j] == 0 and np.array_equal(outs[j].astype(np.uint32), gpu_out[j]))
+        used = min(threads, nl)
+        out["literal"] = {
+            "value": nl / t_lit,
+            "unit": "queries/s",
+            "cores": used,
+            "kind": "port",
+            "sample": f"{nl} InsPIRe^2 queries of the timed workload, one per thread, {t_lit:.2f} s wall on "
+                      f"{used} threads; per query: an NTT context per column, mul_ntt external products, "
+                      f"PackParams::new with all {D} brute-force automorphism tables and 4d monomials, "
+                      f"OfflinePackingKeys::generate; bit-exact vs GPU: {ok}/{nl}",
+            "bit_exact_sampled": f"{ok}/{nl}",
+        }
+    return out
 
 
 def main():
@@ -194,6 +258,9 @@ def main():
                     help="launches of the batched client-side extract leg (0 skips it)")
     ap.add_argument("--extract-batch", type=int, default=4096, help="responses per extract launch")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-literal-queries", type=int, default=-1,
+                    help="queries of the literal CPU baseline (nothing hoisted), one per thread; -1 (default): "
+                         "one per oracle thread, 0 skips it")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (a known-size kernel for "
                          "calibrating rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -240,6 +307,10 @@ def main():
         tuning = ctx.autotune()
     log(f"tuning: ntt_fwd_2048 {tuning['ntt_fwd_2048']}, chain {tuning['chain_values_per_thread']} values/thread "
         f"{tuning.get('measured', {})}")
+    # measured issue rates of the arithmetic itself (register-resident probes), the roof beside the
+    # figure derived from the published FP32 rate
+    RATES.update({k_: round(v, 3) for k_, v in ctx.measure_rates().items()})
+    log(f"measured issue rates (T/s): {RATES}")
     t0 = time.perf_counter()
     ctx.load_entries(0, db, 32)
     log(f"database resident: {nshards} shards, {ctx.device_bytes / 2**30:.1f} GiB on device, "
@@ -475,6 +546,10 @@ def main():
         "measured": f"HIP events on each lane's stream over the timed region ({n_lanes} lanes, "
                     "sharing CUs with the other lanes' kernels)",
         "avg_launch_ms_alone": round(iso_chain_ms, 4),
+        "peak_measured": RATES.get("smmul_tops"),
+        "frac_measured": round(chain_tmodmul / RATES["smmul_tops"], 4) if RATES.get("smmul_tops") else None,
+        "peak_measured_note": "the signed Montgomery product's measured issue rate (the chain's digit transforms "
+                              "and multiply-accumulates are signed)",
     }
     roofline_hbm = {
         "bound": "hbm",
@@ -521,7 +596,13 @@ def main():
         "lanes_sweep": sweep,
         "graphs_ab": graphs_ab,
         "tuning": tuning,
+        "measured_issue_rates_tops": dict(RATES),
         "modmul_per_query": MODMUL_QUERY,
+        "modmul_per_query_reference_algorithm": MODMUL_QUERY_REF_ALGORITHM,
+        "modmul_note": "modmul_per_query counts the work this path does (database digits stored in the NTT "
+                       "domain, step 1 by Horner's rule on residue 0, the last gadget digit's transform "
+                       "recovered); the reference's algorithm as written does "
+                       f"{MODMUL_QUERY_REF_ALGORITHM} per query; the roofline uses the former",
         "roofline": roofline,
         "roofline_hbm_front": roofline_hbm,
         "kernel_ms_per_step": {k_: round(ms / n, 4) for k_, (ms, n) in breakdown.items() if n},
@@ -536,7 +617,8 @@ def main():
         step(args.steps - 1)  # the batch of this query set, responses in pout
         gout = dout.download(np.zeros((B, 2, K, D), dtype=np.uint32))
         log("cpu baseline (oracle) ...")
-        line["cpu_baseline"] = cpu_baseline(db, (shards, b, seeds, y), gout, args.cpu_seconds, w_seed)
+        line["cpu_baseline"] = cpu_baseline(db, (shards, b, seeds, y), gout, args.cpu_seconds, w_seed,
+                                            args.cpu_literal_queries)
     for _, db_, ds_, dy_ in dev:
         db_.free(), ds_.free(), dy_.free()
     for o_ in louts:
@@ -622,6 +704,8 @@ def tree_leg(args, M, ctx, db, dev, qsets, log):
             "peak": round(MODMUL_PEAK_T, 2),
             "unit": "Tmodmul/s",
             "frac": round(achieved / MODMUL_PEAK_T, 4),
+            "peak_measured": RATES.get("mmul_tops"),
+            "frac_measured": round(achieved / RATES["mmul_tops"], 4) if RATES.get("mmul_tops") else None,
             "traffic": measured_traffic("k_tree_leaves", B),
             "algorithmic_per_launch": f"{mm_leaves} modmul/query x {B} queries",
             "avg_launch_ms": round(lv_avg * 1e3, 4),
@@ -711,10 +795,14 @@ def extract_leg(args, M, ctx, log):
         "steps": args.extract_steps,
         "avg_launch_ms": round(avg * 1e3, 4),
         "roofline": {
-            "bound": "valu" if mm_t / MODMUL_PEAK_T >= gbs / HBM_PEAK_GBS else "hbm",
+            "bound": "valu" if mm_t / (RATES.get("mad64_tops") or MODMUL_PEAK_T) >= gbs / HBM_PEAK_GBS else "hbm",
             "kernel": "k_extract",
             "achieved_modmul": round(mm_t, 3), "peak_modmul": round(MODMUL_PEAK_T, 2), "unit_modmul": "Tmodmul/s",
             "frac_modmul": round(mm_t / MODMUL_PEAK_T, 4),
+            "peak_mad64_measured": RATES.get("mad64_tops"), "unit_mad64": "T multiply-add/s",
+            "frac_mad64_measured": round(mm_t / RATES["mad64_tops"], 4) if RATES.get("mad64_tops") else None,
+            "mad64_note": "each term of the kernel's dot products is one 64-bit multiply-add, not a modular "
+                          "product: frac_mad64_measured is its roof; frac_modmul prices it as modmuls",
             "achieved_hbm": round(gbs, 1), "peak_hbm": HBM_PEAK_GBS, "unit_hbm": "GB/s",
             "frac_hbm": round(gbs / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic("k_extract", E),

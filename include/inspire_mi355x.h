@@ -467,6 +467,13 @@ int inspire_ctx_profile(inspire_ctx *ctx, const char *kernel, double *total_ms, 
 int inspire_ctx_autotune(inspire_ctx *ctx, const char *path, char *report, uint64_t capacity, uint64_t *len);
 int inspire_ctx_load_tuning(inspire_ctx *ctx, const char *path);
 int inspire_ctx_set_kernel_choice(inspire_ctx *ctx, const char *name, int value);
+/* Measured issue rates, T operations/s over the whole device, of the
+ * arithmetic the integer kernels are priced against (register-resident
+ * probes, no memory traffic): tops[0] the Montgomery modular product of the
+ * transforms, tops[1] its signed form, tops[2] the 64-bit multiply-add of
+ * the extract kernel.  n >= 3.  No counterpart in the reference: it is the
+ * measured roof beside the published one. */
+int inspire_ctx_measure_rates(inspire_ctx *ctx, double *tops, uint32_t n);
 int inspire_ctx_get_tuning(const inspire_ctx *ctx, char *out, uint64_t capacity, uint64_t *len);
 
 /* ---- math entry points (NttContext, external_product) ---- */

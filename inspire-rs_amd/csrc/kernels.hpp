@@ -82,4 +82,10 @@ hipError_t launch_rows_seeded(const uint32_t* seeds, const uint32_t* bsrc, uint3
                               uint32_t* err, const CtxDev& C, const ScaleDev& sc, hipStream_t st,
                               LaunchEv ev = LaunchEv{});
 
+// Issue-rate probes (rates.hip): kind 0 the Montgomery product, 1 the signed
+// Montgomery product, 2 the 64-bit multiply-add; out holds blocks * 256 words.
+hipError_t launch_rate_probe(int kind, uint32_t* out, uint32_t blocks, uint32_t iters, const ModDev& m,
+                             hipStream_t st);
+uint64_t rate_ops_per_launch(uint32_t blocks, uint32_t iters);
+
 }  // namespace inspire

@@ -11,6 +11,7 @@
 // the measurements persist as a small JSON file.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -193,6 +194,25 @@ std::string tuning_json(const inspire_ctx* ctx, const TuneResult* R) {
     return o.str();
 }
 
+// Issue rates of the probes in rates.hip (T operations/s): 8 workgroups of
+// 256 threads per CU, each thread 8 chains of 2048 operations, the mean of
+// 5 launches after a warm one.
+void measure_rates(inspire_ctx* ctx, double* tops) {
+    int dev = 0, cus = 256;
+    hipDeviceProp_t p{};
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&p, dev) == hipSuccess) cus = p.multiProcessorCount;
+    (void)hipGetLastError();
+    const uint32_t blocks = 8u * (uint32_t)std::max(cus, 1), iters = 2048;
+    DevBuf out((size_t)blocks * 256 * 4);
+    for (int kind = 0; kind < 3; kind++) {
+        const double ms = time_launches(ctx, 5, [&] {
+            hip_check(launch_rate_probe(kind, out.get<uint32_t>(), blocks, iters, ctx->dev.m[0], ctx->stream),
+                      "rate probe");
+        });
+        tops[kind] = (double)rate_ops_per_launch(blocks, iters) / (ms * 1e-3) / 1e12;
+    }
+}
+
 int copy_out(const std::string& s, char* out, uint64_t cap, uint64_t* len) {
     if (len) *len = s.size();
     if (!out) return INSPIRE_OK;
@@ -281,6 +301,16 @@ int inspire_ctx_set_kernel_choice(inspire_ctx* ctx, const char* name, int value)
         std::lock_guard<std::recursive_mutex> g(ctx->mu);
         graph_cache_clear(ctx->graphs);
         apply_choice(ctx, name, value);
+        return INSPIRE_OK;
+    })
+}
+
+int inspire_ctx_measure_rates(inspire_ctx* ctx, double* tops, uint32_t n) {
+    INSPIRE_TRY({
+        if (!ctx || !tops) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "null argument"};
+        if (n < 3) throw Error{INSPIRE_ERR_CAPACITY, "the rate probes report 3 values"};
+        std::lock_guard<std::recursive_mutex> g(ctx->mu);
+        measure_rates(ctx, tops);
         return INSPIRE_OK;
     })
 }

@@ -202,6 +202,7 @@ EXPORTS = [
     ("inspire_ctx_load_tuning", C.c_int, [C.c_void_p, C.c_char_p]),
     ("inspire_ctx_set_kernel_choice", C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
     ("inspire_ctx_get_tuning", C.c_int, [C.c_void_p, C.c_char_p, C.c_uint64, _u64p]),
+    ("inspire_ctx_measure_rates", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_uint32]),
     ("inspire_ctx_graph_stats", C.c_int, [C.c_void_p, _u64p, _u64p]),
     ("inspire_register_client_keys", C.c_int, [C.c_void_p, C.c_uint32, _u64p]),
     ("inspire_server_create", C.c_int, [C.c_void_p, C.POINTER(_ServerConfig), C.POINTER(C.c_void_p)]),
@@ -600,6 +601,13 @@ class Context:
         buf = C.create_string_buffer(1024)
         _raise(lib().inspire_ctx_get_tuning(self._h, buf, 1024, C.byref(n)))
         return json.loads(buf.value.decode())
+
+    def measure_rates(self) -> dict:
+        """Measured issue rates (inspire_ctx_measure_rates), T operations/s:
+        the Montgomery product, its signed form, the 64-bit multiply-add."""
+        t = (C.c_double * 3)()
+        _raise(lib().inspire_ctx_measure_rates(self._h, t, 3))
+        return {"mmul_tops": t[0], "smmul_tops": t[1], "mad64_tops": t[2]}
 
     def set_graphs(self, on: bool):
         """Graph replay of repeated device batches (on by default)."""

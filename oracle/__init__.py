@@ -104,6 +104,14 @@ def _load():
             [P, C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(_u64p), C.c_uint32, C.POINTER(_u64p),
              C.POINTER(_u64p), C.POINTER(_u64p), C.POINTER(C.c_int)],
         ),
+        "or_respond_inspiring_literal": (
+            C.c_int, [P, C.c_void_p, _u64p, C.c_uint32, _u64p, _u64p, _u64p, _u64p],
+        ),
+        "or_respond_inspiring_literal_many": (
+            None,
+            [P, C.c_void_p, C.c_uint32, C.POINTER(_u64p), C.c_uint32, C.POINTER(_u64p), C.POINTER(_u64p),
+             C.POINTER(_u64p), C.POINTER(C.c_int)],
+        ),
         "or_num_threads": (C.c_int, []),
         "or_sample_extract_coeff0": (None, [P, _u64p, _u64p, _u64p]),
         "or_homomorphic_automorph": (None, [P, C.c_void_p, _u64p, C.c_uint32, _u64p, C.c_uint32, _u64p]),
@@ -366,6 +374,34 @@ class Oracle:
         rc = (C.c_int * n)()
         lib.or_respond_inspiring_many(C.byref(self.cp), self.ntt, pk, n, arr(keep[:n]), ncols, arr(keep[n:2 * n]),
                                       arr(keep[2 * n:]), arr(outs), rc)
+        return list(rc), outs
+
+    def respond_inspiring_literal(self, w_seed, shard_polys, rgsw, y_body):
+        """respond_inspiring with nothing hoisted (per query: NTT contexts,
+        literal external products, PackParams::new with every brute-force
+        automorphism table, OfflinePackingKeys::generate) -> (rc, ct, work);
+        work = (brute-force retries, tables built)."""
+        shard_polys = np.ascontiguousarray(shard_polys, dtype=np.uint64)
+        ptr, keep = _p8(w_seed)
+        ct = self.zeros(2)
+        work = np.zeros(2, dtype=np.uint64)
+        rc = lib.or_respond_inspiring_literal(C.byref(self.cp), ptr, _p64(shard_polys), shard_polys.shape[0],
+                                              _p64(np.ascontiguousarray(rgsw)),
+                                              _p64(np.ascontiguousarray(y_body, dtype=np.uint64)), _p64(ct),
+                                              _p64(work))
+        return rc, ct, (int(work[0]), int(work[1]))
+
+    def respond_inspiring_literal_many(self, w_seed, shards, rgsws, y_bodies):
+        """Many literal respond_inspiring calls, one per OpenMP thread at a time."""
+        n = len(shards)
+        ncols = shards[0].shape[0]
+        ptr, keep_seed = _p8(w_seed)
+        keep = [np.ascontiguousarray(x, dtype=np.uint64) for x in list(shards) + list(rgsws) + list(y_bodies)]
+        outs = [self.zeros(2) for _ in range(n)]
+        arr = lambda xs: (_u64p * n)(*[_p64(x) for x in xs])
+        rc = (C.c_int * n)()
+        lib.or_respond_inspiring_literal_many(C.byref(self.cp), ptr, n, arr(keep[:n]), ncols, arr(keep[n:2 * n]),
+                                              arr(keep[2 * n:]), arr(outs), rc)
         return list(rc), outs
 
     # -- tree packing (inspiring/automorph_pack.rs)

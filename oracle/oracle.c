@@ -777,10 +777,10 @@ or_pack *or_pack_new(const or_params *pr, const or_ntt *c, uint32_t gamma, const
     return or_pack_new_mode(pr, c, gamma, w_seed, 0);
 }
 
-or_pack *or_pack_new_mode(const or_params *pr, const or_ntt *c, uint32_t gamma, const uint8_t w_seed[32],
-                          int residue_wise) {
-    const uint32_t n = pr->d, two_n = 2 * n, ell = pr->gadget_len;
-    const size_t P = psz(pr);
+/* PackParams::new's scalars (inspiring2.rs:115-136): generator, its n powers
+ * mod 2n, mod_inv_gamma and its NTT as a constant polynomial. */
+static or_pack *pack_alloc(const or_params *pr, const or_ntt *c, uint32_t gamma, int residue_wise) {
+    const uint32_t n = pr->d, two_n = 2 * n;
     or_pack *pk = (or_pack *)calloc(1, sizeof(or_pack));
     pk->pr = *pr;
     pk->residue_wise = residue_wise;
@@ -792,30 +792,27 @@ or_pack *or_pack_new_mode(const or_params *pr, const or_ntt *c, uint32_t gamma, 
     for (uint32_t i = 0; i < n; i++) { pk->gen_pows[i] = (uint32_t)v; v = (v * pk->generator) % two_n; }
     pk->mod_inv_gamma = mod_inverse_u64(gamma, pr->q);             /* :131-132 */
     pk->mod_inv_ntt = palloc(pr, 1);                               /* :135-136 */
-    {
-        uint64_t *cf = (uint64_t *)calloc(n, sizeof(uint64_t));
-        cf[0] = pk->mod_inv_gamma;
-        or_poly_from_coeffs(pr, cf, pk->mod_inv_ntt);
-        or_ntt_forward(c, pk->mod_inv_ntt);
-        free(cf);
-    }
+    uint64_t *cf = (uint64_t *)calloc(n, sizeof(uint64_t));
+    cf[0] = pk->mod_inv_gamma;
+    or_poly_from_coeffs(pr, cf, pk->mod_inv_ntt);
+    or_ntt_forward(c, pk->mod_inv_ntt);
+    free(cf);
     pk->tables = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)gamma * n);
-    for (uint32_t i = 0; i < gamma; i++)
-        or_automorph_table(n, 2 * table_index(pk->gen_pows[i]) + 1, pk->tables + (size_t)i * n);
-    /* the NTT-form monomials packing_offline step 1 reads (PackParams.monomials_ntt,
-     * inspiring2.rs:143-158), kept per (i, j) */
     pk->mono = palloc(pr, (size_t)gamma * gamma);
-    for (uint32_t i = 0; i < gamma; i++)
-        for (uint32_t j = 0; j < gamma; j++) {
-            uint32_t e = (uint32_t)(((uint64_t)j * pk->gen_pows[(n - i) % n]) % (2ull * n));
-            monomial_ntt(pk, e, pk->mono + ((size_t)i * gamma + j) * P);
-        }
-    /* OfflinePackingKeys::generate (inspiring2.rs:448-506) */
+    return pk;
+}
+
+/* OfflinePackingKeys::generate (inspiring2.rs:448-506): the mask from the
+ * seed, its NTT, and its rotations by the tables of g^i, i < gamma - 1. */
+static void pack_keys(or_pack *pk, const uint8_t w_seed[32]) {
+    const or_params *pr = &pk->pr;
+    const uint32_t n = pr->d, ell = pr->gadget_len, gamma = pk->gamma;
+    const size_t P = psz(pr);
     pk->w_mask = palloc(pr, ell);
     or_generate_mask(pr, w_seed, pk->w_mask);
     uint64_t *w_mask = palloc(pr, ell);
     memcpy(w_mask, pk->w_mask, (size_t)ell * P * 8);
-    for (uint32_t k = 0; k < ell; k++) or_ntt_forward(c, w_mask + (size_t)k * P);
+    for (uint32_t k = 0; k < ell; k++) or_ntt_forward(pk->c, w_mask + (size_t)k * P);
     uint32_t nrot = gamma > 0 ? gamma - 1 : 0;
     pk->w_all_ntt = palloc(pr, (size_t)(nrot ? nrot : 1) * ell);
     for (uint32_t i = 0; i < nrot; i++)
@@ -823,6 +820,25 @@ or_pack *or_pack_new_mode(const or_params *pr, const or_ntt *c, uint32_t gamma, 
             pack_automorph_ntt(pk, w_mask + (size_t)k * P, pk->tables + (size_t)i * n,
                                pk->w_all_ntt + ((size_t)i * ell + k) * P);
     free(w_mask);
+}
+
+/* The query-independent tables built once (the hoisted form): the closed-form
+ * automorphism tables of the gamma generator powers packing reads, and the
+ * gamma^2 NTT-form monomials step 1 reads (PackParams.monomials_ntt,
+ * inspiring2.rs:143-158), kept per (i, j). */
+or_pack *or_pack_new_mode(const or_params *pr, const or_ntt *c, uint32_t gamma, const uint8_t w_seed[32],
+                          int residue_wise) {
+    const uint32_t n = pr->d;
+    const size_t P = psz(pr);
+    or_pack *pk = pack_alloc(pr, c, gamma, residue_wise);
+    for (uint32_t i = 0; i < gamma; i++)
+        or_automorph_table(n, 2 * table_index(pk->gen_pows[i]) + 1, pk->tables + (size_t)i * n);
+    for (uint32_t i = 0; i < gamma; i++)
+        for (uint32_t j = 0; j < gamma; j++) {
+            uint32_t e = (uint32_t)(((uint64_t)j * pk->gen_pows[(n - i) % n]) % (2ull * n));
+            monomial_ntt(pk, e, pk->mono + ((size_t)i * gamma + j) * P);
+        }
+    pack_keys(pk, w_seed);
     return pk;
 }
 
@@ -960,6 +976,117 @@ void or_respond_inspiring_many(const or_params *pr, const or_ntt *c, const or_pa
 #pragma omp parallel for schedule(dynamic, 1)
     for (uint32_t i = 0; i < n; i++)
         rc[i] = or_respond_inspiring(pr, c, pk, shards[i], ncols, rgsw[i], y_body[i], NULL, 0, ct_out[i]);
+}
+
+/* respond_inspiring as the reference runs it (respond.rs:284-386), nothing
+ * hoisted, for the CPU baseline's literal form:
+ *  - every column's external product under an NTT context of its own
+ *    (crs.params.ntt_context(), respond.rs:319-321; ntt.rs:116-164) with one
+ *    mul_ntt per row product (external_product.rs:122-150);
+ *  - PackParams::new in full (inspiring2.rs:107-182) under its own context:
+ *    the n generator powers, mod_inv's NTT, the 4n monomials (X^j and -X^j,
+ *    coefficient and NTT form) and the brute-force NTT-domain automorphism
+ *    table of every odd t < 2n (inspiring2.rs:208-262); packing reads its
+ *    tables and monomials out of these;
+ *  - OfflinePackingKeys::generate (inspiring2.rs:448-506), packing_offline,
+ *    generate_rotations and packing_online under the query's context
+ *    (respond.rs:293).
+ * One thread.  Returns or_respond_inspiring's code (and the same ciphertext),
+ * or -4 when a brute-force table disagrees with the closed form.  work[0..1]
+ * (may be NULL) receive the brute-force retries and the tables built. */
+int or_respond_inspiring_literal(const or_params *pr, const uint8_t w_seed[32], const uint64_t *shard_polys,
+                                 uint32_t ncols, const uint64_t *rgsw, const uint64_t *y_body, uint64_t *ct_out,
+                                 uint64_t *work) {
+    const size_t P = psz(pr);
+    const uint32_t d = pr->d, n = pr->d, ell = pr->gadget_len;
+    if (ncols == 0) { memset(ct_out, 0, 2 * P * 8); return 1; }
+    or_ntt *ctx = or_ntt_new(d, pr->moduli, pr->K);                      /* :293 */
+    uint64_t *cols = palloc(pr, 2 * (size_t)ncols);
+    for (uint32_t col = 0; col < ncols; col++) {                         /* :316-324 */
+        or_ntt *local = or_ntt_new(d, pr->moduli, pr->K);
+        uint64_t *rl = palloc(pr, 2);
+        trivial_encrypt(pr, shard_polys + (size_t)col * P, rl);
+        or_external_product(pr, local, rl, rgsw, cols + (size_t)col * 2 * P, 1);
+        free(rl);
+        or_ntt_free(local);
+    }
+    uint64_t *a_ct = palloc(pr, ncols), *bcf = (uint64_t *)calloc(d, sizeof(uint64_t));
+    uint64_t *b_poly = palloc(pr, 1);
+    const uint64_t inv = crt_inv(pr);
+    for (uint32_t col = 0; col < ncols; col++) {
+        memcpy(a_ct + (size_t)col * P, cols + (size_t)col * 2 * P, P * 8); /* :344-347 */
+        if (col < d) bcf[col] = poly_coeff_inv(pr, cols + (size_t)col * 2 * P + P, 0, inv); /* :350-356 */
+    }
+    or_poly_from_coeffs(pr, bcf, b_poly);
+
+    /* PackParams::new(&crs.params, num_columns) (:360) */
+    or_ntt *pctx = or_ntt_new(d, pr->moduli, pr->K);                     /* inspiring2.rs:111 */
+    or_pack *pk = pack_alloc(pr, pctx, ncols, 0);
+    uint64_t *mono = palloc(pr, n), *mono_ntt = palloc(pr, n);           /* :139-158 */
+    uint64_t *neg = palloc(pr, n), *neg_ntt = palloc(pr, n);
+    uint64_t *cf = (uint64_t *)calloc(n, sizeof(uint64_t));
+    for (uint32_t j = 0; j < n; j++) {
+        memset(cf, 0, (size_t)n * 8);
+        cf[j] = 1;
+        or_poly_from_coeffs(pr, cf, mono + (size_t)j * P);
+        memcpy(mono_ntt + (size_t)j * P, mono + (size_t)j * P, P * 8);
+        or_ntt_forward(pctx, mono_ntt + (size_t)j * P);
+        cf[j] = pr->q - 1;
+        or_poly_from_coeffs(pr, cf, neg + (size_t)j * P);
+        memcpy(neg_ntt + (size_t)j * P, neg + (size_t)j * P, P * 8);
+        or_ntt_forward(pctx, neg_ntt + (size_t)j * P);
+    }
+    free(cf);
+    uint32_t *all = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)n * n); /* :208-262, table (t-1)/2 */
+    uint64_t retries = 0;
+    for (uint32_t t = 1; t < 2 * n; t += 2)
+        retries += (uint64_t)or_automorph_table_bruteforce(pr, pctx, t, 0x5eed0000ull + t, all + (size_t)table_index(t) * n);
+    int bad = 0;
+    uint32_t *closed = (uint32_t *)malloc(sizeof(uint32_t) * n);
+    for (uint32_t i = 0; i < ncols; i++) {
+        const uint32_t *tab = all + (size_t)table_index(pk->gen_pows[i]) * n;
+        or_automorph_table(n, 2 * table_index(pk->gen_pows[i]) + 1, closed);
+        if (memcmp(tab, closed, sizeof(uint32_t) * n)) bad = 1;
+        memcpy(pk->tables + (size_t)i * n, tab, sizeof(uint32_t) * n);
+        for (uint32_t j = 0; j < ncols; j++) {                           /* packing_offline's index, :955-967 */
+            const uint32_t e = (uint32_t)(((uint64_t)j * pk->gen_pows[(n - i) % n]) % (2ull * n));
+            memcpy(pk->mono + ((size_t)i * ncols + j) * P, (e < n ? mono_ntt : neg_ntt) + (size_t)(e % n) * P, P * 8);
+        }
+    }
+    free(closed);
+    pk->c = ctx;  /* packing runs under the query's context (:293, :361) */
+    pack_keys(pk, w_seed);                                               /* :361 */
+    if (work) { work[0] = retries; work[1] = n; }
+    int rc = 0;
+    if (bad) {
+        rc = -4;
+    } else if (y_body == NULL) {
+        rc = -3;
+    } else {
+        uint64_t *a_hat = palloc(pr, 1);
+        uint32_t nrot = ncols - 1;
+        uint64_t *bold_t = palloc(pr, (size_t)(nrot ? nrot : 1) * ell);
+        uint64_t *ya = palloc(pr, (size_t)(nrot ? nrot : 1) * ell);
+        or_packing_offline(pk, a_ct, a_hat, bold_t);                     /* :362 */
+        or_generate_rotations(pk, y_body, ya);                           /* :365-377 */
+        or_packing_online(pk, a_hat, bold_t, ya, nrot, b_poly, ct_out);  /* :380 */
+        free(a_hat); free(bold_t); free(ya);
+    }
+    or_pack_free(pk);
+    free(mono); free(mono_ntt); free(neg); free(neg_ntt); free(all);
+    or_ntt_free(pctx);
+    or_ntt_free(ctx);
+    free(cols); free(a_ct); free(bcf); free(b_poly);
+    return rc;
+}
+
+/* n literal respond_inspiring calls, one query per thread. */
+void or_respond_inspiring_literal_many(const or_params *pr, const uint8_t w_seed[32], uint32_t n,
+                                       const uint64_t *const *shards, uint32_t ncols, const uint64_t *const *rgsw,
+                                       const uint64_t *const *y_body, uint64_t *const *ct_out, int *rc) {
+#pragma omp parallel for schedule(dynamic, 1)
+    for (uint32_t i = 0; i < n; i++)
+        rc[i] = or_respond_inspiring_literal(pr, w_seed, shards[i], ncols, rgsw[i], y_body[i], ct_out[i], NULL);
 }
 
 int or_num_threads(void) {

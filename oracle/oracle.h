@@ -116,6 +116,17 @@ int or_respond_inspiring(const or_params *pr, const or_ntt *c, const or_pack *pk
 void or_respond_inspiring_many(const or_params *pr, const or_ntt *c, const or_pack *pk, uint32_t n,
                                const uint64_t *const *shards, uint32_t ncols, const uint64_t *const *rgsw,
                                const uint64_t *const *y_body, uint64_t *const *ct_out, int *rc);
+/* respond_inspiring as the reference runs it, nothing hoisted (per query:
+ * NTT contexts, literal external products, PackParams::new with the
+ * brute-force table of every odd t < 2d, OfflinePackingKeys::generate); one
+ * thread.  Same response as or_respond_inspiring; -4 when a brute-force table
+ * differs from the closed form.  work[2] (may be NULL): retries, tables. */
+int or_respond_inspiring_literal(const or_params *pr, const uint8_t w_seed[32], const uint64_t *shard_polys,
+                                 uint32_t ncols, const uint64_t *rgsw, const uint64_t *y_body, uint64_t *ct_out,
+                                 uint64_t *work);
+void or_respond_inspiring_literal_many(const or_params *pr, const uint8_t w_seed[32], uint32_t n,
+                                       const uint64_t *const *shards, uint32_t ncols, const uint64_t *const *rgsw,
+                                       const uint64_t *const *y_body, uint64_t *const *ct_out, int *rc);
 int or_num_threads(void);
 
 /* ---- tree packing: inspiring/automorph_pack.rs ---- */

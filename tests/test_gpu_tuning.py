@@ -96,3 +96,14 @@ def test_autotune_report_persists(M, tmp_path):
         ctx2.load_tuning(path)
         t = ctx2.tuning()
         assert t["ntt_fwd_2048"] == want_ntt and t["chain_values_per_thread"] == want_v
+
+
+def test_measured_issue_rates_are_plausible(M):
+    """The register-resident probes report device-wide rates: a Montgomery
+    product is three multiply instructions, so it issues slower than one
+    64-bit multiply-add, and both lie within what 256 CUs can issue."""
+    with M.Context(M.InspireParams(), W_SEED) as ctx:
+        r = ctx.measure_rates()
+        say("rates", r)
+        assert 1.0 < r["mmul_tops"] < 100.0 and 1.0 < r["smmul_tops"] < 100.0
+        assert r["mad64_tops"] > r["mmul_tops"]

@@ -309,6 +309,30 @@ def test_inspiring_y_all_and_y_body_agree_single(single):
     assert (c1 == c2).all()
 
 
+@pytest.mark.parametrize("params", [O.single_modulus_params(256), O.Params(d=256)])
+def test_inspiring_literal_equals_hoisted(params):
+    """The literal respond_inspiring (per query: NTT contexts, mul_ntt
+    external products, PackParams::new with the brute-force table of every
+    odd t < 2d, OfflinePackingKeys::generate) gives the hoisted form's
+    ciphertext, built every table, and (single modulus) decrypts to the entry."""
+    o = O.Oracle(params)
+    w = bytes(range(32))
+    db = random_db(256, 32, seed=17, values_below_256=True)
+    shards = encode_db(o, db, 32)
+    cl = Client(o, 18)
+    yb, _ = cl.packing_keys(16, w, with_y_all=False)
+    rows = cl.query_rows(77)
+    rc, want = o.respond_inspiring(w, shards[0], rows, y_body=yb)
+    rc2, got, (retries, tables) = o.respond_inspiring_literal(w, shards[0], rows, yb)
+    assert rc == rc2 == 0 and tables == params.d and retries >= 0
+    assert np.array_equal(got, want)
+    if params.K == 1:
+        m = o.decrypt(got, cl.sk)
+        assert O.reconstruct_entry([int(m[c]) for c in range(16)], 32) == db[77 * 32:78 * 32]
+    rcs, outs = o.respond_inspiring_literal_many(w, [shards[0]] * 2, [rows, rows], [yb, yb])
+    assert rcs == [0, 0] and all(np.array_equal(x, want) for x in outs)
+
+
 def test_inspiring_crt_does_not_decrypt_but_residue_wise_does():
     """Settles the 2-modulus question: the reference's residue-1 rebuild makes the
     packed response undecryptable; permuting each residue (corrected mode) decrypts."""
