@@ -101,6 +101,15 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
     const uint32_t ycount = A.y_count ? A.y_count[qi] : gamma - 1;
     const Mod M{C.m[k].q, C.m[k].q2, C.m[k].qinv};
     const uint32_t q0 = C.m[0].q, q1 = C.m[1].q;
+    // SINV: the steps' inverse transforms are signed (intt4_s; the host's
+    // chain_signed_ok also requires 8 q < 2^31)
+    constexpr bool SINV = SGN && V == 4;
+    SInvScale sinv{};
+    if constexpr (SINV) {
+        sinv.r1 = C.mont[k];
+        sinv.cm = csub(shoup_mul(inv.c[k], C.mont[k], C.montp[k], M.q), M.q);
+        sinv.cwm = csub(shoup_mul(inv.cw[k], C.mont[k], C.montp[k], M.q), M.q);
+    }
     uint32_t* db = dbuf + k * DB * LP;
     uint32_t* rb = db;  // the inverse transform's exchange: this residue's first digit buffer
     const uint32_t* tw = twf + k * N;
@@ -213,16 +222,21 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
 #pragma unroll
             for (int e = 0; e < V; e++) Rs[e] = R[e];
         }
-        intt_v<LOGN, V>(R, ts, rb, tw, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
+        if constexpr (SINV) {
+            intt4_s<LOGN>(R, ts, rb, tw, M, sinv);
+#pragma unroll
+            for (int e = 0; e < V; e++) R[e] = sfix(R[e], M.q);
+        } else {
+            intt_v<LOGN, V>(R, ts, rb, tw, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
+#pragma unroll
+            for (int e = 0; e < V; e++) R[e] = csub(R[e], M.q);
+        }
         const int cix = lds_ix(ts);
         // the halves swap coefficients through buffers of their own, so the digit
         // transforms may write their exchange buffers while the other half still reads
         uint32_t* xb = xbuf + k * LP;
 #pragma unroll
-        for (int e = 0; e < V; e++) {
-            R[e] = csub(R[e], M.q);
-            xb[lds_step(cix, e, T)] = R[e];
-        }
+        for (int e = 0; e < V; e++) xb[lds_step(cix, e, T)] = R[e];
         lds_barrier();
         const uint32_t* ro = xbuf + (1 - k) * LP;  // the other residue's coefficients
         uint32_t dg[ELL][V];
@@ -238,7 +252,7 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
             for (int j = 0; j < ND; j++) {
                 const uint32_t sh = j * C.gadget_log;
                 const uint32_t v = sh < 64 ? (uint32_t)((x >> sh) & C.gadget_mask) : 0u;
-                dg[j][e] = C.small_digits ? v : reduce32(v, M.q, C.onep[k]);
+                dg[j][e] = (SGN || C.small_digits) ? v : reduce32(v, M.q, C.onep[k]);  // SGN implies small digits
             }
         }
         if constexpr (ND > 0) ntt_v<LOGN, V, ND, SGN>(reinterpret_cast<uint32_t(&)[ND][V]>(dg), ts, db, tw, M);
@@ -281,46 +295,61 @@ k_pack_chain_crt(PackArgs A, CtxDev C, ScaleDev inv) {
                     load_slots_v<V>(Y + (((size_t)i * ELL + j) * 2 + k) * N, ts, yv[j]);
             }
         }
+        // this residue's r_pre values: one wave-uniform choice per step, not per value
+        uint32_t ri[V];
+        if (k == 0) {
 #pragma unroll
-        for (int e = 0; e < V; e++) {
-            if constexpr (SGN) {
+            for (int e = 0; e < V; e++) ri[e] = rv[e];
+        } else if (A.residue_wise) {
+#pragma unroll
+            for (int e = 0; e < V; e++) ri[e] = rv1[e];
+        } else {
+#pragma unroll
+            for (int e = 0; e < V; e++) {
                 // r_pre's value enters the sum as r * rmul: residue 1 of the reference's automorphism
                 // is residue 0's 2^64 Montgomery value mod q1, times 2^-64 (rmul = 2^-32 mod q1)
-                uint32_t ri;
-                if (k == 0) ri = rv[e];
-                else if (A.residue_wise) ri = rv1[e];
-                else ri = csub(csub(shoup_mul(rv[e], A.q0r64, A.q0r64p, q0), q0), q1);
-                int64_t ar = (int64_t)(int32_t)ri * (int32_t)A.rmul[k];
-                int64_t ab = (int64_t)(int32_t)S[e] * (int32_t)A.smul[k];
-#pragma unroll
-                for (int j = 0; j < ELL; j++) {
-                    const int64_t x = (int32_t)dg[j][e];
-                    ar += x * (int32_t)w[j][e];
-                    if (use_y) ab += x * (int32_t)yv[j][e];
-                }
-                // reduced sums in (-2q, 2q): + 2q, one conditional subtraction -> [0, 2q)
-                R[e] = csub(smont(ar, M) + M.q2, M.q2);
-                if (use_y) S[e] = csub(smont(ab, M) + M.q2, M.q2);
-            } else {
-                uint32_t ri;
-                if (k == 0) ri = rv[e];
-                else if (A.residue_wise) ri = rv1[e];
-                else {
+                if constexpr (SGN) {
+                    ri[e] = csub(csub(shoup_mul(rv[e], A.q0r64, A.q0r64p, q0), q0), q1);
+                } else {
                     const uint32_t ref = csub(shoup_mul(rv[e], A.q0r64, A.q0r64p, q0), q0);
                     const uint32_t v1 = csub(shoup_mul(ref, 1u, A.one_q1p, q1), q1);
-                    ri = csub(shoup_mul(v1, A.q1r64i, A.q1r64ip, q1), q1);
+                    ri[e] = csub(shoup_mul(v1, A.q1r64i, A.q1r64ip, q1), q1);
                 }
-                uint64_t ar = 0, ab = 0;
-#pragma unroll
-                for (int j = 0; j < ELL; j++) {
-                    const uint32_t x = LAZY ? dg[j][e] : csub(csub(dg[j][e], M.q2), M.q);
-                    ar += (uint64_t)x * w[j][e];
-                    if (use_y) ab += (uint64_t)x * yv[j][e];
-                }
-                R[e] = csub(ri + csub(mont_reduce(ar, M.q, M.qinv), M.q), M.q);
-                if (use_y) S[e] = csub(S[e] + csub(mont_reduce(ab, M.q, M.qinv), M.q), M.q);
             }
         }
+        // the multiply-accumulates, with and without the online sum (use_y is
+        // wave-uniform; one branch per step)
+        auto mac = [&](auto with_y) {
+            constexpr bool WY = decltype(with_y)::value;
+#pragma unroll
+            for (int e = 0; e < V; e++) {
+                if constexpr (SGN) {
+                    int64_t ar = (int64_t)(int32_t)ri[e] * (int32_t)A.rmul[k];
+                    int64_t ab = (int64_t)(int32_t)S[e] * (int32_t)A.smul[k];
+#pragma unroll
+                    for (int j = 0; j < ELL; j++) {
+                        const int64_t x = (int32_t)dg[j][e];
+                        ar += x * (int32_t)w[j][e];
+                        if constexpr (WY) ab += x * (int32_t)yv[j][e];
+                    }
+                    // reduced sums in (-2q, 2q): + 2q, one conditional subtraction -> [0, 2q)
+                    R[e] = csub(smont(ar, M) + M.q2, M.q2);
+                    if constexpr (WY) S[e] = csub(smont(ab, M) + M.q2, M.q2);
+                } else {
+                    uint64_t ar = 0, ab = 0;
+#pragma unroll
+                    for (int j = 0; j < ELL; j++) {
+                        const uint32_t x = LAZY ? dg[j][e] : csub(csub(dg[j][e], M.q2), M.q);
+                        ar += (uint64_t)x * w[j][e];
+                        if constexpr (WY) ab += (uint64_t)x * yv[j][e];
+                    }
+                    R[e] = csub(ri[e] + csub(mont_reduce(ar, M.q, M.qinv), M.q), M.q);
+                    if constexpr (WY) S[e] = csub(S[e] + csub(mont_reduce(ab, M.q, M.qinv), M.q), M.q);
+                }
+            }
+        };
+        if (use_y) mac(std::true_type{});
+        else mac(std::false_type{});
     }
 
     // a_hat = INTT(R[0]); b = b_poly + INTT(S)
@@ -388,6 +417,9 @@ static hipError_t chain_crt_launch(const PackArgs& A, uint32_t Q, const CtxDev& 
 // q0 < 2 q1 (a residue mod q0 is reduced mod q1 by one subtraction).
 static bool chain_signed_ok(const CtxDev& C, int logn, int ell, bool rec) {
     if (C.m[0].q >= 2ull * C.m[1].q) return false;
+    if (!C.small_digits) return false;  // the signed kernel takes the gadget digits unreduced
+    // the signed inverse transforms hold sums below 8 q in int32
+    if (8ull * C.m[0].q >= (1ull << 31) || 8ull * C.m[1].q >= (1ull << 31)) return false;
     for (int k = 0; k < 2; k++) {
         const uint64_t q = C.m[k].q;
         if (!signed_digits_ok(q, C.small_digits ? C.gadget_mask : q - 1, logn, rec ? ell - 1 : ell, rec ? 2 * q : 0,

@@ -592,6 +592,112 @@ __device__ __forceinline__ void intt4(uint32_t (&x)[4], int t, uint32_t* rb, con
     intt4_passes<LOGN, 0>(x, t, rb, tw, M, c, cp, cw, cwp);
 }
 
+// ---------------------------------------------------------------------------
+// Signed inverse transform (four values per thread) for callers whose inputs
+// lie in [0, 2q) and who take outputs in (-q, q), for q < 2^28.  The
+// Gentleman-Sande butterflies reduce only the difference:
+//   gs_s: X, Y -> X + Y, (-w)(X - Y) = w(Y - X) in (-q, q)  (mirrored twiddle).
+// Bounds: the radix-2 stage keeps its sums in [0, 2q) by one conditional
+// subtraction; a radix-4 pass takes |x| < 2q to sums below 4q and, in its
+// second stage, x[0] (the sum of two sums) below 8q < 2^31 while x[1] stays
+// below 2q and the differences below q; x[0] is then reduced to (-q, q) by a
+// signed Montgomery product with 2^32 mod q, so every pass starts below 2q.
+// The last stage multiplies the sum and the difference (below 8q) by the
+// output scales in Montgomery form.  8q < 2^31 also keeps every product
+// below 2^31 q, which smont needs.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void gs_s(uint32_t& X, uint32_t& Y, uint32_t wm, const Mod& M) {
+    const uint32_t d = Y - X;
+    X = X + Y;
+    Y = smmul(d, wm, M);
+}
+
+// (-q, q) two's complement -> [0, q)
+__device__ __forceinline__ uint32_t sfix(uint32_t x, uint32_t q) {
+    return x + ((uint32_t)((int32_t)x >> 31) & q);
+}
+
+// Scales of the signed inverse: r1 = 2^32 mod q (reduces x[0] after each pass),
+// cm, cwm = the last stage's scales times 2^32 mod q.
+struct SInvScale {
+    uint32_t r1, cm, cwm;
+};
+
+template <int LOGN, int P>
+__device__ __forceinline__ void intt4s_passes(uint32_t (&x)[4], int t, uint32_t* rb, const uint32_t* tw, const Mod& M,
+                                              const SInvScale& sc) {
+    using L = Lay4<LOGN>;
+    constexpr int N = L::N;
+    if constexpr (P < L::R4) {
+        constexpr int KR = L::inv_kind(P - 1), KW = L::inv_kind(P);
+        const int t0 = (1 << L::REM) << (2 * P);
+        const int i2 = (int)((unsigned)t / (unsigned)t0), r = (int)((unsigned)t % (unsigned)t0);
+        if (!(L::REM == 0 && P == 0)) {
+            const int bix = xl_ix<KR>(4 * t0 * i2 + r);
+#pragma unroll
+            for (int e = 0; e < 4; e++) x[e] = rb[xl_step<KR>(bix, e, t0)];
+        }
+        const int mA = N / (2 * t0), mB = N / (4 * t0);
+#pragma unroll
+        for (int h = 0; h < 2; h++) gs_s(x[2 * h], x[2 * h + 1], inv_tw(tw, mA, 2 * i2 + h), M);
+        if (P < L::R4 - 1) {
+            const uint32_t w = inv_tw(tw, mB, i2);
+            gs_s(x[0], x[2], w, M);
+            gs_s(x[1], x[3], w, M);
+            x[0] = smmul(x[0], sc.r1, M);  // below 8q -> (-q, q)
+            const int bix = xl_ix<KW>(4 * t0 * i2 + r);
+#pragma unroll
+            for (int e = 0; e < 4; e++) rb[xl_step<KW>(bix, e, t0)] = x[e];
+            if (L::inv_wave_local(P + 1))
+                wave_lds_sync();
+            else
+                lds_barrier();
+            intt4s_passes<LOGN, P + 1>(x, t, rb, tw, M, sc);
+        } else {
+            // last stage (m = 1) with the output scales: X' = c (X + Y), Y' = cw (X - Y)
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                const uint32_t X = x[k], Y = x[k + 2];
+                x[k] = smmul(X + Y, sc.cm, M);
+                x[k + 2] = smmul(X - Y, sc.cwm, M);
+            }
+        }
+    }
+}
+
+// Signed inverse NTT of one residue: x in [0, 2q) at the slot positions of
+// thread t -> x in (-q, q) at its coefficient positions, scaled as intt4.
+template <int LOGN>
+__device__ __forceinline__ void intt4_s(uint32_t (&x)[4], int t, uint32_t* rb, const uint32_t* tw, const Mod& M,
+                                        const SInvScale& sc) {
+    using L = Lay4<LOGN>;
+    constexpr int N = L::N;
+    if constexpr (L::REM != 0) {
+        constexpr int KW = L::inv_kind(-1);
+        // radix-2 stage on [0, 2q): the sums kept below 2q, the differences signed
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            uint32_t& X = x[2 * h];
+            uint32_t& Y = x[2 * h + 1];
+            const uint32_t d = Y - X;
+            X = csub(X + Y, M.q2);
+            Y = smmul(d, inv_tw(tw, N / 2, 2 * t + h), M);
+        }
+        const int ix = xl_ix<KW>(4 * t);
+        if constexpr (KW == 2) {
+            *reinterpret_cast<uint2*>(&rb[ix]) = make_uint2(x[0], x[1]);
+            *reinterpret_cast<uint2*>(&rb[ix + 2]) = make_uint2(x[2], x[3]);
+        } else {
+            *reinterpret_cast<uint4*>(&rb[ix]) = make_uint4(x[0], x[1], x[2], x[3]);
+        }
+        if (L::inv_wave_local(0))
+            wave_lds_sync();
+        else
+            lds_barrier();
+    }
+    intt4s_passes<LOGN, 0>(x, t, rb, tw, M, sc);
+}
+
 // Either layout by its number of values per thread.
 template <int LOGN, int V>
 struct LayV {
