@@ -240,7 +240,7 @@ def main():
     ap.add_argument("--lanes", type=int, default=0,
                     help="lanes (inspire_ctx_create_lane) submitting batches from their own threads; 0 (default): "
                          "tuned, the count among --tune-lanes with the highest rate over --tune-steps untimed steps")
-    ap.add_argument("--tune-lanes", type=str, default="1,2,3", help="lane counts the lane tuning tries")
+    ap.add_argument("--tune-lanes", type=str, default="1,2,3,4", help="lane counts the lane tuning tries")
     ap.add_argument("--tune-steps", type=int, default=96, help="untimed steps per lane count when tuning lanes")
     ap.add_argument("--tuning-file", type=str, default="",
                     help="where the tuning report (kernel variants and lanes) is written; with --reuse-tuning, "

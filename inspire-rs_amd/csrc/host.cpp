@@ -253,8 +253,8 @@ int inspire_ctx_create(const inspire_params* params, const uint8_t w_seed[32], i
         ctx->logn = logn;
         std::memcpy(ctx->w_seed, w_seed, 32);
         for (uint32_t k = 0; k < ctx->K; k++) build_modulus(ctx->hm[k], params->crt_moduli[k], ctx->N, logn);
-        // device twiddle tables: per modulus tw, twp, itw, itwp
-        const size_t per = (size_t)4 * ctx->N;
+        // device twiddle tables: per modulus tw, twp, itw, itwp, twm
+        const size_t per = (size_t)5 * ctx->N;
         ctx->tables.alloc(per * ctx->K * sizeof(uint32_t));
         std::vector<uint32_t> h(per * ctx->K);
         for (uint32_t k = 0; k < ctx->K; k++) {
@@ -263,6 +263,8 @@ int inspire_ctx_create(const inspire_params* params, const uint8_t w_seed[32], i
             std::copy(ctx->hm[k].twp.begin(), ctx->hm[k].twp.end(), b + ctx->N);
             std::copy(ctx->hm[k].itw.begin(), ctx->hm[k].itw.end(), b + 2 * ctx->N);
             std::copy(ctx->hm[k].itwp.begin(), ctx->hm[k].itwp.end(), b + 3 * ctx->N);
+            const uint64_t q = ctx->hm[k].q, r32 = (1ull << 32) % q;
+            for (uint32_t i = 0; i < ctx->N; i++) b[4 * ctx->N + i] = (uint32_t)mulmod64(ctx->hm[k].tw[i], r32, q);
         }
         hip_check(hipMemcpy(ctx->tables.get(), h.data(), h.size() * 4, hipMemcpyHostToDevice), "upload tables");
         CtxDev& D = ctx->dev;
@@ -288,6 +290,7 @@ int inspire_ctx_create(const inspire_params* params, const uint8_t w_seed[32], i
             D.m[k].twp = base + ctx->N;
             D.m[k].itw = base + 2 * ctx->N;
             D.m[k].itwp = base + 3 * ctx->N;
+            D.m[k].twm = base + 4 * ctx->N;
             D.delta[k] = (uint32_t)(ctx->delta % q);
             D.deltap[k] = shoup_pre(D.delta[k], q);
             D.onep[k] = shoup_pre(1, q);

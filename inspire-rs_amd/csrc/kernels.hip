@@ -13,6 +13,7 @@
 #include "kernels.hpp"
 #include "ntt_mfma.hpp"
 #include "ntt.hpp"
+#include "reg_ntt.hpp"
 
 namespace inspire {
 
@@ -350,15 +351,23 @@ k_expand_seeds(const uint32_t* __restrict__ seeds, uint32_t* __restrict__ out, s
 // ---------------------------------------------------------------------------
 // Seeded RGSW rows in one pass (d = 2048, 256 threads = the expansion's draw
 // lanes = the NTT's N/8): row i's a is expanded from its seed into LDS as
-// above, b is read from bsrc + i * KN, both are transformed and scaled to
-// Montgomery form, and written to rows + i * 2KN (a) and + KN (b).
+// above, b is read from bsrc + i * KN straight into registers, and per
+// residue a and b are transformed together by the register-resident radix-8
+// NTT (eight values per thread, Montgomery-form twiddles from the global
+// table), scaled to Montgomery form, and written to rows + i * 2KN (a) and
+// + KN (b), each thread's eight consecutive slots as two 16-byte stores.
+// LDS: three polynomial buffers (residue 0's a in 0, residue 1's in 2; the
+// transform of residue k exchanges through buffers k, k + 1 once its a is in
+// registers), 27 KB, which leaves a CU running the packing chain's
+// workgroup room for one of these.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
 k_rows_seeded(const uint32_t* __restrict__ seeds, const uint32_t* __restrict__ bsrc, uint32_t* __restrict__ rows,
               uint32_t* __restrict__ err, CtxDev C, ScaleDev sc) {
     using S = NttShape<11>;
     constexpr int N = S::N, LP = S::LDS;
-    __shared__ __align__(16) uint32_t poly[2 * kMaxK][LP];  // a residues, then b residues
+    static_assert(kMaxK == 2, "a buffers at 0 and 2");
+    __shared__ __align__(16) uint32_t poly[3][LP];  // a of residue 0, exchange, a of residue 1
     __shared__ uint32_t sh[4];
     __shared__ uint32_t s_last;
     const int tid = threadIdx.x;
@@ -366,10 +375,6 @@ k_rows_seeded(const uint32_t* __restrict__ seeds, const uint32_t* __restrict__ b
     uint32_t key[8];
 #pragma unroll
     for (int i = 0; i < 8; i++) key[i] = seeds[(size_t)blockIdx.x * 8 + i];
-    const uint32_t* b = bsrc + (size_t)blockIdx.x * KN;
-    for (uint32_t k = 0; k < C.K; k++)
-#pragma unroll
-        for (int j = 0; j < 8; j++) poly[kMaxK + k][lds_ix(tid + j * S::T)] = b[(size_t)k * N + tid + j * S::T];
     // Draws are generated in chunks of up to 256 ChaCha blocks (8 draws per
     // thread).  For the last residue the chunk is sized to the draws still
     // needed plus a margin (a short chunk just continues in the next one), so
@@ -406,7 +411,7 @@ k_rows_seeded(const uint32_t* __restrict__ seeds, const uint32_t* __restrict__ b
 #pragma unroll
             for (int j = 0; j < 8; j++) {
                 if (flags >> j & 1) {
-                    if (rank < need) poly[k][lds_ix(filled + rank)] = (uint32_t)__umul64hi(v[j], q);
+                    if (rank < need) poly[2 * k][lds_ix(filled + rank)] = (uint32_t)__umul64hi(v[j], q);
                     if (rank == need - 1) s_last = 8 * tid + j;
                     rank++;
                 }
@@ -429,17 +434,29 @@ k_rows_seeded(const uint32_t* __restrict__ seeds, const uint32_t* __restrict__ b
         if (tid == 0) atomicOr(err, 1u);
         return;
     }
-    __syncthreads();
+    __syncthreads();  // the a residues and the twiddles
+    const uint32_t* b = bsrc + (size_t)blockIdx.x * KN;
     uint32_t* out = rows + (size_t)blockIdx.x * 2 * KN;
     for (uint32_t kk = 0; kk < C.K; kk++) {
-        const ModDev& M = C.m[kk];
-        for (int ab = 0; ab < 2; ab++) {
-            uint32_t* s = poly[ab * kMaxK + kk];
-            ntt_fwd_lds<11>(s, tid, M, BlockSync{});
-            uint32_t* o = out + (size_t)ab * KN + (size_t)kk * N;
+        const Mod M{C.m[kk].q, C.m[kk].q2, C.m[kk].qinv};
+        uint32_t x[2][8];  // a, b of residue kk at the coefficient positions tid + e N/8
 #pragma unroll
-            for (int j = 0; j < 8; j++)
-                o[tid + j * S::T] = csub(shoup_mul(s[lds_ix(tid + j * S::T)], sc.c[kk], sc.cp[kk], M.q), M.q);
+        for (int e = 0; e < 8; e++) {
+            x[0][e] = poly[2 * kk][lds_ix(tid + e * S::T)];
+            x[1][e] = b[(size_t)kk * N + tid + e * S::T];
+        }
+        // every thread holds its a values and is past the previous transform's
+        // exchange reads before buffers kk, kk + 1 are written
+        __syncthreads();
+        ntt_digits<11, 2, false>(x, tid, poly[kk], C.m[kk].twm, M);
+#pragma unroll
+        for (int ab = 0; ab < 2; ab++) {
+            uint32_t y[8];
+#pragma unroll
+            for (int e = 0; e < 8; e++) y[e] = csub(shoup_mul(x[ab][e], sc.c[kk], sc.cp[kk], M.q), M.q);
+            uint4* o = reinterpret_cast<uint4*>(out + (size_t)ab * KN + (size_t)kk * N + 8 * tid);
+            o[0] = make_uint4(y[0], y[1], y[2], y[3]);
+            o[1] = make_uint4(y[4], y[5], y[6], y[7]);
         }
     }
 }

@@ -29,6 +29,7 @@ struct ModDev {
     uint32_t pad_;
     const uint32_t *tw, *twp;    // [N] psi^br(m), Shoup companions
     const uint32_t *itw, *itwp;  // [N] psi^-br(m), Shoup companions
+    const uint32_t* twm;         // [N] psi^br(m) 2^32 mod q (Montgomery form, the register transforms')
 };
 
 template <int LOGN>
