@@ -12,6 +12,25 @@ namespace inspire {
 constexpr int kMaxK = 2;      // CRT moduli
 constexpr int kMaxEll = 4;    // gadget digits
 
+// Raw buffer view of a wave-uniform base address (gfx9 descriptor: stride 0,
+// 32-bit records, 2^31 - 1 bytes).  A load or store takes the lane's byte
+// offset in one VGPR and the uniform word offset in an SGPR, so a kernel that
+// reads many arrays at uniform strides spends no vector instructions on
+// addresses; through a pointer read from memory (a query's shard address)
+// the compiler would instead form a 64-bit address per lane and per load.
+struct BufU32 {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ __forceinline__ explicit BufU32(const void* base)
+        : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, 0x7fffffff, 0x00020000)) {}
+    // word `off` (uniform) past the base, plus the lane's byte offset vb
+    __device__ __forceinline__ uint32_t ld(uint32_t vb, uint32_t off) const {
+        return __builtin_amdgcn_raw_buffer_load_b32(r, vb, off * 4u, 0);
+    }
+    __device__ __forceinline__ void st(uint32_t vb, uint32_t off, uint32_t v) const {
+        __builtin_amdgcn_raw_buffer_store_b32(v, r, vb, off * 4u, 0);
+    }
+};
+
 // Constant tables of the matrix-core forward NTT at d = 2048 (ntt_mfma.hip).
 struct MfmaNttTables {
     const uint32_t* tw1;  // [K][2048] psi^j 2^32 mod q (the negacyclic twist, Montgomery form)

@@ -91,24 +91,37 @@ __device__ __forceinline__ uint32_t wave_sum_scatter(uint32_t (&x)[G], uint32_t 
     for (; off >= 1; off >>= 1) x[0] = front_addmod(x[0], __shfl_xor(x[0], off, 64), q);
     return x[0];
 }
+// Ring degree and residue count of a front-end instantiation: compile-time
+// for the common shape (so every load offset is a constant added to a
+// wave-uniform base), read from the context otherwise (LOGN = 0, KC = 0).
+template <int LOGN, int KC>
+struct FrontDims {
+    const int n, K;
+    __device__ __forceinline__ explicit FrontDims(const CtxDev& C)
+        : n(LOGN ? 1 << LOGN : (int)C.n), K(KC ? KC : (int)C.K) {}
+};
+
 // One residue of the fused front end for the thread's slot s: the column
 // values (a only when WANT_A), the twisted DFT of the a values into r_pre,
-// and this wave's b sums.
-template <int LG, bool WANT_A, int ELL>
-__device__ __forceinline__ void front_residue(int k, const uint32_t* R, const uint32_t* Dq, uint32_t* r_pre,
-                                              uint64_t* bpart, const uint32_t* twist, const uint2* dftw,
-                                              const CtxDev& C, uint32_t qi, uint32_t blk, int s, uint32_t cls) {
+// and this wave's b sums.  R, Dq and the tables are wave-uniform bases in
+// global memory (buffer views); s is the lane's slot, sb = 4 s its byte offset.
+template <int LG, bool WANT_A, int ELL, int LOGN, int KC>
+__device__ __forceinline__ void front_residue(int k, const BufU32& R, const BufU32& Dq, const BufU32& rp,
+                                              uint64_t* bpart, const BufU32& twist, const uint2* dftw,
+                                              const CtxDev& C, uint32_t qi, uint32_t blk, uint32_t sb,
+                                              uint32_t cls) {
     constexpr int G = 1 << LG;
     constexpr int CH = G < 8 ? G : 8;  // columns whose digits are loaded together
-    const int n = (int)C.n, nb = n / 256;
+    const FrontDims<LOGN, KC> F(C);
+    const int n = F.n, nb = n / 256;
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const size_t KN = (size_t)C.K * n;
+    const uint32_t KN = (uint32_t)(F.K * n);
     const uint32_t q = C.m[k].q, q2 = 2 * q, qinv = C.m[k].qinv;
     uint32_t ca[ELL], cb[ELL];
 #pragma unroll
     for (int i = 0; i < ELL; i++) {
-        ca[i] = WANT_A ? R[((size_t)(2 * i) * C.K + k) * n] : 0u;
-        cb[i] = R[((size_t)(2 * i + 1) * C.K + k) * n];
+        ca[i] = WANT_A ? R.ld(sb, (uint32_t)((2 * i) * F.K + k) * n) : 0u;
+        cb[i] = R.ld(sb, (uint32_t)((2 * i + 1) * F.K + k) * n);
     }
     uint32_t v[G], bv[G];
 #pragma unroll
@@ -118,7 +131,7 @@ __device__ __forceinline__ void front_residue(int k, const uint32_t* R, const ui
 #pragma unroll
         for (int jj = 0; jj < CH; jj++)
 #pragma unroll
-            for (int i = 0; i < ELL; i++) dv[jj][i] = Dq[((((size_t)(j0 + jj)) * ELL + i) * C.K + k) * n];
+            for (int i = 0; i < ELL; i++) dv[jj][i] = Dq.ld(sb, (uint32_t)(((j0 + jj) * ELL + i) * F.K + k) * n);
 #pragma unroll
         for (int jj = 0; jj < CH; jj++) {
             const int j = j0 + jj;
@@ -132,7 +145,7 @@ __device__ __forceinline__ void front_residue(int k, const uint32_t* R, const ui
             if constexpr (WANT_A) {
                 const uint32_t a = csub(mont_reduce(aa, q, qinv), q);
                 // twist by gamma^-1 w_s^j (Montgomery form) and place at the bit-reversed index
-                const uint32_t tw = twist[((size_t)k * G + j) * n + s];
+                const uint32_t tw = twist.ld(sb, (uint32_t)(k * G + j) * n);
                 int p = 0;
 #pragma unroll
                 for (int b = 0; b < LG; b++) p |= ((j >> b) & 1) << (LG - 1 - b);
@@ -162,16 +175,15 @@ __device__ __forceinline__ void front_residue(int k, const uint32_t* R, const ui
                     }
                 }
         }
-        uint32_t* out = r_pre + (size_t)qi * G * KN + (size_t)k * n + s;
 #pragma unroll
-        for (int i = 0; i < G; i++) out[(size_t)i * KN] = csub(csub(v[i], q2), q);
+        for (int i = 0; i < G; i++) rp.st(sb, (uint32_t)i * KN + (uint32_t)k * n, csub(csub(v[i], q2), q));
     }
     // this wave's sums of bhat_j over its 64 slots, part blk * 4 + wave
     const uint32_t sum = wave_sum_scatter<G>(bv, lane, q);
     constexpr uint32_t LPV = 64 / G;  // lanes holding each value
     if (lane % LPV == 0) {
         const uint32_t j = lane / LPV, part = blk * 4 + wave;
-        bpart[(((size_t)qi * nb * 4 + part) * G + j) * C.K + k] = sum;
+        bpart[(((size_t)qi * nb * 4 + part) * G + j) * F.K + k] = sum;
     }
 }
 
@@ -181,28 +193,31 @@ __device__ __forceinline__ void front_residue(int k, const uint32_t* R, const ui
 // before it gives the words the reference's sum after it gives) into the
 // query's block O [1 + gamma][2][K][N] (combined, then the columns), still
 // in the NTT domain; the chain kernel's role-1 workgroups transform them.
-template <int LG, int ELL>
-__device__ __forceinline__ void front_plain(const uint32_t* R, const uint32_t* Dq, uint32_t* O, const CtxDev& C) {
+template <int LG, int ELL, int LOGN, int KC>
+__device__ __forceinline__ void front_plain(const BufU32& R, const BufU32& Dq, uint32_t* O, const CtxDev& C,
+                                            uint32_t sb) {
     constexpr int G = 1 << LG;
     constexpr int CH = G < 8 ? G : 8;
-    const int n = (int)C.n;
-    const size_t KN = (size_t)C.K * n;
-    for (int k = 0; k < (int)C.K; k++) {
+    const FrontDims<LOGN, KC> F(C);
+    const int n = F.n;
+    const uint32_t KN = (uint32_t)(F.K * n);
+    const BufU32 Og(O);
+    for (int k = 0; k < F.K; k++) {
         const uint32_t q = C.m[k].q, qinv = C.m[k].qinv;
         uint32_t ca[ELL], cb[ELL];
 #pragma unroll
         for (int i = 0; i < ELL; i++) {
-            ca[i] = R[((size_t)(2 * i) * C.K + k) * n];
-            cb[i] = R[((size_t)(2 * i + 1) * C.K + k) * n];
+            ca[i] = R.ld(sb, (uint32_t)((2 * i) * F.K + k) * n);
+            cb[i] = R.ld(sb, (uint32_t)((2 * i + 1) * F.K + k) * n);
         }
-        uint32_t sa = 0, sb = 0;
+        uint32_t suma = 0, sumb = 0;
 #pragma unroll
         for (int j0 = 0; j0 < G; j0 += CH) {
             uint32_t dv[CH][ELL];
 #pragma unroll
             for (int jj = 0; jj < CH; jj++)
 #pragma unroll
-                for (int i = 0; i < ELL; i++) dv[jj][i] = Dq[((((size_t)(j0 + jj)) * ELL + i) * C.K + k) * n];
+                for (int i = 0; i < ELL; i++) dv[jj][i] = Dq.ld(sb, (uint32_t)(((j0 + jj) * ELL + i) * F.K + k) * n);
 #pragma unroll
             for (int jj = 0; jj < CH; jj++) {
                 uint64_t aa = 0, bb = 0;
@@ -212,43 +227,46 @@ __device__ __forceinline__ void front_plain(const uint32_t* R, const uint32_t* D
                     bb += (uint64_t)dv[jj][i] * cb[i];
                 }
                 const uint32_t a = csub(mont_reduce(aa, q, qinv), q), b = csub(mont_reduce(bb, q, qinv), q);
-                uint32_t* oc = O + (size_t)(2 + 2 * (j0 + jj)) * KN + (size_t)k * n;
-                oc[0] = a;
-                oc[KN] = b;
-                sa = csub(sa + a, q);
-                sb = csub(sb + b, q);
+                const uint32_t oc = (uint32_t)(2 + 2 * (j0 + jj)) * KN + (uint32_t)k * n;
+                Og.st(sb, oc, a);
+                Og.st(sb, oc + KN, b);
+                suma = csub(suma + a, q);
+                sumb = csub(sumb + b, q);
             }
         }
-        O[(size_t)k * n] = sa;
-        O[KN + (size_t)k * n] = sb;
+        Og.st(sb, (uint32_t)k * n, suma);
+        Og.st(sb, KN + (uint32_t)k * n, sumb);
     }
 }
 
 // At most 64 VGPRs (eight waves per SIMD): a 256-query launch is 8192 waves,
 // which then all fit on the 256 CUs at once instead of leaving a second,
 // nearly empty round of workgroups.
-template <int LG, bool RW, int ELL>
+template <int LG, bool RW, int ELL, int LOGN, int KC>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
 k_pack_front(const uint64_t* __restrict__ qdig, const uint32_t* __restrict__ rows, uint32_t* __restrict__ r_pre,
              uint64_t* __restrict__ bpart, const uint32_t* __restrict__ twist, const uint2* __restrict__ dftw,
              PackRoles roles, uint32_t* __restrict__ out, CtxDev C) {
     constexpr int G = 1 << LG;
-    const int n = (int)C.n, nb = n / 256, tid = threadIdx.x;
+    const FrontDims<LOGN, KC> F(C);
+    const int n = F.n, nb = n / 256, tid = threadIdx.x;
     const uint32_t qi = blockIdx.x / nb, blk = blockIdx.x % nb;
-    const int s = blk * 256 + tid;
-    const size_t KN = (size_t)C.K * n;
-    const uint32_t* R = rows + (size_t)qi * ELL * 2 * KN + s;
+    const uint32_t s = blk * 256 + tid;
+    const size_t KN = (size_t)F.K * n;
+    const uint32_t sb = 4 * s;
+    const BufU32 R(rows + (size_t)qi * ELL * 2 * KN);
     // the host points a missing shard at a valid one and reports it
-    const uint32_t* Dq = reinterpret_cast<const uint32_t*>(qdig[qi]) + s;
+    const BufU32 Dq(reinterpret_cast<const uint32_t*>(qdig[qi]));
     if (roles.role && roles.role[qi]) {
-        front_plain<LG, ELL>(R, Dq, out + roles.qout[qi] + s, C);
+        front_plain<LG, ELL, LOGN, KC>(R, Dq, out + roles.qout[qi], C, sb);
         return;
     }
-    const uint32_t cls = ((2u * (__brev((uint32_t)s) >> (32 - C.logn)) + 1u) & (G - 1)) >> 1;
-    front_residue<LG, true, ELL>(0, R, Dq, r_pre, bpart, twist, dftw, C, qi, blk, s, cls);
+    const uint32_t cls = ((2u * (__brev(s) >> (32 - C.logn)) + 1u) & (G - 1)) >> 1;
+    const BufU32 tw(twist), rp(r_pre + (size_t)qi * G * KN);
+    front_residue<LG, true, ELL, LOGN, KC>(0, R, Dq, rp, bpart, tw, dftw, C, qi, blk, sb, cls);
     // in the reference's residue handling only residue 0 of r_pre is read later
-    for (int k = 1; k < (int)C.K; k++)
-        front_residue<LG, RW, ELL>(k, R, Dq, r_pre, bpart, twist, dftw, C, qi, blk, s, cls);
+    for (int k = 1; k < F.K; k++)
+        front_residue<LG, RW, ELL, LOGN, KC>(k, R, Dq, rp, bpart, tw, dftw, C, qi, blk, sb, cls);
 }
 
 // Step 1 before the automorphism:
@@ -453,18 +471,29 @@ hipError_t launch_pack_b0(const uint32_t* cols, uint64_t* bpart, uint32_t Q, uin
     return hipGetLastError();
 }
 
+template <int LG, int ELL, int LOGN, int KC>
+static hipError_t front_launch(const uint64_t* qdig, const uint32_t* rows, uint32_t* r_pre, uint64_t* bpart,
+                               const uint32_t* twist, const uint2* dftw, const PackRoles& roles, uint32_t* out,
+                               uint32_t Q, bool rw, const CtxDev& C, hipStream_t st, LaunchEv ev) {
+    const dim3 grid(Q * (C.n / 256));
+    if (rw)
+        hipExtLaunchKernelGGL((k_pack_front<LG, true, ELL, LOGN, KC>), grid, dim3(256), 0, st, ev.start, ev.stop, 0,
+                              qdig, rows, r_pre, bpart, twist, dftw, roles, out, C);
+    else
+        hipExtLaunchKernelGGL((k_pack_front<LG, false, ELL, LOGN, KC>), grid, dim3(256), 0, st, ev.start, ev.stop, 0,
+                              qdig, rows, r_pre, bpart, twist, dftw, roles, out, C);
+    return hipGetLastError();
+}
+
 template <int LG, int ELL>
 static hipError_t front_t(const uint64_t* qdig, const uint32_t* rows, uint32_t* r_pre, uint64_t* bpart,
                           const uint32_t* twist, const uint2* dftw, const PackRoles& roles, uint32_t* out, uint32_t Q,
                           bool rw, const CtxDev& C, hipStream_t st, LaunchEv ev) {
-    const dim3 grid(Q * (C.n / 256));
-    if (rw)
-        hipExtLaunchKernelGGL((k_pack_front<LG, true, ELL>), grid, dim3(256), 0, st, ev.start, ev.stop, 0, qdig, rows,
-                              r_pre, bpart, twist, dftw, roles, out, C);
-    else
-        hipExtLaunchKernelGGL((k_pack_front<LG, false, ELL>), grid, dim3(256), 0, st, ev.start, ev.stop, 0, qdig,
-                              rows, r_pre, bpart, twist, dftw, roles, out, C);
-    return hipGetLastError();
+    // d = 2048 with two moduli (the parameter set of every InsPIRe^2 workload here): constant strides
+    if constexpr (LG == 4 && ELL == 3)
+        if (C.logn == 11 && C.K == 2)
+            return front_launch<LG, ELL, 11, 2>(qdig, rows, r_pre, bpart, twist, dftw, roles, out, Q, rw, C, st, ev);
+    return front_launch<LG, ELL, 0, 0>(qdig, rows, r_pre, bpart, twist, dftw, roles, out, Q, rw, C, st, ev);
 }
 
 template <int LG>
