@@ -239,9 +239,12 @@ def main():
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--lanes", type=int, default=0,
                     help="lanes (inspire_ctx_create_lane) submitting batches from their own threads; 0 (default): "
-                         "tuned, the count among --tune-lanes with the highest rate over --tune-steps untimed steps")
+                         "tuned, the count among --tune-lanes with the highest median rate over --tune-rounds "
+                         "interleaved runs of --tune-steps untimed steps")
     ap.add_argument("--tune-lanes", type=str, default="1,2,3,4", help="lane counts the lane tuning tries")
-    ap.add_argument("--tune-steps", type=int, default=96, help="untimed steps per lane count when tuning lanes")
+    ap.add_argument("--tune-steps", type=int, default=96, help="untimed steps per lane count and round when tuning lanes")
+    ap.add_argument("--tune-rounds", type=int, default=5,
+                    help="rounds of the lane tuning (every candidate once per round, the median rate kept)")
     ap.add_argument("--tuning-file", type=str, default="",
                     help="where the tuning report (kernel variants and lanes) is written; with --reuse-tuning, "
                          "read from instead of measuring")
@@ -392,9 +395,15 @@ def main():
     elif prior and "lanes" in prior:
         n_lanes = int(prior["lanes"])
     else:
+        # every candidate once per round, rounds interleaved, so a drift of the clocks or a
+        # slow run moves all candidates alike; the median of each candidate's rates is kept
+        runs = {nl: [] for nl in cand}
+        for _ in range(max(1, args.tune_rounds)):
+            for nl in cand:
+                el, _ = run_lanes(lanes[:nl], louts[:nl], args.tune_steps)
+                runs[nl].append(B * args.tune_steps / el)
         for nl in cand:
-            el, _ = run_lanes(lanes[:nl], louts[:nl], args.tune_steps)
-            lane_rates[nl] = round(B * args.tune_steps / el, 1)
+            lane_rates[nl] = round(float(np.median(runs[nl])), 1)
         # the fewest lanes within 3 % of the best rate: more lanes only add latency beyond that
         best = max(lane_rates.values())
         n_lanes = min(nl for nl, r in lane_rates.items() if r >= 0.97 * best)
