@@ -360,10 +360,16 @@ k_expand_seeds(const uint32_t* __restrict__ seeds, uint32_t* __restrict__ out, s
 // transform of residue k exchanges through buffers k, k + 1 once its a is in
 // registers), 27 KB, which leaves a CU running the packing chain's
 // workgroup room for one of these.
+// Residues of a: a row whose query reads only residue 0 of a (InspiRING
+// packing through the fused front end in the reference's residue handling)
+// expands and transforms that residue alone: its draws come first in the
+// stream, so the residue-0 words are the same, and residue 1's a is left
+// unwritten.  Row r expands KA = (a_full && a_full[r / ell]) ? K : a_res.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
 k_rows_seeded(const uint32_t* __restrict__ seeds, const uint32_t* __restrict__ bsrc, uint32_t* __restrict__ rows,
-              uint32_t* __restrict__ err, CtxDev C, ScaleDev sc) {
+              uint32_t* __restrict__ err, CtxDev C, ScaleDev sc, const uint32_t* __restrict__ a_full,
+              uint32_t a_res, uint32_t ell) {
     using S = NttShape<11>;
     constexpr int N = S::N, LP = S::LDS;
     static_assert(kMaxK == 2, "a buffers at 0 and 2");
@@ -379,11 +385,12 @@ k_rows_seeded(const uint32_t* __restrict__ seeds, const uint32_t* __restrict__ b
     // thread).  For the last residue the chunk is sized to the draws still
     // needed plus a margin (a short chunk just continues in the next one), so
     // the rejected draws of the last residue do not cost a whole chunk.
+    const uint32_t KA = (a_full && a_full[blockIdx.x / ell]) ? C.K : a_res;  // residues of a expanded
     uint32_t k = 0, filled = 0;
     uint64_t pos = 0, c0 = 0;  // first unconsumed draw; first draw of the chunk
-    for (int chunk = 0; chunk < kExpandMaxChunks && k < C.K; chunk++) {
+    for (int chunk = 0; chunk < kExpandMaxChunks && k < KA; chunk++) {
         uint32_t active = 256;
-        if (k == C.K - 1) {
+        if (k == KA - 1) {
             const uint32_t need_now = N - filled;
             active = min(256u, (need_now + need_now / 8 + 64 + 7) / 8);
         }
@@ -393,7 +400,7 @@ k_rows_seeded(const uint32_t* __restrict__ seeds, const uint32_t* __restrict__ b
         uint64_t v[8];
 #pragma unroll
         for (int j = 0; j < 8; j++) v[j] = on ? ((uint64_t)w[2 * j] | ((uint64_t)w[2 * j + 1] << 32)) : 0;
-        while (k < C.K) {
+        while (k < KA) {
             const uint64_t q = C.m[k].q;
             const uint64_t zone = (q << __clzll(q)) - 1;
             uint32_t flags = 0, cnt = 0;
@@ -430,33 +437,44 @@ k_rows_seeded(const uint32_t* __restrict__ seeds, const uint32_t* __restrict__ b
         }
         c0 += 8 * (uint64_t)active;
     }
-    if (k < C.K) {  // draw budget exhausted (never in practice): report, write nothing
+    if (k < KA) {  // draw budget exhausted (never in practice): report, write nothing
         if (tid == 0) atomicOr(err, 1u);
         return;
     }
     __syncthreads();  // the a residues and the twiddles
     const uint32_t* b = bsrc + (size_t)blockIdx.x * KN;
     uint32_t* out = rows + (size_t)blockIdx.x * 2 * KN;
+    // this thread's eight consecutive slots of poly ab (0: a, 1: b) of residue kk, Montgomery form
+    auto store = [&](const uint32_t (&x)[8], int ab, uint32_t kk, const Mod& M) {
+        uint32_t y[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) y[e] = csub(shoup_mul(x[e], sc.c[kk], sc.cp[kk], M.q), M.q);
+        uint4* o = reinterpret_cast<uint4*>(out + (size_t)ab * KN + (size_t)kk * N + 8 * tid);
+        o[0] = make_uint4(y[0], y[1], y[2], y[3]);
+        o[1] = make_uint4(y[4], y[5], y[6], y[7]);
+    };
     for (uint32_t kk = 0; kk < C.K; kk++) {
         const Mod M{C.m[kk].q, C.m[kk].q2, C.m[kk].qinv};
-        uint32_t x[2][8];  // a, b of residue kk at the coefficient positions tid + e N/8
+        if (kk < KA) {
+            uint32_t x[2][8];  // a, b of residue kk at the coefficient positions tid + e N/8
 #pragma unroll
-        for (int e = 0; e < 8; e++) {
-            x[0][e] = poly[2 * kk][lds_ix(tid + e * S::T)];
-            x[1][e] = b[(size_t)kk * N + tid + e * S::T];
-        }
-        // every thread holds its a values and is past the previous transform's
-        // exchange reads before buffers kk, kk + 1 are written
-        __syncthreads();
-        ntt_digits<11, 2, false>(x, tid, poly[kk], C.m[kk].twm, M);
+            for (int e = 0; e < 8; e++) {
+                x[0][e] = poly[2 * kk][lds_ix(tid + e * S::T)];
+                x[1][e] = b[(size_t)kk * N + tid + e * S::T];
+            }
+            // every thread holds its a values and is past the previous transform's
+            // exchange reads before buffers kk, kk + 1 are written
+            __syncthreads();
+            ntt_digits<11, 2, false>(x, tid, poly[kk], C.m[kk].twm, M);
+            store(x[0], 0, kk, M);
+            store(x[1], 1, kk, M);
+        } else {  // b alone (a of this residue is not read)
+            uint32_t x[1][8];
 #pragma unroll
-        for (int ab = 0; ab < 2; ab++) {
-            uint32_t y[8];
-#pragma unroll
-            for (int e = 0; e < 8; e++) y[e] = csub(shoup_mul(x[ab][e], sc.c[kk], sc.cp[kk], M.q), M.q);
-            uint4* o = reinterpret_cast<uint4*>(out + (size_t)ab * KN + (size_t)kk * N + 8 * tid);
-            o[0] = make_uint4(y[0], y[1], y[2], y[3]);
-            o[1] = make_uint4(y[4], y[5], y[6], y[7]);
+            for (int e = 0; e < 8; e++) x[0][e] = b[(size_t)kk * N + tid + e * S::T];
+            __syncthreads();  // past the previous transform's exchange reads of buffer kk
+            ntt_digits<11, 1, false>(x, tid, poly[kk], C.m[kk].twm, M);
+            store(x[0], 1, kk, M);
         }
     }
 }
@@ -565,11 +583,13 @@ hipError_t launch_sum_cols(uint32_t* resp, uint32_t Q, uint32_t cols, const CtxD
 }
 
 hipError_t launch_rows_seeded(const uint32_t* seeds, const uint32_t* bsrc, uint32_t* rows, size_t count,
-                              uint32_t* err, const CtxDev& C, const ScaleDev& sc, hipStream_t st, LaunchEv ev) {
+                              uint32_t* err, const CtxDev& C, const ScaleDev& sc, hipStream_t st, LaunchEv ev,
+                              uint32_t a_res, const uint32_t* a_full) {
     if (!count) return hipSuccess;
     if (C.logn != 11 || count > 0x7fffffffu) return hipErrorInvalidValue;
+    if (a_res < 1 || a_res > C.K) a_res = C.K;
     hipExtLaunchKernelGGL(k_rows_seeded, dim3((unsigned)count), dim3(256), 0, st, ev.start, ev.stop, 0, seeds, bsrc,
-                          rows, err, C, sc);
+                          rows, err, C, sc, a_full, a_res, C.ell);
     return hipGetLastError();
 }
 

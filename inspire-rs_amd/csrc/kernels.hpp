@@ -96,10 +96,12 @@ hipError_t launch_expand_seeds(const uint32_t* seeds, uint32_t* out, size_t coun
 hipError_t launch_sum_cols(uint32_t* resp, uint32_t Q, uint32_t cols, const CtxDev& C, hipStream_t st);
 // d = 2048: seeded rows expanded and transformed in one pass.  Row i: seed
 // seeds + 8i, b at bsrc + i*K*N; out rows + i*2*K*N (a, then b), Montgomery
-// form via sc.  Returns hipErrorInvalidValue for other degrees.
+// form via sc.  Residues of a: a_res (1..K) for every row, or K for the rows
+// of the queries q (row i of query i / ell) with a_full[q] != 0; the a of the
+// other residues is not written.  Returns hipErrorInvalidValue for other degrees.
 hipError_t launch_rows_seeded(const uint32_t* seeds, const uint32_t* bsrc, uint32_t* rows, size_t count,
                               uint32_t* err, const CtxDev& C, const ScaleDev& sc, hipStream_t st,
-                              LaunchEv ev = LaunchEv{});
+                              LaunchEv ev = LaunchEv{}, uint32_t a_res = kMaxK, const uint32_t* a_full = nullptr);
 
 // Issue-rate probes (rates.hip): kind 0 the Montgomery product, 1 the signed
 // Montgomery product, 2 the 64-bit multiply-add; out holds blocks * 256 words.

@@ -294,6 +294,12 @@ bool pack_mixed_supported(inspire_ctx* ctx, uint32_t gamma) {
     return pack_chain_crt_supported(ctx->dev) && pack_tables(ctx, gamma).fast_front;
 }
 
+uint32_t pack_rows_a_residues(inspire_ctx* ctx, uint32_t gamma) {
+    const bool rw = (ctx->params.flags & INSPIRE_FLAG_RESIDUE_WISE_AUTOMORPHISM) != 0;
+    if (ctx->K != 2 || rw) return ctx->K;
+    return pack_tables(ctx, gamma).fast_front ? 1u : ctx->K;
+}
+
 void pack_run(inspire_ctx* ctx, uint32_t gamma, uint32_t Q, const uint32_t* rows, const uint64_t* d_qdig,
               const uint32_t* d_y, const uint64_t* d_yoff, const uint32_t* d_ymode, const uint32_t* d_ycount,
               uint32_t* d_out, bool y_coeff, const PackWs& ws, const PackRoles& roles) {

@@ -102,6 +102,10 @@ void pack_run(inspire_ctx* ctx, uint32_t gamma, uint32_t Q, const uint32_t* rows
 // (PackRoles) for databases of gamma columns: the fused front end applies
 // (gamma = 2^k, gamma^2 | 2d) and the two-modulus chain kernel runs.
 bool pack_mixed_supported(inspire_ctx* ctx, uint32_t gamma);
+// Residues of the RGSW rows' a that packing gamma columns reads: 1 when the
+// fused front end runs in the reference's residue handling with two moduli
+// (only residue 0 of step 1 is kept, and only its a enters), K otherwise.
+uint32_t pack_rows_a_residues(inspire_ctx* ctx, uint32_t gamma);
 // True when the packing chain transforms coefficient-domain y_body itself;
 // otherwise pack_run_device_ybody needs y_ws for the transformed keys.
 bool pack_ybody_in_chain(const inspire_ctx* ctx);

@@ -415,7 +415,8 @@ void external_product_batch(inspire_ctx* ctx, uint64_t count, const uint64_t* rl
 // Every row is transformed (NTT, Montgomery form) from its source into
 // rows_ws [Q][ell][2][K][N]; no copy of the inputs is made.
 void stage_rows_from(inspire_ctx* ctx, uint32_t Q, uint32_t S, uint32_t* rows_ws, const uint32_t* seeds,
-                     const uint32_t* bsrc, const uint32_t* fsrc, uint32_t* d_err) {
+                     const uint32_t* bsrc, const uint32_t* fsrc, uint32_t* d_err, uint32_t a_res,
+                     const uint32_t* a_full) {
     const size_t KN = (size_t)ctx->K * ctx->N, ell = ctx->ell;
     uint64_t mont[kMaxK];
     for (uint32_t k = 0; k < ctx->K; k++) mont[k] = mont_of(ctx, k);
@@ -423,7 +424,7 @@ void stage_rows_from(inspire_ctx* ctx, uint32_t Q, uint32_t S, uint32_t* rows_ws
     if (S && ctx->logn == 11) {  // expansion and both transforms in one pass
         ProfLaunch pl(ctx, P_EXPAND);
         hip_check(launch_rows_seeded(seeds, bsrc, rows_ws, (size_t)S * ell, d_err, ctx->dev, sc, ctx->stream,
-                                     LaunchEv{pl.start(), pl.stop()}),
+                                     LaunchEv{pl.start(), pl.stop()}, a_res ? a_res : ctx->K, a_full),
                   "seeded rows");
     } else if (S) {
         {
@@ -499,7 +500,8 @@ struct HostRows {
 // (stage_rows_from).  Device layout, each part 256-byte aligned: digit
 // addresses [Q] u64, error word, seeds [S][ell][8], b rows [S][ell][K][N],
 // full rows [Q-S][ell][2][K][N].
-HostRows upload_rows(inspire_ctx* ctx, const inspire_query* qs, const std::vector<BatchItem>& items, uint32_t S) {
+HostRows upload_rows(inspire_ctx* ctx, const inspire_query* qs, const std::vector<BatchItem>& items, uint32_t S,
+                     uint32_t a_res) {
     const size_t KN = (size_t)ctx->K * ctx->N, ell = ctx->ell;
     const uint32_t Q = (uint32_t)items.size();
     auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
@@ -529,7 +531,8 @@ HostRows upload_rows(inspire_ctx* ctx, const inspire_query* qs, const std::vecto
     uint32_t* rows = ctx->ws1.get<uint32_t>();
     uint32_t* d_err = reinterpret_cast<uint32_t*>(d + o_err);
     stage_rows_from(ctx, Q, S, rows, reinterpret_cast<const uint32_t*>(d + o_seeds),
-                    reinterpret_cast<const uint32_t*>(d + o_b), reinterpret_cast<const uint32_t*>(d + o_full), d_err);
+                    reinterpret_cast<const uint32_t*>(d + o_b), reinterpret_cast<const uint32_t*>(d + o_full), d_err,
+                    a_res);
     return HostRows{rows, reinterpret_cast<const uint64_t*>(d), d_err};
 }
 
@@ -645,7 +648,8 @@ void respond_batch(inspire_ctx* ctx, uint32_t n, const inspire_query* qs, const 
             const uint32_t Q = (uint32_t)items.size();
             uint32_t S = 0;
             while (S < Q && qs[items[S].idx].seeded) S++;
-            const HostRows in = upload_rows(ctx, qs, items, S);
+            const HostRows in =
+                upload_rows(ctx, qs, items, S, pass == KIND_INSPIRING ? pack_rows_a_residues(ctx, cols) : 0);
             if (pass == KIND_INSPIRING) {
                 pack_respond_host(ctx, cols, items.data(), Q, in.rows, in.qdig, in.err, qs, rs, status);
                 continue;
@@ -778,7 +782,9 @@ void respond_batch_device(inspire_ctx* ctx, const inspire_device_batch& b, uint8
     auto enqueue = [&] {
         hip_check(hipMemcpyAsync(d_qdig, hdig, (size_t)Q * 8 + 8, hipMemcpyHostToDevice, ctx->stream), "digits");
         // rows: transformed from the caller's buffers into the workspace layout
-        if (b.seeded) stage_rows_from(ctx, Q, Q, rows, b.seeds, b.rows, nullptr, d_err);
+        if (b.seeded)
+            stage_rows_from(ctx, Q, Q, rows, b.seeds, b.rows, nullptr, d_err,
+                            packed ? pack_rows_a_residues(ctx, db.cols) : 0);
         else stage_rows_from(ctx, Q, 0, rows, nullptr, nullptr, b.rows, d_err);
         if (tree) tree_run(ctx, db.cols, Q, rows, d_qdig, b.out);
         else if (!packed) run_respond0(ctx, db.cols, Q, rows, d_qdig, b.out);

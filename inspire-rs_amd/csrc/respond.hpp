@@ -40,9 +40,12 @@ void respond_batch(inspire_ctx* ctx, uint32_t n, const inspire_query* qs, const 
 
 // RGSW rows ell..2ell-1 into rows_ws [Q][ell][2][K][N] (NTT, Montgomery
 // form): queries [0, S) seeded (a expanded from seeds [S][ell][8], b in bsrc
-// [S][ell][K][N]), [S, Q) full rows fsrc [Q-S][ell][2][K][N].
+// [S][ell][K][N]), [S, Q) full rows fsrc [Q-S][ell][2][K][N].  a_res: the
+// residues of a the kernels after it read (0: all; 1: residue 0, see
+// pack_rows_a_residues), except for the seeded queries t with a_full[t] != 0.
 void stage_rows_from(inspire_ctx* ctx, uint32_t Q, uint32_t S, uint32_t* rows_ws, const uint32_t* seeds,
-                     const uint32_t* bsrc, const uint32_t* fsrc, uint32_t* d_err);
+                     const uint32_t* bsrc, const uint32_t* fsrc, uint32_t* d_err, uint32_t a_res = 0,
+                     const uint32_t* a_full = nullptr);
 
 // Copies `words` device words and the 32-bit error word back into pinned
 // staging (ctx->hout), waits for the stream and returns the staged words.

@@ -246,9 +246,12 @@ void run_batch(inspire_server* s, Lane& L, std::vector<std::shared_ptr<SReq>>& b
         const uint32_t Q = (uint32_t)g.reqs.size();
         uint32_t* grows = rows + g.o_rows;
         uint32_t* d_err = reinterpret_cast<uint32_t*>(inbox + g.o_err);
+        // packed queries read residue 0 of a only (the InsPIRe^0 ones of a mixed group all of it)
         stage_rows_from(ctx, Q, g.S, grows, reinterpret_cast<const uint32_t*>(inbox + g.o_seeds),
                         reinterpret_cast<const uint32_t*>(inbox + g.o_b),
-                        reinterpret_cast<const uint32_t*>(inbox + g.o_full), d_err);
+                        reinterpret_cast<const uint32_t*>(inbox + g.o_full), d_err,
+                        g.packed ? pack_rows_a_residues(ctx, g.cols) : 0,
+                        g.mixed ? reinterpret_cast<const uint32_t*>(inbox + g.o_role) : nullptr);
         const uint64_t* d_qdig = reinterpret_cast<const uint64_t*>(inbox + g.o_dig);
         PackRoles roles;
         if (g.mixed) {
