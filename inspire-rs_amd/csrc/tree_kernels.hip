@@ -116,6 +116,28 @@ __device__ __forceinline__ uint32_t aut_slot(uint32_t p, uint32_t t) {
     return __brev((e - 1u) >> 1) >> (32 - LOGN);
 }
 
+// Scales of the signed inverse transform of residue k (reg_ntt.hpp SInvScale):
+// 2^32 mod q and the last stage's output scales in Montgomery form.
+__device__ __forceinline__ SInvScale tree_sinv(const CtxDev& C, const ScaleDev& inv, int k) {
+    const uint32_t q = C.m[k].q;
+    SInvScale s;
+    s.r1 = C.mont[k];
+    s.cm = csub(shoup_mul(inv.c[k], C.mont[k], C.montp[k], q), q);
+    s.cwm = csub(shoup_mul(inv.cw[k], C.mont[k], C.montp[k], q), q);
+    return s;
+}
+
+// Logical block of hardware block b in a grid of n: the dispatcher hands
+// consecutive blocks to the 8 XCDs in turn, each with its own L2, so the
+// blocks XCD x receives (x, x + 8, ...) are given one contiguous range of
+// logical ids.  The column workgroups of a query then share one L2 for the
+// query's RGSW rows, which would otherwise be fetched once per XCD.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t n) {
+    constexpr uint32_t kXcd = 8;
+    const uint32_t x = b % kXcd, i = b / kXcd, per = n / kXcd, rem = n % kXcd;
+    return x * per + (x < rem ? x : rem) + i;
+}
+
 // One level of pack_lwes_inner on the node held by this workgroup (see the
 // file comment).  a0/a1: s0.a/s1.a at this thread's coefficient positions,
 // h0/h1: s0.a^/s1.a^ and b0/b1: s0.b/s1.b at its slot positions (canonical).
@@ -259,9 +281,17 @@ __device__ __forceinline__ void tree_level(uint32_t (&a0)[kTreeV], const uint32_
             b0[e] = addq(addq(b0[e], bt[e], q), csub(mont_reduce(sb, q, M.qinv), q), q);
         }
     }
-    intt_v<LOGN, V>(A, t, ab, L.twf + k * N, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
+    if constexpr (SGN) {
+        // signed inverse transform (A in [0, 2q), 8 q < 2^31 checked on the host): one
+        // reduction per radix-4 pass, output in (-q, q)
+        intt4_s<LOGN>(A, t, ab, L.twf + k * N, M, tree_sinv(C, inv, k));
 #pragma unroll
-    for (int e = 0; e < V; e++) a0[e] = addq(a0[e], csub(A[e], q), q);
+        for (int e = 0; e < V; e++) a0[e] = addq(a0[e], sfix(A[e], q), q);
+    } else {
+        intt_v<LOGN, V>(A, t, ab, L.twf + k * N, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
+#pragma unroll
+        for (int e = 0; e < V; e++) a0[e] = addq(a0[e], csub(A[e], q), q);
+    }
 }
 
 template <int LOGN, int K, int ELL, bool LAZY, bool REC, bool SGN>
@@ -274,7 +304,8 @@ k_tree_leaves(TreeArgs A, CtxDev C, ScaleDev inv) {
     extern __shared__ __align__(16) uint32_t lds[];
     const TreeLds<LOGN, K, ELL, REC> L(lds);
     const int tid = threadIdx.x, k = residue_of<T>(tid), t = tid - k * T;
-    const uint32_t qi = blockIdx.x / A.m, s = blockIdx.x - qi * A.m;
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t qi = blk / A.m, s = blk - qi * A.m;
     const Mod M{C.m[k].q, C.m[k].q2, C.m[k].qinv};
     const uint32_t q = M.q;
 
@@ -325,9 +356,15 @@ k_tree_leaves(TreeArgs A, CtxDev C, ScaleDev inv) {
         a[e] = h[e];
         b[e] = 0;
     }
-    intt_v<LOGN, V>(a, t, L.abuf + k * LP, L.twf + k * N, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
+    if constexpr (SGN) {
+        intt4_s<LOGN>(a, t, L.abuf + k * LP, L.twf + k * N, M, tree_sinv(C, inv, k));  // a < 2q in
 #pragma unroll
-    for (int e = 0; e < V; e++) a[e] = csub(a[e], q);
+        for (int e = 0; e < V; e++) a[e] = sfix(a[e], q);
+    } else {
+        intt_v<LOGN, V>(a, t, L.abuf + k * LP, L.twf + k * N, M, inv.c[k], inv.cp[k], inv.cw[k], inv.cwp[k]);
+#pragma unroll
+        for (int e = 0; e < V; e++) a[e] = csub(a[e], q);
+    }
 
     if (A.m == 1) {  // a single LWE is not tree-packed (automorph_pack.rs:436-443)
         uint32_t* o = A.out + (size_t)qi * A.out_words;
@@ -373,7 +410,8 @@ k_tree_combine(TreeArgs A, CtxDev C, ScaleDev inv) {
     const uint32_t level = A.level_lo;
     const uint32_t half = 1u << (LOGN - level);  // nodes at this level: s < min(m, half)
     const uint32_t nn = A.m < half ? A.m : half;
-    const uint32_t qi = blockIdx.x / nn, s = blockIdx.x - qi * nn;
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t qi = blk / nn, s = blk - qi * nn;
     const bool odd = s + half < A.m;
     const uint32_t q = C.m[k].q;
     const uint32_t* ev = A.node + ((size_t)qi * A.m + s) * 3 * KN + (size_t)k * N;
@@ -496,7 +534,8 @@ static hipError_t tree_lazy(const TreeArgs& A, bool leaves, uint32_t blocks, con
     const bool rec = (uint64_t)C.gadget_log * ELL >= 64 || C.q <= (1ull << (C.gadget_log * ELL));
     // signed digit path (ct_s transforms, signed Montgomery sums): tau(s1.a^) < q enters the
     // recovered digit, nothing is folded into the multiply-accumulates
-    bool sgn = true;
+    // the signed inverse transforms hold sums below 8 q in int32
+    bool sgn = 8 * qmax < (1ull << 31);
     for (int kk = 0; kk < K; kk++) {
         const uint64_t q = C.m[kk].q;
         sgn = sgn && signed_digits_ok(q, C.small_digits ? C.gadget_mask : q - 1, LOGN, rec ? ELL - 1 : ELL,
