@@ -572,6 +572,10 @@ def main():
         "avg_launch_ms": round(fd_avg_s * 1e3, 4),
         "measured": "HIP events, untimed single-lane pass (the kernel alone on the GPU)",
         "avg_launch_ms_timed_region": round(prof[front][0] / max(prof[front][1], 1), 4),
+        # the device's measured streaming-read rate (16-byte loads over 1 GiB), the practical roof
+        "peak_measured": round(RATES["hbm_read_tbs"] * 1e3, 1) if RATES.get("hbm_read_tbs") else None,
+        "frac_measured": (round(FRONT_BYTES * B / fd_avg_s / 1e9 / (RATES["hbm_read_tbs"] * 1e3), 4)
+                          if RATES.get("hbm_read_tbs") else None),
     }
     line = {
         "metric": METRIC,
@@ -814,6 +818,7 @@ def extract_leg(args, M, ctx, log):
                           "product: frac_mad64_measured is its roof; frac_modmul prices it as modmuls",
             "achieved_hbm": round(gbs, 1), "peak_hbm": HBM_PEAK_GBS, "unit_hbm": "GB/s",
             "frac_hbm": round(gbs / HBM_PEAK_GBS, 4),
+            "frac_hbm_measured": round(gbs / (RATES["hbm_read_tbs"] * 1e3), 4) if RATES.get("hbm_read_tbs") else None,
             "traffic": measured_traffic("k_extract", E),
             "algorithmic_per_launch": f"{mm} modmul and {nbytes} B per response x {E} responses",
         },

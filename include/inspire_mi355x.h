@@ -471,8 +471,10 @@ int inspire_ctx_set_kernel_choice(inspire_ctx *ctx, const char *name, int value)
  * arithmetic the integer kernels are priced against (register-resident
  * probes, no memory traffic): tops[0] the Montgomery modular product of the
  * transforms, tops[1] its signed form, tops[2] the 64-bit multiply-add of
- * the extract kernel.  n >= 3.  No counterpart in the reference: it is the
- * measured roof beside the published one. */
+ * the extract kernel; with n >= 4, tops[3] the streaming read rate of HBM in
+ * TB/s (16-byte loads over 1 GiB, allocated for the call).  n >= 3.  No
+ * counterpart in the reference: it is the measured roof beside the
+ * published one. */
 int inspire_ctx_measure_rates(inspire_ctx *ctx, double *tops, uint32_t n);
 int inspire_ctx_get_tuning(const inspire_ctx *ctx, char *out, uint64_t capacity, uint64_t *len);
 

@@ -108,5 +108,7 @@ hipError_t launch_rows_seeded(const uint32_t* seeds, const uint32_t* bsrc, uint3
 hipError_t launch_rate_probe(int kind, uint32_t* out, uint32_t blocks, uint32_t iters, const ModDev& m,
                              hipStream_t st);
 uint64_t rate_ops_per_launch(uint32_t blocks, uint32_t iters);
+// Streaming read of `bytes` (a multiple of 16) at src; out holds blocks * 256 words.
+hipError_t launch_read_probe(const void* src, size_t bytes, uint32_t* out, uint32_t blocks, hipStream_t st);
 
 }  // namespace inspire

@@ -4,7 +4,10 @@
 //     one 32-bit multiply by -q^-1, one multiply-add by q and the shift);
 //   - the signed Montgomery product of the signed digit transforms (smmul);
 //   - the 64-bit multiply-add (32x32 + 64) the extract kernel's dot products
-//     are made of.
+//     are made of;
+//   - and the streaming read rate of HBM (16-byte loads over a buffer far
+//     larger than the caches), the practical roof of the HBM-bound kernels
+//     beside the published peak.
 // Each thread runs CH independent dependency chains of `iters` operations;
 // one vector store per thread, taken only when the chains' XOR equals a value
 // the host never passes, keeps the results live.
@@ -58,7 +61,25 @@ __global__ void __launch_bounds__(256) k_rate_mad64(uint32_t* out, uint32_t iter
     if (acc == never) out[blockIdx.x * blockDim.x + threadIdx.x] = acc;
 }
 
+// Every 16-byte word of p once, grid-stride, XOR-reduced per thread.
+__global__ void __launch_bounds__(256) k_rate_read(const uint4* __restrict__ p, size_t n16, uint32_t* out,
+                                                   uint32_t never) {
+    uint32_t acc = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x) {
+        const uint4 v = p[i];
+        acc ^= v.x ^ v.y ^ v.z ^ v.w;
+    }
+    if (acc == never) out[blockIdx.x * blockDim.x + threadIdx.x] = acc;
+}
+
 }  // namespace
+
+// Streaming read of `bytes` (a multiple of 16) at src; out: blocks * 256 words.
+hipError_t launch_read_probe(const void* src, size_t bytes, uint32_t* out, uint32_t blocks, hipStream_t st) {
+    hipLaunchKernelGGL(k_rate_read, dim3(blocks), dim3(256), 0, st, reinterpret_cast<const uint4*>(src), bytes / 16,
+                       out, 0xfffffffdu);
+    return hipGetLastError();
+}
 
 uint64_t rate_ops_per_launch(uint32_t blocks, uint32_t iters) {
     return (uint64_t)blocks * 256 * iters * kRateChains;

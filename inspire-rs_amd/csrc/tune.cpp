@@ -197,7 +197,7 @@ std::string tuning_json(const inspire_ctx* ctx, const TuneResult* R) {
 // Issue rates of the probes in rates.hip (T operations/s): 8 workgroups of
 // 256 threads per CU, each thread 8 chains of 2048 operations, the mean of
 // 5 launches after a warm one.
-void measure_rates(inspire_ctx* ctx, double* tops) {
+void measure_rates(inspire_ctx* ctx, double* tops, uint32_t n) {
     int dev = 0, cus = 256;
     hipDeviceProp_t p{};
     if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&p, dev) == hipSuccess) cus = p.multiProcessorCount;
@@ -210,6 +210,16 @@ void measure_rates(inspire_ctx* ctx, double* tops) {
                       "rate probe");
         });
         tops[kind] = (double)rate_ops_per_launch(blocks, iters) / (ms * 1e-3) / 1e12;
+    }
+    if (n > 3) {  // HBM streaming read: 1 GiB (zeros), TB/s
+        const size_t bytes = (size_t)1 << 30;
+        DevBuf src(bytes);
+        hip_check(hipMemsetAsync(src.get(), 0, bytes, ctx->stream), "read probe fill");
+        const double ms = time_launches(ctx, 5, [&] {
+            hip_check(launch_read_probe(src.get(), bytes, out.get<uint32_t>(), blocks, ctx->stream), "read probe");
+        });
+        tops[3] = (double)bytes / (ms * 1e-3) / 1e12;
+        hip_check(hipStreamSynchronize(ctx->stream), "read probe");
     }
 }
 
@@ -310,7 +320,7 @@ int inspire_ctx_measure_rates(inspire_ctx* ctx, double* tops, uint32_t n) {
         if (!ctx || !tops) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "null argument"};
         if (n < 3) throw Error{INSPIRE_ERR_CAPACITY, "the rate probes report 3 values"};
         std::lock_guard<std::recursive_mutex> g(ctx->mu);
-        measure_rates(ctx, tops);
+        measure_rates(ctx, tops, n);
         return INSPIRE_OK;
     })
 }

@@ -603,11 +603,12 @@ class Context:
         return json.loads(buf.value.decode())
 
     def measure_rates(self) -> dict:
-        """Measured issue rates (inspire_ctx_measure_rates), T operations/s:
-        the Montgomery product, its signed form, the 64-bit multiply-add."""
-        t = (C.c_double * 3)()
-        _raise(lib().inspire_ctx_measure_rates(self._h, t, 3))
-        return {"mmul_tops": t[0], "smmul_tops": t[1], "mad64_tops": t[2]}
+        """Measured rates (inspire_ctx_measure_rates): T operations/s of the
+        Montgomery product, its signed form and the 64-bit multiply-add, and
+        the streaming read rate of HBM in TB/s."""
+        t = (C.c_double * 4)()
+        _raise(lib().inspire_ctx_measure_rates(self._h, t, 4))
+        return {"mmul_tops": t[0], "smmul_tops": t[1], "mad64_tops": t[2], "hbm_read_tbs": t[3]}
 
     def set_graphs(self, on: bool):
         """Graph replay of repeated device batches (on by default)."""

@@ -101,9 +101,12 @@ def test_autotune_report_persists(M, tmp_path):
 def test_measured_issue_rates_are_plausible(M):
     """The register-resident probes report device-wide rates: a Montgomery
     product is three multiply instructions, so it issues slower than one
-    64-bit multiply-add, and both lie within what 256 CUs can issue."""
+    64-bit multiply-add, and both lie within what 256 CUs can issue.  The
+    streaming read rate lies below the published 8 TB/s of HBM3E and above
+    a quarter of it."""
     with M.Context(M.InspireParams(), W_SEED) as ctx:
         r = ctx.measure_rates()
         say("rates", r)
         assert 1.0 < r["mmul_tops"] < 100.0 and 1.0 < r["smmul_tops"] < 100.0
         assert r["mad64_tops"] > r["mmul_tops"]
+        assert 2.0 < r["hbm_read_tbs"] < 8.0
