@@ -242,9 +242,13 @@ def main():
                          "tuned, the count among --tune-lanes with the highest median rate over --tune-rounds "
                          "interleaved runs of --tune-steps untimed steps")
     ap.add_argument("--tune-lanes", type=str, default="1,2,3,4", help="lane counts the lane tuning tries")
-    ap.add_argument("--tune-steps", type=int, default=96, help="untimed steps per lane count and round when tuning lanes")
-    ap.add_argument("--tune-rounds", type=int, default=5,
-                    help="rounds of the lane tuning (every candidate once per round, the median rate kept)")
+    ap.add_argument("--tune-steps", type=int, default=0,
+                    help="untimed steps per lane count and round when tuning lanes; 0 (default): as many as the "
+                         "timed region has, at most 96 (a short timed region favours fewer lanes: filling and "
+                         "draining the lanes' pipelines weighs more)")
+    ap.add_argument("--tune-rounds", type=int, default=0,
+                    help="rounds of the lane tuning (every candidate once per round, the median rate kept); "
+                         "0 (default): enough for about 480 steps per candidate, 5 to 25")
     ap.add_argument("--tuning-file", type=str, default="",
                     help="where the tuning report (kernel variants and lanes) is written; with --reuse-tuning, "
                          "read from instead of measuring")
@@ -390,6 +394,7 @@ def main():
         run_lanes(lanes, louts, 2 * len(lanes))
     # Lanes: the headline's lane count is tuned (untimed) unless given.
     lane_rates = {}
+    t_steps = t_rounds = 0
     if args.lanes:
         n_lanes = args.lanes
     elif prior and "lanes" in prior:
@@ -397,11 +402,13 @@ def main():
     else:
         # every candidate once per round, rounds interleaved, so a drift of the clocks or a
         # slow run moves all candidates alike; the median of each candidate's rates is kept
+        t_steps = args.tune_steps if args.tune_steps > 0 else max(1, min(args.steps, 96))
+        t_rounds = args.tune_rounds if args.tune_rounds > 0 else min(25, max(5, 480 // t_steps))
         runs = {nl: [] for nl in cand}
-        for _ in range(max(1, args.tune_rounds)):
+        for _ in range(t_rounds):
             for nl in cand:
-                el, _ = run_lanes(lanes[:nl], louts[:nl], args.tune_steps)
-                runs[nl].append(B * args.tune_steps / el)
+                el, _ = run_lanes(lanes[:nl], louts[:nl], t_steps)
+                runs[nl].append(B * t_steps / el)
         for nl in cand:
             lane_rates[nl] = round(float(np.median(runs[nl])), 1)
         # the fewest lanes within 3 % of the best rate: more lanes only add latency beyond that
@@ -410,6 +417,7 @@ def main():
     tuning["lanes"] = n_lanes
     if lane_rates:
         tuning.setdefault("measured", {})["lanes_qps"] = {str(k): v for k, v in lane_rates.items()}
+        tuning["measured"]["lanes_tuning"] = f"median of {t_rounds} interleaved rounds of {t_steps} steps"
     log(f"lanes: {n_lanes} {lane_rates}")
     if args.tuning_file and not args.reuse_tuning:
         with open(args.tuning_file, "w") as f:
