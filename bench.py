@@ -46,7 +46,7 @@ NTT = (D // 2) * 11
 MODMUL_CHAIN = ((GAMMA - 1) * (2 + 2 * (ELL - 1)) * NTT + (GAMMA - 1) * 2 * K * D
                 + (GAMMA - 1) * 2 * ELL * K * D + 4 * NTT + ELL * K * NTT)
 MODMUL_QUERY = (
-    2 * ELL * K * NTT            # RGSW rows ell..2ell-1, a and b
+    ELL * (K + 1) * NTT          # RGSW rows ell..2ell-1: a of residue 0 and b of both (residue 1 of a is not read)
     + GAMMA * 2 * ELL * K * D    # first dimension MAC
     + GAMMA * GAMMA * D          # step 1 (residue 0)
     + MODMUL_CHAIN
@@ -621,8 +621,8 @@ def main():
         "modmul_per_query": MODMUL_QUERY,
         "modmul_per_query_reference_algorithm": MODMUL_QUERY_REF_ALGORITHM,
         "modmul_note": "modmul_per_query counts the work this path does (database digits stored in the NTT "
-                       "domain, step 1 by Horner's rule on residue 0, the last gadget digit's transform "
-                       "recovered); the reference's algorithm as written does "
+                       "domain, step 1 on residue 0 only, the rows' a expanded and transformed for residue 0 "
+                       "only, the last gadget digit's transform recovered); the reference's algorithm as written does "
                        f"{MODMUL_QUERY_REF_ALGORITHM} per query; the roofline uses the former",
         "roofline": roofline,
         "roofline_hbm_front": roofline_hbm,
