@@ -365,7 +365,11 @@ k_expand_seeds(const uint32_t* __restrict__ seeds, uint32_t* __restrict__ out, s
 // expands and transforms that residue alone: its draws come first in the
 // stream, so the residue-0 words are the same, and residue 1's a is left
 // unwritten.  Row r expands KA = (a_full && a_full[r / ell]) ? K : a_res.
+// SGN: the transforms run on two's-complement words with butterflies that
+// reduce nothing (one reduction of every value after six stages), when the
+// host's bound check (signed_full_ok) passes for every modulus.
 // ---------------------------------------------------------------------------
+template <bool SGN>
 __global__ void __launch_bounds__(256)
 k_rows_seeded(const uint32_t* __restrict__ seeds, const uint32_t* __restrict__ bsrc, uint32_t* __restrict__ rows,
               uint32_t* __restrict__ err, CtxDev C, ScaleDev sc, const uint32_t* __restrict__ a_full,
@@ -447,8 +451,15 @@ k_rows_seeded(const uint32_t* __restrict__ seeds, const uint32_t* __restrict__ b
     // this thread's eight consecutive slots of poly ab (0: a, 1: b) of residue kk, Montgomery form
     auto store = [&](const uint32_t (&x)[8], int ab, uint32_t kk, const Mod& M) {
         uint32_t y[8];
+        if constexpr (SGN) {
+            // the scale c in Montgomery form: one signed product gives x c in (-q, q)
+            const uint32_t cm = csub(shoup_mul(sc.c[kk], C.mont[kk], C.montp[kk], M.q), M.q);
 #pragma unroll
-        for (int e = 0; e < 8; e++) y[e] = csub(shoup_mul(x[e], sc.c[kk], sc.cp[kk], M.q), M.q);
+            for (int e = 0; e < 8; e++) y[e] = sfix(smmul(x[e], cm, M), M.q);
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; e++) y[e] = csub(shoup_mul(x[e], sc.c[kk], sc.cp[kk], M.q), M.q);
+        }
         uint4* o = reinterpret_cast<uint4*>(out + (size_t)ab * KN + (size_t)kk * N + 8 * tid);
         o[0] = make_uint4(y[0], y[1], y[2], y[3]);
         o[1] = make_uint4(y[4], y[5], y[6], y[7]);
@@ -465,7 +476,8 @@ k_rows_seeded(const uint32_t* __restrict__ seeds, const uint32_t* __restrict__ b
             // every thread holds its a values and is past the previous transform's
             // exchange reads before buffers kk, kk + 1 are written
             __syncthreads();
-            ntt_digits<11, 2, false>(x, tid, poly[kk], C.m[kk].twm, M);
+            if constexpr (SGN) ntt_digits<11, 2, true, 1>(x, tid, poly[kk], C.m[kk].twm, M, C.mont[kk]);
+            else ntt_digits<11, 2, false>(x, tid, poly[kk], C.m[kk].twm, M);
             store(x[0], 0, kk, M);
             store(x[1], 1, kk, M);
         } else {  // b alone (a of this residue is not read)
@@ -473,7 +485,8 @@ k_rows_seeded(const uint32_t* __restrict__ seeds, const uint32_t* __restrict__ b
 #pragma unroll
             for (int e = 0; e < 8; e++) x[0][e] = b[(size_t)kk * N + tid + e * S::T];
             __syncthreads();  // past the previous transform's exchange reads of buffer kk
-            ntt_digits<11, 1, false>(x, tid, poly[kk], C.m[kk].twm, M);
+            if constexpr (SGN) ntt_digits<11, 1, true, 1>(x, tid, poly[kk], C.m[kk].twm, M, C.mont[kk]);
+            else ntt_digits<11, 1, false>(x, tid, poly[kk], C.m[kk].twm, M);
             store(x[0], 1, kk, M);
         }
     }
@@ -588,8 +601,15 @@ hipError_t launch_rows_seeded(const uint32_t* seeds, const uint32_t* bsrc, uint3
     if (!count) return hipSuccess;
     if (C.logn != 11 || count > 0x7fffffffu) return hipErrorInvalidValue;
     if (a_res < 1 || a_res > C.K) a_res = C.K;
-    hipExtLaunchKernelGGL(k_rows_seeded, dim3((unsigned)count), dim3(256), 0, st, ev.start, ev.stop, 0, seeds, bsrc,
-                          rows, err, C, sc, a_full, a_res, C.ell);
+    // canonical inputs (below q) through 11 signed stages with one reduction after six
+    bool sgn = true;
+    for (uint32_t k = 0; k < C.K; k++) sgn = sgn && signed_full_ok(C.m[k].q, C.m[k].q - 1, 11, 6);
+    if (sgn)
+        hipExtLaunchKernelGGL(k_rows_seeded<true>, dim3((unsigned)count), dim3(256), 0, st, ev.start, ev.stop, 0,
+                              seeds, bsrc, rows, err, C, sc, a_full, a_res, C.ell);
+    else
+        hipExtLaunchKernelGGL(k_rows_seeded<false>, dim3((unsigned)count), dim3(256), 0, st, ev.start, ev.stop, 0,
+                              seeds, bsrc, rows, err, C, sc, a_full, a_res, C.ell);
     return hipGetLastError();
 }
 

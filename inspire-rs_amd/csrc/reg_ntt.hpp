@@ -121,6 +121,23 @@ inline bool signed_digits_ok(uint64_t q, uint64_t digit_max, int logn, int nd, u
     return red(((u128)nd * B + last + extra) * (q - 1)) < 2 * q;
 }
 
+// Host: bounds of a signed forward transform of inputs below in_max with one
+// reduction to (-q, q) after `first` stages (ntt_digits' RED): true when all
+// logn stages stay inside int32 and the outputs times a constant below q fit
+// one signed Montgomery product (below 2^31 q).
+inline bool signed_full_ok(uint64_t q, uint64_t in_max, int logn, int first) {
+    using u128 = unsigned __int128;
+    if (2 * q >= (1ull << 31)) return false;
+    auto red = [&](u128 t) { return (uint64_t)((t + ((u128)q << 31)) >> 32); };
+    uint64_t B = in_max;
+    for (int s = 0; s < logn; s++) {
+        if (s == first) B = red((u128)B * (q - 1));
+        B += red((u128)B * (q - 1));
+        if (B >= (1ull << 31)) return false;
+    }
+    return true;
+}
+
 template <bool SGN>
 __device__ __forceinline__ void ct_b(uint32_t& X, uint32_t& Y, uint32_t wm, const Mod& M) {
     if constexpr (SGN) ct_s(X, Y, wm, M);
@@ -272,10 +289,13 @@ __device__ __forceinline__ void fwd_r8(uint32_t (&x)[ELL][8], int m0, int i, con
 // thread t -> x at its slot positions (< 4q).  db is [ELL][LP] LDS; no thread
 // may still be reading it.  SGN: the inputs and outputs are two's-complement
 // words and the butterflies reduce nothing (ct_s): the caller guarantees that
-// LOGN stages keep the values inside int32.
-template <int LOGN, int ELL, bool SGN = false>
+// LOGN stages keep the values inside int32.  RED >= 0 (with SGN): after
+// radix-8 pass RED every value is brought back to (-q, q) by a signed
+// Montgomery product with r1 = 2^32 mod q, for inputs of full size
+// (signed_full_ok bounds the stages on either side).
+template <int LOGN, int ELL, bool SGN = false, int RED = -1>
 __device__ __forceinline__ void ntt_digits(uint32_t (&x)[ELL][8], int t, uint32_t* db, const uint32_t* tw,
-                                           const Mod& M) {
+                                           const Mod& M, uint32_t r1 = 0) {
     using L = Lay<LOGN>;
     constexpr int N = L::N, LP = L::LP;
     int m0 = 1;
@@ -291,6 +311,14 @@ __device__ __forceinline__ void ntt_digits(uint32_t (&x)[ELL][8], int t, uint32_
                 for (int e = 0; e < 8; e++) x[j][e] = db[j * LP + lds_step(bix, e, st)];
         }
         fwd_r8<ELL, SGN>(x, m0, i, tw, M);
+        if constexpr (SGN && RED >= 0) {
+            if (p == RED) {
+#pragma unroll
+                for (int j = 0; j < ELL; j++)
+#pragma unroll
+                    for (int e = 0; e < 8; e++) x[j][e] = smmul(x[j][e], r1, M);
+            }
+        }
         if (!(L::REM == 0 && p == L::R8 - 1)) {
 #pragma unroll
             for (int j = 0; j < ELL; j++)
