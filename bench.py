@@ -565,6 +565,9 @@ def main():
         "measured": f"HIP events on each lane's stream over the timed region ({n_lanes} lanes, "
                     "sharing CUs with the other lanes' kernels)",
         "avg_launch_ms_alone": round(iso_chain_ms, 4),
+        # the same algorithmic work over the launch time of the kernel alone on the GPU (one lane)
+        "frac_alone": (round(MODMUL_CHAIN * B / (iso_chain_ms * 1e-3) / 1e12 / MODMUL_PEAK_T, 4)
+                       if iso_chain_ms else None),
         "peak_measured": RATES.get("smmul_tops"),
         "frac_measured": round(chain_tmodmul / RATES["smmul_tops"], 4) if RATES.get("smmul_tops") else None,
         "peak_measured_note": "the signed Montgomery product's measured issue rate (the chain's digit transforms "
