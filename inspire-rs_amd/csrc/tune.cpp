@@ -93,8 +93,14 @@ void tune_ntt(inspire_ctx* ctx, TuneResult& R) {
                                      ctx->stream),
                   "tune ntt");
     };
-    R.ntt_valu_ms = time_launches(ctx, 5, [&] { run(0, d0); });
-    R.ntt_mfma_ms = time_launches(ctx, 5, [&] { run(1, d1); });
+    // alternating rounds, the fastest round of each variant kept: a transient
+    // slowdown that hits one variant's launches does not decide the choice
+    double best[2] = {1e30, 1e30};
+    for (int r = 0; r < 5; r++)
+        for (uint32_t v = 0; v < 2; v++)
+            best[v] = std::min(best[v], time_launches(ctx, 3, [&] { run(v, v ? d1 : d0); }));
+    R.ntt_valu_ms = best[0];
+    R.ntt_mfma_ms = best[1];
     std::vector<uint32_t> o0(words), o1(words);
     hip_check(hipMemcpy(o0.data(), d0.get(), words * 4, hipMemcpyDeviceToHost), "tune download");
     hip_check(hipMemcpy(o1.data(), d1.get(), words * 4, hipMemcpyDeviceToHost), "tune download");
