@@ -353,6 +353,16 @@ def main():
     cand = sorted({int(x) for x in args.tune_lanes.split(",") if x.strip()} | ({args.lanes} if args.lanes else set()))
     lanes = [ctx] + [ctx.lane() for _ in range(max(cand + [1]) - 1)]
     louts = [dout] + [M.DeviceArray(B * 2 * K * D * 4) for _ in lanes[1:]]
+    # The headline's regime from priming on: HIP events on each lane's stream
+    # bracket the chain kernel of every batch (its roofline), and a recurring
+    # batch replays the HIP graphs captured for it, the chain a graph of its
+    # own between the two events.  Priming, lane tuning, warmup and the timed
+    # region all run this way, so the graphs are captured before the timed region.
+    def headline_regime(ln, on=True):
+        ln.set_profiling_kernels(["pack_chain"])
+        ln.set_profiling(on)
+    for ln in lanes:
+        headline_regime(ln)
     def run_lanes(lns, outs, steps):
         """steps batches taken in turn by the lanes' threads: (elapsed s, per-step latencies ms)"""
         lat = []
@@ -429,15 +439,17 @@ def main():
     for j, ln in enumerate(lanes):
         for i in range(args.warmup):
             step(i + j, c=ln, out=louts[j])
-    # Timed region: HIP events bracket only the two kernels the rooflines are
-    # computed for (every bracket costs host time inside the step).
+    # Timed region: HIP events bracket only the kernel the roofline is computed
+    # for (every bracket costs host time inside the step).
     kernels = ("pack_chain", "pack_front", "first_dim", "pack_step1", "pack_b0", "expand_seeds", "rows_ntt", "y_ntt")
     for ln in lanes:
-        ln.set_profiling_kernels(["pack_chain", "pack_front"])
-        ln.set_profiling(True)
         for name in kernels:
             ln.profile(name, reset=True)
+    caps0 = [ln.graph_stats() for ln in lanes]
     elapsed, lat = run_lanes(lanes, louts, args.steps)
+    caps1 = [ln.graph_stats() for ln in lanes]
+    headline_graphs = {"captures": int(sum(b[0] - a[0] for a, b in zip(caps0, caps1))),
+                       "replays": int(sum(b[1] - a[1] for a, b in zip(caps0, caps1)))}
     n_lanes = len(lanes)
     prof = {}
     for name in kernels:
@@ -447,12 +459,13 @@ def main():
         ln.set_profiling(False)
     qps = B * args.steps / elapsed
     log(f"batched: {qps:.0f} queries/s, {elapsed / args.steps * 1e3:.3f} ms/step over {n_lanes} lanes, "
-        f"p50 {percentile(lat, 50):.3f} ms")
+        f"p50 {percentile(lat, 50):.3f} ms, graphs {headline_graphs}")
     # The same timed region without profiling events, with graph replay of the
-    # repeated batches on and then off (a profiled call always runs eagerly).
+    # repeated batches on and then off.
     graphs_ab = {}
     for on in (True, False):
         for j, ln in enumerate(lanes):
+            ln.set_profiling(False)
             ln.set_graphs(on)
             for i in range(max(args.warmup, 2 * n_sets)):  # every query set seen twice: captured
                 step(i + j, c=ln, out=louts[j])
@@ -464,44 +477,67 @@ def main():
             "p50_ms": round(percentile(lt, 50), 3), "p99_ms": round(percentile(lt, 99), 3),
             "graph_replays": int(sum(b[1] - a[1] for a, b in zip(caps0, caps1)))}
         log(f"  no profiling, graphs {'on' if on else 'off'}: {graphs_ab['graphs_on' if on else 'graphs_off']}")
-    for ln in lanes:
+    # the headline region once more (same K steps) after the others, with the
+    # front end's launches bracketed too: the spread between two measurements
+    # of the same thing in one process, and the front end's launch time there
+    for j, ln in enumerate(lanes):
         ln.set_graphs(True)
-    # the headline region once more (profiled, same K steps) after the others:
-    # the spread between two measurements of the same thing in one process
-    for ln in lanes:
         ln.set_profiling_kernels(["pack_chain", "pack_front"])
         ln.set_profiling(True)
-    el, lt = run_lanes(lanes, louts, args.steps)
-    for ln in lanes:
+        for i in range(max(args.warmup, 2 * n_sets)):
+            step(i + j, c=ln, out=louts[j])
         for name in kernels:
             ln.profile(name, reset=True)
+    el, lt = run_lanes(lanes, louts, args.steps)
+    prof_repeat = {}
+    for name in kernels:
+        tot = [ln.profile(name, reset=True) for ln in lanes]
+        prof_repeat[name] = (sum(x[0] for x in tot), sum(x[1] for x in tot))
+    for ln in lanes:
         ln.set_profiling(False)
-    graphs_ab["profiled_repeat"] = {"value": round(B * args.steps / el, 1), "ms_per_step": round(el / args.steps * 1e3, 3),
-                                    "p50_ms": round(percentile(lt, 50), 3), "p99_ms": round(percentile(lt, 99), 3)}
-    log(f"  profiled repeat: {graphs_ab['profiled_repeat']}")
-    # the same batches from other numbers of lanes (more lanes: more throughput, longer latency)
+    graphs_ab["profiled_repeat"] = {
+        "value": round(B * args.steps / el, 1), "ms_per_step": round(el / args.steps * 1e3, 3),
+        "p50_ms": round(percentile(lt, 50), 3), "p99_ms": round(percentile(lt, 99), 3),
+        "pack_chain_avg_ms": round(prof_repeat["pack_chain"][0] / max(prof_repeat["pack_chain"][1], 1), 4),
+        "pack_front_avg_ms": round(prof_repeat["pack_front"][0] / max(prof_repeat["pack_front"][1], 1), 4)}
+    log(f"  profiled repeat (chain and front end bracketed): {graphs_ab['profiled_repeat']}")
+    # the same batches from other numbers of lanes (more lanes: more throughput,
+    # longer latency), in the headline's regime
     sweep = []
     lanes, louts = lanes + spare, louts + spare_outs
+    for j, ln in enumerate(lanes):
+        headline_regime(ln)
+        for i in range(max(args.warmup, 2 * n_sets)):
+            step(i + j, c=ln, out=louts[j])
     for nl in sorted({int(x) for x in args.lanes_sweep.split(",") if x.strip()} | set(cand)):
         if nl < 1 or nl == n_lanes:
             continue
         while len(lanes) < nl:
             lanes.append(ctx.lane())
             louts.append(M.DeviceArray(B * 2 * K * D * 4))
-            for i in range(args.warmup):
-                step(i, c=lanes[-1], out=louts[-1])
+            headline_regime(lanes[-1])
+            for i in range(max(args.warmup, 2 * n_sets)):
+                step(i + len(lanes) - 1, c=lanes[-1], out=louts[-1])
         el, lt = run_lanes(lanes[:nl], louts[:nl], args.steps)
         sweep.append({"lanes": nl, "value": round(B * args.steps / el, 1), "ms_per_step": round(el / args.steps * 1e3, 3),
                       "p50_ms": round(percentile(lt, 50), 3), "p99_ms": round(percentile(lt, 99), 3)})
         log(f"  {nl} lanes: {sweep[-1]['value']:.0f} queries/s, p50 {sweep[-1]['p50_ms']} ms, "
             f"p99 {sweep[-1]['p99_ms']} ms")
-    # untimed: every kernel of the step, for the breakdown
-    ctx.set_profiling(True)
+    # untimed: every kernel of the step alone on the GPU (one lane, eager
+    # launches whose events are recorded by the kernels' own dispatches), for
+    # the breakdown
+    for ln in lanes:
+        ln.set_profiling(False)
+    ctx.set_graphs(False)
     ctx.set_profiling_kernels(None)
+    for name in kernels:
+        ctx.profile(name, reset=True)
+    ctx.set_profiling(True)
     for i in range(min(args.steps, 5)):
         step(i)
     breakdown = {name: ctx.profile(name, reset=True) for name in kernels}
     ctx.set_profiling(False)
+    ctx.set_graphs(True)
     for k_, (ms, n) in breakdown.items():
         if n:
             log(f"  {k_:13s} {ms / n:8.3f} ms/launch x {n}")
@@ -562,8 +598,10 @@ def main():
         "traffic": measured_traffic("k_pack_chain_crt", B),
         "algorithmic_per_launch": f"{MODMUL_CHAIN} modmul/query x {B} queries",
         "avg_launch_ms": round(chain_avg_s * 1e3, 4),
-        "measured": f"HIP events on each lane's stream over the timed region ({n_lanes} lanes, "
-                    "sharing CUs with the other lanes' kernels)",
+        "measured": f"HIP events on each lane's stream over the timed region, recorded around the "
+                    f"chain's own graph launch in every batch ({n_lanes} lanes, sharing CUs with the other "
+                    "lanes' kernels)",
+        "timed_region_graphs": headline_graphs,
         "avg_launch_ms_alone": round(iso_chain_ms, 4),
         # the same algorithmic work over the launch time of the kernel alone on the GPU (one lane)
         "frac_alone": (round(MODMUL_CHAIN * B / (iso_chain_ms * 1e-3) / 1e12 / MODMUL_PEAK_T, 4)
@@ -584,7 +622,8 @@ def main():
         "algorithmic_per_launch": f"{FRONT_BYTES} B/query x {B} queries",
         "avg_launch_ms": round(fd_avg_s * 1e3, 4),
         "measured": "HIP events, untimed single-lane pass (the kernel alone on the GPU)",
-        "avg_launch_ms_timed_region": round(prof[front][0] / max(prof[front][1], 1), 4),
+        # its launches in the profiled repeat of the timed region (beside other lanes' kernels)
+        "avg_launch_ms_timed_region": round(prof_repeat[front][0] / max(prof_repeat[front][1], 1), 4),
         # the device's measured streaming-read rate (16-byte loads over 1 GiB), the practical roof
         "peak_measured": round(RATES["hbm_read_tbs"] * 1e3, 1) if RATES.get("hbm_read_tbs") else None,
         "frac_measured": (round(FRONT_BYTES * B / fd_avg_s / 1e9 / (RATES["hbm_read_tbs"] * 1e3), 4)

@@ -128,18 +128,35 @@ struct ClientPool {
 
 // A device batch's launch sequence captured once and replayed (hipGraph):
 // the key is every value its launches take as arguments (batch shape,
-// caller buffers, workspaces, staging, resident tables).
-struct GraphEntry {
-    std::vector<uint64_t> key;
+// caller buffers, workspaces, staging, resident tables) and which kernels
+// are profiled.  A profiled kernel is a segment of its own, replayed
+// between two events recorded on the stream (events recorded inside a
+// graph do not time its kernels on this runtime); unprofiled, the whole
+// sequence is one segment.
+struct GraphSeg {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    int prof = -1;  // the ProfId this segment's events time, or -1
+};
+struct GraphEntry {
+    std::vector<uint64_t> key;
+    std::vector<GraphSeg> segs;
 };
 struct GraphCache {
     bool enabled = true;
     std::vector<std::vector<uint64_t>> last_eager;  // keys of the last calls run eagerly
     std::vector<GraphEntry> entries;    // least recently used first
     uint64_t replays = 0, captures = 0;
+    // while a profiled sequence is captured: the segments closed so far, and
+    // the first error of a segment boundary (profiling scopes end in
+    // destructors, which report here instead of throwing)
+    bool segmenting = false;
+    std::vector<GraphSeg> pending;
+    hipError_t seg_error = hipSuccess;
 };
+// Ends the capture of the current segment and begins the next: the closed
+// segment is kept (with `prof`, the ProfId it holds, or -1) when it has nodes.
+void graph_segment(inspire_ctx* ctx, int prof) noexcept;
 void graph_cache_clear(GraphCache& G);
 
 struct HostModulus {
@@ -191,6 +208,8 @@ inline inspire_ctx* root(inspire_ctx* c) { return c->owner ? c->owner : c; }
 inline const inspire_ctx* root(const inspire_ctx* c) { return c->owner ? c->owner : c; }
 
 // Records HIP events around the launches in its scope when profiling is on.
+// While a profiled sequence is captured into a graph, the scope is a segment
+// of its own instead (graph_segment at both ends) and records no event.
 class ProfScope {
    public:
     ProfScope(inspire_ctx* ctx, int id);
@@ -200,11 +219,13 @@ class ProfScope {
     inspire_ctx* ctx_;
     int id_;
     hipEvent_t a_ = nullptr;
+    bool seg_ = false;
 };
 // Events for one kernel launch that records them itself
 // (hipExtLaunchKernelGGL): the span is the kernel's execution, without the
 // host's submission time that a scope around a launch on an idle stream
-// includes.  start()/stop() are null when profiling is off.
+// includes.  start()/stop() are null when profiling is off, and while a
+// profiled sequence is captured (the launch is then a segment of its own).
 class ProfLaunch {
    public:
     ProfLaunch(inspire_ctx* ctx, int id);
@@ -216,6 +237,7 @@ class ProfLaunch {
     inspire_ctx* ctx_;
     int id_;
     hipEvent_t a_ = nullptr, b_ = nullptr;
+    bool seg_ = false;
 };
 // Folds finished spans into the totals (synchronises the stream).
 void prof_collect(inspire_ctx* ctx);

@@ -33,6 +33,11 @@ const char* const kProfNames[P_COUNT] = {"rows_ntt", "expand_seeds", "first_dim"
 
 ProfScope::ProfScope(inspire_ctx* ctx, int id) : ctx_(ctx), id_(id) {
     if (!ctx_->prof.on || ctx_->prof.suspended || !((ctx_->prof.mask >> id) & 1u)) return;
+    if (ctx_->graphs.segmenting) {
+        seg_ = true;
+        graph_segment(ctx_, -1);
+        return;
+    }
     auto& P = ctx_->prof;
     if (P.pool.empty()) {
         hipEvent_t e;
@@ -45,6 +50,7 @@ ProfScope::ProfScope(inspire_ctx* ctx, int id) : ctx_(ctx), id_(id) {
 }
 
 ProfScope::~ProfScope() {
+    if (seg_) graph_segment(ctx_, id_);
     if (!a_) return;
     auto& P = ctx_->prof;
     hipEvent_t b;
@@ -60,6 +66,11 @@ ProfScope::~ProfScope() {
 
 ProfLaunch::ProfLaunch(inspire_ctx* ctx, int id) : ctx_(ctx), id_(id) {
     if (!ctx_->prof.on || ctx_->prof.suspended || !((ctx_->prof.mask >> id) & 1u)) return;
+    if (ctx_->graphs.segmenting) {
+        seg_ = true;
+        graph_segment(ctx_, -1);
+        return;
+    }
     auto& P = ctx_->prof;
     hipEvent_t e[2];
     for (int j = 0; j < 2; j++) {
@@ -75,6 +86,7 @@ ProfLaunch::ProfLaunch(inspire_ctx* ctx, int id) : ctx_(ctx), id_(id) {
 }
 
 ProfLaunch::~ProfLaunch() {
+    if (seg_) graph_segment(ctx_, id_);
     if (a_) ctx_->prof.spans.push_back({id_, a_, b_});
 }
 
@@ -96,24 +108,59 @@ void prof_collect(inspire_ctx* ctx) {
     P.spans.clear();
 }
 
-void graph_cache_clear(GraphCache& G) {
-    for (auto& e : G.entries) {
-        if (e.exec) (void)hipGraphExecDestroy(e.exec);
-        if (e.graph) (void)hipGraphDestroy(e.graph);
+static void destroy_segs(std::vector<GraphSeg>& segs) {
+    for (auto& sg : segs) {
+        if (sg.exec) (void)hipGraphExecDestroy(sg.exec);
+        if (sg.graph) (void)hipGraphDestroy(sg.graph);
     }
+    segs.clear();
+}
+
+void graph_cache_clear(GraphCache& G) {
+    for (auto& e : G.entries) destroy_segs(e.segs);
     G.entries.clear();
     G.last_eager.clear();
+}
+
+void graph_segment(inspire_ctx* ctx, int prof) noexcept {
+    GraphCache& G = ctx->graphs;
+    if (G.seg_error != hipSuccess) return;  // the capture is abandoned after the call
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    size_t nodes = 0;
+    if (e == hipSuccess) e = hipGraphGetNodes(g, nullptr, &nodes);
+    if (e == hipSuccess && nodes) G.pending.push_back(GraphSeg{g, nullptr, prof});
+    else if (g) (void)hipGraphDestroy(g);
+    if (e == hipSuccess) e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        G.seg_error = e;
+    }
 }
 
 namespace {
 
 constexpr size_t kGraphCacheEntries = 8;
 
+// Launches a captured sequence; a profiled segment runs between two events
+// recorded on the stream.
+void launch_segs(inspire_ctx* ctx, const GraphEntry& e) {
+    for (const auto& sg : e.segs) {
+        if (sg.prof >= 0) {
+            ProfScope ps(ctx, sg.prof);
+            hip_check(hipGraphLaunch(sg.exec, ctx->stream), "graph launch");
+        } else {
+            hip_check(hipGraphLaunch(sg.exec, ctx->stream), "graph launch");
+        }
+    }
+}
+
 // Runs `enqueue` (every stream operation of one device batch) eagerly, or
 // replays the graph captured for `key`.  A key seen on one of the last
 // eager calls is captured now, so one-off shapes never pay for a capture.  When
-// capture or instantiation is refused the call runs eagerly.  Profiling is
-// off here (profiled calls run eagerly), so no events enter a graph.
+// capture or instantiation is refused the call runs eagerly.  With profiling
+// on, each profiled kernel is captured as a segment of its own (the key
+// carries the profiled set).
 template <typename F>
 void run_with_graphs(inspire_ctx* ctx, const std::vector<uint64_t>& key, F&& enqueue) {
     GraphCache& G = ctx->graphs;
@@ -121,10 +168,9 @@ void run_with_graphs(inspire_ctx* ctx, const std::vector<uint64_t>& key, F&& enq
         if (G.entries[i].key != key) continue;
         GraphEntry e = std::move(G.entries[i]);  // most recently used goes last
         G.entries.erase(G.entries.begin() + (ptrdiff_t)i);
-        const hipError_t le = hipGraphLaunch(e.exec, ctx->stream);
         G.entries.push_back(std::move(e));
-        hip_check(le, "graph launch");
         G.replays++;
+        launch_segs(ctx, G.entries.back());
         return;
     }
     auto seen = std::find(G.last_eager.begin(), G.last_eager.end(), key);
@@ -141,33 +187,52 @@ void run_with_graphs(inspire_ctx* ctx, const std::vector<uint64_t>& key, F&& enq
         enqueue();
         return;
     }
-    hipGraph_t g = nullptr;
+    destroy_segs(G.pending);
+    G.seg_error = hipSuccess;
+    G.segmenting = ctx->prof.on && !ctx->prof.suspended;
     try {
         enqueue();
     } catch (...) {
+        G.segmenting = false;
+        hipGraph_t g = nullptr;
         (void)hipStreamEndCapture(ctx->stream, &g);
         (void)hipGetLastError();
         if (g) (void)hipGraphDestroy(g);
+        destroy_segs(G.pending);
         throw;
     }
-    hip_check(hipStreamEndCapture(ctx->stream, &g), "end capture");
+    G.segmenting = false;
+    graph_segment(ctx, -1);  // closes the last segment (and begins one more, ended here)
+    bool ok = G.seg_error == hipSuccess;
+    {
+        hipGraph_t g = nullptr;
+        if (hipStreamEndCapture(ctx->stream, &g) != hipSuccess) (void)hipGetLastError();
+        if (g) (void)hipGraphDestroy(g);
+    }
     GraphEntry e;
     e.key = key;
-    e.graph = g;
-    if (hipGraphInstantiate(&e.exec, g, nullptr, nullptr, 0) != hipSuccess) {
-        (void)hipGetLastError();
-        (void)hipGraphDestroy(g);
+    e.segs = std::move(G.pending);
+    G.pending.clear();
+    for (auto& sg : e.segs) {
+        if (!ok) break;
+        if (hipGraphInstantiate(&sg.exec, sg.graph, nullptr, nullptr, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            sg.exec = nullptr;
+            ok = false;
+        }
+    }
+    if (!ok) {  // run this call eagerly
+        destroy_segs(e.segs);
         enqueue();
         return;
     }
     if (G.entries.size() >= kGraphCacheEntries) {  // evict the least recently used
-        (void)hipGraphExecDestroy(G.entries.front().exec);
-        (void)hipGraphDestroy(G.entries.front().graph);
+        destroy_segs(G.entries.front().segs);
         G.entries.erase(G.entries.begin());
     }
     G.entries.push_back(std::move(e));
     G.captures++;
-    hip_check(hipGraphLaunch(G.entries.back().exec, ctx->stream), "graph launch");
+    launch_segs(ctx, G.entries.back());
 }
 
 }  // namespace
@@ -791,9 +856,7 @@ void respond_batch_device(inspire_ctx* ctx, const inspire_device_batch& b, uint8
         else pack_run_device_ybody(ctx, db.cols, Q, rows, d_qdig, b.y_body, b.out, pws, y_ws);
         hip_check(hipMemcpyAsync(herr, d_err, 4, hipMemcpyDeviceToHost, ctx->stream), "err");
     };
-    // Events recorded inside a replayed graph do not time its kernels on this
-    // runtime, so a profiled call runs eagerly.
-    if (!scratch && ctx->graphs.enabled && !(ctx->prof.on && !ctx->prof.suspended)) {
+    if (!scratch && ctx->graphs.enabled) {
         // every value the launches take: the batch's shape and buffers, the
         // workspaces (grown by an earlier call of this shape), the resident
         // tables they read, and which kernels are bracketed by profiling events
@@ -806,7 +869,8 @@ void respond_batch_device(inspire_ctx* ctx, const inspire_device_batch& b, uint8
             reinterpret_cast<uint64_t>(ctx->ws2.get()), reinterpret_cast<uint64_t>(ctx->ws3.get()),
             reinterpret_cast<uint64_t>(ctx->ws4.get()), reinterpret_cast<uint64_t>(ctx->ws5.get()),
             reinterpret_cast<uint64_t>(ctx->ws6.get()), reinterpret_cast<uint64_t>(ctx->pin.get()),
-            reinterpret_cast<uint64_t>(R->galois.get()), reinterpret_cast<uint64_t>(R->yhat.get())};
+            reinterpret_cast<uint64_t>(R->galois.get()), reinterpret_cast<uint64_t>(R->yhat.get()),
+            ctx->prof.on && !ctx->prof.suspended ? (uint64_t)ctx->prof.mask | 1ull << 32 : 0ull};
         run_with_graphs(ctx, key, enqueue);
     } else {
         enqueue();

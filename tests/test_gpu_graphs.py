@@ -3,8 +3,9 @@ same batch shape and buffers arriving again are captured into a HIP graph
 and replayed.  Every replay, with new contents in the same buffers and the
 program's own eager work (another shape, uploads, synchronisations) between
 replays, equals the eager launches bit for bit, for InsPIRe^2 (fused front
-end), InsPIRe^0 and tree packing; a profiled batch runs eagerly so that its
-events time the kernels."""
+end), InsPIRe^0 and tree packing.  A profiled batch is captured with the
+profiled kernel as a graph of its own, replayed between two events on the
+stream: it matches too, and its events time about what the eager launch's do."""
 from __future__ import annotations
 
 import sys
@@ -83,16 +84,22 @@ def test_graph_replays_equal_eager(M, variant, mode):
                 caps, reps = ctx.graph_stats()
                 say("variant", variant, "mode", mode, "rep", rep, "captures", caps, "replays", reps)
                 assert caps == rep + 1 and reps == 2 * (rep + 1)
-            # a profiled batch runs eagerly (its events time the kernels) and matches too
+            # profiled batches: eager on the first sighting of the (new) key, then
+            # captured with the profiled kernel as a segment of its own, then replayed
             name = "tree_leaves" if mode else ("pack_chain" if variant else "first_dim")
             ctx.set_profiling(True)
             ctx.set_profiling_kernels([name])
             ctx.profile(name, reset=True)
-            for _ in range(3):
+            assert np.array_equal(run(shards), want)
+            ms_eager, n_eager = ctx.profile(name, reset=True)
+            for _ in range(5):
                 assert np.array_equal(run(shards), want)
             ms, launches = ctx.profile(name, reset=True)
-            say(name, "launches", launches, "ms", ms, "stats", ctx.graph_stats())
-            assert launches == 3 and ms > 0 and ctx.graph_stats() == (4, 8)
+            say(name, "eager ms", ms_eager, "captured + replayed ms", ms, "launches", launches,
+                "stats", ctx.graph_stats())
+            assert n_eager == 1 and launches == 5 and ms > 0 and ctx.graph_stats() == (5, 12)
+            # the segment's events around a graph launch time the same kernel
+            assert 0.5 * ms_eager < ms / launches < 2.0 * ms_eager + 0.05
         finally:
             ctx.set_profiling(False)
             for a in (drows, dseeds, dy, dout, other):
