@@ -354,10 +354,11 @@ def main():
     lanes = [ctx] + [ctx.lane() for _ in range(max(cand + [1]) - 1)]
     louts = [dout] + [M.DeviceArray(B * 2 * K * D * 4) for _ in lanes[1:]]
     # The headline's regime from priming on: HIP events on each lane's stream
-    # bracket the chain kernel of every batch (its roofline), and a recurring
-    # batch replays the HIP graphs captured for it, the chain a graph of its
-    # own between the two events.  Priming, lane tuning, warmup and the timed
-    # region all run this way, so the graphs are captured before the timed region.
+    # time the chain kernel of every batch (its roofline), and a recurring
+    # batch replays the HIP graphs captured for it, the chain launched between
+    # the graphs before and after it (its events are recorded by its own
+    # dispatch).  Priming, lane tuning, warmup and the timed region all run
+    # this way, so the graphs are captured before the timed region.
     def headline_regime(ln, on=True):
         ln.set_profiling_kernels(["pack_chain"])
         ln.set_profiling(on)
@@ -598,8 +599,8 @@ def main():
         "traffic": measured_traffic("k_pack_chain_crt", B),
         "algorithmic_per_launch": f"{MODMUL_CHAIN} modmul/query x {B} queries",
         "avg_launch_ms": round(chain_avg_s * 1e3, 4),
-        "measured": f"HIP events on each lane's stream over the timed region, recorded around the "
-                    f"chain's own graph launch in every batch ({n_lanes} lanes, sharing CUs with the other "
+        "measured": f"HIP events on each lane's stream over the timed region, recorded by the chain's own "
+                    f"dispatch between the batch's graph launches ({n_lanes} lanes, sharing CUs with the other "
                     "lanes' kernels)",
         "timed_region_graphs": headline_graphs,
         "avg_launch_ms_alone": round(iso_chain_ms, 4),

@@ -235,10 +235,12 @@ int inspire_respond_batch_device(inspire_ctx *ctx, const inspire_device_batch *b
  * serving loop that reuses its buffers submits one graph per batch instead
  * of each kernel and copy.  The buffers' contents may change between calls;
  * results are those of the eager launches.  While profiling is enabled
- * (inspire_ctx_set_profiling), each profiled kernel is captured as a graph
- * of its own and replayed between two HIP events recorded on the stream
- * (the profiled set is part of the batch's key).  enable = 0 runs every
- * batch eagerly and drops the captured graphs.  Neither call has a counterpart in
+ * (inspire_ctx_set_profiling), the profiled kernels split the captured
+ * sequence: a kernel whose events its own dispatch records is launched
+ * between the graphs at every replay, other profiled kernels are graphs of
+ * their own between two HIP events recorded on the stream (the profiled set
+ * is part of the batch's key).  enable = 0 runs every batch eagerly and
+ * drops the captured graphs.  Neither call has a counterpart in
  * the reference (which has no device). */
 int inspire_ctx_set_graphs(inspire_ctx *ctx, int enable);
 int inspire_ctx_graph_stats(const inspire_ctx *ctx, uint64_t *captures, uint64_t *replays);

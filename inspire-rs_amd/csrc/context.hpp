@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -129,14 +130,22 @@ struct ClientPool {
 // A device batch's launch sequence captured once and replayed (hipGraph):
 // the key is every value its launches take as arguments (batch shape,
 // caller buffers, workspaces, staging, resident tables) and which kernels
-// are profiled.  A profiled kernel is a segment of its own, replayed
-// between two events recorded on the stream (events recorded inside a
-// graph do not time its kernels on this runtime); unprofiled, the whole
-// sequence is one segment.
+// are profiled.  Events recorded inside a graph do not time its kernels on
+// this runtime, so a profiled kernel splits the sequence: a launch that
+// records its own events (prof_launch) is replayed eagerly between the
+// graph segments before and after it, a profiled scope (ProfScope) is a
+// graph segment of its own between two events recorded on the stream.
+// Unprofiled, the whole sequence is one graph.
+// A kernel launch given the events it records (null: none).
+using LaunchFn = std::function<hipError_t(LaunchEv)>;
 struct GraphSeg {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int prof = -1;  // the ProfId this segment's events time, or -1
+    // a profiled launch that records its own events: kept out of the graphs
+    // and launched between them at every replay (graph and exec are null)
+    LaunchFn eager;
+    const char* what = nullptr;
 };
 struct GraphEntry {
     std::vector<uint64_t> key;
@@ -221,24 +230,14 @@ class ProfScope {
     hipEvent_t a_ = nullptr;
     bool seg_ = false;
 };
-// Events for one kernel launch that records them itself
-// (hipExtLaunchKernelGGL): the span is the kernel's execution, without the
-// host's submission time that a scope around a launch on an idle stream
-// includes.  start()/stop() are null when profiling is off, and while a
-// profiled sequence is captured (the launch is then a segment of its own).
-class ProfLaunch {
-   public:
-    ProfLaunch(inspire_ctx* ctx, int id);
-    ~ProfLaunch();
-    hipEvent_t start() const { return a_; }
-    hipEvent_t stop() const { return b_; }
-
-   private:
-    inspire_ctx* ctx_;
-    int id_;
-    hipEvent_t a_ = nullptr, b_ = nullptr;
-    bool seg_ = false;
-};
+// One kernel launch that records its own events (hipExtLaunchKernelGGL):
+// `launch` enqueues the kernel with the events it is given, null when the
+// kernel is not profiled.  A profiled span is the kernel's execution,
+// without the host's submission time that a scope around a launch on an idle
+// stream includes.  While a profiled sequence is captured into graphs, the
+// launch is not captured: it is kept (with the values it captured) to run
+// between the graph segments at every replay, with fresh events.
+void prof_launch(inspire_ctx* ctx, int id, const char* what, LaunchFn launch);
 // Folds finished spans into the totals (synchronises the stream).
 void prof_collect(inspire_ctx* ctx);
 

@@ -71,8 +71,9 @@ void extract_batch(inspire_ctx* ctx, const inspire_extract_batch& b) {
     A.sk = b.secret_keys;
     A.entries = b.entries;
     {
-        ProfLaunch pl(ctx, P_EXTRACT);
-        hip_check(launch_extract(A, ctx->dev, ctx->stream, pl.start(), pl.stop()), "extract");
+        prof_launch(ctx, P_EXTRACT, "extract", [A, dev = ctx->dev, st = ctx->stream](LaunchEv ev) {
+            return launch_extract(A, dev, st, ev.start, ev.stop);
+        });
     }
     hip_check(hipStreamSynchronize(ctx->stream), "extract sync");
 }

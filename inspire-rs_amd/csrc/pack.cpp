@@ -319,11 +319,13 @@ void pack_run(inspire_ctx* ctx, uint32_t gamma, uint32_t Q, const uint32_t* rows
         d_bpart = reinterpret_cast<uint64_t*>(d_rpre + rpre_words);
     }
     if (T.fast_front) {
-        ProfLaunch pl(ctx, P_FRONT);
-        hip_check(launch_pack_front(d_qdig, rows, d_rpre, d_bpart, T.twist.get<uint32_t>(), T.dftw.get<uint2>(), Q,
-                                    gamma, rw || K == 1, ctx->dev, ctx->stream, LaunchEv{pl.start(), pl.stop()},
-                                    roles, d_out),
-                  "pack front");
+        const uint32_t* twist = T.twist.get<uint32_t>();
+        const uint2* dftw = T.dftw.get<uint2>();
+        const bool both = rw || K == 1;
+        prof_launch(ctx, P_FRONT, "pack front", [=, dev = ctx->dev, st = ctx->stream](LaunchEv ev) {
+            return launch_pack_front(d_qdig, rows, d_rpre, d_bpart, twist, dftw, Q, gamma, both, dev, st, ev, roles,
+                                     d_out);
+        });
     } else {
         uint32_t* d_cols = ws.cols;
         if (!d_cols) {
@@ -393,10 +395,10 @@ void pack_run(inspire_ctx* ctx, uint32_t gamma, uint32_t Q, const uint32_t* rows
         A.q1r64ip = shoup_pre(A.q1r64i, (uint32_t)q1);
         A.one_q1p = shoup_pre(1, (uint32_t)q1);
     }
-    ProfLaunch pl(ctx, P_CHAIN);
-    hip_check(launch_pack_chain_scaled(A, Q, ctx->dev, scale_inv(ctx, nullptr), ctx->stream,
-                                       LaunchEv{pl.start(), pl.stop()}),
-              "pack chain");
+    prof_launch(ctx, P_CHAIN, "pack chain",
+                [A, Q, dev = ctx->dev, sc = scale_inv(ctx, nullptr), st = ctx->stream](LaunchEv ev) {
+                    return launch_pack_chain_scaled(A, Q, dev, sc, st, ev);
+                });
 }
 
 static void upload_meta(inspire_ctx* ctx, uint32_t Q, const std::vector<uint64_t>& y_off,

@@ -64,14 +64,13 @@ ProfScope::~ProfScope() {
     P.spans.push_back({id_, a_, b});
 }
 
-ProfLaunch::ProfLaunch(inspire_ctx* ctx, int id) : ctx_(ctx), id_(id) {
-    if (!ctx_->prof.on || ctx_->prof.suspended || !((ctx_->prof.mask >> id) & 1u)) return;
-    if (ctx_->graphs.segmenting) {
-        seg_ = true;
-        graph_segment(ctx_, -1);
-        return;
-    }
-    auto& P = ctx_->prof;
+static bool prof_wants(const inspire_ctx* ctx, int id) {
+    return ctx->prof.on && !ctx->prof.suspended && ((ctx->prof.mask >> id) & 1u);
+}
+
+// The launch with two events from the pool, its span kept for prof_collect.
+static void timed_launch(inspire_ctx* ctx, int id, const char* what, const LaunchFn& launch) {
+    auto& P = ctx->prof;
     hipEvent_t e[2];
     for (int j = 0; j < 2; j++) {
         if (P.pool.empty()) {
@@ -81,13 +80,33 @@ ProfLaunch::ProfLaunch(inspire_ctx* ctx, int id) : ctx_(ctx), id_(id) {
             P.pool.pop_back();
         }
     }
-    a_ = e[0];
-    b_ = e[1];
+    const hipError_t err = launch(LaunchEv{e[0], e[1]});
+    if (err != hipSuccess) {
+        P.pool.push_back(e[0]);
+        P.pool.push_back(e[1]);
+        hip_check(err, what);
+    }
+    P.spans.push_back({id, e[0], e[1]});
 }
 
-ProfLaunch::~ProfLaunch() {
-    if (seg_) graph_segment(ctx_, id_);
-    if (a_) ctx_->prof.spans.push_back({id_, a_, b_});
+void prof_launch(inspire_ctx* ctx, int id, const char* what, LaunchFn launch) {
+    if (!prof_wants(ctx, id)) {
+        hip_check(launch(LaunchEv{}), what);
+        return;
+    }
+    GraphCache& G = ctx->graphs;
+    if (G.segmenting) {
+        graph_segment(ctx, -1);  // the graph segment before it
+        if (G.seg_error == hipSuccess) {
+            GraphSeg sg;
+            sg.prof = id;
+            sg.eager = std::move(launch);
+            sg.what = what;
+            G.pending.push_back(std::move(sg));
+        }
+        return;
+    }
+    timed_launch(ctx, id, what, launch);
 }
 
 void prof_collect(inspire_ctx* ctx) {
@@ -142,11 +161,15 @@ namespace {
 
 constexpr size_t kGraphCacheEntries = 8;
 
-// Launches a captured sequence; a profiled segment runs between two events
-// recorded on the stream.
+// Launches a captured sequence: graph segments, a profiled one between two
+// events recorded on the stream, and the profiled launches kept out of the
+// graphs, each with its own events.
 void launch_segs(inspire_ctx* ctx, const GraphEntry& e) {
     for (const auto& sg : e.segs) {
-        if (sg.prof >= 0) {
+        if (sg.eager) {
+            if (prof_wants(ctx, sg.prof)) timed_launch(ctx, sg.prof, sg.what, sg.eager);
+            else hip_check(sg.eager(LaunchEv{}), sg.what);
+        } else if (sg.prof >= 0) {
             ProfScope ps(ctx, sg.prof);
             hip_check(hipGraphLaunch(sg.exec, ctx->stream), "graph launch");
         } else {
@@ -159,8 +182,8 @@ void launch_segs(inspire_ctx* ctx, const GraphEntry& e) {
 // replays the graph captured for `key`.  A key seen on one of the last
 // eager calls is captured now, so one-off shapes never pay for a capture.  When
 // capture or instantiation is refused the call runs eagerly.  With profiling
-// on, each profiled kernel is captured as a segment of its own (the key
-// carries the profiled set).
+// on, the profiled kernels split the capture into segments (the key carries
+// the profiled set).
 template <typename F>
 void run_with_graphs(inspire_ctx* ctx, const std::vector<uint64_t>& key, F&& enqueue) {
     GraphCache& G = ctx->graphs;
@@ -215,6 +238,7 @@ void run_with_graphs(inspire_ctx* ctx, const std::vector<uint64_t>& key, F&& enq
     G.pending.clear();
     for (auto& sg : e.segs) {
         if (!ok) break;
+        if (!sg.graph) continue;  // a launch kept out of the graphs
         if (hipGraphInstantiate(&sg.exec, sg.graph, nullptr, nullptr, 0) != hipSuccess) {
             (void)hipGetLastError();
             sg.exec = nullptr;
@@ -487,10 +511,11 @@ void stage_rows_from(inspire_ctx* ctx, uint32_t Q, uint32_t S, uint32_t* rows_ws
     for (uint32_t k = 0; k < ctx->K; k++) mont[k] = mont_of(ctx, k);
     const ScaleDev sc = scale_fwd(ctx, mont);
     if (S && ctx->logn == 11) {  // expansion and both transforms in one pass
-        ProfLaunch pl(ctx, P_EXPAND);
-        hip_check(launch_rows_seeded(seeds, bsrc, rows_ws, (size_t)S * ell, d_err, ctx->dev, sc, ctx->stream,
-                                     LaunchEv{pl.start(), pl.stop()}, a_res ? a_res : ctx->K, a_full),
-                  "seeded rows");
+        const size_t count = (size_t)S * ell;
+        const uint32_t ka = a_res ? a_res : ctx->K;
+        prof_launch(ctx, P_EXPAND, "seeded rows", [=, dev = ctx->dev, st = ctx->stream](LaunchEv ev) {
+            return launch_rows_seeded(seeds, bsrc, rows_ws, count, d_err, dev, sc, st, ev, ka, a_full);
+        });
     } else if (S) {
         {
             ProfScope ps(ctx, P_EXPAND);

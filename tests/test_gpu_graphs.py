@@ -3,9 +3,11 @@ same batch shape and buffers arriving again are captured into a HIP graph
 and replayed.  Every replay, with new contents in the same buffers and the
 program's own eager work (another shape, uploads, synchronisations) between
 replays, equals the eager launches bit for bit, for InsPIRe^2 (fused front
-end), InsPIRe^0 and tree packing.  A profiled batch is captured with the
-profiled kernel as a graph of its own, replayed between two events on the
-stream: it matches too, and its events time about what the eager launch's do."""
+end), InsPIRe^0 and tree packing.  A profiled batch is captured in segments
+split at the profiled kernel (the chain launched between the graphs with its
+own events; the first dimension and the tree leaves graphs of their own
+between two events on the stream): it matches too, and its events time
+about what the eager launch's do."""
 from __future__ import annotations
 
 import sys
@@ -98,7 +100,7 @@ def test_graph_replays_equal_eager(M, variant, mode):
             say(name, "eager ms", ms_eager, "captured + replayed ms", ms, "launches", launches,
                 "stats", ctx.graph_stats())
             assert n_eager == 1 and launches == 5 and ms > 0 and ctx.graph_stats() == (5, 12)
-            # the segment's events around a graph launch time the same kernel
+            # the replayed kernel's events time about what the eager launch's do
             assert 0.5 * ms_eager < ms / launches < 2.0 * ms_eager + 0.05
         finally:
             ctx.set_profiling(False)
