@@ -255,6 +255,9 @@ def main():
                     help="where the tuning report (kernel variants and lanes) is written; with --reuse-tuning, "
                          "read from instead of measuring")
     ap.add_argument("--reuse-tuning", action="store_true")
+    ap.add_argument("--chain-window-us", type=int, default=-1,
+                    help="inspire_ctx_set_chain_window: chains of lanes submitted within this many microseconds "
+                         "of each other run one after another; -1 (default): the library's default")
     ap.add_argument("--lanes-sweep", type=str, default="1,3",
                     help="other lane counts timed after the headline (reported as lanes_sweep)")
     ap.add_argument("--db-log2", type=int, default=26)
@@ -305,6 +308,8 @@ def main():
     log(f"generating {n_entries} x 32 B database")
     db = np.frombuffer(rng.bytes(n_entries * 32), dtype=np.uint8)
     ctx = M.Context(M.InspireParams(), w_seed)
+    if args.chain_window_us >= 0:
+        ctx.set_chain_window(args.chain_window_us)
     # Kernel variants (matrix-core vs VALU forward NTT, packing-chain width):
     # timed on the device by the library's tuner, the faster kept.
     prior = None

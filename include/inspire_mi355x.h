@@ -243,6 +243,18 @@ int inspire_respond_batch_device(inspire_ctx *ctx, const inspire_device_batch *b
  * drops the captured graphs.  Neither call has a counterpart in
  * the reference (which has no device). */
 int inspire_ctx_set_graphs(inspire_ctx *ctx, int enable);
+/* Packing chains of lanes submitted together (set on the owner context;
+ * default 60 us).  The packing chain of an InspiRING batch fills every
+ * compute unit with one workgroup per query.  Lanes whose batches arrive at
+ * the same moment run their chains at the same time, each beside the
+ * others', then their row expansions and front ends at the same time, and
+ * tend to stay in step batch after batch.  A chain launch submitted less
+ * than window_us after the previous one, from another lane of the same
+ * owner, therefore waits for it on the GPU (a stream wait on an event, no
+ * host synchronisation), so that the lanes fall out of step.  Chains
+ * submitted further apart are not ordered.  window_us = 0 turns the
+ * ordering off.  No counterpart in the reference. */
+int inspire_ctx_set_chain_window(inspire_ctx *ctx, uint32_t window_us);
 int inspire_ctx_graph_stats(const inspire_ctx *ctx, uint64_t *captures, uint64_t *replays);
 
 /* inspire_respond_batch_device without any allocation: no hipMalloc,

@@ -3,6 +3,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+#include <chrono>
 #include <cstdint>
 #include <functional>
 #include <map>
@@ -210,6 +212,16 @@ struct inspire_ctx {
     std::vector<uint32_t> hbuf;
     inspire::Profiler prof;
     inspire::GraphCache graphs;
+    // Packing chains of an owner and its lanes submitted within chain_window_us
+    // of each other run one after another: the later launch waits on the
+    // event recorded after the earlier one, whichever stream ran it.
+    // chain_done is this context's; the owner keeps the last one recorded and
+    // when its chain was submitted.
+    hipEvent_t chain_done = nullptr;
+    std::mutex chain_mu;  // owner: guards last_chain, last_submit
+    hipEvent_t last_chain = nullptr;
+    std::chrono::steady_clock::time_point last_submit{};
+    std::atomic<uint32_t> chain_window_us{60};  // owner: 0 turns the ordering off
 };
 
 namespace inspire {
@@ -237,7 +249,13 @@ class ProfScope {
 // stream includes.  While a profiled sequence is captured into graphs, the
 // launch is not captured: it is kept (with the values it captured) to run
 // between the graph segments at every replay, with fresh events.
-void prof_launch(inspire_ctx* ctx, int id, const char* what, LaunchFn launch);
+// ordered: a packing-chain launch, run after a chain of the owner's other
+// lanes submitted just before it (inspire_ctx::chain_window_us); while a
+// sequence is captured into graphs it is kept out of them like a profiled
+// launch.
+void prof_launch(inspire_ctx* ctx, int id, const char* what, LaunchFn launch, bool ordered = false);
+// The chain ordering applies to this context's calls.
+bool chain_order_on(const inspire_ctx* ctx);
 // Folds finished spans into the totals (synchronises the stream).
 void prof_collect(inspire_ctx* ctx);
 

@@ -159,6 +159,7 @@ inspire_ctx* make_lane_ctx(inspire_ctx* owner) {
     c->device = owner->device;
     c->owner = owner;
     hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "lane stream");
+    hip_check(hipEventCreateWithFlags(&c->chain_done, hipEventDisableTiming), "chain event");
     return c.release();
 }
 
@@ -325,6 +326,7 @@ int inspire_ctx_create(const inspire_params* params, const uint8_t w_seed[32], i
             }
         }
         hip_check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "stream create");
+        hip_check(hipEventCreateWithFlags(&ctx->chain_done, hipEventDisableTiming), "chain event");
         *out = ctx.release();
         return INSPIRE_OK;
     })
@@ -339,6 +341,14 @@ void inspire_ctx_destroy(inspire_ctx* ctx) {
         (void)hipEventDestroy(s.b);
     }
     for (auto e : ctx->prof.pool) (void)hipEventDestroy(e);
+    if (ctx->chain_done) {
+        inspire_ctx* R = root(ctx);
+        {
+            std::lock_guard<std::mutex> lk(R->chain_mu);
+            if (R->last_chain == ctx->chain_done) R->last_chain = nullptr;
+        }
+        (void)hipEventDestroy(ctx->chain_done);
+    }
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -379,6 +389,17 @@ int inspire_ctx_set_graphs(inspire_ctx* ctx, int enable) {
         std::lock_guard<std::recursive_mutex> g(ctx->mu);
         graph_cache_clear(ctx->graphs);
         ctx->graphs.enabled = enable != 0;
+        return INSPIRE_OK;
+    })
+}
+
+int inspire_ctx_set_chain_window(inspire_ctx* ctx, uint32_t window_us) {
+    INSPIRE_TRY({
+        if (!ctx) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "null argument"};
+        if (ctx->owner) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "the chain window is set on the owner context"};
+        std::lock_guard<std::mutex> lk(ctx->chain_mu);
+        ctx->chain_window_us = window_us;
+        ctx->last_chain = nullptr;
         return INSPIRE_OK;
     })
 }

@@ -130,6 +130,14 @@ GenInfo gen_info(uint32_t n, uint32_t gamma) {  // inspiring2.rs:114-128
 
 }  // namespace
 
+// Copies a table to the device on the calling context's stream (a lane may
+// build tables while another lane captures a graph, and this runtime refuses
+// null-stream copies while any capture is open).
+static void upload_table(inspire_ctx* ctx, const DevBuf& dst, const void* src, size_t bytes, const char* what) {
+    hip_check(hipMemcpyAsync(dst.get(), src, bytes, hipMemcpyHostToDevice, ctx->stream), what);
+    hip_check(hipStreamSynchronize(ctx->stream), what);
+}
+
 PackTables& pack_tables(inspire_ctx* ctx, uint32_t gamma) {
     // The map is shared by the owner and its lanes: lookups and the first
     // build of a gamma's tables happen under its own lock.  Entries are never
@@ -237,8 +245,8 @@ PackTables& pack_tables(inspire_ctx* ctx, uint32_t gamma) {
             }
             T->twist.alloc(twist.size() * 4);
             T->dftw.alloc(dftw.size() * 4);
-            hip_check(hipMemcpy(T->twist.get(), twist.data(), twist.size() * 4, hipMemcpyHostToDevice), "twist");
-            hip_check(hipMemcpy(T->dftw.get(), dftw.data(), dftw.size() * 4, hipMemcpyHostToDevice), "dftw");
+            upload_table(ctx, T->twist, twist.data(), twist.size() * 4, "twist");
+            upload_table(ctx, T->dftw, dftw.data(), dftw.size() * 4, "dftw");
         }
     }
     if (!T->fast_front) {
@@ -255,12 +263,12 @@ PackTables& pack_tables(inspire_ctx* ctx, uint32_t gamma) {
             }
         }
         T->stepu.alloc(stepu.size() * 4);
-        hip_check(hipMemcpy(T->stepu.get(), stepu.data(), stepu.size() * 4, hipMemcpyHostToDevice), "step1 u");
+        upload_table(ctx, T->stepu, stepu.data(), stepu.size() * 4, "step1 u");
     }
     T->perm.alloc(perm.size() * 4);
     T->w_all.alloc(w_all.size() * 4);
-    hip_check(hipMemcpy(T->perm.get(), perm.data(), perm.size() * 4, hipMemcpyHostToDevice), "perm");
-    hip_check(hipMemcpy(T->w_all.get(), w_all.data(), w_all.size() * 4, hipMemcpyHostToDevice), "w_all");
+    upload_table(ctx, T->perm, perm.data(), perm.size() * 4, "perm");
+    upload_table(ctx, T->w_all, w_all.data(), w_all.size() * 4, "w_all");
     T->gen_pows = G.gen_pows;
     for (uint32_t i = 0; i + 1 < gamma; i++) T->odd_gens &= (G.gen_pows[i] & 1) != 0;
     auto& ref = *T;
@@ -395,10 +403,12 @@ void pack_run(inspire_ctx* ctx, uint32_t gamma, uint32_t Q, const uint32_t* rows
         A.q1r64ip = shoup_pre(A.q1r64i, (uint32_t)q1);
         A.one_q1p = shoup_pre(1, (uint32_t)q1);
     }
-    prof_launch(ctx, P_CHAIN, "pack chain",
-                [A, Q, dev = ctx->dev, sc = scale_inv(ctx, nullptr), st = ctx->stream](LaunchEv ev) {
-                    return launch_pack_chain_scaled(A, Q, dev, sc, st, ev);
-                });
+    prof_launch(
+        ctx, P_CHAIN, "pack chain",
+        [A, Q, dev = ctx->dev, sc = scale_inv(ctx, nullptr), st = ctx->stream](LaunchEv ev) {
+            return launch_pack_chain_scaled(A, Q, dev, sc, st, ev);
+        },
+        true);
 }
 
 static void upload_meta(inspire_ctx* ctx, uint32_t Q, const std::vector<uint64_t>& y_off,

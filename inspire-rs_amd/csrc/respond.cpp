@@ -89,13 +89,41 @@ static void timed_launch(inspire_ctx* ctx, int id, const char* what, const Launc
     P.spans.push_back({id, e[0], e[1]});
 }
 
-void prof_launch(inspire_ctx* ctx, int id, const char* what, LaunchFn launch) {
-    if (!prof_wants(ctx, id)) {
-        hip_check(launch(LaunchEv{}), what);
-        return;
-    }
+bool chain_order_on(const inspire_ctx* ctx) { return root(ctx)->chain_window_us.load() > 0 && ctx->chain_done; }
+
+// `launch`, after the owner's previous chain launch when that one was
+// submitted from another context less than the window before (a wait on its
+// event in this stream, no host synchronisation); this context's event is
+// then recorded as the last.  Lanes that submit together would otherwise run
+// their chains together, each beside the others' (every workgroup of a chain
+// fills a compute unit), and keep doing so batch after batch, their front
+// ends idle meanwhile; chains submitted further apart overlap freely.
+static LaunchFn in_chain_order(inspire_ctx* ctx, LaunchFn launch) {
+    return [ctx, launch = std::move(launch)](LaunchEv ev) -> hipError_t {
+        inspire_ctx* R = root(ctx);
+        std::lock_guard<std::mutex> lk(R->chain_mu);
+        const auto now = std::chrono::steady_clock::now();
+        const auto window = std::chrono::microseconds(R->chain_window_us.load());
+        if (R->last_chain && R->last_chain != ctx->chain_done && now - R->last_submit < window) {
+            const hipError_t e = hipStreamWaitEvent(ctx->stream, R->last_chain, 0);
+            if (e != hipSuccess) return e;
+        }
+        hipError_t e = launch(ev);
+        if (e == hipSuccess) e = hipEventRecord(ctx->chain_done, ctx->stream);
+        if (e == hipSuccess) {
+            R->last_chain = ctx->chain_done;
+            R->last_submit = now;
+        }
+        return e;
+    };
+}
+
+void prof_launch(inspire_ctx* ctx, int id, const char* what, LaunchFn launch, bool ordered) {
+    ordered = ordered && chain_order_on(ctx);
+    if (ordered) launch = in_chain_order(ctx, std::move(launch));
+    const bool timed = prof_wants(ctx, id);
     GraphCache& G = ctx->graphs;
-    if (G.segmenting) {
+    if (G.segmenting && (timed || ordered)) {
         graph_segment(ctx, -1);  // the graph segment before it
         if (G.seg_error == hipSuccess) {
             GraphSeg sg;
@@ -106,7 +134,8 @@ void prof_launch(inspire_ctx* ctx, int id, const char* what, LaunchFn launch) {
         }
         return;
     }
-    timed_launch(ctx, id, what, launch);
+    if (timed) timed_launch(ctx, id, what, launch);
+    else hip_check(launch(LaunchEv{}), what);
 }
 
 void prof_collect(inspire_ctx* ctx) {
@@ -141,6 +170,13 @@ void graph_cache_clear(GraphCache& G) {
     G.last_eager.clear();
 }
 
+// Thread-local: other threads' calls are not restricted by this capture.
+// (This runtime still refuses null-stream operations of other threads while
+// any capture is open, with hipErrorStreamCaptureImplicit; callers that copy
+// concurrently use streams of their own.  In relaxed mode such a copy crashed
+// the runtime instead.)
+constexpr hipStreamCaptureMode kCaptureMode = hipStreamCaptureModeThreadLocal;
+
 void graph_segment(inspire_ctx* ctx, int prof) noexcept {
     GraphCache& G = ctx->graphs;
     if (G.seg_error != hipSuccess) return;  // the capture is abandoned after the call
@@ -150,7 +186,7 @@ void graph_segment(inspire_ctx* ctx, int prof) noexcept {
     if (e == hipSuccess) e = hipGraphGetNodes(g, nullptr, &nodes);
     if (e == hipSuccess && nodes) G.pending.push_back(GraphSeg{g, nullptr, prof});
     else if (g) (void)hipGraphDestroy(g);
-    if (e == hipSuccess) e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) e = hipStreamBeginCapture(ctx->stream, kCaptureMode);
     if (e != hipSuccess) {
         (void)hipGetLastError();
         G.seg_error = e;
@@ -166,7 +202,7 @@ constexpr size_t kGraphCacheEntries = 8;
 // graphs, each with its own events.
 void launch_segs(inspire_ctx* ctx, const GraphEntry& e) {
     for (const auto& sg : e.segs) {
-        if (sg.eager) {
+        if (sg.eager) {  // a profiled or ordered launch
             if (prof_wants(ctx, sg.prof)) timed_launch(ctx, sg.prof, sg.what, sg.eager);
             else hip_check(sg.eager(LaunchEv{}), sg.what);
         } else if (sg.prof >= 0) {
@@ -204,7 +240,7 @@ void run_with_graphs(inspire_ctx* ctx, const std::vector<uint64_t>& key, F&& enq
         return;
     }
     G.last_eager.erase(seen);
-    if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    if (hipStreamBeginCapture(ctx->stream, kCaptureMode) != hipSuccess) {
         (void)hipGetLastError();
         G.enabled = false;  // this runtime cannot capture: stay eager
         enqueue();
@@ -212,17 +248,21 @@ void run_with_graphs(inspire_ctx* ctx, const std::vector<uint64_t>& key, F&& enq
     }
     destroy_segs(G.pending);
     G.seg_error = hipSuccess;
-    G.segmenting = ctx->prof.on && !ctx->prof.suspended;
+    G.segmenting = (ctx->prof.on && !ctx->prof.suspended) || chain_order_on(ctx);
     try {
         enqueue();
     } catch (...) {
+        // the capture failed (invalidated, or an operation refused while it
+        // was open): drop it and run this call eagerly, which reports any
+        // error of its own
         G.segmenting = false;
         hipGraph_t g = nullptr;
         (void)hipStreamEndCapture(ctx->stream, &g);
         (void)hipGetLastError();
         if (g) (void)hipGraphDestroy(g);
         destroy_segs(G.pending);
-        throw;
+        enqueue();
+        return;
     }
     G.segmenting = false;
     graph_segment(ctx, -1);  // closes the last segment (and begins one more, ended here)
@@ -895,7 +935,8 @@ void respond_batch_device(inspire_ctx* ctx, const inspire_device_batch& b, uint8
             reinterpret_cast<uint64_t>(ctx->ws4.get()), reinterpret_cast<uint64_t>(ctx->ws5.get()),
             reinterpret_cast<uint64_t>(ctx->ws6.get()), reinterpret_cast<uint64_t>(ctx->pin.get()),
             reinterpret_cast<uint64_t>(R->galois.get()), reinterpret_cast<uint64_t>(R->yhat.get()),
-            ctx->prof.on && !ctx->prof.suspended ? (uint64_t)ctx->prof.mask | 1ull << 32 : 0ull};
+            (ctx->prof.on && !ctx->prof.suspended ? (uint64_t)ctx->prof.mask | 1ull << 32 : 0ull) |
+                (chain_order_on(ctx) ? 1ull << 33 : 0ull)};
         run_with_graphs(ctx, key, enqueue);
     } else {
         enqueue();

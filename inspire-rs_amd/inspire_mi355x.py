@@ -217,6 +217,7 @@ EXPORTS = [
     ("inspire_server_get_stats", C.c_int, [C.c_void_p, C.POINTER(_ServerStats)]),
     ("inspire_server_destroy", None, [C.c_void_p]),
     ("inspire_ctx_set_profiling", C.c_int, [C.c_void_p, C.c_int]),
+    ("inspire_ctx_set_chain_window", C.c_int, [C.c_void_p, C.c_uint32]),
     ("inspire_ctx_set_profiling_kernels", C.c_int, [C.c_void_p, C.c_char_p]),
     (
         "inspire_ctx_profile",
@@ -620,6 +621,10 @@ class Context:
         _raise(lib().inspire_ctx_graph_stats(self._h, C.byref(a), C.byref(b)))
         return a.value, b.value
 
+    def set_chain_window(self, window_us: int):
+        """A chain launch submitted within window_us of another lane's waits for it (0: off)."""
+        _raise(lib().inspire_ctx_set_chain_window(self._h, int(window_us)))
+
     def set_profiling(self, on: bool):
         _raise(lib().inspire_ctx_set_profiling(self._h, 1 if on else 0))
 
@@ -784,7 +789,11 @@ class Server:
 
 class DeviceArray:
     """A device allocation made through the HIP runtime the library uses
-    (for inspire_respond_batch_device inputs and outputs)."""
+    (for inspire_respond_batch_device inputs and outputs).  Copies are
+    synchronous null-stream copies: while a lane captures a batch into a
+    graph on another thread, this runtime refuses them
+    (hipErrorStreamCaptureImplicit), so copy before or after concurrent
+    batches, not during them."""
 
     _hip = None
 
@@ -798,6 +807,12 @@ class DeviceArray:
             h.hipFree.argtypes = [C.c_void_p]
             cls._hip = h
         return cls._hip
+
+    @classmethod
+    def _copy(cls, dst: int, src: int, nbytes: int, kind: int, what: str):
+        rc = cls.hip().hipMemcpy(dst, src, nbytes, kind)
+        if rc != 0:
+            raise PirError(ERR_DEVICE, f"hipMemcpy {what} failed: {rc}")
 
     def __init__(self, nbytes: int):
         p = C.c_void_p()
@@ -817,15 +832,11 @@ class DeviceArray:
     def upload(self, a: np.ndarray):
         a = np.ascontiguousarray(a)
         assert a.nbytes <= self.nbytes
-        rc = self.hip().hipMemcpy(self.ptr, a.ctypes.data, a.nbytes, 1)
-        if rc != 0:
-            raise PirError(ERR_DEVICE, f"hipMemcpy H2D failed: {rc}")
+        self._copy(self.ptr, a.ctypes.data, a.nbytes, 1, "H2D")
 
     def download(self, a: np.ndarray) -> np.ndarray:
         assert a.flags["C_CONTIGUOUS"] and a.nbytes <= self.nbytes
-        rc = self.hip().hipMemcpy(a.ctypes.data, self.ptr, a.nbytes, 2)
-        if rc != 0:
-            raise PirError(ERR_DEVICE, f"hipMemcpy D2H failed: {rc}")
+        self._copy(a.ctypes.data, self.ptr, a.nbytes, 2, "D2H")
         return a
 
     def free(self):
