@@ -104,17 +104,22 @@ static LaunchFn in_chain_order(inspire_ctx* ctx, LaunchFn launch) {
         std::lock_guard<std::mutex> lk(R->chain_mu);
         const auto now = std::chrono::steady_clock::now();
         const auto window = std::chrono::microseconds(R->chain_window_us.load());
-        if (R->last_chain && R->last_chain != ctx->chain_done && now - R->last_submit < window) {
-            const hipError_t e = hipStreamWaitEvent(ctx->stream, R->last_chain, 0);
-            if (e != hipSuccess) return e;
-        }
-        hipError_t e = launch(ev);
-        if (e == hipSuccess) e = hipEventRecord(ctx->chain_done, ctx->stream);
-        if (e == hipSuccess) {
+        // The order only schedules: when this runtime refuses the wait (it does
+        // while the stream the event was recorded on is capturing a graph), or
+        // the record, the chain runs unordered and the result is the same.
+        if (R->last_chain && R->last_chain != ctx->chain_done && now - R->last_submit < window &&
+            hipStreamWaitEvent(ctx->stream, R->last_chain, 0) != hipSuccess)
+            (void)hipGetLastError();
+        const hipError_t e = launch(ev);
+        if (e != hipSuccess) return e;
+        if (hipEventRecord(ctx->chain_done, ctx->stream) == hipSuccess) {
             R->last_chain = ctx->chain_done;
             R->last_submit = now;
+        } else {
+            (void)hipGetLastError();
+            R->last_chain = nullptr;
         }
-        return e;
+        return hipSuccess;
     };
 }
 
