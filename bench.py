@@ -657,6 +657,7 @@ def main():
             "entry_bytes": 32,
             "ring_dim": D,
             "gadget": "2^20 x 3",
+            "chain_window_us": ctx.chain_window(),
         },
         "p50_ms": round(percentile(lat, 50), 3),
         "p99_ms": round(percentile(lat, 99), 3),

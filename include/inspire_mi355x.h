@@ -255,6 +255,8 @@ int inspire_ctx_set_graphs(inspire_ctx *ctx, int enable);
  * submitted further apart are not ordered.  window_us = 0 turns the
  * ordering off.  No counterpart in the reference. */
 int inspire_ctx_set_chain_window(inspire_ctx *ctx, uint32_t window_us);
+/* The owner's current window (a lane reports its owner's). */
+int inspire_ctx_chain_window(const inspire_ctx *ctx, uint32_t *window_us);
 int inspire_ctx_graph_stats(const inspire_ctx *ctx, uint64_t *captures, uint64_t *replays);
 
 /* inspire_respond_batch_device without any allocation: no hipMalloc,

@@ -404,6 +404,14 @@ int inspire_ctx_set_chain_window(inspire_ctx* ctx, uint32_t window_us) {
     })
 }
 
+int inspire_ctx_chain_window(const inspire_ctx* ctx, uint32_t* window_us) {
+    INSPIRE_TRY({
+        if (!ctx || !window_us) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "null argument"};
+        *window_us = root(ctx)->chain_window_us.load();
+        return INSPIRE_OK;
+    })
+}
+
 int inspire_ctx_graph_stats(const inspire_ctx* ctx, uint64_t* captures, uint64_t* replays) {
     INSPIRE_TRY({
         if (!ctx || !captures || !replays) throw Error{INSPIRE_ERR_INVALID_ARGUMENT, "null argument"};

@@ -218,6 +218,7 @@ EXPORTS = [
     ("inspire_server_destroy", None, [C.c_void_p]),
     ("inspire_ctx_set_profiling", C.c_int, [C.c_void_p, C.c_int]),
     ("inspire_ctx_set_chain_window", C.c_int, [C.c_void_p, C.c_uint32]),
+    ("inspire_ctx_chain_window", C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
     ("inspire_ctx_set_profiling_kernels", C.c_int, [C.c_void_p, C.c_char_p]),
     (
         "inspire_ctx_profile",
@@ -624,6 +625,12 @@ class Context:
     def set_chain_window(self, window_us: int):
         """A chain launch submitted within window_us of another lane's waits for it (0: off)."""
         _raise(lib().inspire_ctx_set_chain_window(self._h, int(window_us)))
+
+    def chain_window(self) -> int:
+        """The owner's chain window in microseconds."""
+        w = C.c_uint32()
+        _raise(lib().inspire_ctx_chain_window(self._h, C.byref(w)))
+        return int(w.value)
 
     def set_profiling(self, on: bool):
         _raise(lib().inspire_ctx_set_profiling(self._h, 1 if on else 0))

@@ -114,7 +114,9 @@ def test_chain_window_keeps_every_response(window_us):
     per = 2 * K * d
     with M.Context(M.InspireParams(), W_SEED) as ctx:
         ctx.load_database(0, shards)
+        assert ctx.chain_window() == 60  # the default
         ctx.set_chain_window(window_us)
+        assert ctx.chain_window() == window_us
         dy = M.DeviceArray.from_numpy(np.stack([yb.astype(np.uint32)] * n))
         dev, want = [], []
         for _ in range(L):
@@ -158,6 +160,7 @@ def test_chain_window_keeps_every_response(window_us):
                     got = outs[j][r].download(np.zeros((n, per), np.uint32))
                     assert np.array_equal(got, want[j]), (rep, j, r)
                 assert lanes[j].graph_stats()[1] >= 1
+            assert lanes[1].chain_window() == window_us  # a lane reports its owner's
             with pytest.raises(M.PirError) as e:
                 lanes[0].set_chain_window(0)
             assert e.value.code == M.ERR_INVALID_ARGUMENT
