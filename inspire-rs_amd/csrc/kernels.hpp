@@ -26,6 +26,10 @@ struct BufU32 {
     __device__ __forceinline__ uint32_t ld(uint32_t vb, uint32_t off) const {
         return __builtin_amdgcn_raw_buffer_load_b32(r, vb, off * 4u, 0);
     }
+    // the same, non-temporal (data read once: kept out of the caches' way)
+    __device__ __forceinline__ uint32_t ldnt(uint32_t vb, uint32_t off) const {
+        return __builtin_amdgcn_raw_buffer_load_b32(r, vb, off * 4u, 2);
+    }
     __device__ __forceinline__ void st(uint32_t vb, uint32_t off, uint32_t v) const {
         __builtin_amdgcn_raw_buffer_store_b32(v, r, vb, off * 4u, 0);
     }

@@ -131,7 +131,7 @@ __device__ __forceinline__ void front_residue(int k, const BufU32& R, const BufU
 #pragma unroll
         for (int jj = 0; jj < CH; jj++)
 #pragma unroll
-            for (int i = 0; i < ELL; i++) dv[jj][i] = Dq.ld(sb, (uint32_t)(((j0 + jj) * ELL + i) * F.K + k) * n);
+            for (int i = 0; i < ELL; i++) dv[jj][i] = Dq.ldnt(sb, (uint32_t)(((j0 + jj) * ELL + i) * F.K + k) * n);
 #pragma unroll
         for (int jj = 0; jj < CH; jj++) {
             const int j = j0 + jj;
@@ -217,7 +217,7 @@ __device__ __forceinline__ void front_plain(const BufU32& R, const BufU32& Dq, u
 #pragma unroll
             for (int jj = 0; jj < CH; jj++)
 #pragma unroll
-                for (int i = 0; i < ELL; i++) dv[jj][i] = Dq.ld(sb, (uint32_t)(((j0 + jj) * ELL + i) * F.K + k) * n);
+                for (int i = 0; i < ELL; i++) dv[jj][i] = Dq.ldnt(sb, (uint32_t)(((j0 + jj) * ELL + i) * F.K + k) * n);
 #pragma unroll
             for (int jj = 0; jj < CH; jj++) {
                 uint64_t aa = 0, bb = 0;

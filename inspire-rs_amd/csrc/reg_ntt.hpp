@@ -377,6 +377,20 @@ __device__ __forceinline__ void load_slots_v(const uint32_t* p, int t, uint32_t 
     }
 }
 
+// The same, non-temporal: data read once (kept out of the caches' way).
+template <int V>
+__device__ __forceinline__ void load_slots_v_nt(const uint32_t* p, int t, uint32_t (&v)[V]) {
+    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+    for (int g = 0; g < V / 4; g++) {
+        const u32x4 a = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p + V * t + 4 * g));
+        v[4 * g + 0] = a.x;
+        v[4 * g + 1] = a.y;
+        v[4 * g + 2] = a.z;
+        v[4 * g + 3] = a.w;
+    }
+}
+
 template <int LOGN>
 struct Lay4;
 template <int LOGN, int ELL, bool SGN>

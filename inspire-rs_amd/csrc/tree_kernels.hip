@@ -316,7 +316,7 @@ k_tree_leaves(TreeArgs A, CtxDev C, ScaleDev inv) {
     uint32_t dv[ELL][kTreeV], ra[ELL][kTreeV], rb[ELL][kTreeV];
 #pragma unroll
     for (int j = 0; j < ELL; j++) {
-        load_slots_v<V>(dig + (size_t)j * KN, t, dv[j]);
+        load_slots_v_nt<V>(dig + (size_t)j * KN, t, dv[j]);
         load_slots_v<V>(rw + (size_t)j * 2 * KN, t, ra[j]);
         load_slots_v<V>(rw + (size_t)j * 2 * KN + KN, t, rb[j]);
     }
