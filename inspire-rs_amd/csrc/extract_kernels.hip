@@ -76,10 +76,10 @@ __global__ void __launch_bounds__(kThreads, (K << LOGN) <= 4096 ? 4 : 2) k_extra
     // modes), so their latencies overlap
     uint32_t sv[SPT], av[TPL];
 #pragma unroll
-    for (uint32_t j = 0; j < SPT; j++) sv[j] = s[threadIdx.x + j * kThreads];
+    for (uint32_t j = 0; j < SPT; j++) sv[j] = __builtin_nontemporal_load(s + threadIdx.x + j * kThreads);
     if constexpr (!COLS) {
 #pragma unroll
-        for (uint32_t r = 0; r < TPL; r++) av[r] = resp[(size_t)k * N + lo + lane + 64 * r];
+        for (uint32_t r = 0; r < TPL; r++) av[r] = __builtin_nontemporal_load(resp + (size_t)k * N + lo + lane + 64 * r);
     }
 #pragma unroll
     for (uint32_t j = 0; j < SPT; j++) {
