@@ -471,7 +471,7 @@ k_rows_seeded(const uint32_t* __restrict__ seeds, const uint32_t* __restrict__ b
 #pragma unroll
             for (int e = 0; e < 8; e++) {
                 x[0][e] = poly[2 * kk][lds_ix(tid + e * S::T)];
-                x[1][e] = b[(size_t)kk * N + tid + e * S::T];
+                x[1][e] = __builtin_nontemporal_load(b + (size_t)kk * N + tid + e * S::T);
             }
             // every thread holds its a values and is past the previous transform's
             // exchange reads before buffers kk, kk + 1 are written
@@ -483,7 +483,7 @@ k_rows_seeded(const uint32_t* __restrict__ seeds, const uint32_t* __restrict__ b
         } else {  // b alone (a of this residue is not read)
             uint32_t x[1][8];
 #pragma unroll
-            for (int e = 0; e < 8; e++) x[0][e] = b[(size_t)kk * N + tid + e * S::T];
+            for (int e = 0; e < 8; e++) x[0][e] = __builtin_nontemporal_load(b + (size_t)kk * N + tid + e * S::T);
             __syncthreads();  // past the previous transform's exchange reads of buffer kk
             if constexpr (SGN) ntt_digits<11, 1, true, 1>(x, tid, poly[kk], C.m[kk].twm, M, C.mont[kk]);
             else ntt_digits<11, 1, false>(x, tid, poly[kk], C.m[kk].twm, M);

@@ -431,11 +431,11 @@ k_tree_combine(TreeArgs A, CtxDev C, ScaleDev inv) {
     {
         // a^ and b: s0 = even + y^ odd, s1 = even - y^ odd, slot by slot
         uint32_t eh[kTreeV], oh[kTreeV], eb[kTreeV], ob[kTreeV], yh[kTreeV];
-        load_slots_v<V>(ev + KN, t, eh);
-        load_slots_v<V>(ev + 2 * KN, t, eb);
+        load_slots_v_nt<V>(ev + KN, t, eh);
+        load_slots_v_nt<V>(ev + 2 * KN, t, eb);
         if (odd) {
-            load_slots_v<V>(od + KN, t, oh);
-            load_slots_v<V>(od + 2 * KN, t, ob);
+            load_slots_v_nt<V>(od + KN, t, oh);
+            load_slots_v_nt<V>(od + 2 * KN, t, ob);
             load_slots_v<V>(A.yhat + ((size_t)(level - 1) * K + k) * N, t, yh);
         }
         const Mod M{q, C.m[k].q2, C.m[k].qinv};
